@@ -1,0 +1,258 @@
+#!/usr/bin/env python3
+"""Benchmark of the render hot path on one MI355X.
+
+Workload (default "plain-1080p"): a seeded synthetic flame at 1920x1080,
+oversample 1, DE (9, 0, 0.4), gamma 4, vibrancy 1. A plain render at quality
+4000 is the reference image; the plain path's quality is then the smallest q
+(ladder + bisection) whose render reaches >= 38 dB PSNR (8-bit RGB) against
+it. One step = one frame at that q: reset -> chaos -> DE -> tonemap, with
+every input resident in HBM (the genome blobs are a few KB of uploads).
+value = ms/frame (lower is better); Giters/s is reported beside it.
+
+Prints ONE JSON line. --gpus above 1 exits: this build targets one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+from pyr3_amd import pack, synth  # noqa: E402
+from pyr3_amd.renderer import Renderer, pack_genome  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+REF_SEED = 0x9E3779B9
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def schedule(samples: int, fuse: int, max_walkers: int = 1 << 19, min_ratio: int = 4):
+    """Walkers x iters for a throughput launch: as many walkers as the chip
+    holds, but at least min_ratio x fuse post-fuse turns per walker."""
+    walkers = max(256, min(max_walkers, samples // (min_ratio * fuse)))
+    walkers = (walkers + 255) // 256 * 256
+    iters = -(-samples // walkers)
+    return walkers, iters
+
+
+class Frame:
+    """One plain-path frame of a genome at a quality (reset, chaos, DE, tonemap)."""
+
+    def __init__(self, r: Renderer, g, fuse: int):
+        self.r, self.g, self.fuse = r, g, fuse
+        self.packed = pack_genome(g)
+
+    def render(self, quality: float, seed: int):
+        samples_target = int(round(quality * self.r.width * self.r.height))
+        walkers, iters = schedule(samples_target, self.fuse)
+        self.r.reset(self.g)
+        s = self.r.iterate(self.g, seed, walkers, iters, fuse=self.fuse, packed=self.packed)
+        self.r.present(self.g, s)
+        return walkers, iters, s
+
+
+def psnr(a, b):
+    d = a[..., :3].astype(np.float64) - b[..., :3].astype(np.float64)
+    mse = float(np.mean(d * d))
+    return float("inf") if mse == 0 else 10.0 * math.log10(255.0 ** 2 / mse)
+
+
+def pick_genome(r: Renderer, fuse: int, width: int, height: int, k0: int = 0):
+    """First seeded genome whose attractor covers >= 5% of the frame at q20."""
+    for k in range(k0, k0 + 64):
+        g = synth.bench_genome(k, width, height)
+        f = Frame(r, g, fuse)
+        f.render(20, REF_SEED + k)
+        img = r.read_image()
+        h = r.read_histogram()
+        cover = float(np.mean(h[..., 3] > 0))
+        if cover >= 0.05 and img[..., :3].max() > 0:
+            return k, g, cover
+    raise RuntimeError("no synthetic genome covered 5% of the frame")
+
+
+def quality_to_target(frame: Frame, ref_img, target_db: float):
+    ladder = [5, 10, 20, 40, 80, 160, 320, 640, 1280, 2560]
+    prev, found, scores = None, None, {}
+    for q in ladder:
+        frame.render(q, 1000 + q)
+        scores[q] = psnr(frame.r.read_image(), ref_img)
+        log(f"q={q}: {scores[q]:.2f} dB")
+        if scores[q] >= target_db:
+            found = q
+            break
+        prev = q
+    if found is None:
+        return None, scores
+    lo, hi = (prev or 1), found
+    while hi - lo > max(1, lo // 8):
+        mid = (lo + hi) // 2
+        frame.render(mid, 1000 + mid)
+        scores[mid] = psnr(frame.r.read_image(), ref_img)
+        log(f"q={mid}: {scores[mid]:.2f} dB")
+        if scores[mid] >= target_db:
+            hi = mid
+        else:
+            lo = mid
+    return hi, scores
+
+
+def cpu_baseline(g, width, height, quality, fuse, threads, budget_s):
+    """Oracle (reference-faithful C++ restatement) timed on host cores."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+    oracle.load()
+    e = pack.expand_genome_for_gpu(g)
+    xf, ds, pal = pack.pack_xforms(e), pack.pack_xform_distrib(e), pack.pack_palette(e)
+    # calibrate the chaos rate on a small probe, then size the sample to the budget
+    probe = 2_000_000
+    walkers = 1024
+    iters = probe // walkers
+    u = pack.chaos_uniforms(e, width, height, 1, iters, fuse, 7, walkers)
+    t0 = time.perf_counter()
+    oracle.chaos_mt(xf, ds, u, pal, threads)
+    rate = walkers * (iters + fuse) / (time.perf_counter() - t0)
+    iters = max(64, int(rate * budget_s * 0.6 / walkers) - fuse)
+    u = pack.chaos_uniforms(e, width, height, 1, iters, fuse, 7, walkers)
+    t0 = time.perf_counter()
+    hist = oracle.chaos_mt(xf, ds, u, pal, threads)
+    t_chaos = time.perf_counter() - t0
+    samples_cpu = walkers * iters
+    samples_full = quality * width * height
+    t_chaos_full = t_chaos * (samples_full / samples_cpu) * ((iters + fuse) / iters)
+    k1, k2 = pack.calibration(g.scale, samples_cpu, g.tonemap.brightness, 1)
+    t0 = time.perf_counter()
+    filt = oracle.density(hist, pack.de_uniforms(width, height, g.density.max_rad,
+                                                 g.density.min_rad, g.density.curve, k1, k2, 1),
+                          threads)
+    taps = pack.spatial_kernel(0.5, 1)
+    oracle.tonemap(pack.viz_uniforms(width, height, k1, k2, g.tonemap, 1, len(taps)), taps,
+                   filtered=filt, threads=threads)
+    t_img = time.perf_counter() - t0
+    ms = (t_chaos_full + t_img) * 1e3
+    return {
+        "value": round(ms, 1), "unit": "ms/frame", "cores": threads, "kind": "port",
+        "sample": (f"oracle chaos {walkers}x({iters}+{fuse}) turns = {samples_cpu} samples at "
+                   f"{width}x{height} in {t_chaos:.2f}s ({rate / 1e6:.1f} Miters/s), extrapolated "
+                   f"linearly to q={quality} ({samples_full:.3g} samples); DE+tonemap measured at "
+                   f"full size in {t_img:.2f}s"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--ref-quality", type=float, default=4000)
+    ap.add_argument("--target-db", type=float, default=38.0)
+    ap.add_argument("--fuse", type=int, default=pack.FUSE)
+    ap.add_argument("--quality", type=float, default=None,
+                    help="skip the PSNR search and time this quality")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build targets one MI355X; --gpus {a.gpus} is not supported "
+              "(the multi-GPU design is described, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    W, H = a.width, a.height
+    r = Renderer(W, H, 1)
+    k, g, cover = pick_genome(r, a.fuse, W, H)
+    log(f"genome k={k} ({len(g.xforms)} xforms, cover {cover:.2%})")
+    frame = Frame(r, g, a.fuse)
+
+    scores = {}
+    if a.quality is None:
+        t0 = time.perf_counter()
+        frame.render(a.ref_quality, REF_SEED)
+        ref_img = r.read_image()
+        log(f"reference q={a.ref_quality:g} rendered in {time.perf_counter() - t0:.2f}s")
+        q, scores = quality_to_target(frame, ref_img, a.target_db)
+        if q is None:
+            q = max(scores)
+            log(f"no quality below {q} reached {a.target_db} dB; timing q={q}")
+    else:
+        q = a.quality
+
+    for i in range(a.warmup):
+        frame.render(q, 77 + i)
+    r.sync()
+    r.set_timing(True)
+    chaos_ms = de_ms = tm_ms = reset_ms = 0.0
+    t0 = time.perf_counter()
+    walkers = iters = samples = 0
+    for i in range(a.steps):
+        walkers, iters, samples = frame.render(q, 500 + i)
+        r.sync()
+        t = r.read_timings()
+        chaos_ms += t["chaos"]; de_ms += t["density"]; tm_ms += t["tonemap"]; reset_ms += t["reset"]
+    r.sync()
+    wall = time.perf_counter() - t0
+    ms = wall * 1e3 / a.steps
+    chaos_ms /= a.steps; de_ms /= a.steps; tm_ms /= a.steps; reset_ms /= a.steps
+    giters = walkers * (iters + a.fuse) / (chaos_ms * 1e-3) / 1e9
+    gsamples = samples / (chaos_ms * 1e-3) / 1e9
+    bins = W * H
+    scatter_bytes = 16.0 * samples
+    de_bytes = 32.0 * bins
+    tm_bytes = 16.0 * bins + 4.0 * W * H
+    achieved = scatter_bytes / (chaos_ms * 1e-3) / 1e9
+    out = {
+        "metric": "ms/frame to >=38 dB vs q4000 @1080p (plain path); Giters/s",
+        "value": round(ms, 3),
+        "unit": "ms/frame",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+u32",
+        "data": "synthetic",
+        "config": {"workload": f"plain-{W}x{H}-os1", "genome_seed": k, "quality": q,
+                   "target_db": a.target_db, "ref_quality": a.ref_quality,
+                   "walkers": walkers, "iters": iters, "fuse": a.fuse,
+                   "psnr_ladder": {str(kq): round(v, 2) for kq, v in sorted(scores.items())}},
+        "giters_per_s": round(giters, 3),
+        "gsamples_per_s": round(gsamples, 3),
+        "kernels_ms": {"chaos": round(chaos_ms, 4), "density": round(de_ms, 4),
+                       "tonemap": round(tm_ms, 4), "reset": round(reset_ms, 4)},
+        "roofline": {"bound": "hbm", "kernel": "chaos_kernel", "achieved": round(achieved, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "note": "16 B of u32 atomic scatter per post-fuse sample; the kernel is "
+                             "VALU/atomic-latency bound, not HBM bound"},
+        "de_roofline": {"achieved": round(de_bytes / (de_ms * 1e-3) / 1e9, 1) if de_ms else None,
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(de_bytes / (de_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                        if de_ms else None},
+        "tonemap_roofline": {"achieved": round(tm_bytes / (tm_ms * 1e-3) / 1e9, 1) if tm_ms else None,
+                             "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": round(tm_bytes / (tm_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                             if tm_ms else None},
+    }
+    if not a.no_cpu_baseline:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(g, W, H, q, a.fuse, a.cpu_threads, a.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

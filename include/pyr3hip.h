@@ -1,0 +1,126 @@
+/*
+ * pyr3hip — C ABI of the MI355X (gfx950) render path.
+ *
+ * Drop-in for the reference's stage interface (src/renderer.ts:163-212,
+ * createRenderer / reset / iterate / present / readHistogram) and its pass
+ * drivers (src/chaos.ts ChaosPass, src/density.ts DensityPass,
+ * src/visualize.ts VisualizePass). The host packs the same byte blobs the
+ * reference uploads to WebGPU (464-byte xform records, the u32 xform-pick
+ * table, the 128-byte chaos uniform block, the 32-byte DE block, the 96-byte
+ * tonemap block, the 256 x RGBA f32 palette) and hands them over by pointer.
+ *
+ * Device buffers are owned by the renderer handle. Every call is ordered on
+ * the handle's HIP stream and returns immediately unless it reads back to
+ * host memory. All functions return 0 on success or a negative P3_E* code;
+ * p3_last_error() gives the message. A handle is not re-entrant: one host
+ * thread at a time.
+ */
+#ifndef PYR3HIP_H
+#define PYR3HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define P3_OK 0
+#define P3_E_ARG (-1)      /* invalid argument (null, size mismatch, out of range) */
+#define P3_E_HIP (-2)      /* HIP runtime failure */
+#define P3_E_STATE (-3)    /* call out of order (e.g. present before reset) */
+
+#define P3_XFORM_FLOATS 116
+#define P3_GRAIN 16384
+#define P3_MAX_XFORMS 128
+#define P3_CHAOS_UNIFORM_WORDS 32
+#define P3_DE_UNIFORM_WORDS 8
+#define P3_VIZ_UNIFORM_WORDS 24
+#define P3_TRACE_FLOATS 16
+
+typedef struct p3_renderer p3_renderer;
+
+/* createRenderer (src/renderer.ts:214, buildPipelines :332-376).
+ * width/height: output pixels; oversample >= 1; taps: the spatial-filter
+ * kernel (src/spatial-filter.ts:217-256), n_taps in [3, 17]. Allocates the
+ * super-res histogram (W*os x H*os x 4 u32), the filtered grid (x 4 f32),
+ * the RGBA8 image, and builds the DE kernel-norm LUT (src/density.ts:78-99). */
+int p3_create(int width, int height, int oversample, const float* taps, int n_taps,
+              p3_renderer** out);
+
+/* Renderer.resize (src/renderer.ts:277-292): rebuild buffers at a new size. */
+int p3_resize(p3_renderer* r, int width, int height, int oversample, const float* taps,
+              int n_taps);
+
+/* Renderer.destroy. Null is accepted. */
+void p3_destroy(p3_renderer* r);
+
+/* Renderer.reset (src/renderer.ts:222-225, chaos.ts:309-320): upload the
+ * 256 x RGBA f32 palette LUT and zero the histogram (device-side fill). */
+int p3_reset(p3_renderer* r, const float* palette_rgba);
+
+/* Renderer.iterate -> ChaosPass.dispatch (src/chaos.ts:333-466): one chaos
+ * dispatch accumulating into the histogram.
+ *   xforms:   n_slots x 116 f32 (regular xforms, then the final xform if any)
+ *   distrib:  n_rows x 16384 u32; rows 0..n_rows-2 are "previous xform = i",
+ *             the last row is the no-previous fallback (row 128 in the
+ *             reference's 129-row buffer)
+ *   uniforms: the 32-word chaos uniform block; slot 7 (seed) and slot 12
+ *             (walker count) seed walker w's ISAAC stream on the device, bit
+ *             for bit as packIsaacStates (src/isaac.ts:194-229) would. */
+int p3_iterate(p3_renderer* r, const float* xforms, int n_slots, const uint32_t* distrib,
+               int n_rows, const uint32_t* uniforms);
+
+/* As p3_iterate, but walkers take explicit ISAAC states (walkers x 36 u32,
+ * the reference's isaac_states buffer layout) instead of device seeding. */
+int p3_iterate_states(p3_renderer* r, const float* xforms, int n_slots, const uint32_t* distrib,
+                      int n_rows, const uint32_t* uniforms, const uint32_t* isaac_states);
+
+/* Renderer.present (src/renderer.ts:241-262): DE (when de_uniforms is
+ * non-null, the 8-word DEUniforms block) then the tonemap (the 24-word
+ * VizUniforms block) into the renderer-owned RGBA8 image. */
+int p3_present(p3_renderer* r, const uint32_t* de_uniforms, const uint32_t* viz_uniforms);
+
+/* Readbacks (block until the stream has drained). n is the element count of
+ * the destination and must match the buffer exactly. */
+int p3_read_histogram(p3_renderer* r, uint32_t* out, size_t n);  /* readHistogramRgba */
+int p3_read_filtered(p3_renderer* r, float* out, size_t n);      /* DE output grid */
+int p3_read_image(p3_renderer* r, uint8_t* out, size_t n);       /* W x H x 4 RGBA8 */
+int p3_read_image_f32(p3_renderer* r, float* out, size_t n);     /* pre-quantisation RGBA */
+
+/* Upload a histogram (tests: DE/tonemap on an identical histogram). */
+int p3_write_histogram(p3_renderer* r, const uint32_t* data, size_t n);
+
+/* Trace mode (src/shaders/chaos.wgsl:8412-8429 generalised): record walkers
+ * [0, walkers) for loop indices [0, iters) (fuse included), 16 f32 per entry:
+ * i, pick, pa.xy, pv_pre.xy, pv.xy, bad, z, draws(u32 bits), pick draw (u32
+ * bits), p.xy after jitter, splat.xy. walkers = 0 turns it off. */
+int p3_set_trace(p3_renderer* r, int walkers, int iters);
+int p3_read_trace(p3_renderer* r, float* out, size_t n);
+
+/* Kernel timing with HIP events on the handle's stream: when enabled, each
+ * launch is bracketed and p3_read_timings returns the last durations (ms) of
+ * [seed, chaos, density, tonemap, reset]. */
+int p3_set_timing(p3_renderer* r, int enabled);
+int p3_read_timings(p3_renderer* r, float* out5);
+
+/* Block until every queued call on the handle has completed. */
+int p3_sync(p3_renderer* r);
+
+/* Geometry of the handle: out4 = {width, height, super_w, super_h}. */
+int p3_dims(p3_renderer* r, int* out4);
+
+/* 1 if the kernels implement catalog entry `index` (src/variations.ts:24 V
+ * table), else 0. p3_iterate rejects xform records that use an entry the
+ * kernels do not implement instead of rendering it as (0, 0). */
+int p3_variation_supported(int index);
+
+/* Message for the last failing call on this handle (or of p3_create when
+ * r is null). Never null. */
+const char* p3_last_error(p3_renderer* r);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYR3HIP_H */

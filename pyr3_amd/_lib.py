@@ -1,0 +1,76 @@
+"""ctypes binding of libpyr3hip.so (the C ABI in include/pyr3hip.h).
+
+The library is the only compute path: there is no CPU fallback. Loading fails
+with an ImportError naming the missing file when the in-tree build is absent,
+so a host without the HIP extension stops instead of silently degrading.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libpyr3hip.so"
+
+# Every entry point the header declares: name -> (restype, argtypes).
+_P = C.c_void_p
+_I = C.c_int
+_SZ = C.c_size_t
+_FP = C.POINTER(C.c_float)
+_UP = C.POINTER(C.c_uint32)
+_BP = C.POINTER(C.c_uint8)
+SIGNATURES = {
+    "p3_create": (_I, [_I, _I, _I, _FP, _I, C.POINTER(_P)]),
+    "p3_resize": (_I, [_P, _I, _I, _I, _FP, _I]),
+    "p3_destroy": (None, [_P]),
+    "p3_reset": (_I, [_P, _FP]),
+    "p3_iterate": (_I, [_P, _FP, _I, _UP, _I, _UP]),
+    "p3_iterate_states": (_I, [_P, _FP, _I, _UP, _I, _UP, _UP]),
+    "p3_present": (_I, [_P, _UP, _UP]),
+    "p3_read_histogram": (_I, [_P, _UP, _SZ]),
+    "p3_read_filtered": (_I, [_P, _FP, _SZ]),
+    "p3_read_image": (_I, [_P, _BP, _SZ]),
+    "p3_read_image_f32": (_I, [_P, _FP, _SZ]),
+    "p3_write_histogram": (_I, [_P, _UP, _SZ]),
+    "p3_set_trace": (_I, [_P, _I, _I]),
+    "p3_read_trace": (_I, [_P, _FP, _SZ]),
+    "p3_set_timing": (_I, [_P, _I]),
+    "p3_read_timings": (_I, [_P, _FP]),
+    "p3_sync": (_I, [_P]),
+    "p3_dims": (_I, [_P, C.POINTER(_I)]),
+    "p3_variation_supported": (_I, [_I]),
+    "p3_last_error": (C.c_char_p, [_P]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load the in-tree library once; raise ImportError if it was never built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"pyr3_amd: {LIB_PATH} is missing; build it with `python -m pyr3_amd.build` "
+            "(there is no CPU fallback for the render path)")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class P3Error(RuntimeError):
+    """A non-zero status from the C ABI, carrying p3_last_error()'s text."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"pyr3hip error {code}: {message}")
+        self.code = code
+
+
+def check(code: int, handle=None) -> None:
+    if code != 0:
+        msg = load().p3_last_error(handle)
+        raise P3Error(code, msg.decode() if msg else "")

@@ -1,0 +1,510 @@
+// C ABI of the gfx950 render path (declared in include/pyr3hip.h).
+//
+// A p3_renderer owns one HIP stream and every device buffer of the path:
+// the super-res u32 RGBA histogram, the DE staging (pre-scaled RGBA, radius
+// bytes, per-tile max radius), the filtered grid, the RGBA8 and RGBA f32
+// images, and small uploads (xform records, pick table, palette, taps,
+// kernel-norm LUT, optional ISAAC states and trace). Host blobs are
+// validated before any launch so a malformed pick table or xform index is an
+// error return, never an out-of-bounds device read.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pyr3hip.h"
+#include "p3_launch.h"
+
+
+namespace {
+
+constexpr int kTimers = 5;  // seed, chaos, density, tonemap, reset
+enum { T_SEED = 0, T_CHAOS = 1, T_DE = 2, T_TM = 3, T_RESET = 4 };
+
+std::string g_create_error = "";
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  hipError_t ensure(size_t count) {
+    if (count <= n && p) return hipSuccess;
+    if (p) { (void)hipFree(p); p = nullptr; n = 0; }
+    if (count == 0) return hipSuccess;
+    hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+    if (e == hipSuccess) n = count;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+float bits_f(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
+
+// Catalog entries the chaos kernel implements (index 97 is pre_blur, applied
+// by the chain itself).
+bool supported(uint32_t idx) {
+  return idx <= 2u || idx == 97u;
+}
+
+// Every active slot of every record must name an implemented entry.
+int check_records(p3_renderer* r, const float* xforms, int n_slots);
+
+}  // namespace
+
+struct p3_renderer {
+  int width = 0, height = 0, os = 1, sw = 0, sh = 0;
+  int n_taps = 0;
+  float taps_host[17] = {0};
+  hipStream_t stream = nullptr;
+  DevBuf<uint32_t> hist;
+  DevBuf<float4> filtered, pre, imagef;
+  DevBuf<uint8_t> rad;
+  DevBuf<uint32_t> tile_rmax, image8;
+  DevBuf<float> taps, lut, palette, xforms;
+  DevBuf<uint32_t> distrib, isaac;
+  DevBuf<float> trace;
+  bool palette_set = false;
+  bool have_image = false;
+  bool have_filtered = false;
+  unsigned long long hist_bound = 0;  // upper bound on any histogram word
+  int trace_walkers = 0, trace_iters = 0;
+  bool timing = false;
+  hipEvent_t ev[kTimers][2] = {};
+  bool ev_used[kTimers] = {};
+  std::string err;
+
+  int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    err = buf;
+    return code;
+  }
+  int hip(hipError_t e, const char* what) {
+    if (e == hipSuccess) return P3_OK;
+    return fail(P3_E_HIP, "%s: %s", what, hipGetErrorString(e));
+  }
+  void tic(int t) {
+    if (timing) { (void)hipEventRecord(ev[t][0], stream); }
+  }
+  void toc(int t) {
+    if (timing) { (void)hipEventRecord(ev[t][1], stream); ev_used[t] = true; }
+  }
+};
+
+namespace {
+
+int build(p3_renderer* r, int width, int height, int os, const float* taps, int n_taps) {
+  if (width <= 0 || height <= 0 || os < 1 || os > 16)
+    return r->fail(P3_E_ARG, "invalid size %dx%d oversample %d", width, height, os);
+  if (!taps || n_taps < 3 || n_taps > 17)
+    return r->fail(P3_E_ARG, "taps must hold 3..17 values (got %d)", n_taps);
+  const long long sw = (long long)width * os, sh = (long long)height * os;
+  if (sw * sh > (1LL << 31))
+    return r->fail(P3_E_ARG, "super-res grid %lldx%lld too large", sw, sh);
+  r->width = width; r->height = height; r->os = os; r->sw = (int)sw; r->sh = (int)sh;
+  r->n_taps = n_taps;
+  std::memcpy(r->taps_host, taps, sizeof(float) * n_taps);
+  const size_t bins = (size_t)sw * sh;
+  const size_t tiles = (size_t)((sw + 15) / 16) * ((sh + 15) / 16);
+  int rc;
+  if ((rc = r->hip(r->hist.ensure(bins * 4), "hipMalloc(histogram)"))) return rc;
+  if ((rc = r->hip(r->filtered.ensure(bins), "hipMalloc(filtered)"))) return rc;
+  if ((rc = r->hip(r->pre.ensure(bins), "hipMalloc(de pre)"))) return rc;
+  if ((rc = r->hip(r->rad.ensure(bins), "hipMalloc(de radius)"))) return rc;
+  if ((rc = r->hip(r->tile_rmax.ensure(tiles), "hipMalloc(de tiles)"))) return rc;
+  if ((rc = r->hip(r->image8.ensure((size_t)width * height), "hipMalloc(image)"))) return rc;
+  if ((rc = r->hip(r->imagef.ensure((size_t)width * height), "hipMalloc(image f32)"))) return rc;
+  if ((rc = r->hip(r->taps.ensure(17), "hipMalloc(taps)"))) return rc;
+  if ((rc = r->hip(r->lut.ensure(257), "hipMalloc(lut)"))) return rc;
+  if ((rc = r->hip(r->palette.ensure(1024), "hipMalloc(palette)"))) return rc;
+  if ((rc = r->hip(r->xforms.ensure((size_t)(P3_MAX_XFORMS + 1) * P3_XFORM_FLOATS),
+                   "hipMalloc(xforms)")))
+    return rc;
+  if ((rc = r->hip(r->distrib.ensure((size_t)(P3_MAX_XFORMS + 1) * P3_GRAIN),
+                   "hipMalloc(distrib)")))
+    return rc;
+  if ((rc = r->hip(hipMemcpyAsync(r->taps.p, taps, sizeof(float) * n_taps, hipMemcpyHostToDevice,
+                                  r->stream),
+                   "upload taps")))
+    return rc;
+  // Per-radius kernel sums over the integer disc, sigma = r / 3, in f64
+  // (src/density.ts:78-99), stored as f32.
+  float lut[257];
+  lut[0] = 1.0f;
+  for (int rr = 1; rr <= 256; ++rr) {
+    const double sigma = rr / 3.0, inv = 1.0 / (2.0 * sigma * sigma), r2 = (double)rr * rr;
+    double s = 0.0;
+    for (int dy = -rr; dy <= rr; ++dy)
+      for (int dx = -rr; dx <= rr; ++dx) {
+        const double d2 = (double)dx * dx + (double)dy * dy;
+        if (d2 <= r2) s += std::exp(-d2 * inv);
+      }
+    lut[rr] = (float)s;
+  }
+  if ((rc = r->hip(hipMemcpyAsync(r->lut.p, lut, sizeof lut, hipMemcpyHostToDevice, r->stream),
+                   "upload lut")))
+    return rc;
+  if ((rc = r->hip(hipStreamSynchronize(r->stream), "sync after create"))) return rc;
+  r->palette_set = false;
+  r->have_image = r->have_filtered = false;
+  r->hist_bound = 0;
+  return P3_OK;
+}
+
+void release(p3_renderer* r) {
+  r->hist.release(); r->filtered.release(); r->pre.release(); r->rad.release();
+  r->tile_rmax.release(); r->image8.release(); r->imagef.release(); r->taps.release();
+  r->lut.release(); r->palette.release(); r->xforms.release(); r->distrib.release();
+  r->isaac.release(); r->trace.release();
+}
+
+int check_records(p3_renderer* r, const float* xforms, int n_slots) {
+  for (int s = 0; s < n_slots; ++s) {
+    const float* xf = xforms + (size_t)s * P3_XFORM_FLOATS;
+    const float nvf = xf[7];
+    if (!(nvf >= 0.0f) || nvf > 1e9f)
+      return r->fail(P3_E_ARG, "xform %d: bad variation count", s);
+    const int nv = std::min((int)nvf, 8);
+    for (int k = 0; k < nv; ++k) {
+      const float idxf = xf[20 + 4 * k];
+      if (!(idxf >= 0.0f) || idxf > 1e9f || !supported((uint32_t)idxf))
+        return r->fail(P3_E_ARG, "xform %d slot %d: catalog entry %g is not implemented", s, k,
+                       (double)idxf);
+    }
+  }
+  return P3_OK;
+}
+
+int iterate_impl(p3_renderer* r, const float* xforms, int n_slots, const uint32_t* distrib,
+                 int n_rows, const uint32_t* u, const uint32_t* states) {
+  if (!r) return P3_E_ARG;
+  if (!r->palette_set) return r->fail(P3_E_STATE, "iterate before reset (no palette)");
+  if (!xforms || !distrib || !u) return r->fail(P3_E_ARG, "null argument");
+  if (n_rows < 2 || n_rows > P3_MAX_XFORMS + 1)
+    return r->fail(P3_E_ARG, "n_rows must be 2..%d (got %d)", P3_MAX_XFORMS + 1, n_rows);
+  const int n_regular = n_rows - 1;
+  if (n_slots < n_regular || n_slots > n_regular + 1)
+    return r->fail(P3_E_ARG, "n_slots %d inconsistent with %d regular xforms", n_slots, n_regular);
+  const int32_t final_idx = (int32_t)u[8];
+  if (final_idx != -1 && (final_idx < 0 || final_idx >= n_slots))
+    return r->fail(P3_E_ARG, "final xform slot %d outside %d slots", final_idx, n_slots);
+  if ((int)u[0] != r->sw || (int)u[1] != r->sh)
+    return r->fail(P3_E_ARG, "uniform size %ux%u does not match super-res grid %dx%d", u[0], u[1],
+                   r->sw, r->sh);
+  const uint32_t walkers = u[12];
+  if (walkers == 0) return P3_OK;
+  int rc0 = check_records(r, xforms, n_slots);
+  if (rc0) return rc0;
+  const size_t table = (size_t)n_rows * P3_GRAIN;
+  for (size_t i = 0; i < table; ++i)
+    if (distrib[i] >= (uint32_t)n_regular)
+      return r->fail(P3_E_ARG, "pick table entry %zu = %u outside %d xforms", i, distrib[i],
+                     n_regular);
+  int rc;
+  if ((rc = r->hip(hipMemcpyAsync(r->xforms.p, xforms, sizeof(float) * P3_XFORM_FLOATS * n_slots,
+                                  hipMemcpyHostToDevice, r->stream),
+                   "upload xforms")))
+    return rc;
+  if ((rc = r->hip(hipMemcpyAsync(r->distrib.p, distrib, sizeof(uint32_t) * table,
+                                  hipMemcpyHostToDevice, r->stream),
+                   "upload pick table")))
+    return rc;
+  p3::ChaosArgs a{};
+  a.isaac_in = nullptr;
+  if (states) {
+    if ((rc = r->hip(r->isaac.ensure((size_t)walkers * 36), "hipMalloc(isaac)"))) return rc;
+    if ((rc = r->hip(hipMemcpyAsync(r->isaac.p, states, sizeof(uint32_t) * 36 * walkers,
+                                    hipMemcpyHostToDevice, r->stream),
+                     "upload isaac states")))
+      return rc;
+    a.isaac_in = r->isaac.p;
+  }
+  a.trace = nullptr;
+  if (r->trace_walkers > 0 && r->trace_iters > 0) {
+    const size_t n = (size_t)r->trace_walkers * r->trace_iters * P3_TRACE_FLOATS;
+    if ((rc = r->hip(r->trace.ensure(n), "hipMalloc(trace)"))) return rc;
+    if ((rc = r->hip(p3::launch_fill(r->trace.p, n * 4, 0u, r->stream), "clear trace"))) return rc;
+    a.trace = r->trace.p;
+    a.trace_walkers = (uint32_t)r->trace_walkers;
+    a.trace_iters = (uint32_t)r->trace_iters;
+  }
+  a.xforms = r->xforms.p;
+  a.distrib = r->distrib.p;
+  a.palette = (const float4*)r->palette.p;
+  a.hist = r->hist.p;
+  a.fallback_row = (uint32_t)n_regular;
+  a.width = u[0]; a.height = u[1]; a.iters = u[2]; a.fuse = u[3];
+  a.scale = bits_f(u[4]); a.cx = bits_f(u[5]); a.cy = bits_f(u[6]);
+  a.seed = u[7];
+  a.final_idx = final_idx;
+  const float rot = bits_f(u[9]);
+  a.cos_r = std::cos(rot);
+  a.sin_r = std::sin(rot);
+  a.palette_mode = u[10];
+  a.walkers = walkers;
+  a.jitter = bits_f(u[13]);
+  a.xform_blend = bits_f(u[30]);
+  // Each walker splats at most `iters` times, each adding <= 255 per word.
+  r->hist_bound += 255ULL * (unsigned long long)walkers * (unsigned long long)a.iters;
+  const bool sat = r->hist_bound >= 0xffffffffULL;
+  r->tic(T_CHAOS);
+  if ((rc = r->hip(p3::launch_chaos(a, sat, r->stream), "chaos launch"))) return rc;
+  r->toc(T_CHAOS);
+  r->have_image = false;
+  return P3_OK;
+}
+
+int read_dev(p3_renderer* r, void* dst, const void* src, size_t bytes, const char* what) {
+  int rc = r->hip(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, r->stream), what);
+  if (rc) return rc;
+  return r->hip(hipStreamSynchronize(r->stream), what);
+}
+
+}  // namespace
+
+extern "C" {
+
+int p3_create(int width, int height, int oversample, const float* taps, int n_taps,
+              p3_renderer** out) {
+  if (!out) { g_create_error = "null output handle"; return P3_E_ARG; }
+  *out = nullptr;
+  if (width <= 0 || height <= 0 || oversample < 1 || oversample > 16 || !taps || n_taps < 3 ||
+      n_taps > 17) {
+    g_create_error = "invalid size, oversample or taps";
+    return P3_E_ARG;
+  }
+  p3_renderer* r = new p3_renderer();
+  hipError_t e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    g_create_error = std::string("hipStreamCreate: ") + hipGetErrorString(e);
+    delete r;
+    return P3_E_HIP;
+  }
+  for (int t = 0; t < kTimers; ++t) {
+    (void)hipEventCreate(&r->ev[t][0]);
+    (void)hipEventCreate(&r->ev[t][1]);
+  }
+  int rc = build(r, width, height, oversample, taps, n_taps);
+  if (rc) {
+    g_create_error = r->err;
+    p3_destroy(r);
+    return rc;
+  }
+  *out = r;
+  return P3_OK;
+}
+
+int p3_resize(p3_renderer* r, int width, int height, int oversample, const float* taps,
+              int n_taps) {
+  if (!r) return P3_E_ARG;
+  (void)hipStreamSynchronize(r->stream);
+  release(r);
+  return build(r, width, height, oversample, taps, n_taps);
+}
+
+void p3_destroy(p3_renderer* r) {
+  if (!r) return;
+  if (r->stream) (void)hipStreamSynchronize(r->stream);
+  release(r);
+  for (int t = 0; t < kTimers; ++t) {
+    if (r->ev[t][0]) (void)hipEventDestroy(r->ev[t][0]);
+    if (r->ev[t][1]) (void)hipEventDestroy(r->ev[t][1]);
+  }
+  if (r->stream) (void)hipStreamDestroy(r->stream);
+  delete r;
+}
+
+int p3_reset(p3_renderer* r, const float* palette_rgba) {
+  if (!r) return P3_E_ARG;
+  if (!palette_rgba) return r->fail(P3_E_ARG, "null palette");
+  int rc;
+  if ((rc = r->hip(hipMemcpyAsync(r->palette.p, palette_rgba, sizeof(float) * 1024,
+                                  hipMemcpyHostToDevice, r->stream),
+                   "upload palette")))
+    return rc;
+  r->tic(T_RESET);
+  if ((rc = r->hip(p3::launch_fill(r->hist.p, r->hist.n * 4, 0u, r->stream), "clear histogram")))
+    return rc;
+  r->toc(T_RESET);
+  r->palette_set = true;
+  r->hist_bound = 0;
+  r->have_image = r->have_filtered = false;
+  return P3_OK;
+}
+
+int p3_iterate(p3_renderer* r, const float* xforms, int n_slots, const uint32_t* distrib,
+               int n_rows, const uint32_t* uniforms) {
+  return iterate_impl(r, xforms, n_slots, distrib, n_rows, uniforms, nullptr);
+}
+
+int p3_iterate_states(p3_renderer* r, const float* xforms, int n_slots, const uint32_t* distrib,
+                      int n_rows, const uint32_t* uniforms, const uint32_t* isaac_states) {
+  if (!isaac_states) return r ? r->fail(P3_E_ARG, "null isaac states") : P3_E_ARG;
+  return iterate_impl(r, xforms, n_slots, distrib, n_rows, uniforms, isaac_states);
+}
+
+int p3_present(p3_renderer* r, const uint32_t* de, const uint32_t* viz) {
+  if (!r) return P3_E_ARG;
+  if (!viz) return r->fail(P3_E_ARG, "null tonemap uniforms");
+  if ((int)viz[0] != r->width || (int)viz[1] != r->height || (int)viz[8] != r->os)
+    return r->fail(P3_E_ARG, "tonemap uniforms %ux%u os %u do not match renderer %dx%d os %d",
+                   viz[0], viz[1], viz[8], r->width, r->height, r->os);
+  if ((int)viz[9] != r->n_taps)
+    return r->fail(P3_E_ARG, "tonemap fwidth %u does not match %d taps", viz[9], r->n_taps);
+  int rc;
+  if (de) {
+    if ((int)de[0] != r->sw || (int)de[1] != r->sh)
+      return r->fail(P3_E_ARG, "DE uniforms %ux%u do not match super-res grid %dx%d", de[0], de[1],
+                     r->sw, r->sh);
+    p3::DeArgs a{};
+    a.max_rad = bits_f(de[2]); a.min_rad = bits_f(de[3]); a.curve = bits_f(de[4]);
+    a.k1 = bits_f(de[5]); a.k2 = bits_f(de[6]);
+    if (!(a.max_rad >= 0.0f) || std::isinf(a.max_rad))
+      return r->fail(P3_E_ARG, "DE max radius must be finite and >= 0");
+    a.R = (int)std::ceil(std::min(std::max(a.max_rad, 0.0f), 256.0f));
+    if (a.R > 254) return r->fail(P3_E_ARG, "DE search radius %d above the 254 limit", a.R);
+    a.hist = (const uint4*)r->hist.p;
+    a.pre = r->pre.p;
+    a.rad = r->rad.p;
+    a.tile_rmax = r->tile_rmax.p;
+    a.out = r->filtered.p;
+    a.lut = r->lut.p;
+    a.width = (uint32_t)r->sw; a.height = (uint32_t)r->sh;
+    a.tiles_x = (uint32_t)((r->sw + 15) / 16);
+    a.tiles_y = (uint32_t)((r->sh + 15) / 16);
+    r->tic(T_DE);
+    if ((rc = r->hip(p3::launch_density(a, r->stream), "density launch"))) return rc;
+    r->toc(T_DE);
+    r->have_filtered = true;
+  }
+  p3::TmArgs t{};
+  t.filtered = r->filtered.p;
+  t.hist = (const uint4*)r->hist.p;
+  t.taps = r->taps.p;
+  t.out8 = r->image8.p;
+  t.outf = r->imagef.p;
+  t.width = (uint32_t)r->width; t.height = (uint32_t)r->height;
+  t.os = (uint32_t)r->os; t.fwidth = viz[9];
+  t.k1 = bits_f(viz[2]); t.k2 = bits_f(viz[3]); t.gamma = bits_f(viz[4]);
+  t.vibrancy = bits_f(viz[5]); t.highpow = bits_f(viz[6]); t.linrange = bits_f(viz[7]);
+  t.bg0 = bits_f(viz[12]); t.bg1 = bits_f(viz[13]); t.bg2 = bits_f(viz[14]);
+  t.transparent = viz[21];
+  r->tic(T_TM);
+  if ((rc = r->hip(p3::launch_tonemap(t, de != nullptr, r->stream), "tonemap launch"))) return rc;
+  r->toc(T_TM);
+  r->have_image = true;
+  return P3_OK;
+}
+
+int p3_read_histogram(p3_renderer* r, uint32_t* out, size_t n) {
+  if (!r) return P3_E_ARG;
+  if (!out || n != r->hist.n) return r->fail(P3_E_ARG, "histogram needs %zu words", r->hist.n);
+  return read_dev(r, out, r->hist.p, n * 4, "read histogram");
+}
+
+int p3_read_filtered(p3_renderer* r, float* out, size_t n) {
+  if (!r) return P3_E_ARG;
+  if (!out || n != r->filtered.n * 4)
+    return r->fail(P3_E_ARG, "filtered grid needs %zu floats", r->filtered.n * 4);
+  if (!r->have_filtered) return r->fail(P3_E_STATE, "no DE output yet");
+  return read_dev(r, out, r->filtered.p, n * 4, "read filtered");
+}
+
+int p3_read_image(p3_renderer* r, uint8_t* out, size_t n) {
+  if (!r) return P3_E_ARG;
+  if (!out || n != (size_t)r->width * r->height * 4)
+    return r->fail(P3_E_ARG, "image needs %zu bytes", (size_t)r->width * r->height * 4);
+  if (!r->have_image) return r->fail(P3_E_STATE, "present has not run since the last change");
+  return read_dev(r, out, r->image8.p, n, "read image");
+}
+
+int p3_read_image_f32(p3_renderer* r, float* out, size_t n) {
+  if (!r) return P3_E_ARG;
+  if (!out || n != (size_t)r->width * r->height * 4)
+    return r->fail(P3_E_ARG, "image needs %zu floats", (size_t)r->width * r->height * 4);
+  if (!r->have_image) return r->fail(P3_E_STATE, "present has not run since the last change");
+  return read_dev(r, out, r->imagef.p, n * 4, "read image f32");
+}
+
+int p3_write_histogram(p3_renderer* r, const uint32_t* data, size_t n) {
+  if (!r) return P3_E_ARG;
+  if (!data || n != r->hist.n) return r->fail(P3_E_ARG, "histogram needs %zu words", r->hist.n);
+  int rc = r->hip(hipMemcpyAsync(r->hist.p, data, n * 4, hipMemcpyHostToDevice, r->stream),
+                  "write histogram");
+  if (rc) return rc;
+  unsigned long long mx = 0;
+  for (size_t i = 0; i < n; ++i) mx = std::max<unsigned long long>(mx, data[i]);
+  r->hist_bound = mx;
+  r->have_image = r->have_filtered = false;
+  return r->hip(hipStreamSynchronize(r->stream), "write histogram");
+}
+
+int p3_set_trace(p3_renderer* r, int walkers, int iters) {
+  if (!r) return P3_E_ARG;
+  if (walkers < 0 || iters < 0 || (long long)walkers * iters > (1LL << 26))
+    return r->fail(P3_E_ARG, "trace %d x %d out of range", walkers, iters);
+  r->trace_walkers = walkers;
+  r->trace_iters = iters;
+  return P3_OK;
+}
+
+int p3_read_trace(p3_renderer* r, float* out, size_t n) {
+  if (!r) return P3_E_ARG;
+  const size_t need = (size_t)r->trace_walkers * r->trace_iters * P3_TRACE_FLOATS;
+  if (!out || n != need || need == 0) return r->fail(P3_E_ARG, "trace needs %zu floats", need);
+  if (r->trace.n < need) return r->fail(P3_E_STATE, "no trace recorded");
+  return read_dev(r, out, r->trace.p, need * 4, "read trace");
+}
+
+int p3_set_timing(p3_renderer* r, int enabled) {
+  if (!r) return P3_E_ARG;
+  r->timing = enabled != 0;
+  for (auto& u : r->ev_used) u = false;
+  return P3_OK;
+}
+
+int p3_read_timings(p3_renderer* r, float* out5) {
+  if (!r || !out5) return P3_E_ARG;
+  int rc = r->hip(hipStreamSynchronize(r->stream), "sync for timings");
+  if (rc) return rc;
+  for (int t = 0; t < kTimers; ++t) {
+    out5[t] = 0.0f;
+    if (r->ev_used[t]) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, r->ev[t][0], r->ev[t][1]) == hipSuccess) out5[t] = ms;
+    }
+  }
+  return P3_OK;
+}
+
+int p3_sync(p3_renderer* r) {
+  if (!r) return P3_E_ARG;
+  return r->hip(hipStreamSynchronize(r->stream), "stream sync");
+}
+
+int p3_dims(p3_renderer* r, int* out4) {
+  if (!r || !out4) return P3_E_ARG;
+  out4[0] = r->width; out4[1] = r->height; out4[2] = r->sw; out4[3] = r->sh;
+  return P3_OK;
+}
+
+int p3_variation_supported(int index) { return index >= 0 && supported((uint32_t)index) ? 1 : 0; }
+
+const char* p3_last_error(p3_renderer* r) {
+  if (!r) return g_create_error.c_str();
+  return r->err.c_str();
+}
+
+}  // extern "C"

@@ -1,0 +1,275 @@
+// Chaos-game kernel for gfx950: one walker per lane, ISAAC stream in LDS,
+// u32 RGBA histogram scatter with native global atomics.
+//
+// Behaviour follows chaos_main (src/shaders/chaos.wgsl:8269-8732):
+//   init draws rand_11, rand_11, rand01 -> (x, y, z)
+//   per turn: pick = table[row * 16384 + (irand & 16383)], row = previous
+//   accepted xform or the fallback row; chain A (affine, pre_blur, variation
+//   sum, direct colour, post-affine, colour mix); optional xform blend; bad
+//   check (NaN or |v| > 1e10) with rand_11 reseed and up to four retried
+//   turns that do not advance i (except at i = 0); final-xform lens with an
+//   opacity gate; scale-relative jitter (two rand01); splat after the fuse.
+// The splat adds u32(pal * opacity * 255) and u32(opacity * 255) to the four
+// channels of bin (y * W + x). The reference's adds saturate at 2^32 - 1; a
+// saturating add is order-independent (the final value is min(sum, 2^32-1)),
+// so it is reproduced with a native atomic add whose returned old value
+// detects a wrap, followed by an exchange to 2^32 - 1: after the last wrap
+// of a bin the last operation on it is always that exchange. When the host
+// has proven that no bin can reach 2^32 (bound on 255 x samples since the
+// last reset) the adds are issued without a return value.
+#include "p3_device.h"
+#include "p3_launch.h"
+#include "p3_variations.h"
+
+namespace p3 {
+
+struct Chain {
+  float2 pv, pv_pre, pa;
+  float z;
+};
+
+__device__ inline int nvars_of(const float* xf) {
+  const uint32_t n = f2u_sat(xf[7]);
+  return (int)(n < (uint32_t)kMaxVars ? n : (uint32_t)kMaxVars);
+}
+
+__device__ inline void load_var(const float* xf, int k, VarIn& a) {
+  const float* v = xf + 20 + k * 4;
+  const float* ve = xf + 52 + k * 4;
+  const float* ve2 = xf + 84 + k * 4;
+  a.w = v[1];
+  a.p[0] = v[2]; a.p[1] = v[3];
+  a.p[2] = ve[0]; a.p[3] = ve[1]; a.p[4] = ve[2]; a.p[5] = ve[3];
+  a.p[6] = ve2[0]; a.p[7] = ve2[1]; a.p[8] = ve2[2]; a.p[9] = ve2[3];
+  a.xf = xf;
+}
+
+// chaos.wgsl:8164-8181
+__device__ inline float2 pre_blur(const float* xf, int nv, float2 c, Rng& rng) {
+  for (int k = 0; k < nv; ++k) {
+    const float* v = xf + 20 + k * 4;
+    if (f2u_sat(v[0]) == 97u) {
+      const float r0 = rng.rand01(), r1 = rng.rand01(), r2 = rng.rand01(), r3 = rng.rand01();
+      const float g = v[1] * (r0 + r1 + r2 + r3 - 2.0f);
+      const float r4 = rng.rand01();
+      const float ang = r4 * kTau;
+      c.x = c.x + cosf(ang) * g;
+      c.y = c.y + sinf(ang) * g;
+    }
+  }
+  return c;
+}
+
+// Variation sum with the "last active direct colour wins" override.
+__device__ inline float2 var_sum(const float* xf, int nv, float2 pa, Rng& rng, bool& dc_on,
+                                 float3& dc_rgb) {
+  float2 pv = make_float2(0.0f, 0.0f);
+  VarIn a;
+  const bool dc_flag = xf[11] > 0.5f;
+  for (int k = 0; k < nv; ++k) {
+    const float* v = xf + 20 + k * 4;
+    const int idx = (int)f2u_sat(v[0]);
+    load_var(xf, k, a);
+    if (idx != 97) {
+      const float2 o = apply_variation(idx, pa, a, rng);
+      pv.x = pv.x + o.x;
+      pv.y = pv.y + o.y;
+    }
+    if (dc_flag && a.w > 0.0f) {
+      float3 rgb;
+      if (dc_color_for(idx, pa, a, rgb)) {
+        dc_on = true;
+        dc_rgb = rgb;
+      }
+    }
+  }
+  return pv;
+}
+
+__device__ inline float2 post_affine(const float* xf, float2 pv) {
+  if (xf[15] != 0.0f) {
+    return make_float2(xf[12] * pv.x + xf[13] * pv.y + xf[14],
+                       xf[16] * pv.x + xf[17] * pv.y + xf[18]);
+  }
+  return pv;
+}
+
+// chaos.wgsl:8226-8267
+__device__ inline Chain xform_chain(const float* xf, float x, float y, float z, Rng& rng,
+                                    bool& dc_on, float3& dc_rgb) {
+  Chain c;
+  c.pa = make_float2(xf[0] * x + xf[1] * y + xf[2], xf[4] * x + xf[5] * y + xf[6]);
+  const int nv = nvars_of(xf);
+  const float2 pam = pre_blur(xf, nv, c.pa, rng);
+  const float2 pv = var_sum(xf, nv, pam, rng, dc_on, dc_rgb);
+  c.pv_pre = pv;
+  c.pv = post_affine(xf, pv);
+  c.z = mixf(z, xf[8], xf[9]);
+  return c;
+}
+
+__device__ inline bool is_bad(float2 v) {
+  return isnan(v.x) || isnan(v.y) || fabsf(v.x) > 1e10f || fabsf(v.y) > 1e10f;
+}
+
+template <bool kSat>
+__device__ inline void hist_add(uint32_t* slot, uint32_t d) {
+  if (d == 0u) return;
+  if (kSat) {
+    const uint32_t old = atomicAdd(slot, d);
+    if (old > 0xffffffffu - d) atomicExch(slot, 0xffffffffu);
+  } else {
+    atomicAdd(slot, d);
+  }
+}
+
+template <bool kSat>
+__global__ void __launch_bounds__(256) chaos_kernel(ChaosArgs A) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= A.walkers) return;
+
+  Rng rng;
+  rng.m.base = lds + threadIdx.x;
+  rng.m.stride = (int)blockDim.x;
+  rng.draws = 0u;
+  if (A.isaac_in) {
+    const uint32_t* st = A.isaac_in + (size_t)w * 36u;
+    rng.s.cnt = st[0]; rng.s.a = st[1]; rng.s.b = st[2]; rng.s.c = st[3];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) { rng.m.mem(k) = st[4 + k]; rng.m.rsl(k) = st[20 + k]; }
+  } else {
+    isaac_seed(rng.s, rng.m, A.seed, w);
+  }
+
+  float x = rng.rand11();
+  float y = rng.rand11();
+  float z = rng.rand01();
+  int prev = -1;
+  uint32_t consec_bad = 0u;
+  const uint32_t total = A.iters + A.fuse;
+  const bool tracing = A.trace != nullptr && w < A.trace_walkers;
+
+  for (uint32_t i = 0; i < total; ++i) {
+    const uint32_t row = prev >= 0 ? (uint32_t)prev : A.fallback_row;
+    const uint32_t raw = rng.irand();
+    const uint32_t fn = A.distrib[(size_t)row * kGrain + (raw & (kGrain - 1))];
+    const float* xf = A.xforms + (size_t)fn * kXformFloats;
+    bool dc_on = false;
+    float3 dc_rgb = make_float3(0.0f, 0.0f, 0.0f);
+    const Chain ca = xform_chain(xf, x, y, z, rng, dc_on, dc_rgb);
+    float2 pv = ca.pv;
+    float nz = ca.z;
+    if (A.xform_blend > 0.0f) {
+      if (rng.rand01() < A.xform_blend) {
+        const uint32_t rb = rng.irand();
+        const uint32_t bi = A.distrib[(size_t)A.fallback_row * kGrain + (rb & (kGrain - 1))];
+        bool dc_b = false;
+        float3 rgb_b;
+        const Chain cb = xform_chain(A.xforms + (size_t)bi * kXformFloats, x, y, z, rng, dc_b,
+                                     rgb_b);
+        const float t = rng.rand01();
+        pv.x = mixf(pv.x, cb.pv.x, t);
+        pv.y = mixf(pv.y, cb.pv.y, t);
+        nz = mixf(nz, cb.z, t);
+      }
+    }
+    const bool bad = is_bad(pv);
+    if (bad) {
+      const float rx = rng.rand11();
+      const float ry = rng.rand11();
+      pv = make_float2(rx, ry);
+      consec_bad += 1u;
+    } else {
+      consec_bad = 0u;
+    }
+    float* tr = nullptr;
+    if (tracing && i < A.trace_iters) {
+      tr = A.trace + ((size_t)w * A.trace_iters + i) * kTraceFloats;
+      tr[0] = (float)i; tr[1] = (float)fn;
+      tr[2] = ca.pa.x; tr[3] = ca.pa.y; tr[4] = ca.pv_pre.x; tr[5] = ca.pv_pre.y;
+      tr[6] = pv.x; tr[7] = pv.y; tr[8] = bad ? 1.0f : 0.0f; tr[9] = nz;
+      tr[11] = __uint_as_float(raw);
+      tr[12] = pv.x; tr[13] = pv.y; tr[14] = 0.0f; tr[15] = 0.0f;
+    }
+    const float ppx = pv.x, ppy = pv.y, ppz = nz;
+    if (consec_bad > 0u && consec_bad < 5u) {
+      x = ppx; y = ppy; z = ppz;
+      if (tr) tr[10] = __uint_as_float(rng.draws);
+      if (i > 0u) i -= 1u;
+      continue;
+    }
+    if (consec_bad >= 5u) consec_bad = 0u;
+    prev = (int)fn;
+    float sx = ppx, sy = ppy, sz = ppz;
+    if (A.final_idx >= 0) {
+      const float* fx = A.xforms + (size_t)A.final_idx * kXformFloats;
+      const bool apply = fx[10] == 1.0f ? true : (rng.rand01() < fx[10]);
+      if (apply) {
+        const float2 fpa = make_float2(fx[0] * pv.x + fx[1] * pv.y + fx[2],
+                                       fx[4] * pv.x + fx[5] * pv.y + fx[6]);
+        const int fnv = nvars_of(fx);
+        const float2 fpam = pre_blur(fx, fnv, fpa, rng);
+        float2 fpv = var_sum(fx, fnv, fpam, rng, dc_on, dc_rgb);
+        fpv = post_affine(fx, fpv);
+        const float fz = mixf(nz, fx[8], fx[9]);
+        if (!is_bad(fpv)) { sx = fpv.x; sy = fpv.y; sz = fz; }
+      }
+    }
+    const float mag = fmaxf(fmaxf(fabsf(ppx), fabsf(ppy)), 1e-30f);
+    const float amp = mag * A.jitter;
+    const float jx = (rng.rand01() - 0.5f) * amp;
+    const float jy = (rng.rand01() - 0.5f) * amp;
+    x = ppx + jx; y = ppy + jy; z = ppz;
+    if (tr) {
+      tr[12] = x; tr[13] = y; tr[14] = sx; tr[15] = sy;
+      tr[10] = __uint_as_float(rng.draws);
+    }
+    if (i >= A.fuse) {
+      const float opacity = xf[10];
+      const float dx = sx - A.cx, dy = sy - A.cy;
+      const float rx = dx * A.cos_r - dy * A.sin_r;
+      const float ry = dx * A.sin_r + dy * A.cos_r;
+      const float px = rx * A.scale + (float)A.width * 0.5f;
+      const float py = ry * A.scale + (float)A.height * 0.5f;
+      // floor(p) in [0, W) <=> p in [0, W) for finite p; NaN fails both tests.
+      if (px >= 0.0f && px < (float)A.width && py >= 0.0f && py < (float)A.height) {
+        const uint32_t xi = (uint32_t)floorf(px), yi = (uint32_t)floorf(py);
+        const float cxf = fminf(fmaxf(sz, 0.0f), 1.0f) * 255.0f;
+        float4 pal;
+        if (A.palette_mode == 1u) {
+          uint32_t i0 = f2u_sat(cxf);
+          float frac = cxf - (float)i0;
+          if (i0 >= 255u) { i0 = 254u; frac = 1.0f; }
+          const float4 p0 = A.palette[i0], p1 = A.palette[i0 + 1u];
+          pal = make_float4(mixf(p0.x, p1.x, frac), mixf(p0.y, p1.y, frac),
+                            mixf(p0.z, p1.z, frac), mixf(p0.w, p1.w, frac));
+        } else {
+          const uint32_t pi = min(f2u_sat(cxf), 255u);
+          pal = A.palette[pi];
+        }
+        if (dc_on) { pal.x = dc_rgb.x; pal.y = dc_rgb.y; pal.z = dc_rgb.z; }
+        const float wgt = opacity * 255.0f;
+        uint32_t* b = A.hist + ((size_t)yi * A.width + xi) * 4u;
+        hist_add<kSat>(b + 0, f2u_sat(pal.x * wgt));
+        hist_add<kSat>(b + 1, f2u_sat(pal.y * wgt));
+        hist_add<kSat>(b + 2, f2u_sat(pal.z * wgt));
+        hist_add<kSat>(b + 3, f2u_sat(wgt));
+      }
+    }
+  }
+}
+
+hipError_t launch_chaos(const ChaosArgs& a, bool saturating, hipStream_t stream) {
+  const int block = 256;
+  const unsigned grid = (a.walkers + block - 1) / block;
+  if (grid == 0) return hipSuccess;
+  const size_t shmem = 32u * block * sizeof(uint32_t);
+  if (saturating)
+    chaos_kernel<true><<<grid, block, shmem, stream>>>(a);
+  else
+    chaos_kernel<false><<<grid, block, shmem, stream>>>(a);
+  return hipGetLastError();
+}
+
+}  // namespace p3

@@ -1,0 +1,175 @@
+// Device-side building blocks of the gfx950 render path: ISAAC streams held
+// in LDS, the masked 28-bit rand transforms, range-guarded trig, and the
+// buffer-layout constants shared by the kernels.
+//
+// ISAAC state per walker is [cnt, a, b, c | mem[16] | rsl[16]]. cnt/a/b/c
+// live in VGPRs; mem and rsl are indexed by data-dependent values
+// (mem[(x >> 2) & 15], rsl[--cnt]), which has no cheap per-lane VGPR form on
+// CDNA, so they live in LDS as [word][lane] columns: lane l's word k is at
+// k * stride + l, so any per-lane index hits bank (l mod 64) — conflict-free.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace p3 {
+
+constexpr int kGrain = 16384;
+constexpr int kMaxXforms = 128;
+constexpr int kXformFloats = 116;
+constexpr int kMaxVars = 8;
+constexpr int kTraceFloats = 16;
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kTau = 6.28318530717958647692f;
+constexpr float kEps = 1e-10f;
+
+// ---------------------------------------------------------------- ISAAC --
+
+struct IsaacRegs {
+  uint32_t cnt, a, b, c;
+};
+
+// LDS view of one lane's mem/rsl words.
+struct IsaacLds {
+  uint32_t* base;   // &lds[lane]
+  int stride;       // lanes per block
+  __device__ uint32_t& mem(int k) const { return base[k * stride]; }
+  __device__ uint32_t& rsl(int k) const { return base[(16 + k) * stride]; }
+};
+
+// chaos.wgsl:243-294 / isaac.ts:35-105: one round, 16 outputs.
+__device__ inline void isaac_round(IsaacRegs& s, const IsaacLds& m) {
+  s.c += 1u;
+  uint32_t a = s.a;
+  uint32_t b = s.b + s.c;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const uint32_t x = m.mem(k);
+    if ((k & 3) == 0) a ^= a << 13;
+    else if ((k & 3) == 1) a ^= a >> 6;
+    else if ((k & 3) == 2) a ^= a << 2;
+    else a ^= a >> 16;
+    a += m.mem((k + 8) & 15);
+    const uint32_t y = m.mem((x >> 2) & 15) + a + b;
+    m.mem(k) = y;
+    b = m.mem((y >> 6) & 15) + x;
+    m.rsl(k) = b;
+  }
+  s.a = a;
+  s.b = b;
+}
+
+// chaos.wgsl:297-308: pop rsl[--cnt], refilling with a round when empty.
+__device__ inline uint32_t isaac_irand(IsaacRegs& s, const IsaacLds& m) {
+  if (s.cnt == 0u) {
+    isaac_round(s, m);
+    s.cnt = 15u;
+  } else {
+    s.cnt -= 1u;
+  }
+  return m.rsl((int)s.cnt);
+}
+
+__device__ inline void isaac_mix8(uint32_t v[8]) {
+  uint32_t a = v[0], b = v[1], c = v[2], d = v[3], e = v[4], f = v[5], g = v[6], h = v[7];
+  a ^= b << 11; d += a; b += c;
+  b ^= c >> 2;  e += b; c += d;
+  c ^= d << 8;  f += c; d += e;
+  d ^= e >> 16; g += d; e += f;
+  e ^= f << 10; h += e; f += g;
+  f ^= g >> 4;  a += f; g += h;
+  g ^= h << 8;  b += g; h += a;
+  h ^= a >> 9;  c += h; a += b;
+  v[0] = a; v[1] = b; v[2] = c; v[3] = d; v[4] = e; v[5] = f; v[6] = g; v[7] = h;
+}
+
+// isaac.ts:194-229 (packIsaacStates, one walker) on the device: PCG32
+// XSH-RS words into rsl, the two-pass golden-ratio key schedule, one round.
+__device__ inline void isaac_seed(IsaacRegs& s, const IsaacLds& m, uint32_t seed,
+                                  uint32_t walker) {
+  const uint32_t MUL = 747796405u, INC = 2891336453u;
+  uint32_t pcg = seed ^ (walker * 2654435761u + 1u);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) pcg = pcg * MUL + INC;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const uint32_t st = pcg;
+    pcg = pcg * MUL + INC;
+    const uint32_t shift = (st >> 28) + 4u;
+    const uint32_t word = ((st >> shift) ^ st) * 277803737u;
+    m.rsl(i) = (word >> 22) ^ word;
+  }
+  uint32_t v[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) v[j] = 0x9e3779b9u;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) isaac_mix8(v);
+#pragma unroll
+  for (int i = 0; i < 16; i += 8) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] += m.rsl(i + j);
+    isaac_mix8(v);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) m.mem(i + j) = v[j];
+  }
+#pragma unroll
+  for (int i = 0; i < 16; i += 8) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] += m.mem(i + j);
+    isaac_mix8(v);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) m.mem(i + j) = v[j];
+  }
+  s.a = s.b = s.c = 0u;
+  isaac_round(s, m);
+  s.cnt = 16u;
+}
+
+// A walker's RNG as the variations see it.
+struct Rng {
+  IsaacRegs s;
+  IsaacLds m;
+  uint32_t draws;
+  __device__ uint32_t irand() { ++draws; return isaac_irand(s, m); }
+  // chaos.wgsl:319-331: low 28 bits scaled by 1/(2^28-1) and 1/(2^27-1).
+  __device__ float rand01() {
+    return (float)(irand() & 0x0fffffffu) * (1.0f / 268435455.0f);
+  }
+  __device__ float rand11() {
+    return (float)((int32_t)(irand() & 0x0fffffffu) - 0x07ffffff) * (1.0f / 134217727.0f);
+  }
+};
+
+// ----------------------------------------------------------------- math --
+
+// chaos.wgsl:356-362 integer hash for out-of-range trig arguments.
+__device__ inline float hash01(uint32_t h) {
+  h ^= h >> 17; h *= 0xed5ad4bbu;
+  h ^= h >> 11; h *= 0xac4c1b51u;
+  h ^= h >> 15;
+  return (float)h / 4294967296.0f;
+}
+
+// chaos.wgsl:363-384
+__device__ inline float safe_sin(float a) {
+  if (fabsf(a) <= 1.0e6f) return sinf(a);
+  return sinf(hash01(__float_as_uint(a)) * kTau);
+}
+__device__ inline float safe_cos(float a) {
+  if (fabsf(a) <= 1.0e6f) return cosf(a);
+  return cosf(hash01(__float_as_uint(a)) * kTau);
+}
+__device__ inline float safe_tan(float a) { return safe_sin(a) / safe_cos(a); }
+__device__ inline float safe_tanh(float a) { return tanhf(fminf(fmaxf(a, -20.0f), 20.0f)); }
+
+// WGSL mix(a, b, t) = a * (1 - t) + b * t
+__device__ inline float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+
+// WGSL u32(f32): saturating; NaN and negatives -> 0.
+__device__ inline uint32_t f2u_sat(float f) {
+  if (!(f > 0.0f)) return 0u;
+  if (f >= 4294967296.0f) return 0xffffffffu;
+  return (uint32_t)f;
+}
+
+}  // namespace p3

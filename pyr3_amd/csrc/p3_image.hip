@@ -1,0 +1,317 @@
+// Density estimation and tonemap kernels for gfx950.
+//
+// DE follows density_main (src/shaders/density.wgsl:51-123): every
+// non-empty super-pixel n scatters its log-density-scaled colour over a disc
+// of integer radius r_n (half-even rounded adaptive radius), Gaussian
+// sigma = r_n / 3, normalised by the per-radius kernel sum so each bucket
+// deposits weight 1. The reference evaluates it as a per-output gather over
+// the full (2R+1)^2 box with pow/log/round per visited neighbour. Here it is
+// split in two passes:
+//   de_prep    once per bin: r_n (u8, 255 = empty) and the bin's RGBA times
+//              ls_n / knorm(r_n) (float4), plus the max radius of each
+//              16 x 16 bin tile;
+//   de_gather  per 16 x 16 output tile: the largest radius among bins that
+//              could reach the tile bounds the loop box (Rt <= R), the
+//              (16 + 2 Rt)^2 neighbourhood is staged in LDS, and every output
+//              sums kw * pre_n over its box in the reference's dy-then-dx order.
+// kw = exp(-d^2 / (2 sigma^2)) is the same expression as the reference.
+//
+// The tonemap follows fs (src/shaders/visualize_f32.wgsl:162-271 with DE,
+// src/shaders/visualize_u32.wgsl:148-248 without): separable collapse of
+// fwidth^2 super-pixels (clamp to edge), flam3 alpha/vibrancy/highlight
+// composite, background blend, RGBA8 store with round-to-nearest.
+#include "p3_device.h"
+#include "p3_launch.h"
+
+namespace p3 {
+
+constexpr int kTile = 16;
+constexpr int kLdsRadius = 24;                        // staged-halo limit
+constexpr int kLdsSpan = kTile + 2 * kLdsRadius;      // 64
+constexpr uint8_t kEmpty = 255;
+
+__global__ void __launch_bounds__(256) de_prep(DeArgs A) {
+  __shared__ uint32_t smax;
+  const uint32_t x = blockIdx.x * kTile + threadIdx.x;
+  const uint32_t y = blockIdx.y * kTile + threadIdx.y;
+  if (threadIdx.x == 0 && threadIdx.y == 0) smax = 0u;
+  __syncthreads();
+  if (x < A.width && y < A.height) {
+    const size_t i = (size_t)y * A.width + x;
+    const uint4 h = A.hist[i];
+    const float cnt = (float)h.w;
+    uint8_t r8 = kEmpty;
+    float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (cnt > 0.0f) {
+      const float hits = cnt / 255.0f;
+      const float rf = fminf(fmaxf(A.max_rad / powf(hits + 1.0f, A.curve), A.min_rad), A.max_rad);
+      const int ri = min(max((int)rintf(rf), 0), 256);
+      const float knorm = fmaxf(A.lut[ri], 1e-6f);
+      const float ls = (A.k1 * logf(1.0f + cnt * A.k2)) / cnt;
+      const float s = ls / knorm;
+      p = make_float4((float)h.x * s, (float)h.y * s, (float)h.z * s, cnt * s);
+      // Radii beyond the search box never reach a neighbour inside it; the
+      // gather clips its box to R anyway, so store min(ri, 254).
+      r8 = (uint8_t)min(ri, 254);
+      atomicMax(&smax, (uint32_t)min(ri, A.R));
+    }
+    A.pre[i] = p;
+    A.rad[i] = r8;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && threadIdx.y == 0)
+    A.tile_rmax[blockIdx.y * A.tiles_x + blockIdx.x] = smax;
+}
+
+__device__ inline int tile_bound(const DeArgs& A, int tx0, int ty0) {
+  // Tiles whose bins lie within R of the output tile.
+  const int rt = (A.R + kTile - 1) / kTile;
+  int m = 0;
+  for (int ty = max(ty0 - rt, 0); ty <= min(ty0 + rt, (int)A.tiles_y - 1); ++ty)
+    for (int tx = max(tx0 - rt, 0); tx <= min(tx0 + rt, (int)A.tiles_x - 1); ++tx)
+      m = max(m, (int)A.tile_rmax[ty * A.tiles_x + tx]);
+  return min(m, A.R);
+}
+
+__device__ inline float de_kw(int d2, int r) {
+  const float rad = (float)r;
+  const float sigma = fmaxf(rad / 3.0f, 1e-6f);
+  return expf(-(float)d2 / (2.0f * sigma * sigma));
+}
+
+__global__ void __launch_bounds__(256) de_gather(DeArgs A) {
+  __shared__ float4 s_pre[kLdsSpan * kLdsSpan];
+  __shared__ uint8_t s_rad[kLdsSpan * kLdsSpan];
+  __shared__ int s_rt;
+  const int tid = threadIdx.y * kTile + threadIdx.x;
+  const int x0 = blockIdx.x * kTile, y0 = blockIdx.y * kTile;
+  if (tid == 0) s_rt = tile_bound(A, blockIdx.x, blockIdx.y);
+  __syncthreads();
+  const int Rt = s_rt;
+  const int xi = x0 + threadIdx.x, yi = y0 + threadIdx.y;
+  const int W = (int)A.width, H = (int)A.height;
+  float4 sum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+
+  if (Rt <= kLdsRadius) {
+    const int span = kTile + 2 * Rt;
+    for (int k = tid; k < span * span; k += 256) {
+      const int ly = k / span, lx = k - ly * span;
+      const int gx = x0 - Rt + lx, gy = y0 - Rt + ly;
+      uint8_t r8 = kEmpty;
+      float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
+        const size_t g = (size_t)gy * W + gx;
+        r8 = A.rad[g];
+        p = A.pre[g];
+      }
+      s_rad[k] = r8;
+      s_pre[k] = p;
+    }
+    __syncthreads();
+    if (xi < W && yi < H) {
+      const int cx = threadIdx.x + Rt, cy = threadIdx.y + Rt;
+      for (int dy = -Rt; dy <= Rt; ++dy) {
+        const int rowk = (cy + dy) * span + cx;
+        for (int dx = -Rt; dx <= Rt; ++dx) {
+          const int k = rowk + dx;
+          const int r = s_rad[k];
+          if (r == kEmpty) continue;
+          const int d2 = dx * dx + dy * dy;
+          if (d2 > r * r) continue;
+          const float kw = de_kw(d2, r);
+          const float4 p = s_pre[k];
+          sum.x = sum.x + p.x * kw;
+          sum.y = sum.y + p.y * kw;
+          sum.z = sum.z + p.z * kw;
+          sum.w = sum.w + p.w * kw;
+        }
+      }
+    }
+  } else if (xi < W && yi < H) {
+    for (int dy = -Rt; dy <= Rt; ++dy) {
+      const int ny = yi + dy;
+      if (ny < 0 || ny >= H) continue;
+      for (int dx = -Rt; dx <= Rt; ++dx) {
+        const int nx = xi + dx;
+        if (nx < 0 || nx >= W) continue;
+        const size_t g = (size_t)ny * W + nx;
+        const int r = A.rad[g];
+        if (r == kEmpty) continue;
+        const int d2 = dx * dx + dy * dy;
+        if (d2 > r * r) continue;
+        const float kw = de_kw(d2, r);
+        const float4 p = A.pre[g];
+        sum.x = sum.x + p.x * kw;
+        sum.y = sum.y + p.y * kw;
+        sum.z = sum.z + p.z * kw;
+        sum.w = sum.w + p.w * kw;
+      }
+    }
+  }
+  if (xi < W && yi < H) A.out[(size_t)yi * W + xi] = sum;
+}
+
+hipError_t launch_density(const DeArgs& a, hipStream_t stream) {
+  const dim3 block(kTile, kTile);
+  const dim3 grid(a.tiles_x, a.tiles_y);
+  de_prep<<<grid, block, 0, stream>>>(a);
+  de_gather<<<grid, block, 0, stream>>>(a);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------- tonemap --
+
+__device__ inline float calc_alpha(float d, float g, float lin) {
+  if (d <= 0.0f) return 0.0f;
+  if (d < lin) {
+    const float fv = powf(lin, g);
+    const float frac = d / lin;
+    return (1.0f - frac) * d * (fv / lin) + frac * powf(d, g);
+  }
+  return powf(d, g);
+}
+
+__device__ inline float3 rgb2hsv(float3 c) {
+  const float mx = fmaxf(fmaxf(c.x, c.y), c.z);
+  const float mn = fminf(fminf(c.x, c.y), c.z);
+  const float d = mx - mn;
+  float h = 0.0f;
+  if (d > 0.0f) {
+    if (mx == c.x) h = (c.y - c.z) / d;
+    else if (mx == c.y) h = 2.0f + (c.z - c.x) / d;
+    else h = 4.0f + (c.x - c.y) / d;
+    h *= 60.0f;
+    if (h < 0.0f) h += 360.0f;
+  }
+  return make_float3(h, mx > 0.0f ? d / mx : 0.0f, mx);
+}
+
+__device__ inline float3 hsv2rgb(float3 c) {
+  const float h = c.x / 60.0f, s = c.y, v = c.z;
+  const float i = floorf(h);
+  const float f = h - i;
+  const float p = v * (1.0f - s), q = v * (1.0f - s * f), t = v * (1.0f - s * (1.0f - f));
+  const int ix = (int)i % 6;
+  if (ix == 0) return make_float3(v, t, p);
+  if (ix == 1) return make_float3(q, v, p);
+  if (ix == 2) return make_float3(p, v, t);
+  if (ix == 3) return make_float3(p, q, v);
+  if (ix == 4) return make_float3(t, p, v);
+  return make_float3(v, p, q);
+}
+
+__device__ inline float3 calc_newrgb(float3 c, float ls, float highpow) {
+  if (ls == 0.0f || (c.x == 0.0f && c.y == 0.0f && c.z == 0.0f)) return make_float3(0.f, 0.f, 0.f);
+  const float3 cw = make_float3(c.x / 255.0f, c.y / 255.0f, c.z / 255.0f);
+  const float3 sc = make_float3(ls * cw.x, ls * cw.y, ls * cw.z);
+  float maxa = sc.x, maxc = cw.x;
+  if (sc.y > maxa) { maxa = sc.y; maxc = cw.y; }
+  if (sc.z > maxa) { maxa = sc.z; maxc = cw.z; }
+  if (maxa > 255.0f && highpow >= 0.0f) {
+    const float newls = 255.0f / maxc;
+    const float lsratio = powf(newls / ls, highpow);
+    float3 n = make_float3(newls * cw.x / 255.0f, newls * cw.y / 255.0f, newls * cw.z / 255.0f);
+    float3 hsv = rgb2hsv(n);
+    hsv.y *= lsratio;
+    n = hsv2rgb(hsv);
+    return make_float3(n.x * 255.0f, n.y * 255.0f, n.z * 255.0f);
+  }
+  const float newls = 255.0f / maxc;
+  float adj = -highpow;
+  if (adj > 1.0f) adj = 1.0f;
+  if (maxa <= 255.0f) adj = 1.0f;
+  const float m = (1.0f - adj) * newls + adj * ls;
+  return make_float3(m * cw.x, m * cw.y, m * cw.z);
+}
+
+__device__ inline uint32_t to_u8(float v) {
+  return (uint32_t)(fminf(fmaxf(v, 0.0f), 1.0f) * 255.0f + 0.5f);
+}
+
+template <bool kDe>
+__global__ void __launch_bounds__(256) tonemap_kernel(TmArgs A) {
+  const uint32_t xi = blockIdx.x * 16 + threadIdx.x;
+  const uint32_t yi = blockIdx.y * 16 + threadIdx.y;
+  if (xi >= A.width || yi >= A.height) return;
+  const uint32_t N = A.os, fw = A.fwidth, half = fw / 2u;
+  const int SW = (int)(A.width * N), SH = (int)(A.height * N);
+  const uint32_t cx = xi * N + N / 2u, cy = yi * N + N / 2u;
+  float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, c3 = 0.0f;
+  for (uint32_t dy = 0; dy < fw; ++dy) {
+    const float ky = A.taps[dy];
+    const int sy = min(max((int)cy + (int)dy - (int)half, 0), SH - 1);
+    for (uint32_t dx = 0; dx < fw; ++dx) {
+      const float kx = A.taps[dx];
+      const int sx = min(max((int)cx + (int)dx - (int)half, 0), SW - 1);
+      const size_t sb = (size_t)sy * SW + sx;
+      const float w = kx * ky;
+      if (kDe) {
+        const float4 f = A.filtered[sb];
+        c0 = c0 + f.x * w; c1 = c1 + f.y * w; c2 = c2 + f.z * w; c3 = c3 + f.w * w;
+      } else {
+        const uint4 h = A.hist[sb];
+        const float cnt = (float)h.w;
+        if (cnt > 0.0f) {
+          const float ls = (A.k1 * logf(1.0f + cnt * A.k2)) / cnt;
+          c0 = c0 + ((float)h.x * ls) * w;
+          c1 = c1 + ((float)h.y * ls) * w;
+          c2 = c2 + ((float)h.z * ls) * w;
+          c3 = c3 + (cnt * ls) * w;
+        }
+      }
+    }
+  }
+  float4 o;
+  if (c3 <= 0.0f) {
+    o = A.transparent ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(A.bg0, A.bg1, A.bg2, 1.0f);
+  } else {
+    const float g = 1.0f / A.gamma;
+    const float tmp = c3 / 255.0f;
+    float alpha = calc_alpha(tmp, g, A.linrange);
+    const float ls_a = A.vibrancy * 256.0f * alpha / fmaxf(tmp, 1e-12f);
+    alpha = fminf(fmaxf(alpha, 0.0f), 1.0f);
+    const float3 n = calc_newrgb(make_float3(c0, c1, c2), ls_a, A.highpow);
+    const float cc[3] = {c0, c1, c2};
+    const float nn[3] = {n.x, n.y, n.z};
+    const float bg[3] = {A.bg0, A.bg1, A.bg2};
+    float res[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const float perch = (1.0f - A.vibrancy) * 256.0f * powf(fmaxf(cc[i] / 255.0f, 0.0f), g);
+      const float bgt = A.transparent ? 0.0f : (1.0f - alpha) * 256.0f * bg[i];
+      const float composed = nn[i] + perch + bgt;
+      res[i] = fminf(fmaxf(composed / 256.0f, 0.0f), 1.0f);
+    }
+    o = make_float4(res[0], res[1], res[2], A.transparent ? alpha : 1.0f);
+  }
+  const size_t oi = (size_t)yi * A.width + xi;
+  A.out8[oi] = to_u8(o.x) | (to_u8(o.y) << 8) | (to_u8(o.z) << 16) | (to_u8(o.w) << 24);
+  if (A.outf) A.outf[oi] = o;
+}
+
+hipError_t launch_tonemap(const TmArgs& a, bool de, hipStream_t stream) {
+  const dim3 block(16, 16);
+  const dim3 grid((a.width + 15) / 16, (a.height + 15) / 16);
+  if (de) tonemap_kernel<true><<<grid, block, 0, stream>>>(a);
+  else tonemap_kernel<false><<<grid, block, 0, stream>>>(a);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ fill --
+
+__global__ void fill_u32(uint4* p, size_t n4, uint32_t v) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = i; k < n4; k += stride) p[k] = make_uint4(v, v, v, v);
+}
+
+hipError_t launch_fill(void* p, size_t bytes, uint32_t v, hipStream_t stream) {
+  const size_t n4 = bytes / 16;
+  if (n4 == 0) return hipSuccess;
+  size_t blocks = (n4 + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  fill_u32<<<(unsigned)blocks, 256, 0, stream>>>((uint4*)p, n4, v);
+  return hipGetLastError();
+}
+
+}  // namespace p3

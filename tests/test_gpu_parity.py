@@ -1,0 +1,181 @@
+"""HIP render path vs the CPU oracle, through the C ABI (needs an MI355X).
+
+Bars (each asserted below):
+  * ISAAC stream and xform-pick sequence bit-exact per walker (trace words);
+  * per-walker FP32 trajectories within 1e-5 relative over the first 20 turns;
+  * histograms bit-exact where the arithmetic is exact (+,-,*,/ only, no
+    transcendentals): the u32 adds are order-independent;
+  * DE on an identical histogram within 1e-5 relative; tonemap float output
+    within 1e-5 relative and RGBA8 within 1 LSB;
+  * whole images by PSNR at matched sample counts.
+"""
+import numpy as np
+import pytest
+
+from helpers import blobs, genome, psnr, sierpinski, xform
+from pyr3_amd import pack
+from pyr3_amd.genome import Tonemap
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def R():
+    from pyr3_amd.renderer import Renderer
+    return Renderer
+
+
+def _render_hist_gpu(R, g, W, H, os_=1, iters=100, fuse=20, seed=1, walkers=64, trace=None,
+                     jitter=None):
+    r = R(W, H, os_)
+    r.reset(g)
+    if trace:
+        r.set_trace(*trace)
+    r.iterate(g, seed, walkers, iters, walker_jitter=jitter, fuse=fuse)
+    h = r.read_histogram()
+    tr = r.read_trace() if trace else None
+    return r, h, tr
+
+
+def test_isaac_stream_and_pick_sequence_bit_exact(orc, R):
+    g = genome(sierpinski(), scale=10.0, cx=0.5, cy=0.5)
+    W = H = 32
+    walkers, iters, fuse = 300, 40, 10
+    _, hg, tg = _render_hist_gpu(R, g, W, H, iters=iters, fuse=fuse, seed=0xC0FFEE,
+                                 walkers=walkers, trace=(walkers, iters + fuse))
+    xf, ds, u, pal = blobs(g, W, H, iters=iters, fuse=fuse, seed=0xC0FFEE, walkers=walkers)
+    ho, to = orc.chaos(xf, ds, u, pal, trace_shape=(walkers, iters + fuse))
+    # raw pick draw, pick index and draw counter: bit-exact for every walker/turn
+    assert np.array_equal(tg[..., 11].view(np.uint32), to[..., 11].view(np.uint32))
+    assert np.array_equal(tg[..., 1], to[..., 1])
+    assert np.array_equal(tg[..., 10].view(np.uint32), to[..., 10].view(np.uint32))
+    # linear maps only: trajectories and histogram are exact
+    assert np.array_equal(tg[..., 12:14], to[..., 12:14])
+    assert np.array_equal(hg, ho)
+
+
+def test_isaac_explicit_states_match_device_seeding(orc, R):
+    g = genome(sierpinski(), scale=10.0, cx=0.5, cy=0.5)
+    W = H = 16
+    walkers = 77  # not a multiple of 64
+    st = orc.seed_walkers(4242, walkers)
+    r = R(W, H, 1)
+    r.reset(g)
+    r.iterate(g, 4242, walkers, 50, fuse=5)
+    a = r.read_histogram()
+    r.reset(g)
+    r.iterate(g, 4242, walkers, 50, fuse=5, isaac_states=st)
+    b = r.read_histogram()
+    assert np.array_equal(a, b) and a[..., 3].sum() > 0
+
+
+def _nonlinear_genome():
+    return genome([xform(0.6, (0.6, 0.3, 0.1, -0.2, 0.7, 0.05), vars=((1, 0.5), (2, 0.5)),
+                         color=0.1),
+                   xform(0.4, (-0.5, 0.2, -0.3, 0.4, 0.5, 0.2), vars=((0, 0.7), (1, 0.3)),
+                         color=0.9, post=(0.9, 0.1, 0.0, -0.1, 0.9, 0.0))],
+                  scale=40.0)
+
+
+def test_trajectories_within_1e5_over_20_turns(orc, R):
+    g = _nonlinear_genome()
+    W = H = 64
+    walkers, turns = 256, 20
+    _, _, tg = _render_hist_gpu(R, g, W, H, iters=turns, fuse=0, seed=7, walkers=walkers,
+                                trace=(walkers, turns))
+    xf, ds, u, pal = blobs(g, W, H, iters=turns, fuse=0, seed=7, walkers=walkers)
+    _, to = orc.chaos(xf, ds, u, pal, trace_shape=(walkers, turns))
+    assert np.array_equal(tg[..., 1], to[..., 1])  # same pick sequence
+    for col in (2, 3, 6, 7, 9, 12, 13):
+        a, b = tg[..., col].astype(np.float64), to[..., col].astype(np.float64)
+        rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-6)
+        assert rel.max() <= 1e-5, (col, rel.max())
+
+
+def test_histogram_totals_and_image_psnr(orc, R):
+    g = _nonlinear_genome()
+    W, H = 96, 80
+    walkers, iters = 512, 2000
+    r, hg, _ = _render_hist_gpu(R, g, W, H, iters=iters, fuse=20, seed=11, walkers=walkers)
+    xf, ds, u, pal = blobs(g, W, H, iters=iters, fuse=20, seed=11, walkers=walkers)
+    ho = orc.chaos_mt(xf, ds, u, pal, 8)
+    # in-bounds sample count agrees to within chaotic divergence
+    cg, co = int(hg[..., 3].sum()), int(ho[..., 3].sum())
+    assert abs(cg - co) <= 0.01 * co
+    samples = walkers * iters
+    r.present(g, samples)
+    img_g = r.read_image()
+    k1, k2 = pack.calibration(g.scale, samples, 1.0, 1)
+    fo = orc.density(ho, pack.de_uniforms(W, H, 9, 0, 0.4, k1, k2, 1))
+    _, img_o = orc.tonemap(pack.viz_uniforms(W, H, k1, k2, g.tonemap, 1, 3), pack.spatial_kernel(0.5, 1),
+                           filtered=fo)
+    assert psnr(img_g, img_o) > 30.0
+
+
+@pytest.mark.parametrize("os_", [1, 2])
+def test_density_and_tonemap_on_identical_histogram(orc, R, os_):
+    rng = np.random.default_rng(21 + os_)
+    W, H = 70, 45
+    SW, SH = W * os_, H * os_
+    h = np.zeros((SH, SW, 4), dtype=np.uint32)
+    dense = rng.random((SH, SW)) < 0.5
+    cnt = (rng.integers(1, 3000, (SH, SW)) * 255).astype(np.uint32)
+    h[..., 3] = np.where(dense, cnt, 0)
+    for c in range(3):
+        h[..., c] = (h[..., 3] * rng.random((SH, SW))).astype(np.uint32)
+    g = genome(sierpinski(), scale=30.0)
+    g.tonemap = Tonemap(gamma=4.0, vibrancy=1.0, highlight_power=1.0)
+    samples = 5e6
+    r = R(W, H, os_)
+    r.reset(g)
+    r.write_histogram(h)
+    r.present(g, samples)
+    fg = r.read_filtered()
+    k1, k2 = pack.calibration(g.scale, samples, 1.0, os_)
+    fo = orc.density(h, pack.de_uniforms(SW, SH, 9, 0, 0.4, k1, k2, os_))
+    scale = np.abs(fo).max()
+    assert np.allclose(fg, fo, rtol=1e-5, atol=1e-5 * scale)
+    taps = pack.spatial_kernel(0.5, os_)
+    of, o8 = orc.tonemap(pack.viz_uniforms(W, H, k1, k2, g.tonemap, os_, len(taps)), taps,
+                         filtered=fo)
+    gf, g8 = r.read_image_f32(), r.read_image()
+    assert np.allclose(gf, of, rtol=1e-5, atol=1e-5)
+    assert np.abs(g8.astype(int) - o8.astype(int)).max() <= 1
+    # DE off: raw histogram path
+    r.present(g, samples, force_de_off=True)
+    of2, o82 = orc.tonemap(pack.viz_uniforms(W, H, k1, k2, g.tonemap, os_, len(taps)), taps,
+                           hist=h)
+    assert np.allclose(r.read_image_f32(), of2, rtol=1e-5, atol=1e-5)
+    assert np.abs(r.read_image().astype(int) - o82.astype(int)).max() <= 1
+
+
+def test_saturation_on_device(orc, R):
+    g = genome([xform(1.0, (0, 0, 0, 0, 0, 0))], scale=1.0)
+    W = H = 8
+    h0 = np.zeros((H, W, 4), dtype=np.uint32)
+    h0[H // 2, W // 2] = 0xFFFFFFFF - 1000
+    r = R(W, H, 1)
+    r.reset(g)
+    r.write_histogram(h0)
+    r.iterate(g, 5, 3, 100, fuse=1)
+    out = r.read_histogram()
+    assert out[H // 2, W // 2, 3] == 0xFFFFFFFF and out[H // 2, W // 2, 0] == 0xFFFFFFFF
+
+
+def test_retry_path_on_device(orc, R):
+    g = genome([xform(1.0, (1, 0, 0, 0, 1, 0), vars=((0, 1e12),))])
+    iters = 12
+    _, h, tg = _render_hist_gpu(R, g, 16, 16, iters=iters, fuse=0, walkers=3, trace=(3, iters))
+    xf, ds, u, pal = blobs(g, 16, 16, iters=iters, fuse=0, walkers=3)
+    _, to = orc.chaos(xf, ds, u, pal, trace_shape=(3, iters))
+    assert np.array_equal(tg[..., 10].view(np.uint32), to[..., 10].view(np.uint32))
+    assert np.array_equal(tg[..., 6:8], to[..., 6:8])
+
+
+def test_unsupported_variation_is_an_error(R):
+    from pyr3_amd._lib import P3Error
+    g = genome([xform(1.0, vars=((300, 1.0),))])
+    r = R(16, 16, 1)
+    r.reset(g)
+    with pytest.raises(P3Error):
+        r.iterate(g, 1, 8, 10)

@@ -1,0 +1,135 @@
+"""Oracle chaos loop vs closed forms and the RNG-consumption contract.
+
+The reference has no vectors for the loop (src/shaders/chaos.wgsl:8269-8732);
+these pin it by what the algorithm fixes:
+  * identity map, jitter 0: every walker stays at its first two rand_11 draws,
+    so the histogram is a brute-force sum over walkers;
+  * opacity 0 deposits nothing; at opacity 1 the count channel is 255 x the
+    in-bounds post-fuse samples;
+  * draw count per accepted turn = pick + variation draws + 2 jitter (+ final
+    opacity gate when the final's opacity is not 1); a bad turn adds the two
+    reseed draws and retries up to four times without advancing i (except at
+    i = 0), the fifth bad turn is accepted (chaos.wgsl:8391-8455);
+  * saturation pins at 2^32 - 1 (chaos.wgsl:8120-8135).
+"""
+import numpy as np
+import pytest
+
+from helpers import blobs, genome, sierpinski, xform
+from pyr3_amd import pack
+
+
+def test_identity_map_brute_force(orc):
+    W = H = 32
+    g = genome([xform(1.0, (1, 0, 0, 0, 1, 0), vars=((0, 1.0),), color=0.3, speed=0.0)],
+               scale=W / 2.0)
+    walkers, iters = 50, 37
+    xf, ds, u, pal = blobs(g, W, H, iters=iters, fuse=3, seed=99, walkers=walkers, jitter=0.0)
+    hist, _ = orc.chaos(xf, ds, u, pal)
+    want = np.zeros((H, W, 4), dtype=np.uint64)
+    st = orc.seed_walkers(99, walkers)
+    for w in range(walkers):
+        words, _ = orc.isaac_draw(st[w], 3)
+        x, y = orc.rand_transforms(words[:2])[1]
+        z0 = orc.rand_transforms(words[2:3])[0][0]
+        z = np.float32(z0)
+        # colour coordinate converges to 0.3 * 0 speed -> stays z0
+        px = np.float32(x) * np.float32(W / 2.0) + np.float32(W * 0.5)
+        py = np.float32(y) * np.float32(W / 2.0) + np.float32(H * 0.5)
+        xi, yi = int(np.floor(px)), int(np.floor(py))
+        if 0 <= xi < W and 0 <= yi < H:
+            pi = min(int(np.float32(z) * np.float32(255.0)), 255)
+            c = pal[pi]
+            want[yi, xi, 0] += int(np.float32(c[0]) * np.float32(255.0)) * iters
+            want[yi, xi, 1] += int(np.float32(c[1]) * np.float32(255.0)) * iters
+            want[yi, xi, 2] += int(np.float32(c[2]) * np.float32(255.0)) * iters
+            want[yi, xi, 3] += 255 * iters
+    assert np.array_equal(hist.astype(np.uint64), want)
+
+
+def test_opacity_zero_deposits_nothing(orc):
+    g = genome([xform(1.0, (0.5, 0, 0, 0, 0.5, 0), opacity=0.0)])
+    xf, ds, u, pal = blobs(g, 16, 16, walkers=10)
+    hist, _ = orc.chaos(xf, ds, u, pal)
+    assert not hist.any()
+
+
+def test_count_channel_is_255_per_in_bounds_sample(orc):
+    g = genome(sierpinski(), scale=10.0, cx=0.5, cy=0.5)
+    W = H = 32
+    xf, ds, u, pal = blobs(g, W, H, iters=200, fuse=20, walkers=40)
+    hist, _ = orc.chaos(xf, ds, u, pal)
+    # attractor lies in [0,1]^2 -> all samples in bounds at scale 10 around (0.5, 0.5)
+    assert hist[..., 3].sum() == 255 * 200 * 40
+    assert np.all(hist[..., :3] <= hist[..., 3:4])
+
+
+def _draws_per_turn(trace_w):
+    d = trace_w[:, 10].view(np.uint32).astype(np.int64)
+    return np.diff(np.concatenate([[3], d]))  # 3 init draws precede turn 0
+
+
+def test_draw_contract_plain(orc):
+    g = genome(sierpinski(), scale=10.0)
+    xf, ds, u, pal = blobs(g, 16, 16, iters=30, fuse=5, walkers=4)
+    _, tr = orc.chaos(xf, ds, u, pal, trace_shape=(4, 35))
+    for w in range(4):
+        assert np.all(_draws_per_turn(tr[w]) == 3)  # pick + 2 jitter
+
+
+def test_draw_contract_final_opacity_gate(orc):
+    fin = xform(0.0, (1, 0, 0, 0, 1, 0), opacity=0.5)
+    g = genome(sierpinski(), scale=10.0, final=fin)
+    xf, ds, u, pal = blobs(g, 16, 16, iters=30, fuse=5, walkers=4)
+    _, tr = orc.chaos(xf, ds, u, pal, trace_shape=(4, 35))
+    assert np.all(_draws_per_turn(tr[0]) == 4)  # + final opacity gate
+    fin.opacity = 1.0
+    xf, ds, u, pal = blobs(g, 16, 16, iters=30, fuse=5, walkers=4)
+    _, tr = orc.chaos(xf, ds, u, pal, trace_shape=(4, 35))
+    assert np.all(_draws_per_turn(tr[0]) == 3)
+
+
+def test_retry_window_all_bad(orc):
+    """Every turn is bad: turns 1..4 retry, the 5th is accepted."""
+    g = genome([xform(1.0, (1, 0, 0, 0, 1, 0), vars=((0, 1e12),))])
+    iters, fuse = 12, 0
+    xf, ds, u, pal = blobs(g, 16, 16, iters=iters, fuse=fuse, walkers=1)
+    _, tr = orc.chaos(xf, ds, u, pal, trace_shape=(1, iters))
+    # simulate the loop's bookkeeping
+    draws, i, consec, last = 3, 0, 0, {}
+    while i < iters:
+        draws += 3           # pick + 2 reseed
+        consec += 1
+        if 0 < consec < 5:
+            last[i] = draws
+            if i > 0:
+                i -= 1
+            i += 1
+            continue
+        consec = 0
+        draws += 2           # jitter
+        last[i] = draws
+        i += 1
+    got = tr[0, :, 10].view(np.uint32)
+    assert [int(got[k]) for k in range(iters)] == [last[k] for k in range(iters)]
+    assert np.all(tr[0, :, 8] == 1.0)
+
+
+def test_saturation_pins_at_u32_max(orc):
+    g = genome([xform(1.0, (0, 0, 0, 0, 0, 0))], scale=1.0)  # every sample lands on (0,0)
+    W = H = 8
+    xf, ds, u, pal = blobs(g, W, H, iters=100, fuse=1, walkers=3)
+    hist = np.zeros((H, W, 4), dtype=np.uint32)
+    hist[H // 2, W // 2] = 0xFFFFFFFF - 1000
+    out, _ = orc.chaos(xf, ds, u, pal, hist=hist)
+    assert out[H // 2, W // 2, 3] == 0xFFFFFFFF
+    assert out[H // 2, W // 2, 0] == 0xFFFFFFFF
+    assert out.sum(dtype=np.uint64) == np.uint64(4 * 0xFFFFFFFF)
+
+
+def test_mt_equals_single_thread(orc):
+    g = genome(sierpinski(), scale=12.0, cx=0.5, cy=0.5)
+    xf, ds, u, pal = blobs(g, 24, 20, iters=300, fuse=10, walkers=37)
+    a, _ = orc.chaos(xf, ds, u, pal)
+    b = orc.chaos_mt(xf, ds, u, pal, 5)
+    assert np.array_equal(a, b)
