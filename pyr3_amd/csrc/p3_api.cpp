@@ -52,7 +52,7 @@ float bits_f(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
 // Catalog entries the chaos kernel implements (index 97 is pre_blur, applied
 // by the chain itself).
 bool supported(uint32_t idx) {
-  return idx <= 2u || idx == 97u;
+  return idx <= 98u;
 }
 
 // Every active slot of every record must name an implemented entry.

@@ -33,17 +33,6 @@ __device__ inline int nvars_of(const float* xf) {
   return (int)(n < (uint32_t)kMaxVars ? n : (uint32_t)kMaxVars);
 }
 
-__device__ inline void load_var(const float* xf, int k, VarIn& a) {
-  const float* v = xf + 20 + k * 4;
-  const float* ve = xf + 52 + k * 4;
-  const float* ve2 = xf + 84 + k * 4;
-  a.w = v[1];
-  a.p[0] = v[2]; a.p[1] = v[3];
-  a.p[2] = ve[0]; a.p[3] = ve[1]; a.p[4] = ve[2]; a.p[5] = ve[3];
-  a.p[6] = ve2[0]; a.p[7] = ve2[1]; a.p[8] = ve2[2]; a.p[9] = ve2[3];
-  a.xf = xf;
-}
-
 // chaos.wgsl:8164-8181
 __device__ inline float2 pre_blur(const float* xf, int nv, float2 c, Rng& rng) {
   for (int k = 0; k < nv; ++k) {
@@ -64,20 +53,19 @@ __device__ inline float2 pre_blur(const float* xf, int nv, float2 c, Rng& rng) {
 __device__ inline float2 var_sum(const float* xf, int nv, float2 pa, Rng& rng, bool& dc_on,
                                  float3& dc_rgb) {
   float2 pv = make_float2(0.0f, 0.0f);
-  VarIn a;
   const bool dc_flag = xf[11] > 0.5f;
   for (int k = 0; k < nv; ++k) {
     const float* v = xf + 20 + k * 4;
     const int idx = (int)f2u_sat(v[0]);
-    load_var(xf, k, a);
+    const float w = v[1];
     if (idx != 97) {
-      const float2 o = apply_variation(idx, pa, a, rng);
+      const float2 o = apply_variation(idx, pa, xf, k, w, rng);
       pv.x = pv.x + o.x;
       pv.y = pv.y + o.y;
     }
-    if (dc_flag && a.w > 0.0f) {
+    if (dc_flag && w > 0.0f) {
       float3 rgb;
-      if (dc_color_for(idx, pa, a, rgb)) {
+      if (dc_color_for(idx, pa, xf, k, rgb)) {
         dc_on = true;
         dc_rgb = rgb;
       }
