@@ -26,6 +26,7 @@ _SIGS = {
     "orc_rand_transforms": (None, [_U, C.c_int, _F, _F]),
     "orc_chaos": (None, [_F, _U, C.c_int, _U, _F, _U, C.c_uint32, C.c_uint32, _F, C.c_int,
                          C.c_int]),
+    "orc_chaos_follow": (None, [_F, _U, C.c_int, _U, _F, _U, _F, C.c_int, C.c_int, _F]),
     "orc_chaos_mt": (None, [_F, _U, C.c_int, _U, _F, _U, C.c_int]),
     "orc_kernel_norm_lut": (None, [_F]),
     "orc_density": (None, [_U, _U, _F, _F, C.c_int]),
@@ -118,6 +119,21 @@ def chaos(xforms, distrib, uniforms, palette, hist=None, wb=0, we=None, trace_sh
     load().orc_chaos(p(xf, _F), p(ds, _U), ds.shape[0], p(u, _U), p(pal, _F), p(hist, _U), wb, we,
                      p(tr, _F) if tr is not None else None, tw, ti)
     return hist, tr
+
+
+def chaos_follow(xforms, distrib, uniforms, palette, force):
+    """Replay traced walkers turn by turn from another run's recorded states."""
+    xf = np.ascontiguousarray(xforms, dtype=np.float32)
+    ds = np.ascontiguousarray(distrib, dtype=np.uint32)
+    u = np.ascontiguousarray(uniforms, dtype=np.uint32)
+    pal = np.ascontiguousarray(palette, dtype=np.float32).reshape(-1)
+    fc = np.ascontiguousarray(force, dtype=np.float32)
+    tw, ti = fc.shape[0], fc.shape[1]
+    hist = np.zeros((int(u[1]), int(u[0]), 4), dtype=np.uint32)
+    tr = np.zeros((tw, ti, 16), dtype=np.float32)
+    load().orc_chaos_follow(p(xf, _F), p(ds, _U), ds.shape[0], p(u, _U), p(pal, _F), p(hist, _U),
+                            p(tr, _F), tw, ti, p(fc, _F))
+    return tr
 
 
 def chaos_mt(xforms, distrib, uniforms, palette, threads: int):

@@ -163,9 +163,13 @@ inline void putu(float* t, int i, uint32_t u) { std::memcpy(&t[i], &u, 4); }
 }  // namespace
 
 // Runs walkers [wb, we) of one dispatch and accumulates into `hist`.
+// `force` (optional, trace-shaped): for traced walkers, turn i > 0 starts
+// from the recorded end point of turn i - 1 (x, y at [12, 13], z at [9]),
+// so each turn is a one-step evaluation from the other implementation's
+// state (used to check per-step arithmetic without chaotic amplification).
 void chaos(const float* xforms, const uint32_t* distrib, int n_rows, const uint32_t* uni,
            const float* palette, uint32_t* hist, uint32_t wb, uint32_t we,
-           float* trace, int trace_walkers, int trace_iters) {
+           float* trace, int trace_walkers, int trace_iters, const float* force = nullptr) {
   const Uniforms u = read_uniforms(uni);
   const float cos_r = std::cos(u.rotation);
   const float sin_r = std::sin(u.rotation);
@@ -183,6 +187,10 @@ void chaos(const float* xforms, const uint32_t* distrib, int n_rows, const uint3
     int prev = -1;
     uint32_t consec_bad = 0;
     for (uint32_t i = 0; i < total; ++i) {
+      if (force && i > 0 && (int)w < trace_walkers && (int)i <= trace_iters) {
+        const float* f = force + ((size_t)w * trace_iters + (i - 1)) * 16;
+        x = f[12]; y = f[13]; z = f[9];
+      }
       int row = prev >= 0 ? prev : MAX_XFORMS;
       uint32_t raw = rng.irand();
       uint32_t fn = row_ptr(row)[raw & (GRAIN - 1)];
@@ -322,6 +330,14 @@ void orc_chaos_mt(const float* xforms, const uint32_t* distrib, int n_rows, cons
   for (auto& th : pool) th.join();
   for (int t = 0; t < threads; ++t)
     for (size_t i = 0; i < n; ++i) orc::add_sat(hist[i], priv[t][i]);
+}
+
+// Single-threaded, with each traced turn restarted from `force` (see chaos()).
+void orc_chaos_follow(const float* xforms, const uint32_t* distrib, int n_rows,
+                      const uint32_t* uni, const float* palette, uint32_t* hist, float* trace,
+                      int trace_walkers, int trace_iters, const float* force) {
+  orc::chaos(xforms, distrib, n_rows, uni, palette, hist, 0, (uint32_t)trace_walkers, trace,
+             trace_walkers, trace_iters, force);
 }
 
 }  // extern "C"

@@ -133,3 +133,16 @@ def test_mt_equals_single_thread(orc):
     a, _ = orc.chaos(xf, ds, u, pal)
     b = orc.chaos_mt(xf, ds, u, pal, 5)
     assert np.array_equal(a, b)
+
+
+def test_follow_mode_replays_own_trace(orc):
+    """Replaying a run from its own recorded turn end points reproduces it."""
+    g = genome([xform(0.6, (0.6, 0.3, 0.1, -0.2, 0.7, 0.05), vars=((1, 0.5), (2, 0.5))),
+                xform(0.4, (-0.5, 0.2, -0.3, 0.4, 0.5, 0.2), vars=((0, 0.7), (1, 0.3)))],
+               scale=40.0)
+    xf, ds, u, pal = blobs(g, 32, 32, iters=20, fuse=0, walkers=64)
+    _, tr = orc.chaos(xf, ds, u, pal, trace_shape=(64, 20))
+    rep = orc.chaos_follow(xf, ds, u, pal, tr)
+    clean = ~(tr[..., 8] > 0).any(axis=1)
+    assert clean.sum() > 50
+    assert np.array_equal(rep[clean].view(np.uint32), tr[clean].view(np.uint32))
