@@ -115,6 +115,17 @@ int p3_dims(p3_renderer* r, int* out4);
  * kernels do not implement instead of rendering it as (0, 0). */
 int p3_variation_supported(int index);
 
+/* Chaos-kernel specialisation (no counterpart in the reference, whose WGSL
+ * kernel dispatches variations through a runtime switch): p3_iterate compiles
+ * a kernel per genome shape (variation index of every slot, post-affine
+ * presence, final xform) with hipRTC on first use and caches it for the
+ * process. p3_chaos_prepare compiles ahead of time without a device (for
+ * warm-up outside a timed region); p3_chaos_source returns the generated
+ * source (len = full length; out, when non-null, gets up to cap - 1 bytes). */
+int p3_chaos_prepare(const float* xforms, int n_slots, int n_rows, int final_idx);
+int p3_chaos_source(const float* xforms, int n_slots, int n_rows, int final_idx, char* out,
+                    size_t cap, size_t* len);
+
 /* Message for the last failing call on this handle (or of p3_create when
  * r is null). Never null. */
 const char* p3_last_error(p3_renderer* r);

@@ -37,6 +37,8 @@ SIGNATURES = {
     "p3_read_timings": (_I, [_P, _FP]),
     "p3_sync": (_I, [_P]),
     "p3_dims": (_I, [_P, C.POINTER(_I)]),
+    "p3_chaos_prepare": (_I, [_FP, _I, _I, _I]),
+    "p3_chaos_source": (_I, [_FP, _I, _I, _I, C.c_char_p, _SZ, C.POINTER(_SZ)]),
     "p3_variation_supported": (_I, [_I]),
     "p3_last_error": (C.c_char_p, [_P]),
 }

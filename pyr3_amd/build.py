@@ -17,8 +17,30 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libpyr3hip.so"
-SOURCES = ["p3_api.cpp", "p3_chaos.hip", "p3_image.hip"]
-HEADERS = ["p3_device.h", "p3_launch.h", "p3_variations.h"]
+SOURCES = ["p3_api.cpp", "p3_rtc.cpp", "p3_image.hip"]
+# Device headers embedded as text for the run-time chaos specialisation, in
+# dependency order (their #include lines are dropped).
+RTC_HEADERS = ["p3_args.h", "p3_device.h", "p3_variations.h", "p3_chaos_kernel.h"]
+RTC_INC = CSRC / "p3_rtc_source.inc"
+RTC_PRELUDE = ("typedef unsigned int uint32_t;\ntypedef int int32_t;\n"
+               "typedef unsigned short uint16_t;\ntypedef unsigned char uint8_t;\n"
+               "typedef unsigned long size_t;\n")
+
+
+def write_rtc_source() -> Path:
+    """p3_rtc_source.inc: kRtcSource, the device headers as one string."""
+    text = RTC_PRELUDE
+    for h in RTC_HEADERS:
+        for line in (CSRC / h).read_text().splitlines():
+            if line.startswith("#include") or line.startswith("#pragma once"):
+                continue
+            text += line + "\n"
+    chunks = [text[i:i + 16000] for i in range(0, len(text), 16000)]
+    body = "\n".join('R"P3RTC(' + c + ')P3RTC"' for c in chunks)
+    out = ("// Generated by pyr3_amd/build.py from " + ", ".join(RTC_HEADERS) + ".\n"
+           "static const char kRtcSource[] =\n" + body + ";\n")
+    RTC_INC.write_text(out)
+    return RTC_INC
 
 
 def hipcc() -> str:
@@ -34,10 +56,11 @@ def command(out: Path = LIB):
         "-Wall", "-Wno-unused-function",
         "-I", str(PKG.parent / "include"),
         "-o", str(out),
-    ] + [str(CSRC / s) for s in SOURCES]
+    ] + [str(CSRC / s) for s in SOURCES] + ["-lhiprtc"]
 
 
 def build(verbose: bool = True) -> Path:
+    write_rtc_source()
     tmp = LIB.with_suffix(".so.tmp")
     cmd = command(tmp)
     if verbose:

@@ -259,8 +259,16 @@ int iterate_impl(p3_renderer* r, const float* xforms, int n_slots, const uint32_
   // Each walker splats at most `iters` times, each adding <= 255 per word.
   r->hist_bound += 255ULL * (unsigned long long)walkers * (unsigned long long)a.iters;
   const bool sat = r->hist_bound >= 0xffffffffULL;
+  // Compile (or fetch) the kernel for this genome shape before the timed launch.
+  const std::vector<char>* code = nullptr;
+  std::string rtc_err;
+  if (!p3::chaos_code(xforms, n_regular, final_idx, &code, rtc_err))
+    return r->fail(P3_E_HIP, "chaos kernel specialisation: %s", rtc_err.c_str());
   r->tic(T_CHAOS);
-  if ((rc = r->hip(p3::launch_chaos(a, sat, r->stream), "chaos launch"))) return rc;
+  const hipError_t le = p3::launch_chaos_rtc(a, xforms, n_regular, final_idx, sat, r->stream,
+                                             rtc_err);
+  if (le != hipSuccess)
+    return r->fail(P3_E_HIP, "chaos launch: %s %s", hipGetErrorString(le), rtc_err.c_str());
   r->toc(T_CHAOS);
   r->have_image = false;
   return P3_OK;
@@ -497,6 +505,39 @@ int p3_sync(p3_renderer* r) {
 int p3_dims(p3_renderer* r, int* out4) {
   if (!r || !out4) return P3_E_ARG;
   out4[0] = r->width; out4[1] = r->height; out4[2] = r->sw; out4[3] = r->sh;
+  return P3_OK;
+}
+
+int p3_chaos_prepare(const float* xforms, int n_slots, int n_rows, int final_idx) {
+  if (!xforms || n_rows < 2 || n_rows > P3_MAX_XFORMS + 1) return P3_E_ARG;
+  const int n_regular = n_rows - 1;
+  if (n_slots < n_regular || n_slots > n_regular + 1) return P3_E_ARG;
+  if (final_idx != -1 && (final_idx < 0 || final_idx >= n_slots)) return P3_E_ARG;
+  for (int s = 0; s < n_slots; ++s) {
+    const float* xf = xforms + (size_t)s * P3_XFORM_FLOATS;
+    const int nv = xf[7] >= 0.0f && xf[7] < 8.0f ? (int)xf[7] : (xf[7] >= 8.0f ? 8 : -1);
+    if (nv < 0) return P3_E_ARG;
+    for (int k = 0; k < nv; ++k) {
+      const float idxf = xf[20 + 4 * k];
+      if (!(idxf >= 0.0f) || idxf > 1e9f || !supported((uint32_t)idxf)) return P3_E_ARG;
+    }
+  }
+  const std::vector<char>* code = nullptr;
+  if (!p3::chaos_code(xforms, n_regular, final_idx, &code, g_create_error)) return P3_E_HIP;
+  return P3_OK;
+}
+
+int p3_chaos_source(const float* xforms, int n_slots, int n_rows, int final_idx, char* out,
+                    size_t cap, size_t* len) {
+  if (!xforms || n_rows < 2 || n_rows > P3_MAX_XFORMS + 1 || n_slots < n_rows - 1) return P3_E_ARG;
+  (void)n_slots;
+  const std::string src = p3::chaos_source(xforms, n_rows - 1, final_idx);
+  if (len) *len = src.size();
+  if (out && cap > 0) {
+    const size_t n = std::min(cap - 1, src.size());
+    std::memcpy(out, src.data(), n);
+    out[n] = '\0';
+  }
   return P3_OK;
 }
 

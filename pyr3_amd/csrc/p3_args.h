@@ -1,0 +1,42 @@
+// Argument blocks and LDS layout shared by the host launchers and the
+// device code (including the chaos kernels compiled at run time).
+#pragma once
+
+namespace p3 {
+
+constexpr int kGrain = 16384;      // xform-pick table row length
+constexpr int kMaxXforms = 128;    // regular xforms; the fallback row index
+constexpr int kXformFloats = 116;  // 464-byte xform record
+constexpr int kMaxVars = 8;        // variation slots per xform
+constexpr int kTraceFloats = 16;   // floats per trace entry
+constexpr int kChaosBlock = 256;   // walkers per chaos block
+
+struct ChaosArgs {
+  const float* xforms;
+  const uint32_t* distrib;     // rows 0..n-1 then the fallback row
+  const float4* palette;       // 256 RGBA
+  uint32_t* hist;              // W x H x 4
+  const uint32_t* isaac_in;    // walkers x 36 or null (seed on device)
+  float* trace;
+  uint32_t fallback_row;
+  uint32_t width, height, iters, fuse, seed, walkers;
+  int32_t final_idx;
+  float scale, cx, cy, cos_r, sin_r, jitter, xform_blend;
+  uint32_t palette_mode;
+  uint32_t trace_walkers, trace_iters;
+};
+
+// LDS image of a chaos block's walkers (structure of arrays, kB walkers):
+// ISAAC words [32][kB] (mem 0..15, rsl 16..31) and the per-walker scalars.
+template <int kB>
+struct BlockLds {
+  uint32_t isaac[32 * kB];
+  uint32_t cnt[kB], ra[kB], rb[kB], rc[kB];
+  float x[kB], y[kB], z[kB];
+  int32_t prev[kB];
+  uint32_t consec[kB], iter[kB], draws[kB], raw[kB];
+  uint16_t fn[kB], order[kB];
+  uint32_t bucket[kMaxXforms + 2], base[kMaxXforms + 2];
+};
+
+}  // namespace p3

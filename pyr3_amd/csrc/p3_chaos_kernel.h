@@ -1,5 +1,4 @@
-// Chaos-game kernel for gfx950: one walker per lane, ISAAC stream in LDS,
-// u32 RGBA histogram scatter with native global atomics.
+// Chaos-game kernel body for gfx950, compiled at run time per genome shape.
 //
 // Behaviour follows chaos_main (src/shaders/chaos.wgsl:8269-8732):
 //   init draws rand_11, rand_11, rand01 -> (x, y, z)
@@ -9,16 +8,23 @@
 //   check (NaN or |v| > 1e10) with rand_11 reseed and up to four retried
 //   turns that do not advance i (except at i = 0); final-xform lens with an
 //   opacity gate; scale-relative jitter (two rand01); splat after the fuse.
+//
+// The xform chains come from a generated `Spec` (see p3_rtc.cpp): one
+// straight-line case per xform of the genome's shape (its variation indices
+// per slot, pre_blur slots, post-affine presence), so no variation switch
+// runs per lane and each case reads its xform record at a fixed offset from
+// a kernel-argument pointer (scalar loads).
+//
 // The splat adds u32(pal * opacity * 255) and u32(opacity * 255) to the four
 // channels of bin (y * W + x). The reference's adds saturate at 2^32 - 1; a
 // saturating add is order-independent (the final value is min(sum, 2^32-1)),
-// so it is reproduced with a native atomic add whose returned old value
-// detects a wrap, followed by an exchange to 2^32 - 1: after the last wrap
-// of a bin the last operation on it is always that exchange. When the host
-// has proven that no bin can reach 2^32 (bound on 255 x samples since the
-// last reset) the adds are issued without a return value.
+// so it is reproduced with native atomic adds whose returned old values detect
+// a wrap, followed by an exchange to 2^32 - 1: after the last wrap of a word
+// the last operation on it is always that exchange. When the host has proven
+// that no word can reach 2^32 (bound on 255 x samples since the last reset)
+// the adds are issued without a return value.
+#pragma once
 #include "p3_device.h"
-#include "p3_launch.h"
 #include "p3_variations.h"
 
 namespace p3 {
@@ -28,86 +34,43 @@ struct Chain {
   float z;
 };
 
-__device__ inline int nvars_of(const float* xf) {
-  const uint32_t n = f2u_sat(xf[7]);
-  return (int)(n < (uint32_t)kMaxVars ? n : (uint32_t)kMaxVars);
-}
-
-// chaos.wgsl:8164-8181
-__device__ inline float2 pre_blur(const float* xf, int nv, float2 c, Rng& rng) {
-  for (int k = 0; k < nv; ++k) {
-    const float* v = xf + 20 + k * 4;
-    if (f2u_sat(v[0]) == 97u) {
-      const float r0 = rng.rand01(), r1 = rng.rand01(), r2 = rng.rand01(), r3 = rng.rand01();
-      const float g = v[1] * (r0 + r1 + r2 + r3 - 2.0f);
-      const float r4 = rng.rand01();
-      const float ang = r4 * kTau;
-      c.x = c.x + cosf(ang) * g;
-      c.y = c.y + sinf(ang) * g;
-    }
-  }
+// chaos.wgsl:8164-8181 for one pre_blur slot k of record xf.
+__device__ inline float2 pre_blur_slot(const float* xf, int k, float2 c, Rng& rng) {
+  const float r0 = rng.rand01(), r1 = rng.rand01(), r2 = rng.rand01(), r3 = rng.rand01();
+  const float g = xf[20 + 4 * k + 1] * (r0 + r1 + r2 + r3 - 2.0f);
+  const float r4 = rng.rand01();
+  const float ang = r4 * kTau;
+  c.x = c.x + cosf(ang) * g;
+  c.y = c.y + sinf(ang) * g;
   return c;
-}
-
-// Variation sum with the "last active direct colour wins" override.
-__device__ inline float2 var_sum(const float* xf, int nv, float2 pa, Rng& rng, bool& dc_on,
-                                 float3& dc_rgb) {
-  float2 pv = make_float2(0.0f, 0.0f);
-  const bool dc_flag = xf[11] > 0.5f;
-  for (int k = 0; k < nv; ++k) {
-    const float* v = xf + 20 + k * 4;
-    const int idx = (int)f2u_sat(v[0]);
-    const float w = v[1];
-    if (idx != 97) {
-      const float2 o = apply_variation(idx, pa, xf, k, w, rng);
-      pv.x = pv.x + o.x;
-      pv.y = pv.y + o.y;
-    }
-    if (dc_flag && w > 0.0f) {
-      float3 rgb;
-      if (dc_color_for(idx, pa, xf, k, rgb)) {
-        dc_on = true;
-        dc_rgb = rgb;
-      }
-    }
-  }
-  return pv;
 }
 
 __device__ inline float2 post_affine(const float* xf, float2 pv) {
-  if (xf[15] != 0.0f) {
-    return make_float2(xf[12] * pv.x + xf[13] * pv.y + xf[14],
-                       xf[16] * pv.x + xf[17] * pv.y + xf[18]);
-  }
-  return pv;
-}
-
-// chaos.wgsl:8226-8267
-__device__ inline Chain xform_chain(const float* xf, float x, float y, float z, Rng& rng,
-                                    bool& dc_on, float3& dc_rgb) {
-  Chain c;
-  c.pa = make_float2(xf[0] * x + xf[1] * y + xf[2], xf[4] * x + xf[5] * y + xf[6]);
-  const int nv = nvars_of(xf);
-  const float2 pam = pre_blur(xf, nv, c.pa, rng);
-  const float2 pv = var_sum(xf, nv, pam, rng, dc_on, dc_rgb);
-  c.pv_pre = pv;
-  c.pv = post_affine(xf, pv);
-  c.z = mixf(z, xf[8], xf[9]);
-  return c;
+  return make_float2(xf[12] * pv.x + xf[13] * pv.y + xf[14],
+                     xf[16] * pv.x + xf[17] * pv.y + xf[18]);
 }
 
 __device__ inline bool is_bad(float2 v) {
   return isnan(v.x) || isnan(v.y) || fabsf(v.x) > 1e10f || fabsf(v.y) > 1e10f;
 }
 
+// Four channel adds; returns are checked after all four are issued.
 template <bool kSat>
-__device__ inline void hist_add(uint32_t* slot, uint32_t d) {
-  if (d == 0u) return;
+__device__ inline void hist_add4(uint32_t* b, uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3) {
   if (kSat) {
-    const uint32_t old = atomicAdd(slot, d);
-    if (old > 0xffffffffu - d) atomicExch(slot, 0xffffffffu);
+    const uint32_t o0 = d0 ? atomicAdd(b + 0, d0) : 0u;
+    const uint32_t o1 = d1 ? atomicAdd(b + 1, d1) : 0u;
+    const uint32_t o2 = d2 ? atomicAdd(b + 2, d2) : 0u;
+    const uint32_t o3 = d3 ? atomicAdd(b + 3, d3) : 0u;
+    if (d0 && o0 > 0xffffffffu - d0) atomicExch(b + 0, 0xffffffffu);
+    if (d1 && o1 > 0xffffffffu - d1) atomicExch(b + 1, 0xffffffffu);
+    if (d2 && o2 > 0xffffffffu - d2) atomicExch(b + 2, 0xffffffffu);
+    if (d3 && o3 > 0xffffffffu - d3) atomicExch(b + 3, 0xffffffffu);
   } else {
-    atomicAdd(slot, d);
+    if (d0) atomicAdd(b + 0, d0);
+    if (d1) atomicAdd(b + 1, d1);
+    if (d2) atomicAdd(b + 2, d2);
+    if (d3) atomicAdd(b + 3, d3);
   }
 }
 
@@ -120,15 +83,15 @@ struct Walker {
 
 // One turn of walker `wid` (global id) after its pick `fn` (from draw `raw`)
 // is known: chain, blend, bad check and retry window, final lens, jitter,
-// splat. Returns nothing; advances `wk.iter` unless the turn is retried.
-template <bool kSat>
+// splat. Advances `wk.iter` unless the turn is retried.
+template <bool kSat, class Spec>
 __device__ inline void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32_t fn,
                                 uint32_t raw, uint32_t wid) {
   const uint32_t i = wk.iter;
   const float* xf = A.xforms + (size_t)fn * kXformFloats;
   bool dc_on = false;
   float3 dc_rgb = make_float3(0.0f, 0.0f, 0.0f);
-  const Chain ca = xform_chain(xf, wk.x, wk.y, wk.z, rng, dc_on, dc_rgb);
+  const Chain ca = Spec::chain(fn, A.xforms, wk.x, wk.y, wk.z, rng, dc_on, dc_rgb);
   float2 pv = ca.pv;
   float nz = ca.z;
   if (A.xform_blend > 0.0f) {
@@ -137,8 +100,7 @@ __device__ inline void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32
       const uint32_t bi = A.distrib[(size_t)A.fallback_row * kGrain + (rb & (kGrain - 1))];
       bool dc_b = false;
       float3 rgb_b;
-      const Chain cb = xform_chain(A.xforms + (size_t)bi * kXformFloats, wk.x, wk.y, wk.z, rng,
-                                   dc_b, rgb_b);
+      const Chain cb = Spec::chain(bi, A.xforms, wk.x, wk.y, wk.z, rng, dc_b, rgb_b);
       const float t = rng.rand01();
       pv.x = mixf(pv.x, cb.pv.x, t);
       pv.y = mixf(pv.y, cb.pv.y, t);
@@ -174,16 +136,13 @@ __device__ inline void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32
   if (wk.consec >= 5u) wk.consec = 0u;
   wk.prev = (int)fn;
   float sx = ppx, sy = ppy, sz = ppz;
-  if (A.final_idx >= 0) {
-    const float* fx = A.xforms + (size_t)A.final_idx * kXformFloats;
+  if (Spec::kHasFinal) {
+    const float* fx = A.xforms + (size_t)Spec::kFinalSlot * kXformFloats;
     const bool apply = fx[10] == 1.0f ? true : (rng.rand01() < fx[10]);
     if (apply) {
       const float2 fpa = make_float2(fx[0] * pv.x + fx[1] * pv.y + fx[2],
                                      fx[4] * pv.x + fx[5] * pv.y + fx[6]);
-      const int fnv = nvars_of(fx);
-      const float2 fpam = pre_blur(fx, fnv, fpa, rng);
-      float2 fpv = var_sum(fx, fnv, fpam, rng, dc_on, dc_rgb);
-      fpv = post_affine(fx, fpv);
+      const float2 fpv = Spec::final_sum(A.xforms, fpa, rng, dc_on, dc_rgb);
       const float fz = mixf(nz, fx[8], fx[9]);
       if (!is_bad(fpv)) { sx = fpv.x; sy = fpv.y; sz = fz; }
     }
@@ -224,40 +183,23 @@ __device__ inline void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32
       if (dc_on) { pal.x = dc_rgb.x; pal.y = dc_rgb.y; pal.z = dc_rgb.z; }
       const float wgt = opacity * 255.0f;
       uint32_t* b = A.hist + ((size_t)yi * A.width + xi) * 4u;
-      hist_add<kSat>(b + 0, f2u_sat(pal.x * wgt));
-      hist_add<kSat>(b + 1, f2u_sat(pal.y * wgt));
-      hist_add<kSat>(b + 2, f2u_sat(pal.z * wgt));
-      hist_add<kSat>(b + 3, f2u_sat(wgt));
+      hist_add4<kSat>(b, f2u_sat(pal.x * wgt), f2u_sat(pal.y * wgt), f2u_sat(pal.z * wgt),
+                      f2u_sat(wgt));
     }
   }
 }
-
-// LDS image of a block's walkers (structure of arrays, kB walkers):
-// ISAAC words [32][kB] (mem 0..15, rsl 16..31) and the per-walker scalars.
-template <int kB>
-struct BlockLds {
-  uint32_t isaac[32 * kB];
-  uint32_t cnt[kB], ra[kB], rb[kB], rc[kB];
-  float x[kB], y[kB], z[kB];
-  int32_t prev[kB];
-  uint32_t consec[kB], iter[kB], draws[kB], raw[kB];
-  uint16_t fn[kB], order[kB];
-  uint32_t bucket[kMaxXforms + 2], base[kMaxXforms + 2];
-};
 
 // Persistent block of kB walkers. Every turn:
 //   A  lane t draws walker t's pick and counts it into an LDS bucket per xform
 //   B  one wave turns the bucket counts into offsets
 //   C  walkers are listed in xform order
 //   D  lane t runs the turn of the t-th walker in that order
-// so the lanes of a wave mostly share one xform and its variation code.
+// so the lanes of a wave mostly share one xform and its straight-line code.
 // Each walker keeps its own ISAAC column and loop state; only which lane
 // computes it changes, so the per-walker draw sequence, arithmetic and
 // splats are those of the one-lane-per-walker loop.
-template <bool kSat, int kB>
-__global__ void __launch_bounds__(kB) chaos_binned(ChaosArgs A) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  BlockLds<kB>& L = *reinterpret_cast<BlockLds<kB>*>(lds_raw);
+template <bool kSat, int kB, class Spec>
+__device__ inline void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L) {
   const int t = threadIdx.x;
   const uint32_t w0 = blockIdx.x * kB;
   const uint32_t total = A.iters + A.fuse;
@@ -269,6 +211,7 @@ __global__ void __launch_bounds__(kB) chaos_binned(ChaosArgs A) {
     rng.m.base = L.isaac + t;
     rng.m.stride = kB;
     rng.draws = 0u;
+    rng.s.cnt = rng.s.a = rng.s.b = rng.s.c = 0u;
     float x = 0.0f, y = 0.0f, z = 0.0f;
     uint32_t iter = total;
     if (wid < A.walkers) {
@@ -342,7 +285,7 @@ __global__ void __launch_bounds__(kB) chaos_binned(ChaosArgs A) {
         rng.s.cnt = L.cnt[wl]; rng.s.a = L.ra[wl]; rng.s.b = L.rb[wl]; rng.s.c = L.rc[wl];
         rng.draws = L.draws[wl];
         Walker wk{L.x[wl], L.y[wl], L.z[wl], L.prev[wl], L.consec[wl], L.iter[wl]};
-        run_turn<kSat>(A, wk, rng, L.fn[wl], L.raw[wl], w0 + wl);
+        run_turn<kSat, Spec>(A, wk, rng, L.fn[wl], L.raw[wl], w0 + wl);
         L.cnt[wl] = rng.s.cnt; L.ra[wl] = rng.s.a; L.rb[wl] = rng.s.b; L.rc[wl] = rng.s.c;
         L.draws[wl] = rng.draws;
         L.x[wl] = wk.x; L.y[wl] = wk.y; L.z[wl] = wk.z;
@@ -352,21 +295,6 @@ __global__ void __launch_bounds__(kB) chaos_binned(ChaosArgs A) {
     }
     __syncthreads();
   }
-}
-
-constexpr int kChaosBlock = 256;
-
-size_t chaos_lds_bytes() { return sizeof(BlockLds<kChaosBlock>); }
-
-hipError_t launch_chaos(const ChaosArgs& a, bool saturating, hipStream_t stream) {
-  const unsigned grid = (a.walkers + kChaosBlock - 1) / kChaosBlock;
-  if (grid == 0) return hipSuccess;
-  const size_t shmem = sizeof(BlockLds<kChaosBlock>);
-  if (saturating)
-    chaos_binned<true, kChaosBlock><<<grid, kChaosBlock, shmem, stream>>>(a);
-  else
-    chaos_binned<false, kChaosBlock><<<grid, kChaosBlock, shmem, stream>>>(a);
-  return hipGetLastError();
 }
 
 }  // namespace p3

@@ -11,13 +11,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-namespace p3 {
+#include "p3_args.h"
 
-constexpr int kGrain = 16384;
-constexpr int kMaxXforms = 128;
-constexpr int kXformFloats = 116;
-constexpr int kMaxVars = 8;
-constexpr int kTraceFloats = 16;
+namespace p3 {
 
 constexpr float kPi = 3.14159265358979323846f;
 constexpr float kTau = 6.28318530717958647692f;

@@ -1,25 +1,16 @@
 // Launch-side interface between the C ABI (p3_api.cpp) and the kernels:
-// argument blocks passed by value to each kernel, and the launchers.
+// argument blocks passed by value to each kernel, and the launchers. The
+// chaos kernel is compiled per genome shape at run time (p3_rtc.cpp).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-namespace p3 {
+#include <string>
+#include <vector>
 
-struct ChaosArgs {
-  const float* xforms;
-  const uint32_t* distrib;     // rows 0..n-1 then the fallback row
-  const float4* palette;       // 256 RGBA
-  uint32_t* hist;              // W x H x 4
-  const uint32_t* isaac_in;    // walkers x 36 or null (seed on device)
-  float* trace;
-  uint32_t fallback_row;
-  uint32_t width, height, iters, fuse, seed, walkers;
-  int32_t final_idx;
-  float scale, cx, cy, cos_r, sin_r, jitter, xform_blend;
-  uint32_t palette_mode;
-  uint32_t trace_walkers, trace_iters;
-};
+#include "p3_args.h"
+
+namespace p3 {
 
 struct DeArgs {
   const uint4* hist;     // W x H RGBA u32
@@ -45,7 +36,15 @@ struct TmArgs {
   uint32_t transparent;
 };
 
-hipError_t launch_chaos(const ChaosArgs& a, bool saturating, hipStream_t stream);
+// Chaos kernel specialised for the shape of `host_xforms` (p3_rtc.cpp).
+hipError_t launch_chaos_rtc(const ChaosArgs& a, const float* host_xforms, int n_regular,
+                            int final_slot, bool saturating, hipStream_t stream,
+                            std::string& err);
+bool chaos_code(const float* host_xforms, int n_regular, int final_slot,
+                const std::vector<char>** out, std::string& err);
+std::string chaos_source(const float* host_xforms, int n_regular, int final_slot);
+std::string chaos_shape_key(const float* host_xforms, int n_regular, int final_slot);
+size_t chaos_compiled_shapes();
 hipError_t launch_density(const DeArgs& a, hipStream_t stream);
 hipError_t launch_tonemap(const TmArgs& a, bool de, hipStream_t stream);
 hipError_t launch_fill(void* p, size_t bytes, uint32_t v, hipStream_t stream);

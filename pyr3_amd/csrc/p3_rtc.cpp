@@ -1,0 +1,224 @@
+// Run-time specialisation of the chaos kernel with hipRTC.
+//
+// A genome's "shape" is, per xform record, the variation catalog index of
+// every slot and whether a post-affine is present, plus the same for the
+// final xform (or its absence). For each new shape a small source is
+// generated: one `case` per xform with straight-line pre_blur / variation /
+// post-affine code (the catalog switch folds away because the index is a
+// constant), reading weights and parameters from the record at fixed offsets.
+// Weights and parameters stay in the record, so every genome of the same
+// shape (animation frames, quality sweeps) reuses the compiled module. It is
+// compiled for gfx950 with the same flags as the rest of the library, loaded
+// with hipModuleLoadData and cached for the life of the process.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "p3_launch.h"
+
+namespace {
+
+#include "p3_rtc_source.inc"  // defines kRtcSource[] (generated by the build)
+
+struct Module {
+  hipModule_t mod = nullptr;
+  hipFunction_t sat = nullptr, nosat = nullptr;
+};
+
+std::mutex g_mu;
+std::unordered_map<std::string, std::vector<char>> g_code;  // key: shape
+std::unordered_map<std::string, Module> g_mods;             // key: device + shape
+
+uint32_t slot_index(const float* xf, int k) { return (uint32_t)xf[20 + 4 * k]; }
+int slot_count(const float* xf) {
+  const float n = xf[7];
+  return n < 8.0f ? (int)n : 8;
+}
+
+void shape_of(const float* xf, std::ostringstream& key) {
+  const int nv = slot_count(xf);
+  key << nv << (xf[15] != 0.0f ? "p" : "n");
+  for (int k = 0; k < nv; ++k) key << "," << slot_index(xf, k);
+  key << ";";
+}
+
+// Straight-line chain of one record at X + slot * 116.
+void emit_body(std::ostringstream& s, const float* xf, int slot, bool full) {
+  const int nv = slot_count(xf);
+  s << "      const float* xf = X + " << slot * p3::kXformFloats << ";\n";
+  if (full) {
+    s << "      c.pa = make_float2(xf[0] * x + xf[1] * y + xf[2], xf[4] * x + xf[5] * y + xf[6]);\n"
+      << "      float2 pam = c.pa;\n";
+  } else {
+    s << "      float2 pam = pa;\n";
+  }
+  for (int k = 0; k < nv; ++k)
+    if (slot_index(xf, k) == 97u) s << "      pam = pre_blur_slot(xf, " << k << ", pam, rng);\n";
+  s << "      float2 pv = make_float2(0.0f, 0.0f);\n";
+  for (int k = 0; k < nv; ++k) {
+    const uint32_t idx = slot_index(xf, k);
+    if (idx == 97u) continue;
+    s << "      { const float2 o = apply_variation(" << idx << ", pam, xf, " << k << ", xf["
+      << 20 + 4 * k + 1 << "], rng); pv.x = pv.x + o.x; pv.y = pv.y + o.y; }\n";
+  }
+  const bool post = xf[15] != 0.0f;
+  if (full) {
+    s << "      c.pv_pre = pv;\n"
+      << "      c.pv = " << (post ? "post_affine(xf, pv)" : "pv") << ";\n"
+      << "      c.z = mixf(z, xf[8], xf[9]);\n";
+  } else {
+    s << "      return " << (post ? "post_affine(xf, pv)" : "pv") << ";\n";
+  }
+}
+
+std::string generate(const float* xforms, int n_regular, int final_slot) {
+  std::ostringstream s;
+  s << "namespace p3 {\nstruct Spec {\n"
+    << "  static constexpr bool kHasFinal = " << (final_slot >= 0 ? "true" : "false") << ";\n"
+    << "  static constexpr int kFinalSlot = " << (final_slot >= 0 ? final_slot : 0) << ";\n"
+    << "  static __device__ inline Chain chain(uint32_t fn, const float* __restrict__ X, float x,\n"
+    << "      float y, float z, Rng& rng, bool& dc_on, float3& dc_rgb) {\n"
+    << "    (void)dc_on; (void)dc_rgb;\n"
+    << "    Chain c;\n"
+    << "    switch (fn) {\n";
+  for (int j = 0; j < n_regular; ++j) {
+    s << "    case " << j << ": {\n";
+    emit_body(s, xforms + (size_t)j * p3::kXformFloats, j, true);
+    s << "    } break;\n";
+  }
+  s << "    default: __builtin_unreachable();\n"
+    << "    }\n"
+    << "    return c;\n"
+    << "  }\n"
+    << "  static __device__ inline float2 final_sum(const float* __restrict__ X, float2 pa,\n"
+    << "      Rng& rng, bool& dc_on, float3& dc_rgb) {\n"
+    << "    (void)dc_on; (void)dc_rgb;\n";
+  if (final_slot >= 0) {
+    s << "    {\n";
+    emit_body(s, xforms + (size_t)final_slot * p3::kXformFloats, final_slot, false);
+    s << "    }\n";
+  } else {
+    s << "    (void)X; (void)rng; return pa;\n";
+  }
+  s << "  }\n};\n}  // namespace p3\n\n"
+    << "extern \"C\" __global__ void __launch_bounds__(p3::kChaosBlock)\n"
+    << "p3_chaos_sat(p3::ChaosArgs A) {\n"
+    << "  extern __shared__ __align__(16) unsigned char lds_raw[];\n"
+    << "  p3::chaos_binned<true, p3::kChaosBlock, p3::Spec>(A,\n"
+    << "      *reinterpret_cast<p3::BlockLds<p3::kChaosBlock>*>(lds_raw));\n"
+    << "}\n"
+    << "extern \"C\" __global__ void __launch_bounds__(p3::kChaosBlock)\n"
+    << "p3_chaos_nosat(p3::ChaosArgs A) {\n"
+    << "  extern __shared__ __align__(16) unsigned char lds_raw[];\n"
+    << "  p3::chaos_binned<false, p3::kChaosBlock, p3::Spec>(A,\n"
+    << "      *reinterpret_cast<p3::BlockLds<p3::kChaosBlock>*>(lds_raw));\n"
+    << "}\n";
+  return s.str();
+}
+
+bool compile(const std::string& src, std::vector<char>& code, std::string& err) {
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, src.c_str(), "p3_chaos_spec.hip", 0, nullptr, nullptr) !=
+      HIPRTC_SUCCESS) {
+    err = "hiprtcCreateProgram failed";
+    return false;
+  }
+  const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
+                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  const hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof opts / sizeof opts[0]), opts);
+  if (rc != HIPRTC_SUCCESS) {
+    size_t n = 0;
+    hiprtcGetProgramLogSize(prog, &n);
+    std::string log(n, '\0');
+    if (n) hiprtcGetProgramLog(prog, &log[0]);
+    err = std::string("hiprtc compile failed: ") + hiprtcGetErrorString(rc) + "\n" + log;
+    hiprtcDestroyProgram(&prog);
+    return false;
+  }
+  size_t n = 0;
+  hiprtcGetCodeSize(prog, &n);
+  code.resize(n);
+  hiprtcGetCode(prog, code.data());
+  hiprtcDestroyProgram(&prog);
+  return true;
+}
+
+}  // namespace
+
+namespace p3 {
+
+std::string chaos_shape_key(const float* xforms, int n_regular, int final_slot) {
+  std::ostringstream key;
+  for (int j = 0; j < n_regular; ++j) shape_of(xforms + (size_t)j * kXformFloats, key);
+  key << "F";
+  if (final_slot >= 0) {
+    key << final_slot << ":";
+    shape_of(xforms + (size_t)final_slot * kXformFloats, key);
+  }
+  return key.str();
+}
+
+std::string chaos_source(const float* xforms, int n_regular, int final_slot) {
+  return std::string(kRtcSource) + generate(xforms, n_regular, final_slot);
+}
+
+// Code object for this shape, compiled on first use (no device needed).
+bool chaos_code(const float* host_xforms, int n_regular, int final_slot,
+                const std::vector<char>** out, std::string& err) {
+  const std::string key = chaos_shape_key(host_xforms, n_regular, final_slot);
+  std::lock_guard<std::mutex> lock(g_mu);
+  auto it = g_code.find(key);
+  if (it == g_code.end()) {
+    std::vector<char> code;
+    if (!compile(chaos_source(host_xforms, n_regular, final_slot), code, err)) return false;
+    it = g_code.emplace(key, std::move(code)).first;
+  }
+  *out = &it->second;
+  return true;
+}
+
+// Compiles (or fetches) the kernel for this shape and launches it on `stream`.
+hipError_t launch_chaos_rtc(const ChaosArgs& a, const float* host_xforms, int n_regular,
+                            int final_slot, bool saturating, hipStream_t stream,
+                            std::string& err) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  const std::vector<char>* code = nullptr;
+  if (!chaos_code(host_xforms, n_regular, final_slot, &code, err)) return hipErrorInvalidImage;
+  const std::string key =
+      std::to_string(dev) + "|" + chaos_shape_key(host_xforms, n_regular, final_slot);
+  Module m;
+  {
+    std::lock_guard<std::mutex> lock(g_mu);
+    auto it = g_mods.find(key);
+    if (it != g_mods.end()) {
+      m = it->second;
+    } else {
+      if ((e = hipModuleLoadData(&m.mod, code->data())) != hipSuccess) return e;
+      if ((e = hipModuleGetFunction(&m.sat, m.mod, "p3_chaos_sat")) != hipSuccess) return e;
+      if ((e = hipModuleGetFunction(&m.nosat, m.mod, "p3_chaos_nosat")) != hipSuccess) return e;
+      g_mods.emplace(key, m);
+    }
+  }
+  const unsigned grid = (a.walkers + kChaosBlock - 1) / kChaosBlock;
+  if (grid == 0) return hipSuccess;
+  ChaosArgs args = a;
+  void* params[] = {&args};
+  return hipModuleLaunchKernel(saturating ? m.sat : m.nosat, grid, 1, 1, kChaosBlock, 1, 1,
+                               (unsigned)sizeof(BlockLds<kChaosBlock>), stream, params, nullptr);
+}
+
+size_t chaos_compiled_shapes() {
+  std::lock_guard<std::mutex> lock(g_mu);
+  return g_code.size();
+}
+
+}  // namespace p3

@@ -1,0 +1,56 @@
+"""Run-time chaos specialisation compiles for gfx950 without a device.
+
+The chaos kernel is generated per genome shape and compiled with hipRTC; the
+compile needs no GPU, so this runs on the build host: one shape holding every
+implemented catalog entry (one xform per entry, pre_blur included, a post
+affine on every other xform, a final xform), and the benchmark genome.
+"""
+import ctypes as C
+import time
+
+import numpy as np
+
+from helpers import genome, xform
+from pyr3_amd import pack, synth
+from pyr3_amd.variations_supported import SUPPORTED
+
+
+def _prepare(g):
+    from pyr3_amd import _lib
+    lib = _lib.load()
+    e = pack.expand_genome_for_gpu(g)
+    xf = np.ascontiguousarray(pack.pack_xforms(e))
+    n_rows = len(e.xforms) + 1
+    fp = xf.ctypes.data_as(C.POINTER(C.c_float))
+    rc = lib.p3_chaos_prepare(fp, xf.shape[0], n_rows, pack.final_xform_slot(e))
+    n = C.c_size_t()
+    lib.p3_chaos_source(fp, xf.shape[0], n_rows, pack.final_xform_slot(e), None, 0, C.byref(n))
+    buf = C.create_string_buffer(n.value + 1)
+    lib.p3_chaos_source(fp, xf.shape[0], n_rows, pack.final_xform_slot(e), buf, n.value + 1, None)
+    return rc, buf.value.decode(), lib.p3_last_error(None)
+
+
+def test_every_catalog_entry_compiles():
+    idx = sorted(SUPPORTED)
+    xs = [xform(1.0, vars=((i, 1.0),), post=(1, 0, 0, 0, 1, 0) if k % 2 else None)
+          for k, i in enumerate(idx[:127])]
+    g = genome(xs, final=xform(0.0, vars=((13, 1.0), (97, 0.1))))
+    t0 = time.time()
+    rc, src, err = _prepare(g)
+    assert rc == 0, err
+    for i in idx:
+        if i != 97:
+            assert f"apply_variation({i}, pam" in src
+    assert "pre_blur_slot" in src and "post_affine(xf, pv)" in src
+    print(f"compiled {len(idx)} entries in {time.time() - t0:.1f}s")
+
+
+def test_benchmark_genome_compiles():
+    rc, src, err = _prepare(synth.bench_genome(0))
+    assert rc == 0, err
+    assert src.count("case ") >= 12
+
+
+def test_unsupported_entry_is_rejected():
+    rc, _, _ = _prepare(genome([xform(1.0, vars=((300, 1.0),))]))
+    assert rc != 0
