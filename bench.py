@@ -37,10 +37,13 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def schedule(samples: int, fuse: int, max_walkers: int = 1 << 19, min_ratio: int = 4):
-    """Walkers x iters for a throughput launch: as many walkers as the chip
-    holds, but at least min_ratio x fuse post-fuse turns per walker."""
-    walkers = max(256, min(max_walkers, samples // (min_ratio * fuse)))
+WALKER_CANDIDATES = (65536, 131072, 196608, 262144, 393216, 524288)
+
+
+def schedule(samples: int, walkers: int):
+    """Walkers x iters for `samples` post-fuse samples (walkers capped so that
+    each walker splats at least once)."""
+    walkers = max(256, min(walkers, samples))
     walkers = (walkers + 255) // 256 * 256
     iters = -(-samples // walkers)
     return walkers, iters
@@ -49,17 +52,35 @@ def schedule(samples: int, fuse: int, max_walkers: int = 1 << 19, min_ratio: int
 class Frame:
     """One plain-path frame of a genome at a quality (reset, chaos, DE, tonemap)."""
 
-    def __init__(self, r: Renderer, g, fuse: int):
+    def __init__(self, r: Renderer, g, fuse: int, walkers: int = 196608):
         self.r, self.g, self.fuse = r, g, fuse
+        self.walkers = walkers
         self.packed = pack_genome(g)
 
-    def render(self, quality: float, seed: int):
+    def render(self, quality: float, seed: int, walkers: int = None):
         samples_target = int(round(quality * self.r.width * self.r.height))
-        walkers, iters = schedule(samples_target, self.fuse)
+        walkers, iters = schedule(samples_target, walkers or self.walkers)
         self.r.reset(self.g)
         s = self.r.iterate(self.g, seed, walkers, iters, fuse=self.fuse, packed=self.packed)
         self.r.present(self.g, s)
         return walkers, iters, s
+
+
+def tune_walkers(frame: Frame, quality: float):
+    """Autotune the walker count: time the chaos kernel of one frame per
+    candidate (more walkers fill the chip, each adds `fuse` unsplatted turns)."""
+    best, times = None, {}
+    frame.r.set_timing(True)
+    for w in WALKER_CANDIDATES:
+        frame.render(quality, 31, walkers=w)
+        frame.r.sync()
+        ms = frame.r.read_timings()["chaos"]
+        times[w] = round(ms, 3)
+        if best is None or ms < times[best]:
+            best = w
+    frame.r.set_timing(False)
+    log(f"walker autotune at q={quality:g}: {times} -> {best}")
+    return best, times
 
 
 def psnr(a, b):
@@ -176,11 +197,12 @@ def main():
     k, g, cover = pick_genome(r, a.fuse, W, H)
     log(f"genome k={k} ({len(g.xforms)} xforms, cover {cover:.2%})")
     frame = Frame(r, g, a.fuse)
+    frame.walkers, tune = tune_walkers(frame, a.quality or 30)
 
     scores = {}
     if a.quality is None:
         t0 = time.perf_counter()
-        frame.render(a.ref_quality, REF_SEED)
+        frame.render(a.ref_quality, REF_SEED, walkers=max(WALKER_CANDIDATES))
         ref_img = r.read_image()
         log(f"reference q={a.ref_quality:g} rendered in {time.perf_counter() - t0:.2f}s")
         q, scores = quality_to_target(frame, ref_img, a.target_db)
@@ -229,6 +251,7 @@ def main():
         "config": {"workload": f"plain-{W}x{H}-os1", "genome_seed": k, "quality": q,
                    "target_db": a.target_db, "ref_quality": a.ref_quality,
                    "walkers": walkers, "iters": iters, "fuse": a.fuse,
+                   "walker_autotune_ms": {str(k): v for k, v in tune.items()},
                    "psnr_ladder": {str(kq): round(v, 2) for kq, v in sorted(scores.items())}},
         "giters_per_s": round(giters, 3),
         "gsamples_per_s": round(gsamples, 3),

@@ -35,7 +35,7 @@ struct Chain {
 };
 
 // chaos.wgsl:8164-8181 for one pre_blur slot k of record xf.
-__device__ inline float2 pre_blur_slot(const float* xf, int k, float2 c, Rng& rng) {
+P3_INLINE float2 pre_blur_slot(xrec_t xf, int k, float2 c, Rng& rng) {
   const float r0 = rng.rand01(), r1 = rng.rand01(), r2 = rng.rand01(), r3 = rng.rand01();
   const float g = xf[20 + 4 * k + 1] * (r0 + r1 + r2 + r3 - 2.0f);
   const float r4 = rng.rand01();
@@ -45,32 +45,41 @@ __device__ inline float2 pre_blur_slot(const float* xf, int k, float2 c, Rng& rn
   return c;
 }
 
-__device__ inline float2 post_affine(const float* xf, float2 pv) {
+P3_INLINE float2 post_affine(xrec_t xf, float2 pv) {
   return make_float2(xf[12] * pv.x + xf[13] * pv.y + xf[14],
                      xf[16] * pv.x + xf[17] * pv.y + xf[18]);
 }
 
-__device__ inline bool is_bad(float2 v) {
+P3_INLINE bool is_bad(float2 v) {
   return isnan(v.x) || isnan(v.y) || fabsf(v.x) > 1e10f || fabsf(v.y) > 1e10f;
 }
 
-// Four channel adds; returns are checked after all four are issued.
+// Four channel adds at device scope (relaxed ordering: only the final sums
+// matter); with kSat the returned old values are checked after all four are
+// issued.
+P3_INLINE uint32_t add_ret(P3_GLOBAL uint32_t* p, uint32_t d) {
+  return __hip_atomic_fetch_add(p, d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+P3_INLINE void pin_max(P3_GLOBAL uint32_t* p) {
+  (void)__hip_atomic_exchange(p, 0xffffffffu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 template <bool kSat>
-__device__ inline void hist_add4(uint32_t* b, uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3) {
+P3_INLINE void hist_add4(P3_GLOBAL uint32_t* b, uint32_t d0, uint32_t d1, uint32_t d2,
+                         uint32_t d3) {
   if (kSat) {
-    const uint32_t o0 = d0 ? atomicAdd(b + 0, d0) : 0u;
-    const uint32_t o1 = d1 ? atomicAdd(b + 1, d1) : 0u;
-    const uint32_t o2 = d2 ? atomicAdd(b + 2, d2) : 0u;
-    const uint32_t o3 = d3 ? atomicAdd(b + 3, d3) : 0u;
-    if (d0 && o0 > 0xffffffffu - d0) atomicExch(b + 0, 0xffffffffu);
-    if (d1 && o1 > 0xffffffffu - d1) atomicExch(b + 1, 0xffffffffu);
-    if (d2 && o2 > 0xffffffffu - d2) atomicExch(b + 2, 0xffffffffu);
-    if (d3 && o3 > 0xffffffffu - d3) atomicExch(b + 3, 0xffffffffu);
+    const uint32_t o0 = d0 ? add_ret(b + 0, d0) : 0u;
+    const uint32_t o1 = d1 ? add_ret(b + 1, d1) : 0u;
+    const uint32_t o2 = d2 ? add_ret(b + 2, d2) : 0u;
+    const uint32_t o3 = d3 ? add_ret(b + 3, d3) : 0u;
+    if (d0 && o0 > 0xffffffffu - d0) pin_max(b + 0);
+    if (d1 && o1 > 0xffffffffu - d1) pin_max(b + 1);
+    if (d2 && o2 > 0xffffffffu - d2) pin_max(b + 2);
+    if (d3 && o3 > 0xffffffffu - d3) pin_max(b + 3);
   } else {
-    if (d0) atomicAdd(b + 0, d0);
-    if (d1) atomicAdd(b + 1, d1);
-    if (d2) atomicAdd(b + 2, d2);
-    if (d3) atomicAdd(b + 3, d3);
+    if (d0) (void)add_ret(b + 0, d0);
+    if (d1) (void)add_ret(b + 1, d1);
+    if (d2) (void)add_ret(b + 2, d2);
+    if (d3) (void)add_ret(b + 3, d3);
   }
 }
 
@@ -85,22 +94,24 @@ struct Walker {
 // is known: chain, blend, bad check and retry window, final lens, jitter,
 // splat. Advances `wk.iter` unless the turn is retried.
 template <bool kSat, class Spec>
-__device__ inline void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32_t fn,
+P3_INLINE void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32_t fn,
                                 uint32_t raw, uint32_t wid) {
   const uint32_t i = wk.iter;
-  const float* xf = A.xforms + (size_t)fn * kXformFloats;
+  const xrec_t X = (xrec_t)A.xforms;
+  const P3_CONST uint32_t* D = (const P3_CONST uint32_t*)A.distrib;
+  const xrec_t xf = X + (size_t)fn * kXformFloats;
   bool dc_on = false;
   float3 dc_rgb = make_float3(0.0f, 0.0f, 0.0f);
-  const Chain ca = Spec::chain(fn, A.xforms, wk.x, wk.y, wk.z, rng, dc_on, dc_rgb);
+  const Chain ca = Spec::chain(fn, X, wk.x, wk.y, wk.z, rng, dc_on, dc_rgb);
   float2 pv = ca.pv;
   float nz = ca.z;
   if (A.xform_blend > 0.0f) {
     if (rng.rand01() < A.xform_blend) {
       const uint32_t rb = rng.irand();
-      const uint32_t bi = A.distrib[(size_t)A.fallback_row * kGrain + (rb & (kGrain - 1))];
+      const uint32_t bi = D[(size_t)A.fallback_row * kGrain + (rb & (kGrain - 1))];
       bool dc_b = false;
       float3 rgb_b;
-      const Chain cb = Spec::chain(bi, A.xforms, wk.x, wk.y, wk.z, rng, dc_b, rgb_b);
+      const Chain cb = Spec::chain(bi, X, wk.x, wk.y, wk.z, rng, dc_b, rgb_b);
       const float t = rng.rand01();
       pv.x = mixf(pv.x, cb.pv.x, t);
       pv.y = mixf(pv.y, cb.pv.y, t);
@@ -116,9 +127,9 @@ __device__ inline void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32
   } else {
     wk.consec = 0u;
   }
-  float* tr = nullptr;
+  P3_GLOBAL float* tr = nullptr;
   if (A.trace != nullptr && wid < A.trace_walkers && i < A.trace_iters) {
-    tr = A.trace + ((size_t)wid * A.trace_iters + i) * kTraceFloats;
+    tr = (P3_GLOBAL float*)A.trace + ((size_t)wid * A.trace_iters + i) * kTraceFloats;
     tr[0] = (float)i; tr[1] = (float)fn;
     tr[2] = ca.pa.x; tr[3] = ca.pa.y; tr[4] = ca.pv_pre.x; tr[5] = ca.pv_pre.y;
     tr[6] = pv.x; tr[7] = pv.y; tr[8] = bad ? 1.0f : 0.0f; tr[9] = nz;
@@ -137,12 +148,12 @@ __device__ inline void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32
   wk.prev = (int)fn;
   float sx = ppx, sy = ppy, sz = ppz;
   if (Spec::kHasFinal) {
-    const float* fx = A.xforms + (size_t)Spec::kFinalSlot * kXformFloats;
+    const xrec_t fx = X + (size_t)Spec::kFinalSlot * kXformFloats;
     const bool apply = fx[10] == 1.0f ? true : (rng.rand01() < fx[10]);
     if (apply) {
       const float2 fpa = make_float2(fx[0] * pv.x + fx[1] * pv.y + fx[2],
                                      fx[4] * pv.x + fx[5] * pv.y + fx[6]);
-      const float2 fpv = Spec::final_sum(A.xforms, fpa, rng, dc_on, dc_rgb);
+      const float2 fpv = Spec::final_sum(X, fpa, rng, dc_on, dc_rgb);
       const float fz = mixf(nz, fx[8], fx[9]);
       if (!is_bad(fpv)) { sx = fpv.x; sy = fpv.y; sz = fz; }
     }
@@ -167,22 +178,23 @@ __device__ inline void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32
     // floor(p) in [0, W) <=> p in [0, W) for finite p; NaN fails both tests.
     if (px >= 0.0f && px < (float)A.width && py >= 0.0f && py < (float)A.height) {
       const uint32_t xi = (uint32_t)floorf(px), yi = (uint32_t)floorf(py);
+      const P3_CONST float4* PAL = (const P3_CONST float4*)A.palette;
       const float cxf = fminf(fmaxf(sz, 0.0f), 1.0f) * 255.0f;
       float4 pal;
       if (A.palette_mode == 1u) {
         uint32_t i0 = f2u_sat(cxf);
         float frac = cxf - (float)i0;
         if (i0 >= 255u) { i0 = 254u; frac = 1.0f; }
-        const float4 p0 = A.palette[i0], p1 = A.palette[i0 + 1u];
+        const float4 p0 = PAL[i0], p1 = PAL[i0 + 1u];
         pal = make_float4(mixf(p0.x, p1.x, frac), mixf(p0.y, p1.y, frac),
                           mixf(p0.z, p1.z, frac), mixf(p0.w, p1.w, frac));
       } else {
         const uint32_t pi = min(f2u_sat(cxf), 255u);
-        pal = A.palette[pi];
+        pal = PAL[pi];
       }
       if (dc_on) { pal.x = dc_rgb.x; pal.y = dc_rgb.y; pal.z = dc_rgb.z; }
       const float wgt = opacity * 255.0f;
-      uint32_t* b = A.hist + ((size_t)yi * A.width + xi) * 4u;
+      P3_GLOBAL uint32_t* b = (P3_GLOBAL uint32_t*)A.hist + ((size_t)yi * A.width + xi) * 4u;
       hist_add4<kSat>(b, f2u_sat(pal.x * wgt), f2u_sat(pal.y * wgt), f2u_sat(pal.z * wgt),
                       f2u_sat(wgt));
     }
@@ -199,7 +211,7 @@ __device__ inline void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32
 // computes it changes, so the per-walker draw sequence, arithmetic and
 // splats are those of the one-lane-per-walker loop.
 template <bool kSat, int kB, class Spec>
-__device__ inline void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L) {
+P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L) {
   const int t = threadIdx.x;
   const uint32_t w0 = blockIdx.x * kB;
   const uint32_t total = A.iters + A.fuse;
@@ -208,7 +220,7 @@ __device__ inline void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L) {
   {  // seed walker t
     const uint32_t wid = w0 + t;
     Rng rng;
-    rng.m.base = L.isaac + t;
+    rng.m.base = (P3_LDS uint32_t*)(L.isaac + t);
     rng.m.stride = kB;
     rng.draws = 0u;
     rng.s.cnt = rng.s.a = rng.s.b = rng.s.c = 0u;
@@ -216,7 +228,7 @@ __device__ inline void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L) {
     uint32_t iter = total;
     if (wid < A.walkers) {
       if (A.isaac_in) {
-        const uint32_t* st = A.isaac_in + (size_t)wid * 36u;
+        const P3_CONST uint32_t* st = (const P3_CONST uint32_t*)A.isaac_in + (size_t)wid * 36u;
         rng.s.cnt = st[0]; rng.s.a = st[1]; rng.s.b = st[2]; rng.s.c = st[3];
         for (int k = 0; k < 16; ++k) { rng.m.mem(k) = st[4 + k]; rng.m.rsl(k) = st[20 + k]; }
       } else {
@@ -240,14 +252,14 @@ __device__ inline void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L) {
     uint32_t f = idle;
     if (active) {
       Rng rng;
-      rng.m.base = L.isaac + t;
+      rng.m.base = (P3_LDS uint32_t*)(L.isaac + t);
       rng.m.stride = kB;
       rng.s.cnt = L.cnt[t]; rng.s.a = L.ra[t]; rng.s.b = L.rb[t]; rng.s.c = L.rc[t];
       rng.draws = L.draws[t];
       const int prev = L.prev[t];
       const uint32_t row = prev >= 0 ? (uint32_t)prev : A.fallback_row;
       const uint32_t raw = rng.irand();
-      f = A.distrib[(size_t)row * kGrain + (raw & (kGrain - 1))];
+      f = ((const P3_CONST uint32_t*)A.distrib)[(size_t)row * kGrain + (raw & (kGrain - 1))];
       L.cnt[t] = rng.s.cnt; L.ra[t] = rng.s.a; L.rb[t] = rng.s.b; L.rc[t] = rng.s.c;
       L.draws[t] = rng.draws;
       L.raw[t] = raw;
@@ -280,7 +292,7 @@ __device__ inline void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L) {
       const int wl = L.order[t];
       if (L.iter[wl] < total) {
         Rng rng;
-        rng.m.base = L.isaac + wl;
+        rng.m.base = (P3_LDS uint32_t*)(L.isaac + wl);
         rng.m.stride = kB;
         rng.s.cnt = L.cnt[wl]; rng.s.a = L.ra[wl]; rng.s.b = L.rb[wl]; rng.s.c = L.rc[wl];
         rng.draws = L.draws[wl];

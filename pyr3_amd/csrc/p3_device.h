@@ -13,7 +13,17 @@
 
 #include "p3_args.h"
 
+// Address-space-qualified pointers: LDS (3), global (1) and constant (4).
+// Generic pointers would compile to flat instructions; loads through a
+// constant-space pointer at a wave-uniform address become scalar loads.
+#define P3_LDS __attribute__((address_space(3)))
+#define P3_GLOBAL __attribute__((address_space(1)))
+#define P3_CONST __attribute__((address_space(4)))
+#define P3_INLINE __device__ inline __attribute__((always_inline))
+
 namespace p3 {
+
+typedef const P3_CONST float* xrec_t;  // a 116-float xform record
 
 constexpr float kPi = 3.14159265358979323846f;
 constexpr float kTau = 6.28318530717958647692f;
@@ -27,14 +37,14 @@ struct IsaacRegs {
 
 // LDS view of one lane's mem/rsl words.
 struct IsaacLds {
-  uint32_t* base;   // &lds[lane]
-  int stride;       // lanes per block
-  __device__ uint32_t& mem(int k) const { return base[k * stride]; }
-  __device__ uint32_t& rsl(int k) const { return base[(16 + k) * stride]; }
+  P3_LDS uint32_t* base;   // &lds[lane]
+  int stride;              // lanes per block
+  P3_INLINE P3_LDS uint32_t& mem(int k) const { return base[k * stride]; }
+  P3_INLINE P3_LDS uint32_t& rsl(int k) const { return base[(16 + k) * stride]; }
 };
 
 // chaos.wgsl:243-294 / isaac.ts:35-105: one round, 16 outputs.
-__device__ inline void isaac_round(IsaacRegs& s, const IsaacLds& m) {
+P3_INLINE void isaac_round(IsaacRegs& s, const IsaacLds& m) {
   s.c += 1u;
   uint32_t a = s.a;
   uint32_t b = s.b + s.c;
@@ -56,7 +66,7 @@ __device__ inline void isaac_round(IsaacRegs& s, const IsaacLds& m) {
 }
 
 // chaos.wgsl:297-308: pop rsl[--cnt], refilling with a round when empty.
-__device__ inline uint32_t isaac_irand(IsaacRegs& s, const IsaacLds& m) {
+P3_INLINE uint32_t isaac_irand(IsaacRegs& s, const IsaacLds& m) {
   if (s.cnt == 0u) {
     isaac_round(s, m);
     s.cnt = 15u;
@@ -66,7 +76,7 @@ __device__ inline uint32_t isaac_irand(IsaacRegs& s, const IsaacLds& m) {
   return m.rsl((int)s.cnt);
 }
 
-__device__ inline void isaac_mix8(uint32_t v[8]) {
+P3_INLINE void isaac_mix8(uint32_t v[8]) {
   uint32_t a = v[0], b = v[1], c = v[2], d = v[3], e = v[4], f = v[5], g = v[6], h = v[7];
   a ^= b << 11; d += a; b += c;
   b ^= c >> 2;  e += b; c += d;
@@ -81,7 +91,7 @@ __device__ inline void isaac_mix8(uint32_t v[8]) {
 
 // isaac.ts:194-229 (packIsaacStates, one walker) on the device: PCG32
 // XSH-RS words into rsl, the two-pass golden-ratio key schedule, one round.
-__device__ inline void isaac_seed(IsaacRegs& s, const IsaacLds& m, uint32_t seed,
+P3_INLINE void isaac_seed(IsaacRegs& s, const IsaacLds& m, uint32_t seed,
                                   uint32_t walker) {
   const uint32_t MUL = 747796405u, INC = 2891336453u;
   uint32_t pcg = seed ^ (walker * 2654435761u + 1u);
@@ -126,12 +136,12 @@ struct Rng {
   IsaacRegs s;
   IsaacLds m;
   uint32_t draws;
-  __device__ uint32_t irand() { ++draws; return isaac_irand(s, m); }
+  P3_INLINE uint32_t irand() { ++draws; return isaac_irand(s, m); }
   // chaos.wgsl:319-331: low 28 bits scaled by 1/(2^28-1) and 1/(2^27-1).
-  __device__ float rand01() {
+  P3_INLINE float rand01() {
     return (float)(irand() & 0x0fffffffu) * (1.0f / 268435455.0f);
   }
-  __device__ float rand11() {
+  P3_INLINE float rand11() {
     return (float)((int32_t)(irand() & 0x0fffffffu) - 0x07ffffff) * (1.0f / 134217727.0f);
   }
 };
@@ -139,7 +149,7 @@ struct Rng {
 // ----------------------------------------------------------------- math --
 
 // chaos.wgsl:356-362 integer hash for out-of-range trig arguments.
-__device__ inline float hash01(uint32_t h) {
+P3_INLINE float hash01(uint32_t h) {
   h ^= h >> 17; h *= 0xed5ad4bbu;
   h ^= h >> 11; h *= 0xac4c1b51u;
   h ^= h >> 15;
@@ -147,22 +157,22 @@ __device__ inline float hash01(uint32_t h) {
 }
 
 // chaos.wgsl:363-384
-__device__ inline float safe_sin(float a) {
+P3_INLINE float safe_sin(float a) {
   if (fabsf(a) <= 1.0e6f) return sinf(a);
   return sinf(hash01(__float_as_uint(a)) * kTau);
 }
-__device__ inline float safe_cos(float a) {
+P3_INLINE float safe_cos(float a) {
   if (fabsf(a) <= 1.0e6f) return cosf(a);
   return cosf(hash01(__float_as_uint(a)) * kTau);
 }
-__device__ inline float safe_tan(float a) { return safe_sin(a) / safe_cos(a); }
-__device__ inline float safe_tanh(float a) { return tanhf(fminf(fmaxf(a, -20.0f), 20.0f)); }
+P3_INLINE float safe_tan(float a) { return safe_sin(a) / safe_cos(a); }
+P3_INLINE float safe_tanh(float a) { return tanhf(fminf(fmaxf(a, -20.0f), 20.0f)); }
 
 // WGSL mix(a, b, t) = a * (1 - t) + b * t
-__device__ inline float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+P3_INLINE float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
 
 // WGSL u32(f32): saturating; NaN and negatives -> 0.
-__device__ inline uint32_t f2u_sat(float f) {
+P3_INLINE uint32_t f2u_sat(float f) {
   if (!(f > 0.0f)) return 0u;
   if (f >= 4294967296.0f) return 0xffffffffu;
   return (uint32_t)f;

@@ -52,7 +52,7 @@ void shape_of(const float* xf, std::ostringstream& key) {
 // Straight-line chain of one record at X + slot * 116.
 void emit_body(std::ostringstream& s, const float* xf, int slot, bool full) {
   const int nv = slot_count(xf);
-  s << "      const float* xf = X + " << slot * p3::kXformFloats << ";\n";
+  s << "      const xrec_t xf = X + " << slot * p3::kXformFloats << ";\n";
   if (full) {
     s << "      c.pa = make_float2(xf[0] * x + xf[1] * y + xf[2], xf[4] * x + xf[5] * y + xf[6]);\n"
       << "      float2 pam = c.pa;\n";
@@ -83,7 +83,7 @@ std::string generate(const float* xforms, int n_regular, int final_slot) {
   s << "namespace p3 {\nstruct Spec {\n"
     << "  static constexpr bool kHasFinal = " << (final_slot >= 0 ? "true" : "false") << ";\n"
     << "  static constexpr int kFinalSlot = " << (final_slot >= 0 ? final_slot : 0) << ";\n"
-    << "  static __device__ inline Chain chain(uint32_t fn, const float* __restrict__ X, float x,\n"
+    << "  static P3_INLINE Chain chain(uint32_t fn, xrec_t X, float x,\n"
     << "      float y, float z, Rng& rng, bool& dc_on, float3& dc_rgb) {\n"
     << "    (void)dc_on; (void)dc_rgb;\n"
     << "    Chain c;\n"
@@ -97,7 +97,7 @@ std::string generate(const float* xforms, int n_regular, int final_slot) {
     << "    }\n"
     << "    return c;\n"
     << "  }\n"
-    << "  static __device__ inline float2 final_sum(const float* __restrict__ X, float2 pa,\n"
+    << "  static P3_INLINE float2 final_sum(xrec_t X, float2 pa,\n"
     << "      Rng& rng, bool& dc_on, float3& dc_rgb) {\n"
     << "    (void)dc_on; (void)dc_rgb;\n";
   if (final_slot >= 0) {
@@ -110,15 +110,13 @@ std::string generate(const float* xforms, int n_regular, int final_slot) {
   s << "  }\n};\n}  // namespace p3\n\n"
     << "extern \"C\" __global__ void __launch_bounds__(p3::kChaosBlock)\n"
     << "p3_chaos_sat(p3::ChaosArgs A) {\n"
-    << "  extern __shared__ __align__(16) unsigned char lds_raw[];\n"
-    << "  p3::chaos_binned<true, p3::kChaosBlock, p3::Spec>(A,\n"
-    << "      *reinterpret_cast<p3::BlockLds<p3::kChaosBlock>*>(lds_raw));\n"
+    << "  __shared__ p3::BlockLds<p3::kChaosBlock> L;\n"
+    << "  p3::chaos_binned<true, p3::kChaosBlock, p3::Spec>(A, L);\n"
     << "}\n"
     << "extern \"C\" __global__ void __launch_bounds__(p3::kChaosBlock)\n"
     << "p3_chaos_nosat(p3::ChaosArgs A) {\n"
-    << "  extern __shared__ __align__(16) unsigned char lds_raw[];\n"
-    << "  p3::chaos_binned<false, p3::kChaosBlock, p3::Spec>(A,\n"
-    << "      *reinterpret_cast<p3::BlockLds<p3::kChaosBlock>*>(lds_raw));\n"
+    << "  __shared__ p3::BlockLds<p3::kChaosBlock> L;\n"
+    << "  p3::chaos_binned<false, p3::kChaosBlock, p3::Spec>(A, L);\n"
     << "}\n";
   return s.str();
 }
@@ -213,7 +211,7 @@ hipError_t launch_chaos_rtc(const ChaosArgs& a, const float* host_xforms, int n_
   ChaosArgs args = a;
   void* params[] = {&args};
   return hipModuleLaunchKernel(saturating ? m.sat : m.nosat, grid, 1, 1, kChaosBlock, 1, 1,
-                               (unsigned)sizeof(BlockLds<kChaosBlock>), stream, params, nullptr);
+                               0, stream, params, nullptr);
 }
 
 size_t chaos_compiled_shapes() {
