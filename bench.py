@@ -181,6 +181,8 @@ def main():
     ap.add_argument("--ref-quality", type=float, default=4000)
     ap.add_argument("--target-db", type=float, default=38.0)
     ap.add_argument("--fuse", type=int, default=pack.FUSE)
+    ap.add_argument("--walkers", type=int, default=None,
+                    help="skip the walker autotune and use this count")
     ap.add_argument("--quality", type=float, default=None,
                     help="skip the PSNR search and time this quality")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
@@ -197,7 +199,10 @@ def main():
     k, g, cover = pick_genome(r, a.fuse, W, H)
     log(f"genome k={k} ({len(g.xforms)} xforms, cover {cover:.2%})")
     frame = Frame(r, g, a.fuse)
-    frame.walkers, tune = tune_walkers(frame, a.quality or 30)
+    if a.walkers:
+        frame.walkers, tune = a.walkers, {}
+    else:
+        frame.walkers, tune = tune_walkers(frame, a.quality or 30)
 
     scores = {}
     if a.quality is None:
