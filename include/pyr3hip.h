@@ -88,6 +88,11 @@ int p3_read_filtered(p3_renderer* r, float* out, size_t n);      /* DE output gr
 int p3_read_image(p3_renderer* r, uint8_t* out, size_t n);       /* W x H x 4 RGBA8 */
 int p3_read_image_f32(p3_renderer* r, float* out, size_t n);     /* pre-quantisation RGBA */
 
+/* When enabled, p3_present also stores the pre-quantisation RGBA f32 image
+ * (the values the reference writes to its rgba16float target for png16/EXR,
+ * src/visualize.ts:105-156); off by default. */
+int p3_set_image_f32(p3_renderer* r, int enabled);
+
 /* Upload a histogram (tests: DE/tonemap on an identical histogram). */
 int p3_write_histogram(p3_renderer* r, const uint32_t* data, size_t n);
 

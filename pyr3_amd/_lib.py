@@ -33,6 +33,7 @@ SIGNATURES = {
     "p3_write_histogram": (_I, [_P, _UP, _SZ]),
     "p3_set_trace": (_I, [_P, _I, _I]),
     "p3_read_trace": (_I, [_P, _FP, _SZ]),
+    "p3_set_image_f32": (_I, [_P, _I]),
     "p3_set_timing": (_I, [_P, _I]),
     "p3_read_timings": (_I, [_P, _FP]),
     "p3_sync": (_I, [_P]),

@@ -78,6 +78,10 @@ struct p3_renderer {
   unsigned long long hist_bound = 0;  // upper bound on any histogram word
   int trace_walkers = 0, trace_iters = 0;
   bool timing = false;
+  bool want_image_f32 = false;
+  bool have_image_f32 = false;
+  std::vector<float> last_xforms;       // host copies of the last uploads
+  std::vector<uint32_t> last_distrib;
   hipEvent_t ev[kTimers][2] = {};
   bool ev_used[kTimers] = {};
   std::string err;
@@ -158,8 +162,10 @@ int build(p3_renderer* r, int width, int height, int os, const float* taps, int 
     return rc;
   if ((rc = r->hip(hipStreamSynchronize(r->stream), "sync after create"))) return rc;
   r->palette_set = false;
-  r->have_image = r->have_filtered = false;
+  r->have_image = r->have_filtered = r->have_image_f32 = false;
   r->hist_bound = 0;
+  r->last_xforms.clear();
+  r->last_distrib.clear();
   return P3_OK;
 }
 
@@ -205,22 +211,32 @@ int iterate_impl(p3_renderer* r, const float* xforms, int n_slots, const uint32_
                    r->sw, r->sh);
   const uint32_t walkers = u[12];
   if (walkers == 0) return P3_OK;
-  int rc0 = check_records(r, xforms, n_slots);
-  if (rc0) return rc0;
+  const size_t n_xf = (size_t)P3_XFORM_FLOATS * n_slots;
   const size_t table = (size_t)n_rows * P3_GRAIN;
-  for (size_t i = 0; i < table; ++i)
-    if (distrib[i] >= (uint32_t)n_regular)
-      return r->fail(P3_E_ARG, "pick table entry %zu = %u outside %d xforms", i, distrib[i],
-                     n_regular);
   int rc;
-  if ((rc = r->hip(hipMemcpyAsync(r->xforms.p, xforms, sizeof(float) * P3_XFORM_FLOATS * n_slots,
-                                  hipMemcpyHostToDevice, r->stream),
-                   "upload xforms")))
-    return rc;
-  if ((rc = r->hip(hipMemcpyAsync(r->distrib.p, distrib, sizeof(uint32_t) * table,
-                                  hipMemcpyHostToDevice, r->stream),
-                   "upload pick table")))
-    return rc;
+  // Blobs identical to the last upload (successive frames or chunks of one
+  // genome) are neither re-validated nor re-sent.
+  if (r->last_xforms.size() != n_xf ||
+      std::memcmp(r->last_xforms.data(), xforms, n_xf * sizeof(float)) != 0) {
+    if ((rc = check_records(r, xforms, n_slots))) return rc;
+    if ((rc = r->hip(hipMemcpyAsync(r->xforms.p, xforms, n_xf * sizeof(float),
+                                    hipMemcpyHostToDevice, r->stream),
+                     "upload xforms")))
+      return rc;
+    r->last_xforms.assign(xforms, xforms + n_xf);
+  }
+  if (r->last_distrib.size() != table ||
+      std::memcmp(r->last_distrib.data(), distrib, table * sizeof(uint32_t)) != 0) {
+    for (size_t i = 0; i < table; ++i)
+      if (distrib[i] >= (uint32_t)n_regular)
+        return r->fail(P3_E_ARG, "pick table entry %zu = %u outside %d xforms", i, distrib[i],
+                       n_regular);
+    if ((rc = r->hip(hipMemcpyAsync(r->distrib.p, distrib, table * sizeof(uint32_t),
+                                    hipMemcpyHostToDevice, r->stream),
+                     "upload pick table")))
+      return rc;
+    r->last_distrib.assign(distrib, distrib + table);
+  }
   p3::ChaosArgs a{};
   a.isaac_in = nullptr;
   if (states) {
@@ -402,7 +418,7 @@ int p3_present(p3_renderer* r, const uint32_t* de, const uint32_t* viz) {
   t.hist = (const uint4*)r->hist.p;
   t.taps = r->taps.p;
   t.out8 = r->image8.p;
-  t.outf = r->imagef.p;
+  t.outf = r->want_image_f32 ? r->imagef.p : nullptr;
   t.width = (uint32_t)r->width; t.height = (uint32_t)r->height;
   t.os = (uint32_t)r->os; t.fwidth = viz[9];
   t.k1 = bits_f(viz[2]); t.k2 = bits_f(viz[3]); t.gamma = bits_f(viz[4]);
@@ -413,6 +429,7 @@ int p3_present(p3_renderer* r, const uint32_t* de, const uint32_t* viz) {
   if ((rc = r->hip(p3::launch_tonemap(t, de != nullptr, r->stream), "tonemap launch"))) return rc;
   r->toc(T_TM);
   r->have_image = true;
+  r->have_image_f32 = r->want_image_f32;
   return P3_OK;
 }
 
@@ -442,7 +459,8 @@ int p3_read_image_f32(p3_renderer* r, float* out, size_t n) {
   if (!r) return P3_E_ARG;
   if (!out || n != (size_t)r->width * r->height * 4)
     return r->fail(P3_E_ARG, "image needs %zu floats", (size_t)r->width * r->height * 4);
-  if (!r->have_image) return r->fail(P3_E_STATE, "present has not run since the last change");
+  if (!r->have_image || !r->have_image_f32)
+    return r->fail(P3_E_STATE, "no f32 image: enable it with p3_set_image_f32 before present");
   return read_dev(r, out, r->imagef.p, n * 4, "read image f32");
 }
 
@@ -474,6 +492,12 @@ int p3_read_trace(p3_renderer* r, float* out, size_t n) {
   if (!out || n != need || need == 0) return r->fail(P3_E_ARG, "trace needs %zu floats", need);
   if (r->trace.n < need) return r->fail(P3_E_STATE, "no trace recorded");
   return read_dev(r, out, r->trace.p, need * 4, "read trace");
+}
+
+int p3_set_image_f32(p3_renderer* r, int enabled) {
+  if (!r) return P3_E_ARG;
+  r->want_image_f32 = enabled != 0;
+  return P3_OK;
 }
 
 int p3_set_timing(p3_renderer* r, int enabled) {

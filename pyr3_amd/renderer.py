@@ -189,6 +189,10 @@ class Renderer:
         check(self._lib.p3_read_trace(self._h, _fp(out), out.size), self._h)
         return out
 
+    def set_image_f32(self, on: bool) -> None:
+        """Also keep the pre-quantisation RGBA f32 image on each present."""
+        check(self._lib.p3_set_image_f32(self._h, 1 if on else 0), self._h)
+
     def set_timing(self, on: bool) -> None:
         check(self._lib.p3_set_timing(self._h, 1 if on else 0), self._h)
 

@@ -158,6 +158,7 @@ def test_density_and_tonemap_on_identical_histogram(orc, R, os_):
     g.tonemap = Tonemap(gamma=4.0, vibrancy=1.0, highlight_power=1.0)
     samples = 5e6
     r = R(W, H, os_)
+    r.set_image_f32(True)
     r.reset(g)
     r.write_histogram(h)
     r.present(g, samples)
