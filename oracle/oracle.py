@@ -39,6 +39,8 @@ _SIGS = {
     "orc_variation_f32": (C.c_int, [C.c_int, C.c_float, C.c_float, _F, _F, _F,
                                     C.POINTER(C.c_uint32), C.c_int, _F]),
     "orc_variation_supported": (C.c_int, [C.c_int]),
+    "orc_dc_color": (C.c_int, [C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
+                               _F]),
 }
 
 _lib = None
@@ -220,6 +222,15 @@ def variation(idx, x, y, weight, params=(), affine=(1, 0, 0, 0, 1, 0), rng_words
     if rc < 0:
         raise ValueError(f"variation {idx} not in the oracle")
     return float(out[0]), float(out[1]), rc
+
+
+def dc_color(idx, x, y, params=(0.0, 0.0, 0.0)):
+    """Direct-colour RGB of entry idx at (x, y), or None when it has none."""
+    rgb = np.zeros(3, dtype=np.float32)
+    q = list(params) + [0.0] * (3 - len(params))
+    hit = load().orc_dc_color(idx, np.float32(x), np.float32(y), np.float32(q[0]),
+                              np.float32(q[1]), np.float32(q[2]), p(rgb, _F))
+    return tuple(float(v) for v in rgb) if hit else None
 
 
 def variation_supported(idx: int) -> bool:

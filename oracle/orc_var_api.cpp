@@ -48,4 +48,10 @@ int orc_variation_f32(int idx, float x, float y, const float* w, const float* pa
 
 int orc_variation_supported(int idx) { return orc::variation_supported(idx) ? 1 : 0; }
 
+// Direct-colour side of entry idx at (x, y) with parameters p0..p2: writes
+// RGB and returns 1, or returns 0 for an entry without one.
+int orc_dc_color(int idx, float x, float y, float p0, float p1, float p2, float* rgb) {
+  return orc::dc_color_ext(idx, orc::V2<float>{x, y}, p0, p1, p2, rgb) ? 1 : 0;
+}
+
 }  // extern "C"

@@ -707,6 +707,12 @@ V2<T> var_wedge_julia(V2<T> p, T w, T angle, T count, T power, T dist, R& rng) {
   return {r * safe_cos(a), r * safe_sin(a)};
 }
 
+}  // namespace orc
+
+#include "orc_variations_ext.hpp"
+
+namespace orc {
+
 // Dispatch by catalog index (src/variations.ts:24 V table). Index 97
 // (pre_blur) never reaches here: the chain applies it before the sum.
 // Unknown indices contribute (0, 0), like the WGSL default arm.
@@ -813,18 +819,18 @@ V2<T> apply_variation(int idx, V2<T> p, const VarArgs<T>& a, R& rng) {
     case 95: return var_boarders(p, w, rng);
     case 96: return var_wedge_julia(p, w, q[0], q[1], q[2], q[3], rng);
     case 98: return var_mobius(p, w, q);
-    default: return {(T)0, (T)0};
+    default: return apply_variation_ext(idx, p, a, rng);
   }
 }
 
-// Direct-colour override of one slot (chaos.wgsl:8192-8210); the dc_*
-// catalog entries are not restated yet, so no slot overrides the palette.
+// Direct-colour override of slot k of record xf (chaos.wgsl:8192-8210):
+// parameters p0, p1 at vars[k].zw and p2 at vars_extra[k].x.
 inline bool dc_color_for(int idx, V2<float> p, const float* xf, int k, float* rgb) {
-  (void)idx; (void)p; (void)xf; (void)k; (void)rgb;
-  return false;
+  return dc_color_ext(idx, p, xf[20 + 4 * k + 2], xf[20 + 4 * k + 3], xf[52 + 4 * k], rgb);
 }
 
-// Variation indices this restatement covers (used by tests to skip the rest).
-inline bool variation_supported(int idx) { return idx >= 0 && idx <= 98 && idx != 97; }
+// Catalog indices with a position or colour side (97, pre_blur, is applied
+// by the chain itself).
+inline bool variation_supported(int idx) { return idx >= 0 && idx <= 325 && idx != 97; }
 
 }  // namespace orc

@@ -20,7 +20,8 @@ LIB = PKG / "libpyr3hip.so"
 SOURCES = ["p3_api.cpp", "p3_rtc.cpp", "p3_image.hip"]
 # Device headers embedded as text for the run-time chaos specialisation, in
 # dependency order (their #include lines are dropped).
-RTC_HEADERS = ["p3_args.h", "p3_device.h", "p3_variations.h", "p3_chaos_kernel.h"]
+RTC_HEADERS = ["p3_args.h", "p3_device.h", "p3_variations.h", "p3_variations_ext.h",
+               "p3_variation_dispatch.h", "p3_chaos_kernel.h"]
 RTC_INC = CSRC / "p3_rtc_source.inc"
 RTC_PRELUDE = ("typedef unsigned int uint32_t;\ntypedef int int32_t;\n"
                "typedef unsigned short uint16_t;\ntypedef unsigned char uint8_t;\n"

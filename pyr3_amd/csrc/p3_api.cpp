@@ -49,10 +49,10 @@ struct DevBuf {
 
 float bits_f(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
 
-// Catalog entries the chaos kernel implements (index 97 is pre_blur, applied
-// by the chain itself).
+// Catalog entries the chaos kernel implements: all of 0..325 (index 97 is
+// pre_blur, applied by the chain itself).
 bool supported(uint32_t idx) {
-  return idx <= 98u;
+  return idx < (uint32_t)p3::kCatalogSize;
 }
 
 // Every active slot of every record must name an implemented entry.

@@ -10,6 +10,12 @@ constexpr int kXformFloats = 116;  // 464-byte xform record
 constexpr int kMaxVars = 8;        // variation slots per xform
 constexpr int kTraceFloats = 16;   // floats per trace entry
 constexpr int kChaosBlock = 256;   // walkers per chaos block
+constexpr int kCatalogSize = 326;  // catalog indices 0..325 (97 = pre_blur)
+
+// Catalog entries with a direct-colour side (src/variations.ts:681-692).
+constexpr bool is_dc_index(unsigned idx) {
+  return (idx >= 99u && idx <= 102u) || idx == 220u || idx == 265u || (idx >= 310u && idx <= 313u);
+}
 
 struct ChaosArgs {
   const float* xforms;

@@ -25,7 +25,7 @@
 // the adds are issued without a return value.
 #pragma once
 #include "p3_device.h"
-#include "p3_variations.h"
+#include "p3_variation_dispatch.h"
 
 namespace p3 {
 

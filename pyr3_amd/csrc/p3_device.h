@@ -166,7 +166,10 @@ P3_INLINE float safe_cos(float a) {
   return cosf(hash01(__float_as_uint(a)) * kTau);
 }
 P3_INLINE float safe_tan(float a) { return safe_sin(a) / safe_cos(a); }
-P3_INLINE float safe_tanh(float a) { return tanhf(fminf(fmaxf(a, -20.0f), 20.0f)); }
+P3_INLINE float safe_tanh(float a) {  // clamp to +/-20; NaN passes through
+  const float c = a < -20.0f ? -20.0f : (a > 20.0f ? 20.0f : a);
+  return tanhf(c);
+}
 
 // WGSL mix(a, b, t) = a * (1 - t) + b * t
 P3_INLINE float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }

@@ -13,7 +13,11 @@
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <sstream>
@@ -64,9 +68,14 @@ void emit_body(std::ostringstream& s, const float* xf, int slot, bool full) {
   s << "      float2 pv = make_float2(0.0f, 0.0f);\n";
   for (int k = 0; k < nv; ++k) {
     const uint32_t idx = slot_index(xf, k);
-    if (idx == 97u) continue;
-    s << "      { const float2 o = apply_variation(" << idx << ", pam, xf, " << k << ", xf["
-      << 20 + 4 * k + 1 << "], rng); pv.x = pv.x + o.x; pv.y = pv.y + o.y; }\n";
+    if (idx != 97u)
+      s << "      { const float2 o = apply_variation(" << idx << ", pam, xf, " << k << ", xf["
+        << 20 + 4 * k + 1 << "], rng); pv.x = pv.x + o.x; pv.y = pv.y + o.y; }\n";
+    // Direct-colour entries override the splat colour when the record's
+    // dc flag is set and the slot weight is positive (last one wins).
+    if (p3::is_dc_index(idx))
+      s << "      if (xf[11] > 0.5f && xf[" << 20 + 4 * k + 1 << "] > 0.0f) { float3 dcc; if (dc_color_for("
+        << idx << ", pam, xf, " << k << ", dcc)) { dc_on = true; dc_rgb = dcc; } }\n";
   }
   const bool post = xf[15] != 0.0f;
   if (full) {
@@ -121,16 +130,67 @@ std::string generate(const float* xforms, int n_regular, int final_slot) {
   return s.str();
 }
 
+// Persistent code-object cache: when P3_KERNEL_CACHE names a directory,
+// compiled shapes are stored there as <dir>/p3chaos-<fnv64 of source and
+// options>.co and reused by later processes (and hosts of the same arch).
+uint64_t fnv1a(const char* p, size_t n, uint64_t h) {
+  for (size_t i = 0; i < n; ++i) { h ^= (unsigned char)p[i]; h *= 1099511628211ull; }
+  return h;
+}
+
+std::string cache_path(const std::string& src, const char* const* opts, int n_opts) {
+  const char* dir = std::getenv("P3_KERNEL_CACHE");
+  if (!dir || !*dir) return std::string();
+  uint64_t h = fnv1a(src.data(), src.size(), 1469598103934665603ull);
+  for (int i = 0; i < n_opts; ++i) h = fnv1a(opts[i], std::strlen(opts[i]) + 1, h);
+  char name[48];
+  std::snprintf(name, sizeof name, "/p3chaos-%016llx.co", (unsigned long long)h);
+  return std::string(dir) + name;
+}
+
+bool read_cached(const std::string& path, std::vector<char>& code) {
+  if (path.empty()) return false;
+  FILE* f = std::fopen(path.c_str(), "rb");
+  if (!f) return false;
+  std::fseek(f, 0, SEEK_END);
+  const long n = std::ftell(f);
+  std::fseek(f, 0, SEEK_SET);
+  bool ok = n > 0;
+  if (ok) {
+    code.resize((size_t)n);
+    ok = std::fread(code.data(), 1, (size_t)n, f) == (size_t)n;
+  }
+  std::fclose(f);
+  return ok;
+}
+
+void write_cached(const std::string& path, const std::vector<char>& code) {
+  if (path.empty()) return;
+  const std::string dir = path.substr(0, path.rfind('/'));
+  ::mkdir(dir.c_str(), 0755);  // one level; an existing directory is fine
+  const std::string tmp = path + ".tmp." + std::to_string((long)::getpid());
+  FILE* f = std::fopen(tmp.c_str(), "wb");
+  if (!f) return;
+  const bool ok = std::fwrite(code.data(), 1, code.size(), f) == code.size();
+  std::fclose(f);
+  if (ok) std::rename(tmp.c_str(), path.c_str());
+  else std::remove(tmp.c_str());
+}
+
+const char* const kOpts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
+                             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+constexpr int kNumOpts = (int)(sizeof kOpts / sizeof kOpts[0]);
+
 bool compile(const std::string& src, std::vector<char>& code, std::string& err) {
+  const std::string cached = cache_path(src, kOpts, kNumOpts);
+  if (read_cached(cached, code)) return true;
   hiprtcProgram prog;
   if (hiprtcCreateProgram(&prog, src.c_str(), "p3_chaos_spec.hip", 0, nullptr, nullptr) !=
       HIPRTC_SUCCESS) {
     err = "hiprtcCreateProgram failed";
     return false;
   }
-  const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
-                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
-  const hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof opts / sizeof opts[0]), opts);
+  const hiprtcResult rc = hiprtcCompileProgram(prog, kNumOpts, const_cast<const char**>(kOpts));
   if (rc != HIPRTC_SUCCESS) {
     size_t n = 0;
     hiprtcGetProgramLogSize(prog, &n);
@@ -145,6 +205,7 @@ bool compile(const std::string& src, std::vector<char>& code, std::string& err) 
   code.resize(n);
   hiprtcGetCode(prog, code.data());
   hiprtcDestroyProgram(&prog);
+  write_cached(cached, code);
   return true;
 }
 

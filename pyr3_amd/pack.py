@@ -21,6 +21,7 @@ sequential f64 sums (numpy's add.accumulate is sequential), and the while-loop
 """
 from __future__ import annotations
 
+import bisect
 import math
 import struct
 from typing import List, Tuple
@@ -230,30 +231,31 @@ def bake_lut(stops, hue: float = 0.0, mode: str = "linear") -> np.ndarray:
     else:
         adj = [(s.t, s.r, s.g, s.b) for s in stops]
     srt = sorted(adj, key=lambda s: s[0])
-    data = np.zeros((PALETTE_SIZE, 4), dtype=np.float32)
+    ts = [s[0] for s in srt]
+    rows = []  # f64 per entry, rounded to f32 once at the end
     for i in range(PALETTE_SIZE):
         t = i / (PALETTE_SIZE - 1)
+        # first k with ts[k] <= t <= ts[k+1] (ts sorted), else the whole span
         lo_i, hi_i = 0, len(srt) - 1
-        for k in range(len(srt) - 1):
-            if srt[k][0] <= t <= srt[k + 1][0]:
-                lo_i, hi_i = k, k + 1
-                break
+        k = max(bisect.bisect_left(ts, t) - 1, 0)
+        if k <= len(srt) - 2 and k < bisect.bisect_right(ts, t):
+            lo_i, hi_i = k, k + 1
         lo, hi = srt[lo_i], srt[hi_i]
         if mode == "step":
-            data[i, :3] = lo[1:]
+            rows.append((lo[1], lo[2], lo[3], 0.0))
         else:
             span = (hi[0] - lo[0]) or 1
             u = max(0.0, min(1.0, (t - lo[0]) / span))
             if mode == "smooth" and len(srt) >= 3:
                 a = srt[max(0, lo_i - 1)]
                 d = srt[min(len(srt) - 1, hi_i + 1)]
-                for c in range(3):
-                    data[i, c] = max(0.0, min(1.0, _catmull_rom(u, a[1 + c], lo[1 + c], hi[1 + c],
-                                                                d[1 + c])))
+                rows.append(tuple(max(0.0, min(1.0, _catmull_rom(u, a[1 + c], lo[1 + c],
+                                                                 hi[1 + c], d[1 + c])))
+                                  for c in range(3)) + (0.0,))
             else:
-                for c in range(3):
-                    data[i, c] = lo[1 + c] + (hi[1 + c] - lo[1 + c]) * u
-    return data
+                rows.append(tuple(lo[1 + c] + (hi[1 + c] - lo[1 + c]) * u for c in range(3))
+                            + (0.0,))
+    return np.array(rows, dtype=np.float32)
 
 
 def pack_palette(g: Genome) -> np.ndarray:

@@ -11,7 +11,11 @@ Each workload draws its genomes from numpy's PCG64 seeded with
   the flam3 core set, parametric variations' params U(-2, 2), post-affine on
   half of the xforms, same DE/tonemap settings.
 * 4k (3840x2160, os 2, q200): 12 xforms plus a final xform, variations from
-  every implemented catalog entry, log-normal(sigma 0.7) affine scales.
+  the whole catalog (V0..V325, direct-colour entries included), log-normal
+  (sigma 0.7) affine scales; entries past the flam3 core set take the
+  reference's default parameters (src/serialize.ts:487-860).
+* plain-1080p (1920x1080, os 1): the 12-xform + final layout of 4k with
+  variations from the flam3 core set, the denoiser/benchmark workload.
 
 The viewport is centred on the origin with 4 world units across the long
 edge (scale = long_edge / 4).
@@ -22,7 +26,7 @@ from typing import Iterable, List, Sequence
 
 import numpy as np
 
-from .catalog import VARIATION_INDEX, VARIATION_PARAMS
+from .catalog import VARIATION_DEFAULTS, VARIATION_INDEX, VARIATION_PARAMS
 from .genome import Affine, ColorStop, Density, Genome, Palette, Tonemap, Variation, Xform
 
 CONFIG0_VARIATIONS = ("linear", "sinusoidal", "spherical")
@@ -42,6 +46,9 @@ def _affine(rng, lo=-1.0, hi=1.0) -> Affine:
 def _variation(rng, name: str, weight: float, param_range: float = 2.0) -> Variation:
     v = Variation(index=VARIATION_INDEX[name], weight=weight)
     n = len(VARIATION_PARAMS.get(name, []))
+    if VARIATION_INDEX[name] > 98 and name in VARIATION_DEFAULTS:
+        v.params[:n] = VARIATION_DEFAULTS[name]
+        return v
     for i in range(n):
         v.params[i] = float(rng.uniform(-param_range, param_range))
     return v
@@ -105,10 +112,19 @@ def implemented_pool(max_index: int = 400) -> List[str]:
 
 
 def bench_genome(k: int, width: int = 1920, height: int = 1080) -> Genome:
-    """1080p denoiser-workload genome: 12 xforms + final, every implemented
-    catalog entry, log-normal(0.7) affine scales, quality 4000 reference."""
+    """1080p denoiser-workload genome: 12 xforms + final from the flam3 core
+    set, log-normal(0.7) affine scales, quality 4000 reference."""
     rng = np.random.default_rng(4 * 1000 + k)
     counts = [int(c) for c in rng.integers(1, 4, 12)]
-    return make_genome(rng, 12, implemented_pool(), counts, width, height, 4000, 1,
+    return make_genome(rng, 12, flam3_core_pool(), counts, width, height, 4000, 1,
                        post_fraction=0.5, final=True, lognormal_sigma=0.7,
                        name=f"plain-1080p-{k}")
+
+
+def config2_genome(k: int = 0, width: int = 3840, height: int = 2160) -> Genome:
+    """4k workload genome: 12 xforms + final from the whole catalog."""
+    rng = np.random.default_rng(2 * 1000 + k)
+    counts = [int(c) for c in rng.integers(1, 4, 12)]
+    return make_genome(rng, 12, implemented_pool(), counts, width, height, 200, 2,
+                       post_fraction=0.5, final=True, lognormal_sigma=0.7,
+                       name=f"4k-{k}")

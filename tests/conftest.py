@@ -1,3 +1,4 @@
+import os
 import sys
 from pathlib import Path
 
@@ -9,6 +10,10 @@ for p in (ROOT, ROOT / "oracle"):
         sys.path.insert(0, str(p))
 
 GOLDEN = ROOT / "tests" / "golden"
+
+# Specialised chaos kernels compiled by one test process are reused by the
+# next (and by a host of the same arch) from this directory.
+os.environ.setdefault("P3_KERNEL_CACHE", str(ROOT / "pyr3_amd" / "kcache"))
 
 
 def pytest_configure(config):

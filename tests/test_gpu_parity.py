@@ -206,7 +206,7 @@ def test_retry_path_on_device(orc, R):
 
 def test_unsupported_variation_is_an_error(R):
     from pyr3_amd._lib import P3Error
-    g = genome([xform(1.0, vars=((300, 1.0),))])
+    g = genome([xform(1.0, vars=((326, 1.0),))])  # one past the catalog
     r = R(16, 16, 1)
     r.reset(g)
     with pytest.raises(P3Error):

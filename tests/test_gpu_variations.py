@@ -1,12 +1,16 @@
 """Every implemented catalog entry on the GPU vs the oracle's f32 restatement.
 
-One xform carrying the entry (weight 1, the reference test's injected
-parameters, affine scaled x2 so inputs span [-2, 2]^2), one turn per walker,
-4,096 walkers. The trace gives the pre-post-affine output pv_pre of turn 0
-and the walker's draw counter: draws must match exactly (same ISAAC stream,
-same consumption), and pv_pre within 1e-5 relative (scale max(|v|, 1)) on at
-least 99.5% of walkers; the rest may only differ where the f32 evaluation is
-ill-conditioned (tan/sin of a huge argument), which the bound caps.
+Entries are checked in groups of 16: one genome holds one xform per entry
+(weight 1 each, the flam3 fixture parameters for V0..V98 and the reference's
+defaults for V99..V325, affine scaled x2 so inputs span [-2, 2]^2), so a
+group compiles one specialised kernel. 8,192 walkers run one turn each; the
+trace gives, per walker, the picked xform, the affine input, the pre-post
+output pv_pre and the draw counter. Per entry: the pick and the affine input
+are bit-exact, the draw count is exact (same ISAAC stream, same
+consumption), and pv_pre is within 1e-5 relative (scale max(|v|, 1)) on at
+least 99.5% of its walkers. Entries whose value is a discontinuous function
+of f32 rounding (a sin-hash fract, a random integer times a rounded angle)
+are held to the draw count and to a 98% share.
 """
 import json
 
@@ -15,22 +19,30 @@ import pytest
 
 from conftest import GOLDEN
 from helpers import blobs, genome, xform
+from pyr3_amd import catalog as cat
 from pyr3_amd.variations_supported import SUPPORTED
 
 pytestmark = pytest.mark.gpu
 
 CASES = {c["index"]: c for c in json.loads((GOLDEN / "variation_cases.json").read_text()).values()}
 INDICES = sorted(i for i in SUPPORTED if i != 97)
+GROUP = 16
+GROUPS = [INDICES[i:i + GROUP] for i in range(0, len(INDICES), GROUP)]
+HASHY = {cat.VARIATION_INDEX[n] for n in ("circular", "circular2", "waves4", "hypertile2",
+                                          "phoenix_julia")}
 
 
-def _genome(idx):
+def _xform(idx):
     case = CASES.get(idx)
-    params = case["params"] if case else [0.0] * 10
+    if case:
+        params = case["params"]
+    else:
+        params = cat.VARIATION_DEFAULTS.get(cat.VARIATION_NAMES[idx], [])
     aff = (2.0, 0.1, 0.05, -0.1, 2.0, -0.05)
     if case and case["runner"] == "runWithAffine" and idx in (16, 18, 30, 31):
         a, b, c, d, e, f = case["affine"]
         aff = (2.0, b, c, -0.1, 2.0, f)
-    return genome([xform(1.0, aff, vars=((idx, 1.0),), params=[params])], scale=20.0)
+    return xform(1.0, aff, vars=((idx, 1.0),), params=[list(params)])
 
 
 @pytest.fixture(scope="module")
@@ -39,30 +51,42 @@ def R():
     return Renderer
 
 
-@pytest.mark.parametrize("idx", INDICES)
-def test_variation_matches_oracle(orc, R, idx):
-    g = _genome(idx)
+@pytest.mark.parametrize("group", range(len(GROUPS)))
+def test_variation_group_matches_oracle(orc, R, group):
+    idx = GROUPS[group]
+    g = genome([_xform(i) for i in idx], scale=20.0)
     W = H = 32
-    walkers = 4096
+    walkers = 8192
+    seed = 1234 + group
     r = R(W, H, 1)
     r.reset(g)
     r.set_trace(walkers, 1)
-    r.iterate(g, 1234 + idx, walkers, 1, fuse=0)
+    r.iterate(g, seed, walkers, 1, fuse=0)
     tg = r.read_trace()[:, 0]
-    xf, ds, u, pal = blobs(g, W, H, iters=1, fuse=0, seed=1234 + idx, walkers=walkers)
+    xf, ds, u, pal = blobs(g, W, H, iters=1, fuse=0, seed=seed, walkers=walkers)
     _, to = orc.chaos(xf, ds, u, pal, trace_shape=(walkers, 1))
     to = to[:, 0]
-    assert np.array_equal(tg[:, 2:4], to[:, 2:4]), "affine input"
-    assert np.array_equal(tg[:, 10].view(np.uint32), to[:, 10].view(np.uint32)), "draw count"
-    a, b = tg[:, 4:6].astype(np.float64), to[:, 4:6].astype(np.float64)
-    both_nan = np.isnan(a) & np.isnan(b)
-    same_inf = np.isinf(a) & (a == b)
-    fin = np.isfinite(a) & np.isfinite(b)
-    rel = np.where(fin, np.abs(a - b) / np.maximum(np.abs(b), 1.0), 0.0)
-    ok = both_nan | same_inf | (fin & (rel <= 1e-5))
-    frac = ok.all(axis=1).mean()
-    assert frac >= 0.995, (idx, frac, float(np.nanmax(rel)))
     r.destroy()
+    assert np.array_equal(tg[:, 1], to[:, 1]), "pick"
+    assert np.array_equal(tg[:, 2:4], to[:, 2:4]), "affine input"
+    bad = []
+    for j, v in enumerate(idx):
+        m = to[:, 1] == j
+        if m.sum() < 100:
+            bad.append((v, "too few walkers", int(m.sum())))
+            continue
+        if not np.array_equal(tg[m, 10].view(np.uint32), to[m, 10].view(np.uint32)):
+            bad.append((v, "draw count"))
+            continue
+        a, b = tg[m, 4:6].astype(np.float64), to[m, 4:6].astype(np.float64)
+        both_nan = np.isnan(a) & np.isnan(b)
+        same_inf = np.isinf(a) & (a == b)
+        fin = np.isfinite(a) & np.isfinite(b)
+        rel = np.where(fin, np.abs(a - b) / np.maximum(np.abs(b), 1.0), 0.0)
+        ok = (both_nan | same_inf | (fin & (rel <= 1e-5))).all(axis=1).mean()
+        if ok < (0.98 if v in HASHY else 0.995):
+            bad.append((v, cat.VARIATION_NAMES[v], float(ok)))
+    assert not bad, bad
 
 
 def test_pre_blur_on_device(orc, R):
@@ -77,3 +101,31 @@ def test_pre_blur_on_device(orc, R):
     _, to = orc.chaos(xf, ds, u, pal, trace_shape=(walkers, 3))
     assert np.array_equal(tg[..., 10].view(np.uint32), to[..., 10].view(np.uint32))
     assert np.allclose(tg[..., 4:6], to[..., 4:6], rtol=1e-5, atol=1e-6)
+
+
+def test_direct_colour_histogram_bit_exact(orc, R):
+    """Direct-colour entries whose colour is exact f32 arithmetic (dc_linear,
+    dc_gridout, dc_perlin) on linear maps: the splat colours, hence the whole
+    histogram, are bit-exact. Covers the record's dc flag, the slot-weight
+    gate, last-one-wins and the final xform's override."""
+    V = cat.VARIATION_INDEX
+    xs = [xform(1.0, (0.5, 0, 0, 0, 0.5, 0), vars=((0, 1.0), (V["dc_linear"], 0.3)), color=0.1),
+          xform(1.0, (0.5, 0, 0.5, 0, 0.5, 0), vars=((0, 1.0), (V["dc_gridout"], 0.2)),
+                params=[[], [6.0]], color=0.5),
+          xform(1.0, (0.5, 0, 0.25, 0, 0.5, 0.5),
+                vars=((V["dc_perlin"], 0.4), (0, 1.0), (V["dc_gridout"], 0.0)),
+                params=[[3.0, 2.0, 0.25]], color=0.9)]
+    fin = xform(1.0, (1, 0, 0, 0, 1, 0), vars=((0, 1.0), (V["dc_linear"], 0.1)))
+    for final in (None, fin):
+        g = genome(xs, scale=20.0, cx=0.5, cy=0.5, final=final)
+        W = H = 40
+        walkers, iters, fuse, seed = 1024, 60, 5, 77
+        r = R(W, H, 1)
+        r.reset(g)
+        r.iterate(g, seed, walkers, iters, fuse=fuse)
+        hg = r.read_histogram()
+        r.destroy()
+        xf, ds, u, pal = blobs(g, W, H, iters=iters, fuse=fuse, seed=seed, walkers=walkers)
+        assert xf[:, 11].max() == 1.0  # dc flag packed
+        ho, _ = orc.chaos(xf, ds, u, pal, trace_shape=(1, 1))
+        assert np.array_equal(hg, ho), final is not None
