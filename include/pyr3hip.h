@@ -71,6 +71,18 @@ int p3_reset(p3_renderer* r, const float* palette_rgba);
 int p3_iterate(p3_renderer* r, const float* xforms, int n_slots, const uint32_t* distrib,
                int n_rows, const uint32_t* uniforms);
 
+/* Batched launch: n_sub (1..64) dispatches of one genome shape (same
+ * variation slots, post-affine presence and final xform; e.g. the sub-frames
+ * of a motion-blurred frame, src/animate-render.ts:145-160) in one grid. Each
+ * sub-dispatch k has its own records xforms[k], pick table distribs[k] and
+ * uniform block uniforms[k] (seed, walkers, iterations, camera) and numbers
+ * its walkers from 0, so the histogram equals that of n_sub p3_iterate calls.
+ * A sub-dispatch of a different shape is an error (P3_E_ARG). No counterpart
+ * in the reference (it issues one dispatch per sub-frame). */
+int p3_iterate_multi(p3_renderer* r, int n_sub, const float* const* xforms, int n_slots,
+                     const uint32_t* const* distribs, int n_rows,
+                     const uint32_t* const* uniforms);
+
 /* As p3_iterate, but walkers take explicit ISAAC states (walkers x 36 u32,
  * the reference's isaac_states buffer layout) instead of device seeding. */
 int p3_iterate_states(p3_renderer* r, const float* xforms, int n_slots, const uint32_t* distrib,
@@ -104,10 +116,10 @@ int p3_set_trace(p3_renderer* r, int walkers, int iters);
 int p3_read_trace(p3_renderer* r, float* out, size_t n);
 
 /* Kernel timing with HIP events on the handle's stream: when enabled, each
- * launch is bracketed and p3_read_timings returns the last durations (ms) of
- * [seed, chaos, density, tonemap, reset]. */
+ * launch is bracketed and p3_read_timings writes the last durations (ms) of
+ * [seed, chaos, density, tonemap, reset, denoise] (6 floats). */
 int p3_set_timing(p3_renderer* r, int enabled);
-int p3_read_timings(p3_renderer* r, float* out5);
+int p3_read_timings(p3_renderer* r, float* out6);
 
 /* Block until every queued call on the handle has completed. */
 int p3_sync(p3_renderer* r);
@@ -130,6 +142,19 @@ int p3_variation_supported(int index);
 int p3_chaos_prepare(const float* xforms, int n_slots, int n_rows, int final_idx);
 int p3_chaos_source(const float* xforms, int n_slots, int n_rows, int final_idx, char* out,
                     size_t cap, size_t* len);
+
+/* Learned denoiser (no counterpart in the reference): a small U-Net run in
+ * BF16 with MFMA kernels on the handle's stream. Parameters are the float32
+ * weights in the order of pyr3_amd/denoise/model.py (each 3x3 layer's weight
+ * [c_out][c_in][3][3] then bias, then the 1x1 head's weight [3][16] and
+ * bias); p3_denoiser_param_count() gives their number. p3_denoise reads the
+ * renderer's last present (oversample 1, f32 image enabled) and overwrites
+ * its RGBA8 image with the denoised one. */
+typedef struct p3_denoiser p3_denoiser;
+size_t p3_denoiser_param_count(void);
+int p3_denoiser_create(const float* params, size_t n, p3_denoiser** out);
+int p3_denoiser_destroy(p3_denoiser* d);
+int p3_denoise(p3_renderer* r, p3_denoiser* d);
 
 /* Message for the last failing call on this handle (or of p3_create when
  * r is null). Never null. */

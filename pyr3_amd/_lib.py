@@ -24,6 +24,7 @@ SIGNATURES = {
     "p3_destroy": (None, [_P]),
     "p3_reset": (_I, [_P, _FP]),
     "p3_iterate": (_I, [_P, _FP, _I, _UP, _I, _UP]),
+    "p3_iterate_multi": (_I, [_P, _I, C.POINTER(_FP), _I, C.POINTER(_UP), _I, C.POINTER(_UP)]),
     "p3_iterate_states": (_I, [_P, _FP, _I, _UP, _I, _UP, _UP]),
     "p3_present": (_I, [_P, _UP, _UP]),
     "p3_read_histogram": (_I, [_P, _UP, _SZ]),
@@ -36,6 +37,10 @@ SIGNATURES = {
     "p3_set_image_f32": (_I, [_P, _I]),
     "p3_set_timing": (_I, [_P, _I]),
     "p3_read_timings": (_I, [_P, _FP]),
+    "p3_denoiser_param_count": (_SZ, []),
+    "p3_denoiser_create": (_I, [_FP, _SZ, C.POINTER(_P)]),
+    "p3_denoiser_destroy": (_I, [_P]),
+    "p3_denoise": (_I, [_P, _P]),
     "p3_sync": (_I, [_P]),
     "p3_dims": (_I, [_P, C.POINTER(_I)]),
     "p3_chaos_prepare": (_I, [_FP, _I, _I, _I]),

@@ -18,13 +18,14 @@
 #include <vector>
 
 #include "../../include/pyr3hip.h"
+#include "p3_denoise.h"
 #include "p3_launch.h"
 
 
 namespace {
 
-constexpr int kTimers = 5;  // seed, chaos, density, tonemap, reset
-enum { T_SEED = 0, T_CHAOS = 1, T_DE = 2, T_TM = 3, T_RESET = 4 };
+constexpr int kTimers = 6;  // seed, chaos, density, tonemap, reset, denoise
+enum { T_SEED = 0, T_CHAOS = 1, T_DE = 2, T_TM = 3, T_RESET = 4, T_DENOISE = 5 };
 
 std::string g_create_error = "";
 
@@ -72,6 +73,9 @@ struct p3_renderer {
   DevBuf<float> taps, lut, palette, xforms;
   DevBuf<uint32_t> distrib, isaac;
   DevBuf<float> trace;
+  DevBuf<float> mxforms;        // batched dispatch: per-sub-dispatch records
+  DevBuf<uint32_t> mdistrib;    //   and pick tables
+  DevBuf<char> margs;           //   and kernel arguments
   bool palette_set = false;
   bool have_image = false;
   bool have_filtered = false;
@@ -107,7 +111,22 @@ struct p3_renderer {
   }
 };
 
+struct p3_denoiser {
+  DevBuf<uint16_t> w;
+  DevBuf<float> b, head;
+  size_t w_off[p3::kDenoiseLayers] = {}, b_off[p3::kDenoiseLayers] = {};
+  DevBuf<uint16_t> s0a, s0, h1a, s1, h1b, q0, q1, v1;  // activations, grown on demand
+};
+
 namespace {
+
+// f32 -> bf16 bits, round to nearest even.
+uint16_t bf16_bits(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
 
 int build(p3_renderer* r, int width, int height, int os, const float* taps, int n_taps) {
   if (width <= 0 || height <= 0 || os < 1 || os > 16)
@@ -174,6 +193,7 @@ void release(p3_renderer* r) {
   r->tile_rmax.release(); r->image8.release(); r->imagef.release(); r->taps.release();
   r->lut.release(); r->palette.release(); r->xforms.release(); r->distrib.release();
   r->isaac.release(); r->trace.release();
+  r->mxforms.release(); r->mdistrib.release(); r->margs.release();
 }
 
 int check_records(p3_renderer* r, const float* xforms, int n_slots) {
@@ -193,71 +213,44 @@ int check_records(p3_renderer* r, const float* xforms, int n_slots) {
   return P3_OK;
 }
 
-int iterate_impl(p3_renderer* r, const float* xforms, int n_slots, const uint32_t* distrib,
-                 int n_rows, const uint32_t* u, const uint32_t* states) {
-  if (!r) return P3_E_ARG;
-  if (!r->palette_set) return r->fail(P3_E_STATE, "iterate before reset (no palette)");
+// Checks one dispatch's blobs against the renderer (sizes, final slot, uniform
+// grid size, catalog support, pick-table range); `check_tables` = false skips
+// the O(table) scans when the blobs are known to equal validated ones.
+int validate_dispatch(p3_renderer* r, const float* xforms, int n_slots, const uint32_t* distrib,
+                      int n_rows, const uint32_t* u, bool check_records_too, bool check_table,
+                      int& n_regular, int32_t& final_idx) {
   if (!xforms || !distrib || !u) return r->fail(P3_E_ARG, "null argument");
   if (n_rows < 2 || n_rows > P3_MAX_XFORMS + 1)
     return r->fail(P3_E_ARG, "n_rows must be 2..%d (got %d)", P3_MAX_XFORMS + 1, n_rows);
-  const int n_regular = n_rows - 1;
+  n_regular = n_rows - 1;
   if (n_slots < n_regular || n_slots > n_regular + 1)
     return r->fail(P3_E_ARG, "n_slots %d inconsistent with %d regular xforms", n_slots, n_regular);
-  const int32_t final_idx = (int32_t)u[8];
+  final_idx = (int32_t)u[8];
   if (final_idx != -1 && (final_idx < 0 || final_idx >= n_slots))
     return r->fail(P3_E_ARG, "final xform slot %d outside %d slots", final_idx, n_slots);
   if ((int)u[0] != r->sw || (int)u[1] != r->sh)
     return r->fail(P3_E_ARG, "uniform size %ux%u does not match super-res grid %dx%d", u[0], u[1],
                    r->sw, r->sh);
-  const uint32_t walkers = u[12];
-  if (walkers == 0) return P3_OK;
-  const size_t n_xf = (size_t)P3_XFORM_FLOATS * n_slots;
-  const size_t table = (size_t)n_rows * P3_GRAIN;
   int rc;
-  // Blobs identical to the last upload (successive frames or chunks of one
-  // genome) are neither re-validated nor re-sent.
-  if (r->last_xforms.size() != n_xf ||
-      std::memcmp(r->last_xforms.data(), xforms, n_xf * sizeof(float)) != 0) {
-    if ((rc = check_records(r, xforms, n_slots))) return rc;
-    if ((rc = r->hip(hipMemcpyAsync(r->xforms.p, xforms, n_xf * sizeof(float),
-                                    hipMemcpyHostToDevice, r->stream),
-                     "upload xforms")))
-      return rc;
-    r->last_xforms.assign(xforms, xforms + n_xf);
-  }
-  if (r->last_distrib.size() != table ||
-      std::memcmp(r->last_distrib.data(), distrib, table * sizeof(uint32_t)) != 0) {
+  if (check_records_too && (rc = check_records(r, xforms, n_slots))) return rc;
+  if (check_table) {
+    const size_t table = (size_t)n_rows * P3_GRAIN;
     for (size_t i = 0; i < table; ++i)
       if (distrib[i] >= (uint32_t)n_regular)
         return r->fail(P3_E_ARG, "pick table entry %zu = %u outside %d xforms", i, distrib[i],
                        n_regular);
-    if ((rc = r->hip(hipMemcpyAsync(r->distrib.p, distrib, table * sizeof(uint32_t),
-                                    hipMemcpyHostToDevice, r->stream),
-                     "upload pick table")))
-      return rc;
-    r->last_distrib.assign(distrib, distrib + table);
   }
+  return P3_OK;
+}
+
+// Kernel arguments of one dispatch whose records and table are at dxf / dds.
+p3::ChaosArgs dispatch_args(p3_renderer* r, const uint32_t* u, int n_regular, int32_t final_idx,
+                            const float* dxf, const uint32_t* dds) {
   p3::ChaosArgs a{};
   a.isaac_in = nullptr;
-  if (states) {
-    if ((rc = r->hip(r->isaac.ensure((size_t)walkers * 36), "hipMalloc(isaac)"))) return rc;
-    if ((rc = r->hip(hipMemcpyAsync(r->isaac.p, states, sizeof(uint32_t) * 36 * walkers,
-                                    hipMemcpyHostToDevice, r->stream),
-                     "upload isaac states")))
-      return rc;
-    a.isaac_in = r->isaac.p;
-  }
   a.trace = nullptr;
-  if (r->trace_walkers > 0 && r->trace_iters > 0) {
-    const size_t n = (size_t)r->trace_walkers * r->trace_iters * P3_TRACE_FLOATS;
-    if ((rc = r->hip(r->trace.ensure(n), "hipMalloc(trace)"))) return rc;
-    if ((rc = r->hip(p3::launch_fill(r->trace.p, n * 4, 0u, r->stream), "clear trace"))) return rc;
-    a.trace = r->trace.p;
-    a.trace_walkers = (uint32_t)r->trace_walkers;
-    a.trace_iters = (uint32_t)r->trace_iters;
-  }
-  a.xforms = r->xforms.p;
-  a.distrib = r->distrib.p;
+  a.xforms = dxf;
+  a.distrib = dds;
   a.palette = (const float4*)r->palette.p;
   a.hist = r->hist.p;
   a.fallback_row = (uint32_t)n_regular;
@@ -269,13 +262,74 @@ int iterate_impl(p3_renderer* r, const float* xforms, int n_slots, const uint32_
   a.cos_r = std::cos(rot);
   a.sin_r = std::sin(rot);
   a.palette_mode = u[10];
-  a.walkers = walkers;
+  a.walkers = u[12];
   a.jitter = bits_f(u[13]);
   a.xform_blend = bits_f(u[30]);
+  return a;
+}
+
+// Trace buffer (cleared) for the configured walkers x turns, or null.
+int attach_trace(p3_renderer* r, p3::ChaosArgs& a) {
+  if (r->trace_walkers <= 0 || r->trace_iters <= 0) return P3_OK;
+  const size_t n = (size_t)r->trace_walkers * r->trace_iters * P3_TRACE_FLOATS;
+  int rc;
+  if ((rc = r->hip(r->trace.ensure(n), "hipMalloc(trace)"))) return rc;
+  if ((rc = r->hip(p3::launch_fill(r->trace.p, n * 4, 0u, r->stream), "clear trace"))) return rc;
+  a.trace = r->trace.p;
+  a.trace_walkers = (uint32_t)r->trace_walkers;
+  a.trace_iters = (uint32_t)r->trace_iters;
+  return P3_OK;
+}
+
+int iterate_impl(p3_renderer* r, const float* xforms, int n_slots, const uint32_t* distrib,
+                 int n_rows, const uint32_t* u, const uint32_t* states) {
+  if (!r) return P3_E_ARG;
+  if (!r->palette_set) return r->fail(P3_E_STATE, "iterate before reset (no palette)");
+  if (!xforms || !distrib || !u) return r->fail(P3_E_ARG, "null argument");
+  const size_t n_xf = (size_t)P3_XFORM_FLOATS * (n_slots > 0 ? n_slots : 0);
+  const size_t table = (size_t)(n_rows > 0 ? n_rows : 0) * P3_GRAIN;
+  // Blobs identical to the last upload (successive frames or chunks of one
+  // genome) are neither re-validated nor re-sent.
+  const bool new_xf = r->last_xforms.size() != n_xf ||
+                      std::memcmp(r->last_xforms.data(), xforms, n_xf * sizeof(float)) != 0;
+  const bool new_ds = r->last_distrib.size() != table ||
+                      std::memcmp(r->last_distrib.data(), distrib, table * sizeof(uint32_t)) != 0;
+  int n_regular = 0;
+  int32_t final_idx = -1;
+  int rc;
+  if ((rc = validate_dispatch(r, xforms, n_slots, distrib, n_rows, u, new_xf, new_ds, n_regular,
+                              final_idx)))
+    return rc;
+  const uint32_t walkers = u[12];
+  if (walkers == 0) return P3_OK;
+  if (new_xf) {
+    if ((rc = r->hip(hipMemcpyAsync(r->xforms.p, xforms, n_xf * sizeof(float),
+                                    hipMemcpyHostToDevice, r->stream),
+                     "upload xforms")))
+      return rc;
+    r->last_xforms.assign(xforms, xforms + n_xf);
+  }
+  if (new_ds) {
+    if ((rc = r->hip(hipMemcpyAsync(r->distrib.p, distrib, table * sizeof(uint32_t),
+                                    hipMemcpyHostToDevice, r->stream),
+                     "upload pick table")))
+      return rc;
+    r->last_distrib.assign(distrib, distrib + table);
+  }
+  p3::ChaosArgs a = dispatch_args(r, u, n_regular, final_idx, r->xforms.p, r->distrib.p);
+  if (states) {
+    if ((rc = r->hip(r->isaac.ensure((size_t)walkers * 36), "hipMalloc(isaac)"))) return rc;
+    if ((rc = r->hip(hipMemcpyAsync(r->isaac.p, states, sizeof(uint32_t) * 36 * walkers,
+                                    hipMemcpyHostToDevice, r->stream),
+                     "upload isaac states")))
+      return rc;
+    a.isaac_in = r->isaac.p;
+  }
+  if ((rc = attach_trace(r, a))) return rc;
   // Each walker splats at most `iters` times, each adding <= 255 per word.
   r->hist_bound += 255ULL * (unsigned long long)walkers * (unsigned long long)a.iters;
   const bool sat = r->hist_bound >= 0xffffffffULL;
-  // Compile (or fetch) the kernel for this genome shape before the timed launch.
+  // Compile (or fetch) the kernel for this shape before the timed launch.
   const std::vector<char>* code = nullptr;
   std::string rtc_err;
   if (!p3::chaos_code(xforms, n_regular, final_idx, &code, rtc_err))
@@ -285,6 +339,84 @@ int iterate_impl(p3_renderer* r, const float* xforms, int n_slots, const uint32_
                                              rtc_err);
   if (le != hipSuccess)
     return r->fail(P3_E_HIP, "chaos launch: %s %s", hipGetErrorString(le), rtc_err.c_str());
+  r->toc(T_CHAOS);
+  r->have_image = false;
+  return P3_OK;
+}
+
+// n sub-dispatches of one genome shape (motion-blur sub-frames, chunks with
+// their own seeds) in one batched launch; each keeps its own records, pick
+// table, uniforms (seed, walkers, camera) and walker numbering.
+int iterate_multi_impl(p3_renderer* r, int n_sub, const float* const* xforms, int n_slots,
+                       const uint32_t* const* distribs, int n_rows,
+                       const uint32_t* const* uniforms) {
+  if (!r) return P3_E_ARG;
+  if (!r->palette_set) return r->fail(P3_E_STATE, "iterate before reset (no palette)");
+  if (n_sub < 1 || n_sub > p3::kMaxSub)
+    return r->fail(P3_E_ARG, "n_sub must be 1..%d (got %d)", p3::kMaxSub, n_sub);
+  if (!xforms || !distribs || !uniforms) return r->fail(P3_E_ARG, "null argument");
+  int rc, n_regular = 0;
+  int32_t final_idx = -1;
+  std::string key0;
+  for (int k = 0; k < n_sub; ++k) {
+    int nr;
+    int32_t fi;
+    if ((rc = validate_dispatch(r, xforms[k], n_slots, distribs[k], n_rows, uniforms[k], true,
+                                true, nr, fi)))
+      return rc;
+    const std::string key = p3::chaos_shape_key(xforms[k], nr, fi);
+    if (k == 0) {
+      key0 = key;
+      n_regular = nr;
+      final_idx = fi;
+    } else if (key != key0) {
+      return r->fail(P3_E_ARG, "sub-dispatch %d has a different genome shape from sub-dispatch 0",
+                     k);
+    }
+  }
+  const size_t n_xf = (size_t)P3_XFORM_FLOATS * n_slots, table = (size_t)n_rows * P3_GRAIN;
+  if ((rc = r->hip(r->mxforms.ensure(n_xf * n_sub), "hipMalloc(batched xforms)"))) return rc;
+  if ((rc = r->hip(r->mdistrib.ensure(table * n_sub), "hipMalloc(batched pick tables)"))) return rc;
+  if ((rc = r->hip(r->margs.ensure(sizeof(p3::ChaosArgs) * n_sub), "hipMalloc(batched args)")))
+    return rc;
+  std::vector<p3::ChaosArgs> args((size_t)n_sub);
+  p3::MultiArgs M{};
+  M.args = (const p3::ChaosArgs*)r->margs.p;
+  M.n = (uint32_t)n_sub;
+  unsigned long long bound = 0;
+  for (int k = 0; k < n_sub; ++k) {
+    float* dxf = r->mxforms.p + n_xf * k;
+    uint32_t* dds = r->mdistrib.p + table * k;
+    if ((rc = r->hip(hipMemcpyAsync(dxf, xforms[k], n_xf * sizeof(float), hipMemcpyHostToDevice,
+                                    r->stream),
+                     "upload batched xforms")))
+      return rc;
+    if ((rc = r->hip(hipMemcpyAsync(dds, distribs[k], table * sizeof(uint32_t),
+                                    hipMemcpyHostToDevice, r->stream),
+                     "upload batched pick table")))
+      return rc;
+    args[k] = dispatch_args(r, uniforms[k], n_regular, final_idx, dxf, dds);
+    if (k == 0 && (rc = attach_trace(r, args[0]))) return rc;
+    M.start[k + 1] = M.start[k] + (args[k].walkers + p3::kChaosBlock - 1) / p3::kChaosBlock;
+    bound += 255ULL * (unsigned long long)args[k].walkers * (unsigned long long)args[k].iters;
+  }
+  if (M.start[n_sub] == 0) return P3_OK;
+  if ((rc = r->hip(hipMemcpyAsync(r->margs.p, args.data(), sizeof(p3::ChaosArgs) * n_sub,
+                                  hipMemcpyHostToDevice, r->stream),
+                   "upload batched args")))
+    return rc;
+  r->hist_bound += bound;
+  const bool sat = r->hist_bound >= 0xffffffffULL;
+  const std::vector<char>* code = nullptr;
+  std::string rtc_err;
+  if (!p3::chaos_code(xforms[0], n_regular, final_idx, &code, rtc_err))
+    return r->fail(P3_E_HIP, "chaos kernel specialisation: %s", rtc_err.c_str());
+  r->tic(T_CHAOS);
+  const hipError_t le = p3::launch_chaos_rtc_multi(M, xforms[0], n_regular, final_idx, sat,
+                                                   r->stream, rtc_err);
+  if (le != hipSuccess)
+    return r->fail(P3_E_HIP, "batched chaos launch: %s %s", hipGetErrorString(le),
+                   rtc_err.c_str());
   r->toc(T_CHAOS);
   r->have_image = false;
   return P3_OK;
@@ -371,6 +503,12 @@ int p3_reset(p3_renderer* r, const float* palette_rgba) {
 int p3_iterate(p3_renderer* r, const float* xforms, int n_slots, const uint32_t* distrib,
                int n_rows, const uint32_t* uniforms) {
   return iterate_impl(r, xforms, n_slots, distrib, n_rows, uniforms, nullptr);
+}
+
+int p3_iterate_multi(p3_renderer* r, int n_sub, const float* const* xforms, int n_slots,
+                     const uint32_t* const* distribs, int n_rows,
+                     const uint32_t* const* uniforms) {
+  return iterate_multi_impl(r, n_sub, xforms, n_slots, distribs, n_rows, uniforms);
 }
 
 int p3_iterate_states(p3_renderer* r, const float* xforms, int n_slots, const uint32_t* distrib,
@@ -507,15 +645,15 @@ int p3_set_timing(p3_renderer* r, int enabled) {
   return P3_OK;
 }
 
-int p3_read_timings(p3_renderer* r, float* out5) {
-  if (!r || !out5) return P3_E_ARG;
+int p3_read_timings(p3_renderer* r, float* out6) {
+  if (!r || !out6) return P3_E_ARG;
   int rc = r->hip(hipStreamSynchronize(r->stream), "sync for timings");
   if (rc) return rc;
   for (int t = 0; t < kTimers; ++t) {
-    out5[t] = 0.0f;
+    out6[t] = 0.0f;
     if (r->ev_used[t]) {
       float ms = 0.0f;
-      if (hipEventElapsedTime(&ms, r->ev[t][0], r->ev[t][1]) == hipSuccess) out5[t] = ms;
+      if (hipEventElapsedTime(&ms, r->ev[t][0], r->ev[t][1]) == hipSuccess) out6[t] = ms;
     }
   }
   return P3_OK;
@@ -566,6 +704,109 @@ int p3_chaos_source(const float* xforms, int n_slots, int n_rows, int final_idx,
 }
 
 int p3_variation_supported(int index) { return index >= 0 && supported((uint32_t)index) ? 1 : 0; }
+
+// ---------------------------------------------------------------- denoiser
+
+size_t p3_denoiser_param_count(void) {
+  size_t n = p3::kDenoiseHead;
+  for (int l = 0; l < p3::kDenoiseLayers; ++l)
+    n += (size_t)p3::kDenoiseShape[l][0] * p3::kDenoiseShape[l][1] * 9 + p3::kDenoiseShape[l][1];
+  return n;
+}
+
+int p3_denoiser_create(const float* params, size_t n, p3_denoiser** out) {
+  if (!out) return P3_E_ARG;
+  *out = nullptr;
+  if (!params || n != p3_denoiser_param_count()) {
+    g_create_error = "denoiser: parameter count does not match the network";
+    return P3_E_ARG;
+  }
+  p3_denoiser* d = new p3_denoiser();
+  // Pack each 3x3 layer as bf16 [tap][k-chunk][out][16 k] (input channels
+  // zero-padded to a multiple of 16), biases as f32.
+  std::vector<uint16_t> w;
+  std::vector<float> b;
+  const float* q = params;
+  for (int l = 0; l < p3::kDenoiseLayers; ++l) {
+    const int ci = p3::kDenoiseShape[l][0], co = p3::kDenoiseShape[l][1];
+    const int chunks = (ci + 15) / 16;
+    d->w_off[l] = w.size();
+    d->b_off[l] = b.size();
+    w.resize(w.size() + (size_t)9 * chunks * co * 16, 0);
+    uint16_t* dst = w.data() + d->w_off[l];
+    for (int t = 0; t < 9; ++t)
+      for (int kc = 0; kc < chunks; ++kc)
+        for (int o = 0; o < co; ++o)
+          for (int k = 0; k < 16; ++k) {
+            const int c = kc * 16 + k;
+            const float v = c < ci ? q[((size_t)o * ci + c) * 9 + t] : 0.0f;
+            dst[(((size_t)t * chunks + kc) * co + o) * 16 + k] = bf16_bits(v);
+          }
+    q += (size_t)co * ci * 9;
+    b.insert(b.end(), q, q + co);
+    q += co;
+  }
+  std::vector<float> head(q, q + p3::kDenoiseHead);
+  hipError_t e;
+  if ((e = d->w.ensure(w.size())) != hipSuccess || (e = d->b.ensure(b.size())) != hipSuccess ||
+      (e = d->head.ensure(head.size())) != hipSuccess ||
+      (e = hipMemcpy(d->w.p, w.data(), w.size() * 2, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(d->b.p, b.data(), b.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(d->head.p, head.data(), head.size() * 4, hipMemcpyHostToDevice)) !=
+          hipSuccess) {
+    g_create_error = std::string("denoiser upload: ") + hipGetErrorString(e);
+    p3_denoiser_destroy(d);
+    return P3_E_HIP;
+  }
+  *out = d;
+  return P3_OK;
+}
+
+int p3_denoiser_destroy(p3_denoiser* d) {
+  if (!d) return P3_OK;
+  d->w.release(); d->b.release(); d->head.release();
+  for (auto* a : {&d->s0a, &d->s0, &d->h1a, &d->s1, &d->h1b, &d->q0, &d->q1, &d->v1}) a->release();
+  delete d;
+  return P3_OK;
+}
+
+int p3_denoise(p3_renderer* r, p3_denoiser* d) {
+  if (!r) return P3_E_ARG;
+  if (!d) return r->fail(P3_E_ARG, "null denoiser");
+  if (r->os != 1) return r->fail(P3_E_STATE, "the denoiser runs at oversample 1");
+  if (!r->have_image || !r->have_image_f32)
+    return r->fail(P3_E_STATE, "denoise needs a present with the f32 image enabled");
+  const int W = r->width, H = r->height;
+  const size_t W2 = (W + 1) / 2, H2 = (H + 1) / 2, W4 = (W2 + 1) / 2, H4 = (H2 + 1) / 2;
+  const size_t full = (size_t)W * H, half = W2 * H2, qtr = W4 * H4;
+  int rc;
+  if ((rc = r->hip(d->s0a.ensure(full * 16), "hipMalloc(denoise)")) ||
+      (rc = r->hip(d->s0.ensure(full * 16), "hipMalloc(denoise)")) ||
+      (rc = r->hip(d->v1.ensure(full * 16), "hipMalloc(denoise)")) ||
+      (rc = r->hip(d->h1a.ensure(half * 32), "hipMalloc(denoise)")) ||
+      (rc = r->hip(d->s1.ensure(half * 32), "hipMalloc(denoise)")) ||
+      (rc = r->hip(d->h1b.ensure(half * 32), "hipMalloc(denoise)")) ||
+      (rc = r->hip(d->q0.ensure(qtr * 64), "hipMalloc(denoise)")) ||
+      (rc = r->hip(d->q1.ensure(qtr * 64), "hipMalloc(denoise)")))
+    return rc;
+  p3::DenoiseArgs a{};
+  for (int l = 0; l < p3::kDenoiseLayers; ++l) {
+    a.w[l] = d->w.p + d->w_off[l];
+    a.b[l] = d->b.p + d->b_off[l];
+  }
+  a.head = d->head.p;
+  a.img = r->imagef.p;
+  a.hist = r->hist.p;
+  a.out8 = r->image8.p;
+  a.s0a = d->s0a.p; a.s0 = d->s0.p; a.h1a = d->h1a.p; a.s1 = d->s1.p;
+  a.h1b = d->h1b.p; a.q0 = d->q0.p; a.q1 = d->q1.p; a.v1 = d->v1.p;
+  a.width = W;
+  a.height = H;
+  r->tic(T_DENOISE);
+  if ((rc = r->hip(p3::launch_denoise(a, r->stream), "denoise"))) return rc;
+  r->toc(T_DENOISE);
+  return P3_OK;
+}
 
 const char* p3_last_error(p3_renderer* r) {
   if (!r) return g_create_error.c_str();

@@ -32,6 +32,15 @@ struct ChaosArgs {
   uint32_t trace_walkers, trace_iters;
 };
 
+// Batched launch: up to kMaxSub sub-dispatches of one genome shape in one
+// grid; sub-dispatch k owns blocks [start[k], start[k+1]).
+constexpr int kMaxSub = 64;
+struct MultiArgs {
+  const ChaosArgs* args;  // n entries in device memory
+  uint32_t n;
+  uint32_t start[kMaxSub + 1];
+};
+
 // LDS image of a chaos block's walkers (structure of arrays, kB walkers):
 // ISAAC words [32][kB] (mem 0..15, rsl 16..31) and the per-walker scalars.
 template <int kB>

@@ -201,7 +201,14 @@ P3_INLINE void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32_t fn,
   }
 }
 
-// Persistent block of kB walkers. Every turn:
+// Sub-dispatch owning block b of a batched grid.
+P3_INLINE uint32_t multi_find(const MultiArgs& M, uint32_t b) {
+  uint32_t k = 0;
+  while (k + 1 < M.n && b >= M.start[k + 1]) ++k;
+  return k;
+}
+
+// Persistent block of kB walkers (walkers block*kB .. +kB-1 of A). Every turn:
 //   A  lane t draws walker t's pick and counts it into an LDS bucket per xform
 //   B  one wave turns the bucket counts into offsets
 //   C  walkers are listed in xform order
@@ -211,9 +218,9 @@ P3_INLINE void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32_t fn,
 // computes it changes, so the per-walker draw sequence, arithmetic and
 // splats are those of the one-lane-per-walker loop.
 template <bool kSat, int kB, class Spec>
-P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L) {
+P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L, uint32_t block) {
   const int t = threadIdx.x;
-  const uint32_t w0 = blockIdx.x * kB;
+  const uint32_t w0 = block * kB;
   const uint32_t total = A.iters + A.fuse;
   const uint32_t idle = A.fallback_row + 1u;   // bucket after the last xform
 

@@ -40,6 +40,10 @@ struct TmArgs {
 hipError_t launch_chaos_rtc(const ChaosArgs& a, const float* host_xforms, int n_regular,
                             int final_slot, bool saturating, hipStream_t stream,
                             std::string& err);
+// Batched launch of sub-dispatches sharing the shape of `host_xforms`.
+hipError_t launch_chaos_rtc_multi(const MultiArgs& margs, const float* host_xforms, int n_regular,
+                                  int final_slot, bool saturating, hipStream_t stream,
+                                  std::string& err);
 bool chaos_code(const float* host_xforms, int n_regular, int final_slot,
                 const std::vector<char>** out, std::string& err);
 std::string chaos_source(const float* host_xforms, int n_regular, int final_slot);

@@ -33,7 +33,7 @@ namespace {
 
 struct Module {
   hipModule_t mod = nullptr;
-  hipFunction_t sat = nullptr, nosat = nullptr;
+  hipFunction_t sat = nullptr, nosat = nullptr, msat = nullptr, mnosat = nullptr;
 };
 
 std::mutex g_mu;
@@ -120,12 +120,29 @@ std::string generate(const float* xforms, int n_regular, int final_slot) {
     << "extern \"C\" __global__ void __launch_bounds__(p3::kChaosBlock)\n"
     << "p3_chaos_sat(p3::ChaosArgs A) {\n"
     << "  __shared__ p3::BlockLds<p3::kChaosBlock> L;\n"
-    << "  p3::chaos_binned<true, p3::kChaosBlock, p3::Spec>(A, L);\n"
+    << "  p3::chaos_binned<true, p3::kChaosBlock, p3::Spec>(A, L, blockIdx.x);\n"
     << "}\n"
     << "extern \"C\" __global__ void __launch_bounds__(p3::kChaosBlock)\n"
     << "p3_chaos_nosat(p3::ChaosArgs A) {\n"
     << "  __shared__ p3::BlockLds<p3::kChaosBlock> L;\n"
-    << "  p3::chaos_binned<false, p3::kChaosBlock, p3::Spec>(A, L);\n"
+    << "  p3::chaos_binned<false, p3::kChaosBlock, p3::Spec>(A, L, blockIdx.x);\n"
+    << "}\n"
+    // Batched launch: sub-dispatch k (same genome shape, its own records,
+    // pick table, seed, camera and walker count) owns blocks
+    // [start[k], start[k+1]) of one grid.
+    << "extern \"C\" __global__ void __launch_bounds__(p3::kChaosBlock)\n"
+    << "p3_chaos_multi_sat(p3::MultiArgs M) {\n"
+    << "  __shared__ p3::BlockLds<p3::kChaosBlock> L;\n"
+    << "  const uint32_t k = p3::multi_find(M, blockIdx.x);\n"
+    << "  const p3::ChaosArgs A = ((const P3_CONST p3::ChaosArgs*)M.args)[k];\n"
+    << "  p3::chaos_binned<true, p3::kChaosBlock, p3::Spec>(A, L, blockIdx.x - M.start[k]);\n"
+    << "}\n"
+    << "extern \"C\" __global__ void __launch_bounds__(p3::kChaosBlock)\n"
+    << "p3_chaos_multi_nosat(p3::MultiArgs M) {\n"
+    << "  __shared__ p3::BlockLds<p3::kChaosBlock> L;\n"
+    << "  const uint32_t k = p3::multi_find(M, blockIdx.x);\n"
+    << "  const p3::ChaosArgs A = ((const P3_CONST p3::ChaosArgs*)M.args)[k];\n"
+    << "  p3::chaos_binned<false, p3::kChaosBlock, p3::Spec>(A, L, blockIdx.x - M.start[k]);\n"
     << "}\n";
   return s.str();
 }
@@ -243,6 +260,27 @@ bool chaos_code(const float* host_xforms, int n_regular, int final_slot,
   return true;
 }
 
+// Loaded module of this shape on device `dev` (loaded once per device).
+hipError_t module_for(int dev, const float* host_xforms, int n_regular, int final_slot,
+                      const std::vector<char>& code, Module& m) {
+  const std::string key =
+      std::to_string(dev) + "|" + chaos_shape_key(host_xforms, n_regular, final_slot);
+  std::lock_guard<std::mutex> lock(g_mu);
+  auto it = g_mods.find(key);
+  if (it != g_mods.end()) {
+    m = it->second;
+    return hipSuccess;
+  }
+  hipError_t e;
+  if ((e = hipModuleLoadData(&m.mod, code.data())) != hipSuccess) return e;
+  if ((e = hipModuleGetFunction(&m.sat, m.mod, "p3_chaos_sat")) != hipSuccess) return e;
+  if ((e = hipModuleGetFunction(&m.nosat, m.mod, "p3_chaos_nosat")) != hipSuccess) return e;
+  if ((e = hipModuleGetFunction(&m.msat, m.mod, "p3_chaos_multi_sat")) != hipSuccess) return e;
+  if ((e = hipModuleGetFunction(&m.mnosat, m.mod, "p3_chaos_multi_nosat")) != hipSuccess) return e;
+  g_mods.emplace(key, m);
+  return hipSuccess;
+}
+
 // Compiles (or fetches) the kernel for this shape and launches it on `stream`.
 hipError_t launch_chaos_rtc(const ChaosArgs& a, const float* host_xforms, int n_regular,
                             int final_slot, bool saturating, hipStream_t stream,
@@ -252,27 +290,34 @@ hipError_t launch_chaos_rtc(const ChaosArgs& a, const float* host_xforms, int n_
   if (e != hipSuccess) return e;
   const std::vector<char>* code = nullptr;
   if (!chaos_code(host_xforms, n_regular, final_slot, &code, err)) return hipErrorInvalidImage;
-  const std::string key =
-      std::to_string(dev) + "|" + chaos_shape_key(host_xforms, n_regular, final_slot);
   Module m;
-  {
-    std::lock_guard<std::mutex> lock(g_mu);
-    auto it = g_mods.find(key);
-    if (it != g_mods.end()) {
-      m = it->second;
-    } else {
-      if ((e = hipModuleLoadData(&m.mod, code->data())) != hipSuccess) return e;
-      if ((e = hipModuleGetFunction(&m.sat, m.mod, "p3_chaos_sat")) != hipSuccess) return e;
-      if ((e = hipModuleGetFunction(&m.nosat, m.mod, "p3_chaos_nosat")) != hipSuccess) return e;
-      g_mods.emplace(key, m);
-    }
-  }
+  if ((e = module_for(dev, host_xforms, n_regular, final_slot, *code, m)) != hipSuccess) return e;
   const unsigned grid = (a.walkers + kChaosBlock - 1) / kChaosBlock;
   if (grid == 0) return hipSuccess;
   ChaosArgs args = a;
   void* params[] = {&args};
   return hipModuleLaunchKernel(saturating ? m.sat : m.nosat, grid, 1, 1, kChaosBlock, 1, 1,
                                0, stream, params, nullptr);
+}
+
+// Batched launch of n sub-dispatches sharing one genome shape: `dev_args`
+// holds their ChaosArgs in device memory, start[] their first blocks.
+hipError_t launch_chaos_rtc_multi(const MultiArgs& margs, const float* host_xforms, int n_regular,
+                                  int final_slot, bool saturating, hipStream_t stream,
+                                  std::string& err) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  const std::vector<char>* code = nullptr;
+  if (!chaos_code(host_xforms, n_regular, final_slot, &code, err)) return hipErrorInvalidImage;
+  Module m;
+  if ((e = module_for(dev, host_xforms, n_regular, final_slot, *code, m)) != hipSuccess) return e;
+  const unsigned grid = margs.start[margs.n];
+  if (grid == 0) return hipSuccess;
+  MultiArgs args = margs;
+  void* params[] = {&args};
+  return hipModuleLaunchKernel(saturating ? m.msat : m.mnosat, grid, 1, 1, kChaosBlock, 1, 1, 0,
+                               stream, params, nullptr);
 }
 
 size_t chaos_compiled_shapes() {

@@ -1,0 +1,28 @@
+"""Flat float32 parameter vector of the denoiser in the C ABI's order."""
+from __future__ import annotations
+
+import numpy as np
+
+from .model import LAYERS
+
+
+def flat_params(arrays) -> np.ndarray:
+    """arrays: mapping name -> array (an .npz from train.py, or a state dict
+    converted with .numpy()). Order: each 3x3 layer's weight then bias, then
+    the head's weight and bias."""
+    parts = []
+    for n, ci, co, _ in LAYERS:
+        w = np.asarray(arrays[f"{n}.weight"], dtype=np.float32)
+        assert w.shape == (co, ci, 3, 3), (n, w.shape)
+        parts += [w.reshape(-1), np.asarray(arrays[f"{n}.bias"], dtype=np.float32).reshape(-1)]
+    parts += [np.asarray(arrays["out.weight"], dtype=np.float32).reshape(-1),
+              np.asarray(arrays["out.bias"], dtype=np.float32).reshape(-1)]
+    return np.concatenate(parts)
+
+
+def from_module(net) -> np.ndarray:
+    arrays = {f"{n}.{p}": getattr(m, p).detach().float().cpu().numpy()
+              for n, m in net.convs.items() for p in ("weight", "bias")}
+    arrays["out.weight"] = net.out.weight.detach().float().cpu().numpy()
+    arrays["out.bias"] = net.out.bias.detach().float().cpu().numpy()
+    return flat_params(arrays)

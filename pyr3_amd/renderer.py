@@ -125,6 +125,28 @@ class Renderer:
         check(rc, self._h)
         return walkers * iters_per_walker
 
+    def iterate_multi(self, genomes, seeds, walkers, iters_per_walker: int,
+                      walker_jitter: Optional[float] = None, fuse: int = pack.FUSE,
+                      packed=None) -> int:
+        """Several dispatches of one genome shape (e.g. the sub-frames of a
+        motion-blurred frame) in one batched launch; the histogram equals
+        that of one `iterate` per (genome, seed, walkers). Returns samples."""
+        n = len(genomes)
+        if not (len(seeds) == len(walkers) == n):
+            raise ValueError("genomes, seeds and walkers must have the same length")
+        pgs = packed or [pack_genome(g) for g in genomes]
+        us = [np.ascontiguousarray(self.chaos_uniforms(pg.expanded, sd, w, iters_per_walker, fuse,
+                                                       walker_jitter))
+              for pg, sd, w in zip(pgs, seeds, walkers)]
+        n_slots, n_rows = pgs[0].xforms.shape[0], pgs[0].distrib.shape[0]
+        if any(pg.xforms.shape[0] != n_slots or pg.distrib.shape[0] != n_rows for pg in pgs):
+            raise ValueError("sub-dispatches differ in xform count")
+        xfp = (C.POINTER(C.c_float) * n)(*[_fp(pg.xforms) for pg in pgs])
+        dsp = (C.POINTER(C.c_uint32) * n)(*[_up(pg.distrib) for pg in pgs])
+        ufp = (C.POINTER(C.c_uint32) * n)(*[_up(u) for u in us])
+        check(self._lib.p3_iterate_multi(self._h, n, xfp, n_slots, dsp, n_rows, ufp), self._h)
+        return int(sum(walkers)) * iters_per_walker
+
     def present(self, genome: Genome, total_samples: float, force_de_off: bool = False,
                 transparent: bool = False) -> None:
         tm = genome.tonemap or Tonemap()
@@ -197,14 +219,48 @@ class Renderer:
         check(self._lib.p3_set_timing(self._h, 1 if on else 0), self._h)
 
     def read_timings(self):
-        """Last kernel durations (ms): seed, chaos, density, tonemap, reset."""
-        out = (C.c_float * 5)()
+        """Last kernel durations (ms): seed, chaos, density, tonemap, reset, denoise."""
+        out = (C.c_float * 6)()
         check(self._lib.p3_read_timings(self._h, out), self._h)
-        keys = ("seed", "chaos", "density", "tonemap", "reset")
+        keys = ("seed", "chaos", "density", "tonemap", "reset", "denoise")
         return dict(zip(keys, list(out)))
 
     def sync(self) -> None:
         check(self._lib.p3_sync(self._h), self._h)
+
+
+class Denoiser:
+    """The learned denoiser on the GPU (BF16 MFMA kernels)."""
+
+    def __init__(self, params: np.ndarray):
+        self._lib = load()
+        p = np.ascontiguousarray(params, dtype=np.float32).reshape(-1)
+        self._h = C.c_void_p()
+        check(self._lib.p3_denoiser_create(_fp(p), p.size, C.byref(self._h)), None)
+
+    @staticmethod
+    def param_count() -> int:
+        return int(load().p3_denoiser_param_count())
+
+    @classmethod
+    def from_npz(cls, path) -> "Denoiser":
+        from .denoise.weights import flat_params
+        return cls(flat_params(np.load(path, allow_pickle=False)))
+
+    def apply(self, r: Renderer) -> None:
+        """Denoise the renderer's last present (needs set_image_f32(True))."""
+        check(self._lib.p3_denoise(r._h, self._h), r._h)
+
+    def destroy(self) -> None:
+        if self._h:
+            self._lib.p3_denoiser_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 def create_renderer(width: int, height: int, oversample: Optional[int] = None,

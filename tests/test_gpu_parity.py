@@ -211,3 +211,79 @@ def test_unsupported_variation_is_an_error(R):
     r.reset(g)
     with pytest.raises(P3Error):
         r.iterate(g, 1, 8, 10)
+
+
+def _shifted(g, k):
+    import copy
+    h = copy.deepcopy(g)
+    for x in h.xforms:
+        x.affine.c += 0.01 * k
+        x.affine.f -= 0.005 * k
+    return h
+
+
+def test_batched_iterate_equals_sequential(orc, R):
+    """One batched launch of sub-dispatches (own records, seeds, walker counts,
+    one of them empty) leaves the histogram of one iterate per sub-dispatch."""
+    W, H = 64, 48
+    seeds = [(11 + k * 0x9e3779b1) & 0xFFFFFFFF for k in range(4)]
+    walkers = [700, 0, 1300, 256]
+    for base in (_nonlinear_genome(), genome(sierpinski(), scale=20.0, cx=0.5, cy=0.5)):
+        subs = [_shifted(base, k) for k in range(4)]
+        rb = R(W, H, 1)
+        rb.reset(base)
+        rb.iterate_multi(subs, seeds, walkers, 150, fuse=10)
+        hb = rb.read_histogram()
+        rs = R(W, H, 1)
+        rs.reset(base)
+        for g, sd, w in zip(subs, seeds, walkers):
+            rs.iterate(g, sd, w, 150, fuse=10)
+        hs = rs.read_histogram()
+        assert np.array_equal(hb, hs) and hb[..., 3].sum() > 0
+    # exact-arithmetic shape: the oracle's per-sub-dispatch sum is bit-identical
+    ho = np.zeros_like(hb)
+    for g, sd, w in zip(subs, seeds, walkers):
+        if w:
+            xf, ds, u, pal = blobs(g, W, H, iters=150, fuse=10, seed=sd, walkers=w)
+            orc.chaos(xf, ds, u, pal, hist=ho)
+    assert np.array_equal(hb, ho)
+
+
+def test_batched_iterate_rejects_mixed_shapes(R):
+    from pyr3_amd._lib import P3Error
+    a = _nonlinear_genome()
+    b = genome(sierpinski()[:2], scale=40.0)
+    r = R(32, 32, 1)
+    r.reset(a)
+    with pytest.raises((P3Error, ValueError)):
+        r.iterate_multi([a, b], [1, 2], [64, 64], 10)
+
+
+def test_batch_renderer_matches_single_renders(R):
+    """Jobs of mixed sizes and shapes through the batch driver (process-pool
+    compiles, two alternating renderers, motion-blur sub-frames in one
+    batched launch) give the same images as one renderer per job."""
+    from pyr3_amd import batch
+    jobs = batch.batch_workload(n=6, seed=77, motion_fraction=0.34, subframes=4)
+    for j in jobs:
+        j.quality = 4.0
+    br = batch.BatchRenderer(n_streams=2, compile_workers=4)
+    res = br.render(jobs, read_images=True)
+    br.close()
+    for j, out in zip(jobs, res):
+        r = R(j.width, j.height, j.oversample)
+        walkers, iters = batch.gpu_dispatch(j.quality, j.width, j.height)
+        r.reset(j.genome)
+        if j.subframes:
+            counts = batch.apportion(walkers, [1.0] * len(j.subframes))
+            s = 0
+            for i, g in enumerate(j.subframes):
+                if counts[i]:
+                    s += r.iterate(g, (j.seed + i * batch.SUBFRAME_SEED_STEP) & 0xFFFFFFFF,
+                                   counts[i], iters)
+        else:
+            s = r.iterate(j.genome, j.seed, walkers, iters)
+        r.present(j.genome, s)
+        assert s == out.samples
+        assert np.array_equal(r.read_image(), out.image), j.name
+        r.destroy()
