@@ -50,19 +50,24 @@ def schedule(samples: int, walkers: int):
 
 
 class Frame:
-    """One plain-path frame of a genome at a quality (reset, chaos, DE, tonemap)."""
+    """One frame of a genome at a quality: reset, chaos, DE, tonemap (plain
+    path), plus the learned denoiser when one is given (learned path)."""
 
-    def __init__(self, r: Renderer, g, fuse: int, walkers: int = 196608):
+    def __init__(self, r: Renderer, g, fuse: int, walkers: int = 196608, denoiser=None):
         self.r, self.g, self.fuse = r, g, fuse
         self.walkers = walkers
         self.packed = pack_genome(g)
+        self.denoiser = denoiser
 
     def render(self, quality: float, seed: int, walkers: int = None):
         samples_target = int(round(quality * self.r.width * self.r.height))
         walkers, iters = schedule(samples_target, walkers or self.walkers)
+        self.r.set_image_f32(self.denoiser is not None)
         self.r.reset(self.g)
         s = self.r.iterate(self.g, seed, walkers, iters, fuse=self.fuse, packed=self.packed)
         self.r.present(self.g, s)
+        if self.denoiser is not None:
+            self.denoiser.apply(self.r)
         return walkers, iters, s
 
 
@@ -103,8 +108,8 @@ def pick_genome(r: Renderer, fuse: int, width: int, height: int, k0: int = 0):
     raise RuntimeError("no synthetic genome covered 5% of the frame")
 
 
-def quality_to_target(frame: Frame, ref_img, target_db: float):
-    ladder = [5, 10, 20, 40, 80, 160, 320, 640, 1280, 2560]
+def quality_to_target(frame: Frame, ref_img, target_db: float,
+                      ladder=(5, 10, 20, 40, 80, 160, 320, 640, 1280, 2560)):
     prev, found, scores = None, None, {}
     for q in ladder:
         frame.render(q, 1000 + q)
@@ -207,6 +212,47 @@ def batch_main(a):
     print(json.dumps(line), flush=True)
 
 
+WEIGHTS = ROOT / "pyr3_amd" / "denoise" / "weights.npz"
+LEARNED_LADDER = (2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64)
+
+
+def learned_path(r: Renderer, g, a, ref_img, walkers: int):
+    """Low-quality render + the BF16-MFMA denoiser: smallest q reaching the
+    target PSNR, then ms/frame of render + denoise at that q."""
+    from pyr3_amd.renderer import Denoiser
+    den = Denoiser.from_npz(WEIGHTS)
+    fl = Frame(r, g, a.fuse, walkers=walkers, denoiser=den)
+    ql, scores = quality_to_target(fl, ref_img, a.target_db, ladder=LEARNED_LADDER)
+    if ql is None:
+        log(f"learned path: no q up to {max(scores)} reached {a.target_db} dB")
+        return {"reached": False, "psnr_ladder": {str(k): round(v, 2) for k, v in sorted(scores.items())}}
+    for i in range(a.warmup):
+        fl.render(ql, 77 + i)
+    r.sync()
+    r.set_timing(True)
+    dn = 0.0
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        fl.render(ql, 700 + i)
+        r.sync()
+        dn += r.read_timings()["denoise"]
+    wall = time.perf_counter() - t0
+    r.set_timing(False)
+    r.set_image_f32(False)
+    den.destroy()
+    ms = wall * 1e3 / a.steps
+    log(f"learned path: q={ql} {scores[ql]:.2f} dB, {ms:.3f} ms/frame (denoise {dn / a.steps:.3f} ms)")
+    from pyr3_amd.denoise.model import macs_per_pixel
+    flops = 2.0 * macs_per_pixel() * r.width * r.height
+    dms = dn / a.steps
+    return {"reached": True, "ms_per_frame": round(ms, 3), "quality": ql,
+            "psnr_db": round(scores[ql], 2), "denoise_ms": round(dms, 4),
+            "denoise_roofline": {"bound": "mfma", "achieved": round(flops / (dms * 1e-3) / 1e12, 2),
+                                 "peak": 2500.0, "unit": "TFLOP/s",
+                                 "frac": round(flops / (dms * 1e-3) / 1e12 / 2500.0, 5)},
+            "psnr_ladder": {str(k): round(v, 2) for k, v in sorted(scores.items())}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -224,6 +270,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the low-quality + denoiser (learned) path")
     ap.add_argument("--workload", default="plain-1080p", choices=("plain-1080p", "batch"))
     ap.add_argument("--batch-size", type=int, default=512)
     a = ap.parse_args()
@@ -276,6 +324,9 @@ def main():
     chaos_ms /= a.steps; de_ms /= a.steps; tm_ms /= a.steps; reset_ms /= a.steps
     giters = walkers * (iters + a.fuse) / (chaos_ms * 1e-3) / 1e9
     gsamples = samples / (chaos_ms * 1e-3) / 1e9
+    learned = None
+    if a.quality is None and not a.no_learned and WEIGHTS.exists():
+        learned = learned_path(r, g, a, ref_img, frame.walkers)
     bins = W * H
     scatter_bytes = 16.0 * samples
     de_bytes = 32.0 * bins
@@ -317,6 +368,8 @@ def main():
                              "frac": round(tm_bytes / (tm_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
                              if tm_ms else None},
     }
+    if learned is not None:
+        out["learned"] = learned
     if not a.no_cpu_baseline:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(g, W, H, q, a.fuse, a.cpu_threads, a.cpu_budget)

@@ -90,6 +90,27 @@ def _compile_worker(xforms: np.ndarray, n_rows: int, final_slot: int, cache: str
                                 final_slot)
 
 
+def precompile(genomes: Sequence[Genome], workers: Optional[int] = None,
+               cache_dir: Optional[str] = None) -> int:
+    """Compile the chaos kernels of every distinct shape among `genomes` into
+    the code-object cache with a pool of worker processes; returns the number
+    of shapes."""
+    cache_dir = cache_dir or os.environ.get("P3_KERNEL_CACHE") or os.path.join(
+        os.path.dirname(os.path.abspath(__file__)), "kcache")
+    os.environ["P3_KERNEL_CACHE"] = cache_dir
+    shapes = {}
+    for g in genomes:
+        shapes.setdefault(_shape_key(g), pack_for(g))
+    workers = workers or max(1, min(16, (os.cpu_count() or 2) - 1))
+    with ProcessPoolExecutor(max_workers=workers, mp_context=get_context("spawn")) as ex:
+        futs = [ex.submit(_compile_worker, pg.xforms, pg.distrib.shape[0],
+                          pack.final_xform_slot(pg.expanded), cache_dir) for pg in shapes.values()]
+        for f in futs:
+            if f.result() != 0:
+                raise RuntimeError("kernel specialisation failed")
+    return len(shapes)
+
+
 class BatchRenderer:
     def __init__(self, n_streams: int = 2, compile_workers: Optional[int] = None,
                  cache_dir: Optional[str] = None):

@@ -24,6 +24,7 @@ RTC_HEADERS = ["p3_args.h", "p3_device.h", "p3_variations.h", "p3_variations_ext
                "p3_variation_dispatch.h", "p3_chaos_kernel.h"]
 RTC_INC = CSRC / "p3_rtc_source.inc"
 RTC_PRELUDE = ("typedef unsigned int uint32_t;\ntypedef int int32_t;\n"
+               "typedef unsigned long long uint64_t;\n"
                "typedef unsigned short uint16_t;\ntypedef unsigned char uint8_t;\n"
                "typedef unsigned long size_t;\n")
 

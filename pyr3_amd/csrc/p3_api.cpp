@@ -397,7 +397,8 @@ int iterate_multi_impl(p3_renderer* r, int n_sub, const float* const* xforms, in
       return rc;
     args[k] = dispatch_args(r, uniforms[k], n_regular, final_idx, dxf, dds);
     if (k == 0 && (rc = attach_trace(r, args[0]))) return rc;
-    M.start[k + 1] = M.start[k] + (args[k].walkers + p3::kChaosBlock - 1) / p3::kChaosBlock;
+    const uint32_t blk = (uint32_t)p3::chaos_block();
+    M.start[k + 1] = M.start[k] + (args[k].walkers + blk - 1) / blk;
     bound += 255ULL * (unsigned long long)args[k].walkers * (unsigned long long)args[k].iters;
   }
   if (M.start[n_sub] == 0) return P3_OK;

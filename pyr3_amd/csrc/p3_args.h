@@ -51,7 +51,7 @@ struct BlockLds {
   int32_t prev[kB];
   uint32_t consec[kB], iter[kB], draws[kB], raw[kB];
   uint16_t fn[kB], order[kB];
-  uint32_t bucket[kMaxXforms + 2], base[kMaxXforms + 2];
+  uint32_t bucket[kMaxXforms + 2];
 };
 
 }  // namespace p3

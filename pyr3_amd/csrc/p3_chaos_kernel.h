@@ -63,10 +63,21 @@ P3_INLINE uint32_t add_ret(P3_GLOBAL uint32_t* p, uint32_t d) {
 P3_INLINE void pin_max(P3_GLOBAL uint32_t* p) {
   (void)__hip_atomic_exchange(p, 0xffffffffu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-template <bool kSat>
+// Two 64-bit adds of word pairs (R,G) and (B,count): exact when no word can
+// reach 2^32 (the non-saturating case), since then no carry crosses words.
+P3_INLINE void add_pair(P3_GLOBAL uint32_t* p, uint32_t lo, uint32_t hi) {
+  const uint64_t v = (uint64_t)lo | ((uint64_t)hi << 32);
+  if (v)
+    (void)__hip_atomic_fetch_add((P3_GLOBAL uint64_t*)p, v, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+}
+template <bool kSat, bool kPairs>
 P3_INLINE void hist_add4(P3_GLOBAL uint32_t* b, uint32_t d0, uint32_t d1, uint32_t d2,
                          uint32_t d3) {
-  if (kSat) {
+  if (!kSat && kPairs) {
+    add_pair(b + 0, d0, d1);
+    add_pair(b + 2, d2, d3);
+  } else if (kSat) {
     const uint32_t o0 = d0 ? add_ret(b + 0, d0) : 0u;
     const uint32_t o1 = d1 ? add_ret(b + 1, d1) : 0u;
     const uint32_t o2 = d2 ? add_ret(b + 2, d2) : 0u;
@@ -195,8 +206,8 @@ P3_INLINE void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32_t fn,
       if (dc_on) { pal.x = dc_rgb.x; pal.y = dc_rgb.y; pal.z = dc_rgb.z; }
       const float wgt = opacity * 255.0f;
       P3_GLOBAL uint32_t* b = (P3_GLOBAL uint32_t*)A.hist + ((size_t)yi * A.width + xi) * 4u;
-      hist_add4<kSat>(b, f2u_sat(pal.x * wgt), f2u_sat(pal.y * wgt), f2u_sat(pal.z * wgt),
-                      f2u_sat(wgt));
+      hist_add4<kSat, Spec::kPairAdds>(b, f2u_sat(pal.x * wgt), f2u_sat(pal.y * wgt),
+                                       f2u_sat(pal.z * wgt), f2u_sat(wgt));
     }
   }
 }
@@ -210,7 +221,7 @@ P3_INLINE uint32_t multi_find(const MultiArgs& M, uint32_t b) {
 
 // Persistent block of kB walkers (walkers block*kB .. +kB-1 of A). Every turn:
 //   A  lane t draws walker t's pick and counts it into an LDS bucket per xform
-//   B  one wave turns the bucket counts into offsets
+//   B  each wave turns the bucket counts into offsets (redundantly, no barrier)
 //   C  walkers are listed in xform order
 //   D  lane t runs the turn of the t-th walker in that order
 // so the lanes of a wave mostly share one xform and its straight-line code.
@@ -274,25 +285,27 @@ P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L, uint32_t block)
     L.fn[t] = (uint16_t)f;
     const uint32_t pos = atomicAdd(&L.bucket[f], 1u);
     if (__syncthreads_count(active) == 0) break;
-    // B: exclusive scan of the bucket counts (first wave, 64 at a time)
-    if (t < 64) {
-      uint32_t carry = 0u;
+    // B+C: every wave scans the bucket counts itself (64 buckets per step,
+    // carried across steps), takes its walkers' bucket offsets from the lane
+    // holding each bucket, and lists its walkers in xform order.
+    {
+      const int lane = t & 63;
+      uint32_t carry = 0u, mine = 0u;
       for (uint32_t k0 = 0; k0 <= idle; k0 += 64u) {
-        const uint32_t k = k0 + (uint32_t)t;
+        const uint32_t k = k0 + (uint32_t)lane;
         const uint32_t v = k <= idle ? L.bucket[k] : 0u;
         uint32_t incl = v;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
           const uint32_t o = __shfl_up(incl, d, 64);
-          if (t >= d) incl += o;
+          if (lane >= d) incl += o;
         }
-        if (k <= idle) L.base[k] = carry + incl - v;
+        const uint32_t b = __shfl(carry + incl - v, (int)(f & 63u), 64);
+        if ((f >> 6) == (k0 >> 6)) mine = b;
         carry += __shfl(incl, 63, 64);
       }
+      L.order[mine + pos] = (uint16_t)t;
     }
-    __syncthreads();
-    // C: list walkers in xform order
-    L.order[L.base[f] + pos] = (uint16_t)t;
     __syncthreads();
     // D: run the turn of the t-th walker in that order
     {

@@ -49,6 +49,8 @@ bool chaos_code(const float* host_xforms, int n_regular, int final_slot,
 std::string chaos_source(const float* host_xforms, int n_regular, int final_slot);
 std::string chaos_shape_key(const float* host_xforms, int n_regular, int final_slot);
 size_t chaos_compiled_shapes();
+int chaos_block();  // walkers per chaos block
+bool pair_adds();   // u64 pair atomics in the non-saturating kernel
 hipError_t launch_density(const DeArgs& a, hipStream_t stream);
 hipError_t launch_tonemap(const TmArgs& a, bool de, hipStream_t stream);
 hipError_t launch_fill(void* p, size_t bytes, uint32_t v, hipStream_t stream);

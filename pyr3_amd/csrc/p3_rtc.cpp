@@ -90,6 +90,7 @@ void emit_body(std::ostringstream& s, const float* xf, int slot, bool full) {
 std::string generate(const float* xforms, int n_regular, int final_slot) {
   std::ostringstream s;
   s << "namespace p3 {\nstruct Spec {\n"
+    << "  static constexpr bool kPairAdds = " << (p3::pair_adds() ? "true" : "false") << ";\n"
     << "  static constexpr bool kHasFinal = " << (final_slot >= 0 ? "true" : "false") << ";\n"
     << "  static constexpr int kFinalSlot = " << (final_slot >= 0 ? final_slot : 0) << ";\n"
     << "  static P3_INLINE Chain chain(uint32_t fn, xrec_t X, float x,\n"
@@ -117,32 +118,33 @@ std::string generate(const float* xforms, int n_regular, int final_slot) {
     s << "    (void)X; (void)rng; return pa;\n";
   }
   s << "  }\n};\n}  // namespace p3\n\n"
-    << "extern \"C\" __global__ void __launch_bounds__(p3::kChaosBlock)\n"
+    << "constexpr int kBlk = " << p3::chaos_block() << ";\n"
+    << "extern \"C\" __global__ void __launch_bounds__(kBlk)\n"
     << "p3_chaos_sat(p3::ChaosArgs A) {\n"
-    << "  __shared__ p3::BlockLds<p3::kChaosBlock> L;\n"
-    << "  p3::chaos_binned<true, p3::kChaosBlock, p3::Spec>(A, L, blockIdx.x);\n"
+    << "  __shared__ p3::BlockLds<kBlk> L;\n"
+    << "  p3::chaos_binned<true, kBlk, p3::Spec>(A, L, blockIdx.x);\n"
     << "}\n"
-    << "extern \"C\" __global__ void __launch_bounds__(p3::kChaosBlock)\n"
+    << "extern \"C\" __global__ void __launch_bounds__(kBlk)\n"
     << "p3_chaos_nosat(p3::ChaosArgs A) {\n"
-    << "  __shared__ p3::BlockLds<p3::kChaosBlock> L;\n"
-    << "  p3::chaos_binned<false, p3::kChaosBlock, p3::Spec>(A, L, blockIdx.x);\n"
+    << "  __shared__ p3::BlockLds<kBlk> L;\n"
+    << "  p3::chaos_binned<false, kBlk, p3::Spec>(A, L, blockIdx.x);\n"
     << "}\n"
     // Batched launch: sub-dispatch k (same genome shape, its own records,
     // pick table, seed, camera and walker count) owns blocks
     // [start[k], start[k+1]) of one grid.
-    << "extern \"C\" __global__ void __launch_bounds__(p3::kChaosBlock)\n"
+    << "extern \"C\" __global__ void __launch_bounds__(kBlk)\n"
     << "p3_chaos_multi_sat(p3::MultiArgs M) {\n"
-    << "  __shared__ p3::BlockLds<p3::kChaosBlock> L;\n"
+    << "  __shared__ p3::BlockLds<kBlk> L;\n"
     << "  const uint32_t k = p3::multi_find(M, blockIdx.x);\n"
     << "  const p3::ChaosArgs A = ((const P3_CONST p3::ChaosArgs*)M.args)[k];\n"
-    << "  p3::chaos_binned<true, p3::kChaosBlock, p3::Spec>(A, L, blockIdx.x - M.start[k]);\n"
+    << "  p3::chaos_binned<true, kBlk, p3::Spec>(A, L, blockIdx.x - M.start[k]);\n"
     << "}\n"
-    << "extern \"C\" __global__ void __launch_bounds__(p3::kChaosBlock)\n"
+    << "extern \"C\" __global__ void __launch_bounds__(kBlk)\n"
     << "p3_chaos_multi_nosat(p3::MultiArgs M) {\n"
-    << "  __shared__ p3::BlockLds<p3::kChaosBlock> L;\n"
+    << "  __shared__ p3::BlockLds<kBlk> L;\n"
     << "  const uint32_t k = p3::multi_find(M, blockIdx.x);\n"
     << "  const p3::ChaosArgs A = ((const P3_CONST p3::ChaosArgs*)M.args)[k];\n"
-    << "  p3::chaos_binned<false, p3::kChaosBlock, p3::Spec>(A, L, blockIdx.x - M.start[k]);\n"
+    << "  p3::chaos_binned<false, kBlk, p3::Spec>(A, L, blockIdx.x - M.start[k]);\n"
     << "}\n";
   return s.str();
 }
@@ -230,6 +232,27 @@ bool compile(const std::string& src, std::vector<char>& code, std::string& err) 
 
 namespace p3 {
 
+// Walkers per chaos block: 256, or P3_CHAOS_BLOCK (64, 128, 192 or 256),
+// read once per process.
+int chaos_block() {
+  static const int b = [] {
+    const char* e = std::getenv("P3_CHAOS_BLOCK");
+    const int v = e ? std::atoi(e) : kChaosBlock;
+    return (v == 64 || v == 128 || v == 192 || v == 256) ? v : kChaosBlock;
+  }();
+  return b;
+}
+
+// Histogram adds in the non-saturating kernel: four u32 atomics per splat,
+// or two u64 atomics of word pairs when P3_HIST_ADD=pairs.
+bool pair_adds() {
+  static const bool on = [] {
+    const char* e = std::getenv("P3_HIST_ADD");
+    return e && std::strcmp(e, "pairs") == 0;
+  }();
+  return on;
+}
+
 std::string chaos_shape_key(const float* xforms, int n_regular, int final_slot) {
   std::ostringstream key;
   for (int j = 0; j < n_regular; ++j) shape_of(xforms + (size_t)j * kXformFloats, key);
@@ -292,11 +315,12 @@ hipError_t launch_chaos_rtc(const ChaosArgs& a, const float* host_xforms, int n_
   if (!chaos_code(host_xforms, n_regular, final_slot, &code, err)) return hipErrorInvalidImage;
   Module m;
   if ((e = module_for(dev, host_xforms, n_regular, final_slot, *code, m)) != hipSuccess) return e;
-  const unsigned grid = (a.walkers + kChaosBlock - 1) / kChaosBlock;
+  const unsigned blk = (unsigned)chaos_block();
+  const unsigned grid = (a.walkers + blk - 1) / blk;
   if (grid == 0) return hipSuccess;
   ChaosArgs args = a;
   void* params[] = {&args};
-  return hipModuleLaunchKernel(saturating ? m.sat : m.nosat, grid, 1, 1, kChaosBlock, 1, 1,
+  return hipModuleLaunchKernel(saturating ? m.sat : m.nosat, grid, 1, 1, blk, 1, 1,
                                0, stream, params, nullptr);
 }
 
@@ -316,7 +340,7 @@ hipError_t launch_chaos_rtc_multi(const MultiArgs& margs, const float* host_xfor
   if (grid == 0) return hipSuccess;
   MultiArgs args = margs;
   void* params[] = {&args};
-  return hipModuleLaunchKernel(saturating ? m.msat : m.mnosat, grid, 1, 1, kChaosBlock, 1, 1, 0,
+  return hipModuleLaunchKernel(saturating ? m.msat : m.mnosat, grid, 1, 1, chaos_block(), 1, 1, 0,
                                stream, params, nullptr);
 }
 

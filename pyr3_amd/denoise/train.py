@@ -42,13 +42,20 @@ def log(msg):
 
 
 def make_pairs(n: int, width: int, height: int, qualities, ref_q: float, dev):
+    from ..batch import precompile
+    n_genomes = -(-n // len(qualities))
+    cands = [synth.bench_genome(k, width, height)
+             for k in range(GENOME_BASE, GENOME_BASE + n_genomes * 3 // 2 + 8)]
+    t0 = time.time()
+    log(f"compiling {len(cands)} genome shapes ...")
+    precompile(cands)
+    log(f"compiled in {time.time() - t0:.0f}s")
     r = Renderer(width, height, 1)
     r.set_image_f32(True)
     X, Y = [], []
     k = GENOME_BASE
-    t0 = time.time()
-    while len(Y) < n:
-        g = synth.bench_genome(k, width, height)
+    while len(Y) < n and k - GENOME_BASE < len(cands):
+        g = cands[k - GENOME_BASE]
         k += 1
         ref, cnt, _ = render(r, g, 20, 7 + k)
         if np.mean(cnt > 0) < 0.05:
@@ -88,7 +95,7 @@ def train(X, Y, steps: int, crop: int, batch: int, lr: float, dev, seed: int = 0
         loss.backward()
         opt.step()
         sched.step()
-        if step % 250 == 0 or step == steps - 1:
+        if step % 100 == 0 or step == steps - 1:
             log(f"step {step} loss {loss.item():.3e} psnr~{-10 * math.log10(max(loss.item(), 1e-12)):.2f} "
                 f"({time.time() - t0:.0f}s)")
     return net

@@ -40,3 +40,18 @@ def test_zero_head_is_identity():
     net = Denoiser()    # the head starts at zero: output = input RGB
     x = torch.rand(1, 4, 20, 30)
     assert torch.equal(net(x), x[:, :3])
+
+
+def test_shipped_weights_load():
+    from pathlib import Path
+    from pyr3_amd import _lib
+    path = Path(__file__).resolve().parents[1] / "pyr3_amd" / "denoise" / "weights.npz"
+    z = np.load(path, allow_pickle=False)
+    flat = flat_params(z)
+    assert flat.size == _lib.load().p3_denoiser_param_count()
+    assert np.isfinite(flat).all() and np.abs(flat).max() < 100
+    net = Denoiser()
+    net.load_state_dict({**{f"convs.{n}.{p}": torch.from_numpy(z[f"{n}.{p}"])
+                            for n, _, _, _ in LAYERS for p in ("weight", "bias")},
+                         "out.weight": torch.from_numpy(z["out.weight"]),
+                         "out.bias": torch.from_numpy(z["out.bias"])})
