@@ -42,16 +42,17 @@ struct MultiArgs {
 };
 
 // LDS image of a chaos block's walkers (structure of arrays, kB walkers):
-// ISAAC words [32][kB] (mem 0..15, rsl 16..31) and the per-walker scalars.
+// ISAAC words [48][kB] (mem 0..15, two rsl rounds 16..47) and the
+// per-walker scalars: 244 bytes per walker.
 template <int kB>
 struct BlockLds {
-  uint32_t isaac[32 * kB];
+  uint32_t isaac[48 * kB];
   uint32_t cnt[kB], ra[kB], rb[kB], rc[kB];
   float x[kB], y[kB], z[kB];
   int32_t prev[kB];
   uint32_t consec[kB], iter[kB], draws[kB], raw[kB];
   uint16_t fn[kB], order[kB];
-  uint32_t bucket[kMaxXforms + 2];
+  uint32_t bucket[2][kMaxXforms + 2];   // per-xform counts, two turns alternating
 };
 
 }  // namespace p3

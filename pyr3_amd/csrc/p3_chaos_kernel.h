@@ -219,23 +219,39 @@ P3_INLINE uint32_t multi_find(const MultiArgs& M, uint32_t b) {
   return k;
 }
 
-// Persistent block of kB walkers (walkers block*kB .. +kB-1 of A). Every turn:
-//   A  lane t draws walker t's pick and counts it into an LDS bucket per xform
-//   B  each wave turns the bucket counts into offsets (redundantly, no barrier)
-//   C  walkers are listed in xform order
-//   D  lane t runs the turn of the t-th walker in that order
+// Pick of the walker's next turn: row = its previous accepted xform (or the
+// fallback row), column = low 14 bits of the next draw.
+P3_INLINE uint32_t draw_pick(const ChaosArgs& A, Rng& rng, int prev, uint32_t& raw) {
+  const uint32_t row = prev >= 0 ? (uint32_t)prev : A.fallback_row;
+  raw = rng.irand();
+  return ((const P3_CONST uint32_t*)A.distrib)[(size_t)row * kGrain + (raw & (kGrain - 1))];
+}
+
+// Persistent block of kB walkers (walkers block*kB .. +kB-1 of A). Lane t
+// seeds walker t and draws its first pick; then every turn:
+//   1  barrier (the block ends when no walker is left)
+//   2  each wave turns the per-xform pick counts into offsets (redundantly,
+//      no barrier) and lists the walkers its lanes hold in xform order
+//   3  barrier
+//   4  lane t runs the turn of the t-th walker in that order and draws that
+//      walker's next pick, counting it into the other turn's buckets
 // so the lanes of a wave mostly share one xform and its straight-line code.
 // Each walker keeps its own ISAAC column and loop state; only which lane
-// computes it changes, so the per-walker draw sequence, arithmetic and
-// splats are those of the one-lane-per-walker loop.
+// computes it changes, so the per-walker draw sequence (pick, turn, pick,
+// ...), arithmetic and splats are those of the one-lane-per-walker loop.
 template <bool kSat, int kB, class Spec>
 P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L, uint32_t block) {
   const int t = threadIdx.x;
   const uint32_t w0 = block * kB;
   const uint32_t total = A.iters + A.fuse;
-  const uint32_t idle = A.fallback_row + 1u;   // bucket after the last xform
+  const uint32_t idle = A.fallback_row + 1u;   // bucket of finished walkers
 
-  {  // seed walker t
+  for (int k = t; k < 2 * (kMaxXforms + 2); k += kB) (&L.bucket[0][0])[k] = 0u;
+  __syncthreads();
+  int wl = t;          // walker this lane holds
+  uint32_t f = idle;   // its pick
+  uint32_t pos;        // its rank among the walkers with that pick
+  {  // seed walker t and draw its first pick
     const uint32_t wid = w0 + t;
     Rng rng;
     rng.m.base = (P3_LDS uint32_t*)(L.isaac + t);
@@ -248,7 +264,7 @@ P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L, uint32_t block)
       if (A.isaac_in) {
         const P3_CONST uint32_t* st = (const P3_CONST uint32_t*)A.isaac_in + (size_t)wid * 36u;
         rng.s.cnt = st[0]; rng.s.a = st[1]; rng.s.b = st[2]; rng.s.c = st[3];
-        for (int k = 0; k < 16; ++k) { rng.m.mem(k) = st[4 + k]; rng.m.rsl(k) = st[20 + k]; }
+        for (int k = 0; k < 16; ++k) { rng.m.mem(k) = st[4 + k]; rng.m.rsl(0u, k) = st[20 + k]; }
       } else {
         isaac_seed(rng.s, rng.m, A.seed, wid);
       }
@@ -257,43 +273,30 @@ P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L, uint32_t block)
       z = rng.rand01();
       iter = 0u;
     }
+    if (iter < total) {
+      uint32_t raw;
+      f = draw_pick(A, rng, -1, raw);
+      L.raw[t] = raw;
+    }
     L.cnt[t] = rng.s.cnt; L.ra[t] = rng.s.a; L.rb[t] = rng.s.b; L.rc[t] = rng.s.c;
     L.x[t] = x; L.y[t] = y; L.z[t] = z;
     L.prev[t] = -1; L.consec[t] = 0u; L.iter[t] = iter; L.draws[t] = rng.draws;
-  }
-  for (int k = t; k < kMaxXforms + 2; k += kB) L.bucket[k] = 0u;
-  __syncthreads();
-
-  for (;;) {
-    // A: pick for walker t
-    const bool active = L.iter[t] < total;
-    uint32_t f = idle;
-    if (active) {
-      Rng rng;
-      rng.m.base = (P3_LDS uint32_t*)(L.isaac + t);
-      rng.m.stride = kB;
-      rng.s.cnt = L.cnt[t]; rng.s.a = L.ra[t]; rng.s.b = L.rb[t]; rng.s.c = L.rc[t];
-      rng.draws = L.draws[t];
-      const int prev = L.prev[t];
-      const uint32_t row = prev >= 0 ? (uint32_t)prev : A.fallback_row;
-      const uint32_t raw = rng.irand();
-      f = ((const P3_CONST uint32_t*)A.distrib)[(size_t)row * kGrain + (raw & (kGrain - 1))];
-      L.cnt[t] = rng.s.cnt; L.ra[t] = rng.s.a; L.rb[t] = rng.s.b; L.rc[t] = rng.s.c;
-      L.draws[t] = rng.draws;
-      L.raw[t] = raw;
-    }
     L.fn[t] = (uint16_t)f;
-    const uint32_t pos = atomicAdd(&L.bucket[f], 1u);
-    if (__syncthreads_count(active) == 0) break;
-    // B+C: every wave scans the bucket counts itself (64 buckets per step,
+    pos = atomicAdd(&L.bucket[0][f], 1u);
+  }
+
+  for (uint32_t p = 0;; p ^= 1u) {
+    if (__syncthreads_count(f != idle) == 0) break;
+    // every wave scans the bucket counts itself (64 buckets per step,
     // carried across steps), takes its walkers' bucket offsets from the lane
-    // holding each bucket, and lists its walkers in xform order.
+    // holding each bucket, and lists its walkers in xform order; the other
+    // turn's buckets (last read before the previous barrier) are cleared
     {
       const int lane = t & 63;
       uint32_t carry = 0u, mine = 0u;
       for (uint32_t k0 = 0; k0 <= idle; k0 += 64u) {
         const uint32_t k = k0 + (uint32_t)lane;
-        const uint32_t v = k <= idle ? L.bucket[k] : 0u;
+        const uint32_t v = k <= idle ? L.bucket[p][k] : 0u;
         uint32_t incl = v;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
@@ -304,28 +307,34 @@ P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L, uint32_t block)
         if ((f >> 6) == (k0 >> 6)) mine = b;
         carry += __shfl(incl, 63, 64);
       }
-      L.order[mine + pos] = (uint16_t)t;
+      L.order[mine + pos] = (uint16_t)wl;
+      for (int k = t; k < kMaxXforms + 2; k += kB) L.bucket[p ^ 1u][k] = 0u;
     }
     __syncthreads();
-    // D: run the turn of the t-th walker in that order
-    {
-      const int wl = L.order[t];
-      if (L.iter[wl] < total) {
-        Rng rng;
-        rng.m.base = (P3_LDS uint32_t*)(L.isaac + wl);
-        rng.m.stride = kB;
-        rng.s.cnt = L.cnt[wl]; rng.s.a = L.ra[wl]; rng.s.b = L.rb[wl]; rng.s.c = L.rc[wl];
-        rng.draws = L.draws[wl];
-        Walker wk{L.x[wl], L.y[wl], L.z[wl], L.prev[wl], L.consec[wl], L.iter[wl]};
-        run_turn<kSat, Spec>(A, wk, rng, L.fn[wl], L.raw[wl], w0 + wl);
-        L.cnt[wl] = rng.s.cnt; L.ra[wl] = rng.s.a; L.rb[wl] = rng.s.b; L.rc[wl] = rng.s.c;
-        L.draws[wl] = rng.draws;
-        L.x[wl] = wk.x; L.y[wl] = wk.y; L.z[wl] = wk.z;
-        L.prev[wl] = wk.prev; L.consec[wl] = wk.consec; L.iter[wl] = wk.iter;
+    // run the turn of the t-th walker in that order, then its next pick
+    wl = L.order[t];
+    f = idle;
+    if (L.iter[wl] < total) {
+      Rng rng;
+      rng.m.base = (P3_LDS uint32_t*)(L.isaac + wl);
+      rng.m.stride = kB;
+      rng.s.cnt = L.cnt[wl]; rng.s.a = L.ra[wl]; rng.s.b = L.rb[wl]; rng.s.c = L.rc[wl];
+      rng.draws = L.draws[wl];
+      isaac_top_up(rng.s, rng.m);   // the wave's lanes generate rounds together here
+      Walker wk{L.x[wl], L.y[wl], L.z[wl], L.prev[wl], L.consec[wl], L.iter[wl]};
+      run_turn<kSat, Spec>(A, wk, rng, L.fn[wl], L.raw[wl], w0 + wl);
+      if (wk.iter < total) {
+        uint32_t raw;
+        f = draw_pick(A, rng, wk.prev, raw);
+        L.raw[wl] = raw;
       }
-      for (int k = t; k < kMaxXforms + 2; k += kB) L.bucket[k] = 0u;
+      L.cnt[wl] = rng.s.cnt; L.ra[wl] = rng.s.a; L.rb[wl] = rng.s.b; L.rc[wl] = rng.s.c;
+      L.draws[wl] = rng.draws;
+      L.x[wl] = wk.x; L.y[wl] = wk.y; L.z[wl] = wk.z;
+      L.prev[wl] = wk.prev; L.consec[wl] = wk.consec; L.iter[wl] = wk.iter;
+      L.fn[wl] = (uint16_t)f;
     }
-    __syncthreads();
+    pos = atomicAdd(&L.bucket[p ^ 1u][f], 1u);
   }
 }
 

@@ -2,11 +2,13 @@
 // in LDS, the masked 28-bit rand transforms, range-guarded trig, and the
 // buffer-layout constants shared by the kernels.
 //
-// ISAAC state per walker is [cnt, a, b, c | mem[16] | rsl[16]]. cnt/a/b/c
+// ISAAC state per walker is [cnt, a, b, c | mem[16] | rsl[2][16]]. cnt/a/b/c
 // live in VGPRs; mem and rsl are indexed by data-dependent values
 // (mem[(x >> 2) & 15], rsl[--cnt]), which has no cheap per-lane VGPR form on
-// CDNA, so they live in LDS as [word][lane] columns: lane l's word k is at
-// k * stride + l, so any per-lane index hits bank (l mod 64) — conflict-free.
+// CDNA, so they live in LDS as [word][walker] columns: walker w's word k is
+// at k * stride + w. Two rsl rounds are held so that the next round can be
+// generated ahead, at a point where the whole wave runs it together, instead
+// of inside whichever branch happens to draw the 17th value.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -31,20 +33,26 @@ constexpr float kEps = 1e-10f;
 
 // ---------------------------------------------------------------- ISAAC --
 
+// cnt: bits 0-4 the unread values of the current rsl round (0..16), bit 5
+// which of the two rounds is current, bit 6 set when the other round already
+// holds the next 16 values.
 struct IsaacRegs {
   uint32_t cnt, a, b, c;
 };
+constexpr uint32_t kIsaacUnread = 31u, kIsaacHalf = 32u, kIsaacReady = 64u;
 
-// LDS view of one lane's mem/rsl words.
+// LDS view of one walker's mem/rsl words.
 struct IsaacLds {
-  P3_LDS uint32_t* base;   // &lds[lane]
-  int stride;              // lanes per block
+  P3_LDS uint32_t* base;   // &lds[walker]
+  int stride;              // walkers per block
   P3_INLINE P3_LDS uint32_t& mem(int k) const { return base[k * stride]; }
-  P3_INLINE P3_LDS uint32_t& rsl(int k) const { return base[(16 + k) * stride]; }
+  P3_INLINE P3_LDS uint32_t& rsl(uint32_t h, int k) const {
+    return base[(16 + 16 * (int)h + k) * stride];
+  }
 };
 
-// chaos.wgsl:243-294 / isaac.ts:35-105: one round, 16 outputs.
-P3_INLINE void isaac_round(IsaacRegs& s, const IsaacLds& m) {
+// chaos.wgsl:243-294 / isaac.ts:35-105: one round, 16 outputs into rsl[h].
+P3_INLINE void isaac_round(IsaacRegs& s, const IsaacLds& m, uint32_t h) {
   s.c += 1u;
   uint32_t a = s.a;
   uint32_t b = s.b + s.c;
@@ -59,21 +67,39 @@ P3_INLINE void isaac_round(IsaacRegs& s, const IsaacLds& m) {
     const uint32_t y = m.mem((x >> 2) & 15) + a + b;
     m.mem(k) = y;
     b = m.mem((y >> 6) & 15) + x;
-    m.rsl(k) = b;
+    m.rsl(h, k) = b;
   }
   s.a = a;
   s.b = b;
 }
 
-// chaos.wgsl:297-308: pop rsl[--cnt], refilling with a round when empty.
+// chaos.wgsl:297-308: pop rsl[--cnt], refilling with a round when empty
+// (taking the round generated ahead when there is one).
 P3_INLINE uint32_t isaac_irand(IsaacRegs& s, const IsaacLds& m) {
-  if (s.cnt == 0u) {
-    isaac_round(s, m);
-    s.cnt = 15u;
-  } else {
-    s.cnt -= 1u;
+  uint32_t n = s.cnt & kIsaacUnread;
+  uint32_t h = (s.cnt & kIsaacHalf) ? 1u : 0u;
+  uint32_t ready = s.cnt & kIsaacReady;
+  if (n == 0u) {
+    if (ready) {
+      h ^= 1u;
+      ready = 0u;
+    } else {
+      isaac_round(s, m, h);
+    }
+    n = 16u;
   }
-  return m.rsl((int)s.cnt);
+  n -= 1u;
+  s.cnt = n | (h ? kIsaacHalf : 0u) | ready;
+  return m.rsl(h, (int)n);
+}
+
+// Generates the next round into the other rsl half when it is free, so that
+// the next 16 + unread draws need no round.
+P3_INLINE void isaac_top_up(IsaacRegs& s, const IsaacLds& m) {
+  if (!(s.cnt & kIsaacReady)) {
+    isaac_round(s, m, (s.cnt & kIsaacHalf) ? 0u : 1u);
+    s.cnt |= kIsaacReady;
+  }
 }
 
 P3_INLINE void isaac_mix8(uint32_t v[8]) {
@@ -103,7 +129,7 @@ P3_INLINE void isaac_seed(IsaacRegs& s, const IsaacLds& m, uint32_t seed,
     pcg = pcg * MUL + INC;
     const uint32_t shift = (st >> 28) + 4u;
     const uint32_t word = ((st >> shift) ^ st) * 277803737u;
-    m.rsl(i) = (word >> 22) ^ word;
+    m.rsl(0u, i) = (word >> 22) ^ word;
   }
   uint32_t v[8];
 #pragma unroll
@@ -113,7 +139,7 @@ P3_INLINE void isaac_seed(IsaacRegs& s, const IsaacLds& m, uint32_t seed,
 #pragma unroll
   for (int i = 0; i < 16; i += 8) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) v[j] += m.rsl(i + j);
+    for (int j = 0; j < 8; ++j) v[j] += m.rsl(0u, i + j);
     isaac_mix8(v);
 #pragma unroll
     for (int j = 0; j < 8; ++j) m.mem(i + j) = v[j];
@@ -127,7 +153,7 @@ P3_INLINE void isaac_seed(IsaacRegs& s, const IsaacLds& m, uint32_t seed,
     for (int j = 0; j < 8; ++j) m.mem(i + j) = v[j];
   }
   s.a = s.b = s.c = 0u;
-  isaac_round(s, m);
+  isaac_round(s, m, 0u);
   s.cnt = 16u;
 }
 
