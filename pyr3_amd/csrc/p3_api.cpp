@@ -59,6 +59,25 @@ bool supported(uint32_t idx) {
 // Every active slot of every record must name an implemented entry.
 int check_records(p3_renderer* r, const float* xforms, int n_slots);
 
+// Step boundaries of a pick table (n_rows x 16 entries, see ChaosArgs), or
+// false when the table has more than 16 xforms or a row whose entries
+// decrease somewhere (then the kernel reads the table itself).
+bool pick_bounds(const uint32_t* table, int n_rows, std::vector<uint16_t>& out) {
+  if (n_rows > p3::kPickRows) return false;
+  out.assign((size_t)n_rows * p3::kPickCols, (uint16_t)P3_GRAIN);
+  for (int r = 0; r < n_rows; ++r) {
+    const uint32_t* row = table + (size_t)r * P3_GRAIN;
+    uint16_t* b = out.data() + (size_t)r * p3::kPickCols;
+    uint32_t prev = 0;
+    for (uint32_t c = 0; c < (uint32_t)P3_GRAIN; ++c) {
+      const uint32_t v = row[c];
+      if (v < prev) return false;
+      for (; prev < v; ++prev) b[prev] = (uint16_t)c;
+    }
+  }
+  return true;
+}
+
 }  // namespace
 
 struct p3_renderer {
@@ -72,9 +91,14 @@ struct p3_renderer {
   DevBuf<uint32_t> tile_rmax, image8;
   DevBuf<float> taps, lut, palette, xforms;
   DevBuf<uint32_t> distrib, isaac;
+  DevBuf<uint16_t> bounds;      // step boundaries of the pick table, when it has them
+  bool bounds_ok = false;
+  std::vector<uint16_t> bounds_host, mbounds_host;  // staging kept alive past async copies
+  std::vector<p3::ChaosArgs> margs_host;
   DevBuf<float> trace;
   DevBuf<float> mxforms;        // batched dispatch: per-sub-dispatch records
   DevBuf<uint32_t> mdistrib;    //   and pick tables
+  DevBuf<uint16_t> mbounds;     //   and their step boundaries
   DevBuf<char> margs;           //   and kernel arguments
   bool palette_set = false;
   bool have_image = false;
@@ -192,6 +216,7 @@ void release(p3_renderer* r) {
   r->hist.release(); r->filtered.release(); r->pre.release(); r->rad.release();
   r->tile_rmax.release(); r->image8.release(); r->imagef.release(); r->taps.release();
   r->lut.release(); r->palette.release(); r->xforms.release(); r->distrib.release();
+  r->bounds.release(); r->mbounds.release();
   r->isaac.release(); r->trace.release();
   r->mxforms.release(); r->mdistrib.release(); r->margs.release();
 }
@@ -314,9 +339,21 @@ int iterate_impl(p3_renderer* r, const float* xforms, int n_slots, const uint32_
                                     hipMemcpyHostToDevice, r->stream),
                      "upload pick table")))
       return rc;
+    r->bounds_ok = pick_bounds(distrib, n_rows, r->bounds_host);
+    if (r->bounds_ok) {
+      if ((rc = r->hip(r->bounds.ensure((size_t)p3::kPickRows * p3::kPickCols),
+                       "hipMalloc(pick bounds)")))
+        return rc;
+      if ((rc = r->hip(hipMemcpyAsync(r->bounds.p, r->bounds_host.data(),
+                                      r->bounds_host.size() * sizeof(uint16_t),
+                                      hipMemcpyHostToDevice, r->stream),
+                       "upload pick bounds")))
+        return rc;
+    }
     r->last_distrib.assign(distrib, distrib + table);
   }
   p3::ChaosArgs a = dispatch_args(r, u, n_regular, final_idx, r->xforms.p, r->distrib.p);
+  a.pick_bounds = r->bounds_ok ? r->bounds.p : nullptr;
   if (states) {
     if ((rc = r->hip(r->isaac.ensure((size_t)walkers * 36), "hipMalloc(isaac)"))) return rc;
     if ((rc = r->hip(hipMemcpyAsync(r->isaac.p, states, sizeof(uint32_t) * 36 * walkers,
@@ -379,7 +416,12 @@ int iterate_multi_impl(p3_renderer* r, int n_sub, const float* const* xforms, in
   if ((rc = r->hip(r->mdistrib.ensure(table * n_sub), "hipMalloc(batched pick tables)"))) return rc;
   if ((rc = r->hip(r->margs.ensure(sizeof(p3::ChaosArgs) * n_sub), "hipMalloc(batched args)")))
     return rc;
-  std::vector<p3::ChaosArgs> args((size_t)n_sub);
+  const size_t nb = (size_t)p3::kPickRows * p3::kPickCols;
+  if ((rc = r->hip(r->mbounds.ensure(nb * n_sub), "hipMalloc(batched pick bounds)"))) return rc;
+  std::vector<uint16_t>& bounds = r->mbounds_host;
+  bounds.assign(nb * n_sub, (uint16_t)P3_GRAIN);
+  std::vector<p3::ChaosArgs>& args = r->margs_host;
+  args.assign((size_t)n_sub, p3::ChaosArgs{});
   p3::MultiArgs M{};
   M.args = (const p3::ChaosArgs*)r->margs.p;
   M.n = (uint32_t)n_sub;
@@ -396,12 +438,21 @@ int iterate_multi_impl(p3_renderer* r, int n_sub, const float* const* xforms, in
                      "upload batched pick table")))
       return rc;
     args[k] = dispatch_args(r, uniforms[k], n_regular, final_idx, dxf, dds);
+    std::vector<uint16_t> b;
+    if (pick_bounds(distribs[k], n_rows, b)) {
+      std::copy(b.begin(), b.end(), bounds.begin() + nb * k);
+      args[k].pick_bounds = r->mbounds.p + nb * k;
+    }
     if (k == 0 && (rc = attach_trace(r, args[0]))) return rc;
     const uint32_t blk = (uint32_t)p3::chaos_block();
     M.start[k + 1] = M.start[k] + (args[k].walkers + blk - 1) / blk;
     bound += 255ULL * (unsigned long long)args[k].walkers * (unsigned long long)args[k].iters;
   }
   if (M.start[n_sub] == 0) return P3_OK;
+  if ((rc = r->hip(hipMemcpyAsync(r->mbounds.p, bounds.data(), bounds.size() * sizeof(uint16_t),
+                                  hipMemcpyHostToDevice, r->stream),
+                   "upload batched pick bounds")))
+    return rc;
   if ((rc = r->hip(hipMemcpyAsync(r->margs.p, args.data(), sizeof(p3::ChaosArgs) * n_sub,
                                   hipMemcpyHostToDevice, r->stream),
                    "upload batched args")))

@@ -11,6 +11,8 @@ constexpr int kMaxVars = 8;        // variation slots per xform
 constexpr int kTraceFloats = 16;   // floats per trace entry
 constexpr int kChaosBlock = 256;   // walkers per chaos block
 constexpr int kCatalogSize = 326;  // catalog indices 0..325 (97 = pre_blur)
+constexpr int kPickRows = 17;      // pick tables of up to 16 xforms (+ fallback row)
+constexpr int kPickCols = 16;      //   also travel as step boundaries, 16 per row
 
 // Catalog entries with a direct-colour side (src/variations.ts:681-692).
 constexpr bool is_dc_index(unsigned idx) {
@@ -20,6 +22,10 @@ constexpr bool is_dc_index(unsigned idx) {
 struct ChaosArgs {
   const float* xforms;
   const uint32_t* distrib;     // rows 0..n-1 then the fallback row
+  // Null, or the same table as step boundaries (rows with n <= 16 xforms
+  // whose entries never decrease along the row): entry [row][k] = first
+  // column whose pick exceeds k (16384 if none), so pick = #{k : col >= it}.
+  const uint16_t* pick_bounds;
   const float4* palette;       // 256 RGBA
   uint32_t* hist;              // W x H x 4
   const uint32_t* isaac_in;    // walkers x 36 or null (seed on device)
@@ -43,9 +49,10 @@ struct MultiArgs {
 
 // LDS image of a chaos block's walkers (structure of arrays, kB walkers):
 // ISAAC words [48][kB] (mem 0..15, two rsl rounds 16..47) and the
-// per-walker scalars: 244 bytes per walker.
+// per-walker scalars: 244 bytes per walker; the pick boundaries (544 B).
 template <int kB>
 struct BlockLds {
+  alignas(16) uint16_t pick_b[kPickRows * kPickCols];
   uint32_t isaac[48 * kB];
   uint32_t cnt[kB], ra[kB], rb[kB], rc[kB];
   float x[kB], y[kB], z[kB];

@@ -161,7 +161,11 @@ P3_INLINE void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32_t fn,
   if (Spec::kHasFinal) {
     const xrec_t fx = X + (size_t)Spec::kFinalSlot * kXformFloats;
     const bool apply = fx[10] == 1.0f ? true : (rng.rand01() < fx[10]);
-    if (apply) {
+    // During the fuse the final's output feeds neither the walker nor a
+    // splat, so a final that draws nothing is left out there (its opacity
+    // gate draw above is kept).
+    const bool needed = !Spec::kFinalDrawFree || i >= A.fuse || tr != nullptr;
+    if (apply && needed) {
       const float2 fpa = make_float2(fx[0] * pv.x + fx[1] * pv.y + fx[2],
                                      fx[4] * pv.x + fx[5] * pv.y + fx[6]);
       const float2 fpv = Spec::final_sum(X, fpa, rng, dc_on, dc_rgb);
@@ -220,11 +224,27 @@ P3_INLINE uint32_t multi_find(const MultiArgs& M, uint32_t b) {
 }
 
 // Pick of the walker's next turn: row = its previous accepted xform (or the
-// fallback row), column = low 14 bits of the next draw.
-P3_INLINE uint32_t draw_pick(const ChaosArgs& A, Rng& rng, int prev, uint32_t& raw) {
+// fallback row), column = low 14 bits of the next draw. Tables of up to 16
+// xforms with boundaries are read from the block's LDS copy of those
+// (pick = number of boundaries at or below the column), others from HBM.
+template <class Spec, int kB>
+P3_INLINE uint32_t draw_pick(const ChaosArgs& A, const BlockLds<kB>& L, Rng& rng, int prev,
+                             uint32_t& raw) {
   const uint32_t row = prev >= 0 ? (uint32_t)prev : A.fallback_row;
   raw = rng.irand();
-  return ((const P3_CONST uint32_t*)A.distrib)[(size_t)row * kGrain + (raw & (kGrain - 1))];
+  const uint32_t col = raw & (kGrain - 1);
+  if (Spec::kRegular < kPickRows && A.pick_bounds != nullptr) {
+    const uint4* b = (const uint4*)(L.pick_b + row * kPickCols);
+    const uint4 q0 = b[0];
+    const uint4 q1 = Spec::kRegular > 9 ? b[1] : make_uint4(0u, 0u, 0u, 0u);
+    const uint32_t w[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+    uint32_t f = 0u;
+#pragma unroll
+    for (int k = 0; k < Spec::kRegular - 1; ++k)
+      f += col >= ((w[k >> 1] >> (16 * (k & 1))) & 0xffffu) ? 1u : 0u;
+    return f;
+  }
+  return ((const P3_CONST uint32_t*)A.distrib)[(size_t)row * kGrain + col];
 }
 
 // Persistent block of kB walkers (walkers block*kB .. +kB-1 of A). Lane t
@@ -247,6 +267,9 @@ P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L, uint32_t block)
   const uint32_t idle = A.fallback_row + 1u;   // bucket of finished walkers
 
   for (int k = t; k < 2 * (kMaxXforms + 2); k += kB) (&L.bucket[0][0])[k] = 0u;
+  if (Spec::kRegular < kPickRows && A.pick_bounds != nullptr)
+    for (uint32_t k = t; k < (A.fallback_row + 1u) * kPickCols; k += kB)
+      L.pick_b[k] = ((const P3_CONST uint16_t*)A.pick_bounds)[k];
   __syncthreads();
   int wl = t;          // walker this lane holds
   uint32_t f = idle;   // its pick
@@ -275,7 +298,7 @@ P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L, uint32_t block)
     }
     if (iter < total) {
       uint32_t raw;
-      f = draw_pick(A, rng, -1, raw);
+      f = draw_pick<Spec>(A, L, rng, -1, raw);
       L.raw[t] = raw;
     }
     L.cnt[t] = rng.s.cnt; L.ra[t] = rng.s.a; L.rb[t] = rng.s.b; L.rc[t] = rng.s.c;
@@ -325,7 +348,7 @@ P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L, uint32_t block)
       run_turn<kSat, Spec>(A, wk, rng, L.fn[wl], L.raw[wl], w0 + wl);
       if (wk.iter < total) {
         uint32_t raw;
-        f = draw_pick(A, rng, wk.prev, raw);
+        f = draw_pick<Spec>(A, L, rng, wk.prev, raw);
         L.raw[wl] = raw;
       }
       L.cnt[wl] = rng.s.cnt; L.ra[wl] = rng.s.a; L.rb[wl] = rng.s.b; L.rc[wl] = rng.s.c;

@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "p3_launch.h"
+#include "p3_var_draws.h"
 
 namespace {
 
@@ -54,6 +55,15 @@ void shape_of(const float* xf, std::ostringstream& key) {
 }
 
 // Straight-line chain of one record at X + slot * 116.
+// True when the final xform exists and none of its slots draws from the RNG.
+bool final_draw_free(const float* xforms, int final_slot) {
+  if (final_slot < 0) return false;
+  const float* xf = xforms + (size_t)final_slot * p3::kXformFloats;
+  for (int k = 0; k < slot_count(xf); ++k)
+    if (p3::var_draws(slot_index(xf, k))) return false;
+  return true;
+}
+
 void emit_body(std::ostringstream& s, const float* xf, int slot, bool full) {
   const int nv = slot_count(xf);
   s << "      const xrec_t xf = X + " << slot * p3::kXformFloats << ";\n";
@@ -91,8 +101,11 @@ std::string generate(const float* xforms, int n_regular, int final_slot) {
   std::ostringstream s;
   s << "namespace p3 {\nstruct Spec {\n"
     << "  static constexpr bool kPairAdds = " << (p3::pair_adds() ? "true" : "false") << ";\n"
+    << "  static constexpr int kRegular = " << n_regular << ";\n"
     << "  static constexpr bool kHasFinal = " << (final_slot >= 0 ? "true" : "false") << ";\n"
     << "  static constexpr int kFinalSlot = " << (final_slot >= 0 ? final_slot : 0) << ";\n"
+    << "  static constexpr bool kFinalDrawFree = " << (final_draw_free(xforms, final_slot) ? "true" : "false")
+    << ";\n"
     << "  static P3_INLINE Chain chain(uint32_t fn, xrec_t X, float x,\n"
     << "      float y, float z, Rng& rng, bool& dc_on, float3& dc_rgb) {\n"
     << "    (void)dc_on; (void)dc_rgb;\n"

@@ -54,6 +54,52 @@ def test_isaac_stream_and_pick_sequence_bit_exact(orc, R):
     assert np.array_equal(hg, ho)
 
 
+def _linear_xforms(n, rng):
+    """n contractive linear maps with random weights (attractor near the unit square)."""
+    out = []
+    for j in range(n):
+        a = rng.uniform(0.3, 0.6)
+        out.append(xform(float(rng.uniform(0.2, 1.0)),
+                         (a, 0.0, float(rng.uniform(0, 0.5)), 0.0, a, float(rng.uniform(0, 0.5))),
+                         color=j / max(n - 1, 1)))
+    return out
+
+
+@pytest.mark.parametrize("case", ["lds16_xaos", "hbm20", "unsorted_table"])
+def test_pick_paths_bit_exact(orc, R, case):
+    """Picks of tables of up to 16 xforms come from step boundaries in LDS,
+    others (more xforms, or a caller table whose rows are not sorted) from
+    the table itself; both give the table's entry bit for bit."""
+    import dataclasses
+    from pyr3_amd.renderer import pack_genome
+    rng = np.random.default_rng(7)
+    n = 20 if case == "hbm20" else (16 if case == "lds16_xaos" else 5)
+    xs = _linear_xforms(n, rng)
+    if case == "lds16_xaos":
+        for x in xs:
+            x.xaos = [float(v) for v in rng.choice([0.0, 0.5, 1.0, 3.0], size=n)]
+            x.xaos[0] = 1.0
+    g = genome(xs, scale=10.0, cx=0.5, cy=0.5)
+    W = H = 32
+    walkers, iters, fuse, seed = 512, 12, 4, 0xBEEF
+    xf, ds, u, pal = blobs(g, W, H, iters=iters, fuse=fuse, seed=seed, walkers=walkers)
+    pg = pack_genome(g)
+    if case == "unsorted_table":
+        ds = np.ascontiguousarray(ds[:, rng.permutation(ds.shape[1])])
+        pg = dataclasses.replace(pg, distrib=ds)
+    r = R(W, H, 1)
+    r.reset(g)
+    r.set_trace(walkers, iters + fuse)
+    r.iterate(g, seed, walkers, iters, fuse=fuse, packed=pg)
+    tg, hg = r.read_trace(), r.read_histogram()
+    r.destroy()
+    ho, to = orc.chaos(xf, ds, u, pal, trace_shape=(walkers, iters + fuse))
+    assert np.array_equal(tg[..., 11].view(np.uint32), to[..., 11].view(np.uint32))
+    assert np.array_equal(tg[..., 1], to[..., 1])
+    assert len(np.unique(tg[..., 1])) == n
+    assert np.array_equal(hg, ho)
+
+
 def test_isaac_explicit_states_match_device_seeding(orc, R):
     g = genome(sierpinski(), scale=10.0, cx=0.5, cy=0.5)
     W = H = 16
