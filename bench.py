@@ -30,6 +30,21 @@ from pyr3_amd import pack, synth  # noqa: E402
 from pyr3_amd.renderer import Renderer, pack_genome  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+
+
+def profiled_traffic(kernel: str, workload: str, quality, walkers: int):
+    """HBM bytes per launch of `kernel` on this workload from the rocprofv3
+    FETCH_SIZE + WRITE_SIZE summaries under profiles/ (a line
+    `traffic-per-launch <kernel> <workload> q<q> walkers<w>: fetch_kib=F write_kib=W`),
+    or None when no profile covers exactly this configuration."""
+    key = f"traffic-per-launch {kernel} {workload} q{quality:g} walkers{walkers}:"
+    for f in sorted((ROOT / "profiles").glob("*.md")):
+        for line in f.read_text().splitlines():
+            if line.startswith(key):
+                kv = dict(t.split("=") for t in line[len(key):].split())
+                return {"bytes": (float(kv["fetch_kib"]) + float(kv["write_kib"])) * 1024.0,
+                        "source": f"profiles/{f.name}"}
+    return None
 REF_SEED = 0x9E3779B9
 
 
@@ -332,6 +347,8 @@ def main():
     de_bytes = 32.0 * bins
     tm_bytes = 16.0 * bins + 4.0 * W * H
     achieved = scatter_bytes / (chaos_ms * 1e-3) / 1e9
+    traffic = profiled_traffic("p3_chaos_sat" if 255.0 * samples >= 2 ** 32 else "p3_chaos_nosat",
+                               f"plain-{W}x{H}-os1", q, walkers)
     out = {
         "metric": "ms/frame to >=38 dB vs q4000 @1080p (plain path); Giters/s",
         "value": round(ms, 3),
@@ -356,9 +373,13 @@ def main():
                        "tonemap": round(tm_ms, 4), "reset": round(reset_ms, 4)},
         "roofline": {"bound": "hbm", "kernel": "chaos_kernel", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "note": "16 B of u32 atomic scatter per post-fuse sample; the kernel is "
-                             "VALU/atomic-latency bound, not HBM bound"},
+                     "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": round(traffic["bytes"]) if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None,
+                     "note": "achieved = 16 B of u32 atomic scatter per post-fuse sample; "
+                             "traffic = rocprofv3 FETCH_SIZE + WRITE_SIZE of the same launch "
+                             "(atomics leave the L2 at ~21 B per 4-B add); the kernel is "
+                             "VALU/latency bound, not HBM bound"},
         "de_roofline": {"achieved": round(de_bytes / (de_ms * 1e-3) / 1e9, 1) if de_ms else None,
                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(de_bytes / (de_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
