@@ -165,12 +165,17 @@ class Renderer:
               self._h)
 
     def render(self, genome: Genome, seed: Optional[int] = None, force_de_off: bool = False,
-               transparent: bool = False, walker_jitter: Optional[float] = None) -> int:
-        """reset + one iterate sized by computeDispatch + present; returns samples."""
+               transparent: bool = False, walker_jitter: Optional[float] = None,
+               walkers: Optional[int] = None) -> int:
+        """reset + one iterate sized by computeDispatch + present; returns samples.
+        `walkers` pins the walker count (the reference's --walkers override:
+        iters = ceil(samples / walkers)); the default is the reference's
+        schedule of ~1,024 long walkers, which fills only a few CUs, so GPU
+        callers pass a walker count of the order of 10^5."""
         if seed is None:
             seed = random.getrandbits(32)
         spp = genome.quality if genome.quality is not None else pack.DEFAULT_SPP
-        walkers, iters, samples = pack.compute_dispatch(spp, self.width, self.height)
+        walkers, iters, samples = pack.compute_dispatch(spp, self.width, self.height, walkers)
         self.reset(genome)
         self.iterate(genome, seed, walkers, iters, walker_jitter)
         self.present(genome, samples, force_de_off, transparent)
