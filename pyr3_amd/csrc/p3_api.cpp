@@ -139,7 +139,7 @@ struct p3_denoiser {
   DevBuf<uint16_t> w;
   DevBuf<float> b, head;
   size_t w_off[p3::kDenoiseLayers] = {}, b_off[p3::kDenoiseLayers] = {};
-  DevBuf<uint16_t> s0a, s0, h1a, s1, h1b, q0, q1, v1;  // activations, grown on demand
+  DevBuf<uint16_t> s0a, s0, h1a, s1, h1b, q0, q1;  // activations, grown on demand
 };
 
 namespace {
@@ -817,7 +817,7 @@ int p3_denoiser_create(const float* params, size_t n, p3_denoiser** out) {
 int p3_denoiser_destroy(p3_denoiser* d) {
   if (!d) return P3_OK;
   d->w.release(); d->b.release(); d->head.release();
-  for (auto* a : {&d->s0a, &d->s0, &d->h1a, &d->s1, &d->h1b, &d->q0, &d->q1, &d->v1}) a->release();
+  for (auto* a : {&d->s0a, &d->s0, &d->h1a, &d->s1, &d->h1b, &d->q0, &d->q1}) a->release();
   delete d;
   return P3_OK;
 }
@@ -834,7 +834,6 @@ int p3_denoise(p3_renderer* r, p3_denoiser* d) {
   int rc;
   if ((rc = r->hip(d->s0a.ensure(full * 16), "hipMalloc(denoise)")) ||
       (rc = r->hip(d->s0.ensure(full * 16), "hipMalloc(denoise)")) ||
-      (rc = r->hip(d->v1.ensure(full * 16), "hipMalloc(denoise)")) ||
       (rc = r->hip(d->h1a.ensure(half * 32), "hipMalloc(denoise)")) ||
       (rc = r->hip(d->s1.ensure(half * 32), "hipMalloc(denoise)")) ||
       (rc = r->hip(d->h1b.ensure(half * 32), "hipMalloc(denoise)")) ||
@@ -851,7 +850,7 @@ int p3_denoise(p3_renderer* r, p3_denoiser* d) {
   a.hist = r->hist.p;
   a.out8 = r->image8.p;
   a.s0a = d->s0a.p; a.s0 = d->s0.p; a.h1a = d->h1a.p; a.s1 = d->s1.p;
-  a.h1b = d->h1b.p; a.q0 = d->q0.p; a.q1 = d->q1.p; a.v1 = d->v1.p;
+  a.h1b = d->h1b.p; a.q0 = d->q0.p; a.q1 = d->q1.p;
   a.width = W;
   a.height = H;
   r->tic(T_DENOISE);

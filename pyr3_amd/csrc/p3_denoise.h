@@ -20,7 +20,7 @@ struct DenoiseArgs {
   const float4* img;                  // renderer's RGBA f32 image
   const uint32_t* hist;               // renderer's RGBA histogram (oversample 1)
   uint32_t* out8;                     // RGBA8 output
-  uint16_t *s0a, *s0, *h1a, *s1, *h1b, *q0, *q1, *v1;  // activations
+  uint16_t *s0a, *s0, *h1a, *s1, *h1b, *q0, *q1;  // activations
   int width, height;
 };
 
