@@ -26,8 +26,9 @@
 namespace p3 {
 
 constexpr int kTile = 16;
-constexpr int kLdsRadius = 24;                        // staged-halo limit
-constexpr int kLdsSpan = kTile + 2 * kLdsRadius;      // 64
+constexpr int kLdsRadius = 16;                        // staged-halo limit
+constexpr int kLdsSpan = kTile + 2 * kLdsRadius;      // 48
+constexpr int kKwStride = kLdsRadius * kLdsRadius + 1;  // d^2 = 0 .. 256 per radius
 constexpr uint8_t kEmpty = 255;
 
 __global__ void __launch_bounds__(256) de_prep(DeArgs A) {
@@ -82,6 +83,7 @@ __device__ inline float de_kw(int d2, int r) {
 __global__ void __launch_bounds__(256) de_gather(DeArgs A) {
   __shared__ float4 s_pre[kLdsSpan * kLdsSpan];
   __shared__ uint8_t s_rad[kLdsSpan * kLdsSpan];
+  __shared__ float s_kw[(kLdsRadius + 1) * kKwStride];  // de_kw(d^2, r) for d^2 <= r^2
   __shared__ int s_rt;
   const int tid = threadIdx.y * kTile + threadIdx.x;
   const int x0 = blockIdx.x * kTile, y0 = blockIdx.y * kTile;
@@ -94,6 +96,11 @@ __global__ void __launch_bounds__(256) de_gather(DeArgs A) {
 
   if (Rt <= kLdsRadius) {
     const int span = kTile + 2 * Rt;
+    const int kws = Rt * Rt + 1;
+    for (int k = tid; k < (Rt + 1) * kws; k += 256) {
+      const int r = k / kws, d2 = k - r * kws;
+      s_kw[k] = d2 <= r * r ? de_kw(d2, r) : 0.0f;
+    }
     for (int k = tid; k < span * span; k += 256) {
       const int ly = k / span, lx = k - ly * span;
       const int gx = x0 - Rt + lx, gy = y0 - Rt + ly;
@@ -118,7 +125,9 @@ __global__ void __launch_bounds__(256) de_gather(DeArgs A) {
           if (r == kEmpty) continue;
           const int d2 = dx * dx + dy * dy;
           if (d2 > r * r) continue;
-          const float kw = de_kw(d2, r);
+          // r <= R (rint of a radius clamped to max_rad <= R) and every bin of the
+          // box lies in a tile that bounds Rt, so r <= Rt: the table holds it
+          const float kw = r <= Rt ? s_kw[r * kws + d2] : de_kw(d2, r);
           const float4 p = s_pre[k];
           sum.x = sum.x + p.x * kw;
           sum.y = sum.y + p.y * kw;
