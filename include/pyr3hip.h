@@ -67,7 +67,10 @@ int p3_reset(p3_renderer* r, const float* palette_rgba);
  *             reference's 129-row buffer)
  *   uniforms: the 32-word chaos uniform block; slot 7 (seed) and slot 12
  *             (walker count) seed walker w's ISAAC stream on the device, bit
- *             for bit as packIsaacStates (src/isaac.ts:194-229) would. */
+ *             for bit as packIsaacStates (src/isaac.ts:194-229) would; slots
+ *             14-29 select the colour-index capture and the splat colour mode
+ *             (0 palette, 1 flow, 2 trap distance, 3 phase) with its
+ *             parameters, laid out as src/chaos.ts:421-450 writes them. */
 int p3_iterate(p3_renderer* r, const float* xforms, int n_slots, const uint32_t* distrib,
                int n_rows, const uint32_t* uniforms);
 
@@ -96,6 +99,11 @@ int p3_present(p3_renderer* r, const uint32_t* de_uniforms, const uint32_t* viz_
 /* Readbacks (block until the stream has drained). n is the element count of
  * the destination and must match the buffer exactly. */
 int p3_read_histogram(p3_renderer* r, uint32_t* out, size_t n);  /* readHistogramRgba */
+/* Colour-index capture (src/chaos.ts:262-291 readIndexAndCount, idx_sum half):
+ * W_s x H_s u32 sums of clamp(colour, 0, 1) x opacity x 255 per bin, written
+ * by dispatches whose uniform slot 14 (capture_index) is 1; cleared by
+ * p3_reset once allocated. P3_E_STATE before the first capturing dispatch. */
+int p3_read_index(p3_renderer* r, uint32_t* out, size_t n);
 int p3_read_filtered(p3_renderer* r, float* out, size_t n);      /* DE output grid */
 int p3_read_image(p3_renderer* r, uint8_t* out, size_t n);       /* W x H x 4 RGBA8 */
 int p3_read_image_f32(p3_renderer* r, float* out, size_t n);     /* pre-quantisation RGBA */

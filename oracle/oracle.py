@@ -27,6 +27,7 @@ _SIGS = {
     "orc_chaos": (None, [_F, _U, C.c_int, _U, _F, _U, C.c_uint32, C.c_uint32, _F, C.c_int,
                          C.c_int]),
     "orc_chaos_follow": (None, [_F, _U, C.c_int, _U, _F, _U, _F, C.c_int, C.c_int, _F]),
+    "orc_chaos_capture": (None, [_F, _U, C.c_int, _U, _F, _U, _U, C.c_uint32, C.c_uint32]),
     "orc_chaos_mt": (None, [_F, _U, C.c_int, _U, _F, _U, C.c_int]),
     "orc_kernel_norm_lut": (None, [_F]),
     "orc_density": (None, [_U, _U, _F, _F, C.c_int]),
@@ -121,6 +122,21 @@ def chaos(xforms, distrib, uniforms, palette, hist=None, wb=0, we=None, trace_sh
     load().orc_chaos(p(xf, _F), p(ds, _U), ds.shape[0], p(u, _U), p(pal, _F), p(hist, _U), wb, we,
                      p(tr, _F) if tr is not None else None, tw, ti)
     return hist, tr
+
+
+def chaos_capture(xforms, distrib, uniforms, palette):
+    """All walkers single-threaded with the colour-index capture buffer;
+    returns (hist, idx_sum (H, W) u32)."""
+    xf = np.ascontiguousarray(xforms, dtype=np.float32)
+    ds = np.ascontiguousarray(distrib, dtype=np.uint32)
+    u = np.ascontiguousarray(uniforms, dtype=np.uint32)
+    pal = np.ascontiguousarray(palette, dtype=np.float32).reshape(-1)
+    W, H = int(u[0]), int(u[1])
+    hist = np.zeros((H, W, 4), dtype=np.uint32)
+    idx = np.zeros((H, W), dtype=np.uint32)
+    load().orc_chaos_capture(p(xf, _F), p(ds, _U), ds.shape[0], p(u, _U), p(pal, _F),
+                             p(hist, _U), p(idx, _U), 0, int(u[12]))
+    return hist, idx
 
 
 def chaos_follow(xforms, distrib, uniforms, palette, force):

@@ -5,7 +5,10 @@
 //   apply_pre_blur      :8164-8181   (5 rand01 per pre_blur slot)
 //   apply_xform_chain   :8226-8267   (affine -> pre_blur -> sum -> post -> colour)
 //   chaos_main          :8269-8732   (init draws, pick, bad/retry, final lens,
-//                                     jitter, splat, saturating u32 adds :8120-8135)
+//                                     jitter, splat, saturating u32 adds :8120-8135,
+//                                     colour modes flow / trap / phase :8625-8697,
+//                                     capture of the colour index :8720-8728)
+//   hsv_to_rgb          :1662-1666
 // Buffer layouts follow src/chaos.ts:381-456 (uniform slots) and
 // src/genome.ts:328-406 (116-float xform record).
 #include <algorithm>
@@ -51,8 +54,70 @@ struct Uniforms {
   float rotation;
   uint32_t palette_mode, trace_mode, walker_count;
   float jitter;
+  uint32_t capture_index, color_mode;
+  float flow_strength, flow_scale;
+  uint32_t trap_kind, trap_mode;
+  float trap_cx, trap_cy, trap_radius, trap_nx, trap_ny, trap_falloff, trap_freq, trap_strength;
+  float phase_strength, phase_freq;
   float xform_blend;
 };
+
+constexpr float kPi = 3.14159265358979323846f;
+
+// WGSL fract(x) = x - floor(x)
+inline float fractf(float x) { return x - std::floor(x); }
+
+// chaos.wgsl:1662-1666: k = (1, 2/3, 1/3), p = |fract(h + k) * 6 - 3|,
+// rgb = v * mix(1, clamp(p - 1, 0, 1), s).
+inline void hsv_to_rgb(float h, float s, float v, float out[3]) {
+  const float k[3] = {1.0f, 2.0f / 3.0f, 1.0f / 3.0f};
+  for (int c = 0; c < 3; ++c) {
+    const float p = std::fabs(fractf(h + k[c]) * 6.0f - 3.0f);
+    out[c] = v * mixf(1.0f, clampf(p - 1.0f, 0.0f, 1.0f), s);
+  }
+}
+
+// Colour-mode overrides of the splat colour (chaos.wgsl:8625-8697): flow
+// (direction of this turn's displacement -> hue, saturated magnitude -> value),
+// trap distance (palette at a glow / ring coordinate of the distance to a
+// point, circle or line) and phase (arg z -> hue, log-modulus rings -> value),
+// each blended over the palette colour by its strength; alpha is kept.
+inline void colour_mode(const struct Uniforms& u, const float* palette, float sx, float sy,
+                        float p0x, float p0y, float pal[4]) {
+  float rgb[3];
+  float t;
+  if (u.color_mode == 1u) {
+    const float dx = sx - p0x, dy = sy - p0y;
+    const float mag = std::sqrt(dx * dx + dy * dy);
+    if (!(mag > 1e-20f)) return;
+    const float hue = (std::atan2(dy, dx) + kPi) / (2.0f * kPi);
+    const float val = clampf(1.0f - std::exp(-std::max(u.flow_scale, 0.0f) * mag), 0.0f, 1.0f);
+    hsv_to_rgb(hue, 1.0f, val, rgb);
+    t = clampf(u.flow_strength, 0.0f, 1.0f);
+  } else if (u.color_mode == 2u) {
+    const float qx = sx - u.trap_cx, qy = sy - u.trap_cy;
+    float d;
+    if (u.trap_kind == 0u) d = std::sqrt(qx * qx + qy * qy);
+    else if (u.trap_kind == 1u) d = std::fabs(std::sqrt(qx * qx + qy * qy) - u.trap_radius);
+    else d = std::fabs(qx * u.trap_nx + qy * u.trap_ny);
+    float tt;
+    if (u.trap_mode == 0u) tt = clampf(std::exp(-std::max(u.trap_falloff, 0.0f) * d), 0.0f, 1.0f);
+    else tt = fractf(d * std::max(u.trap_freq, 0.0f));
+    const uint32_t idx = std::min<uint32_t>(f2u(tt * 255.0f), 255u);
+    for (int c = 0; c < 3; ++c) rgb[c] = palette[idx * 4 + c];
+    t = clampf(u.trap_strength, 0.0f, 1.0f);
+  } else if (u.color_mode == 3u) {
+    const float mag = std::sqrt(sx * sx + sy * sy);
+    if (!(mag > 1e-20f)) return;
+    const float hue = (std::atan2(sy, sx) + kPi) / (2.0f * kPi);
+    const float ring = fractf(std::log(mag) * std::max(u.phase_freq, 0.0f));
+    hsv_to_rgb(hue, 1.0f, mixf(1.0f, 0.5f, ring), rgb);
+    t = clampf(u.phase_strength, 0.0f, 1.0f);
+  } else {
+    return;
+  }
+  for (int c = 0; c < 3; ++c) pal[c] = mixf(pal[c], rgb[c], t);
+}
 
 Uniforms read_uniforms(const uint32_t* u) {
   Uniforms r;
@@ -61,6 +126,10 @@ Uniforms read_uniforms(const uint32_t* u) {
   r.scale = f(4); r.cx = f(5); r.cy = f(6); r.seed = u[7];
   r.final_idx = (int32_t)u[8]; r.rotation = f(9); r.palette_mode = u[10];
   r.trace_mode = u[11]; r.walker_count = u[12]; r.jitter = f(13);
+  r.capture_index = u[14]; r.color_mode = u[15]; r.flow_strength = f(16); r.flow_scale = f(17);
+  r.trap_kind = u[18]; r.trap_mode = u[19]; r.trap_cx = f(20); r.trap_cy = f(21);
+  r.trap_radius = f(22); r.trap_nx = f(23); r.trap_ny = f(24); r.trap_falloff = f(25);
+  r.trap_freq = f(26); r.trap_strength = f(27); r.phase_strength = f(28); r.phase_freq = f(29);
   r.xform_blend = f(30);
   return r;
 }
@@ -169,7 +238,8 @@ inline void putu(float* t, int i, uint32_t u) { std::memcpy(&t[i], &u, 4); }
 // state (used to check per-step arithmetic without chaotic amplification).
 void chaos(const float* xforms, const uint32_t* distrib, int n_rows, const uint32_t* uni,
            const float* palette, uint32_t* hist, uint32_t wb, uint32_t we,
-           float* trace, int trace_walkers, int trace_iters, const float* force = nullptr) {
+           float* trace, int trace_walkers, int trace_iters, const float* force = nullptr,
+           uint32_t* idx_sum = nullptr) {
   const Uniforms u = read_uniforms(uni);
   const float cos_r = std::cos(u.rotation);
   const float sin_r = std::sin(u.rotation);
@@ -191,6 +261,7 @@ void chaos(const float* xforms, const uint32_t* distrib, int n_rows, const uint3
         const float* f = force + ((size_t)w * trace_iters + (i - 1)) * 16;
         x = f[12]; y = f[13]; z = f[9];
       }
+      const float p0x = x, p0y = y;  // position before this turn's chain
       int row = prev >= 0 ? prev : MAX_XFORMS;
       uint32_t raw = rng.irand();
       uint32_t fn = row_ptr(row)[raw & (GRAIN - 1)];
@@ -286,12 +357,15 @@ void chaos(const float* xforms, const uint32_t* distrib, int n_rows, const uint3
             for (int c = 0; c < 4; ++c) pal[c] = palette[pi * 4 + c];
           }
           if (dc_active) { pal[0] = dc_rgb[0]; pal[1] = dc_rgb[1]; pal[2] = dc_rgb[2]; }
+          colour_mode(u, palette, sx, sy, p0x, p0y, pal);
           float wgt = opacity * 255.0f;
           uint32_t* b = hist + ((size_t)yi * u.width + xi) * 4;
           add_sat(b[0], f2u(pal[0] * wgt));
           add_sat(b[1], f2u(pal[1] * wgt));
           add_sat(b[2], f2u(pal[2] * wgt));
           add_sat(b[3], f2u(wgt));
+          if (u.capture_index == 1u && idx_sum)
+            add_sat(idx_sum[(size_t)yi * u.width + xi], f2u(clampf(sz, 0.0f, 1.0f) * wgt));
         }
       }
     }
@@ -308,6 +382,14 @@ void orc_chaos(const float* xforms, const uint32_t* distrib, int n_rows, const u
                float* trace, int trace_walkers, int trace_iters) {
   orc::chaos(xforms, distrib, n_rows, uni, palette, hist, wb, we, trace, trace_walkers,
              trace_iters);
+}
+
+// As orc_chaos, with the capture buffer (one u32 per bin) for capture_index.
+void orc_chaos_capture(const float* xforms, const uint32_t* distrib, int n_rows,
+                       const uint32_t* uni, const float* palette, uint32_t* hist,
+                       uint32_t* idx_sum, uint32_t wb, uint32_t we) {
+  orc::chaos(xforms, distrib, n_rows, uni, palette, hist, wb, we, nullptr, 0, 0, nullptr,
+             idx_sum);
 }
 
 // Whole dispatch on `threads` std::threads: walkers split in contiguous

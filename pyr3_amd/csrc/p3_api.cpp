@@ -96,6 +96,7 @@ struct p3_renderer {
   std::vector<uint16_t> bounds_host, mbounds_host;  // staging kept alive past async copies
   std::vector<p3::ChaosArgs> margs_host;
   DevBuf<float> trace;
+  DevBuf<uint32_t> idx;         // colour-index capture (one u32 per bin), once requested
   DevBuf<float> mxforms;        // batched dispatch: per-sub-dispatch records
   DevBuf<uint32_t> mdistrib;    //   and pick tables
   DevBuf<uint16_t> mbounds;     //   and their step boundaries
@@ -217,7 +218,7 @@ void release(p3_renderer* r) {
   r->tile_rmax.release(); r->image8.release(); r->imagef.release(); r->taps.release();
   r->lut.release(); r->palette.release(); r->xforms.release(); r->distrib.release();
   r->bounds.release(); r->mbounds.release();
-  r->isaac.release(); r->trace.release();
+  r->isaac.release(); r->trace.release(); r->idx.release();
   r->mxforms.release(); r->mdistrib.release(); r->margs.release();
 }
 
@@ -253,6 +254,8 @@ int validate_dispatch(p3_renderer* r, const float* xforms, int n_slots, const ui
   final_idx = (int32_t)u[8];
   if (final_idx != -1 && (final_idx < 0 || final_idx >= n_slots))
     return r->fail(P3_E_ARG, "final xform slot %d outside %d slots", final_idx, n_slots);
+  if (u[15] > 3u) return r->fail(P3_E_ARG, "colour mode %u is not 0..3", u[15]);
+  if (u[14] > 1u) return r->fail(P3_E_ARG, "capture flag %u is not 0 or 1", u[14]);
   if ((int)u[0] != r->sw || (int)u[1] != r->sh)
     return r->fail(P3_E_ARG, "uniform size %ux%u does not match super-res grid %dx%d", u[0], u[1],
                    r->sw, r->sh);
@@ -290,7 +293,29 @@ p3::ChaosArgs dispatch_args(p3_renderer* r, const uint32_t* u, int n_regular, in
   a.walkers = u[12];
   a.jitter = bits_f(u[13]);
   a.xform_blend = bits_f(u[30]);
+  a.color_mode = u[15];
+  a.flow_strength = bits_f(u[16]); a.flow_scale = bits_f(u[17]);
+  a.trap_kind = u[18]; a.trap_mode = u[19];
+  a.trap_cx = bits_f(u[20]); a.trap_cy = bits_f(u[21]); a.trap_radius = bits_f(u[22]);
+  a.trap_nx = bits_f(u[23]); a.trap_ny = bits_f(u[24]); a.trap_falloff = bits_f(u[25]);
+  a.trap_freq = bits_f(u[26]); a.trap_strength = bits_f(u[27]);
+  a.phase_strength = bits_f(u[28]); a.phase_freq = bits_f(u[29]);
+  a.idx_sum = nullptr;
   return a;
+}
+
+// Capture buffer of the colour index (allocated and cleared on first use,
+// cleared by every reset afterwards) when the uniforms ask for it.
+int attach_capture(p3_renderer* r, p3::ChaosArgs& a, const uint32_t* u) {
+  if (u[14] != 1u) return P3_OK;
+  int rc;
+  if (!r->idx.p) {
+    if ((rc = r->hip(r->idx.ensure((size_t)r->sw * r->sh), "hipMalloc(capture)"))) return rc;
+    if ((rc = r->hip(p3::launch_fill(r->idx.p, r->idx.n * 4, 0u, r->stream), "clear capture")))
+      return rc;
+  }
+  a.idx_sum = r->idx.p;
+  return P3_OK;
 }
 
 // Trace buffer (cleared) for the configured walkers x turns, or null.
@@ -363,6 +388,7 @@ int iterate_impl(p3_renderer* r, const float* xforms, int n_slots, const uint32_
     a.isaac_in = r->isaac.p;
   }
   if ((rc = attach_trace(r, a))) return rc;
+  if ((rc = attach_capture(r, a, u))) return rc;
   // Each walker splats at most `iters` times, each adding <= 255 per word.
   r->hist_bound += 255ULL * (unsigned long long)walkers * (unsigned long long)a.iters;
   const bool sat = r->hist_bound >= 0xffffffffULL;
@@ -444,6 +470,7 @@ int iterate_multi_impl(p3_renderer* r, int n_sub, const float* const* xforms, in
       args[k].pick_bounds = r->mbounds.p + nb * k;
     }
     if (k == 0 && (rc = attach_trace(r, args[0]))) return rc;
+    if ((rc = attach_capture(r, args[k], uniforms[k]))) return rc;
     const uint32_t blk = (uint32_t)p3::chaos_block();
     M.start[k + 1] = M.start[k] + (args[k].walkers + blk - 1) / blk;
     bound += 255ULL * (unsigned long long)args[k].walkers * (unsigned long long)args[k].iters;
@@ -545,6 +572,9 @@ int p3_reset(p3_renderer* r, const float* palette_rgba) {
   r->tic(T_RESET);
   if ((rc = r->hip(p3::launch_fill(r->hist.p, r->hist.n * 4, 0u, r->stream), "clear histogram")))
     return rc;
+  if (r->idx.p &&
+      (rc = r->hip(p3::launch_fill(r->idx.p, r->idx.n * 4, 0u, r->stream), "clear capture")))
+    return rc;
   r->toc(T_RESET);
   r->palette_set = true;
   r->hist_bound = 0;
@@ -627,6 +657,13 @@ int p3_read_histogram(p3_renderer* r, uint32_t* out, size_t n) {
   if (!r) return P3_E_ARG;
   if (!out || n != r->hist.n) return r->fail(P3_E_ARG, "histogram needs %zu words", r->hist.n);
   return read_dev(r, out, r->hist.p, n * 4, "read histogram");
+}
+
+int p3_read_index(p3_renderer* r, uint32_t* out, size_t n) {
+  if (!r) return P3_E_ARG;
+  if (!r->idx.p) return r->fail(P3_E_STATE, "no dispatch has captured the colour index");
+  if (!out || n != r->idx.n) return r->fail(P3_E_ARG, "capture buffer needs %zu words", r->idx.n);
+  return read_dev(r, out, r->idx.p, n * 4, "read capture");
 }
 
 int p3_read_filtered(p3_renderer* r, float* out, size_t n) {

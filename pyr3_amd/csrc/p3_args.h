@@ -36,6 +36,13 @@ struct ChaosArgs {
   float scale, cx, cy, cos_r, sin_r, jitter, xform_blend;
   uint32_t palette_mode;
   uint32_t trace_walkers, trace_iters;
+  // splat colour modes (0 palette, 1 flow, 2 trap distance, 3 phase) and
+  // their parameters; capture of the colour index into idx_sum (or null)
+  uint32_t color_mode, trap_kind, trap_mode;
+  float flow_strength, flow_scale;
+  float trap_cx, trap_cy, trap_radius, trap_nx, trap_ny, trap_falloff, trap_freq, trap_strength;
+  float phase_strength, phase_freq;
+  uint32_t* idx_sum;
 };
 
 // Batched launch: up to kMaxSub sub-dispatches of one genome shape in one

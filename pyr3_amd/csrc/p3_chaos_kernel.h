@@ -94,6 +94,69 @@ P3_INLINE void hist_add4(P3_GLOBAL uint32_t* b, uint32_t d0, uint32_t d1, uint32
   }
 }
 
+// chaos.wgsl:1662-1666: k = (1, 2/3, 1/3), p = |fract(h + k) * 6 - 3|,
+// rgb = v * mix(1, clamp(p - 1, 0, 1), s); fract(x) = x - floor(x).
+P3_INLINE float3 hsv_to_rgb(float h, float s, float v) {
+  const float k[3] = {1.0f, 2.0f / 3.0f, 1.0f / 3.0f};
+  float o[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float f = h + k[c];
+    const float p = fabsf((f - floorf(f)) * 6.0f - 3.0f);
+    o[c] = v * mixf(1.0f, fminf(fmaxf(p - 1.0f, 0.0f), 1.0f), s);
+  }
+  return make_float3(o[0], o[1], o[2]);
+}
+
+// Colour-mode overrides of the splat colour (chaos.wgsl:8625-8697): flow
+// (direction of this turn's displacement from (p0x, p0y) -> hue, saturated
+// magnitude -> value), trap distance (palette at a glow / ring coordinate of
+// the distance to a point, circle or line) and phase (arg z -> hue,
+// log-modulus rings -> value), blended over the palette colour by strength.
+P3_INLINE void colour_mode(const ChaosArgs& A, float sx, float sy, float p0x, float p0y,
+                           float4& pal) {
+  float3 rgb;
+  float t;
+  if (A.color_mode == 1u) {
+    const float dx = sx - p0x, dy = sy - p0y;
+    const float mag = sqrtf(dx * dx + dy * dy);
+    if (!(mag > 1e-20f)) return;
+    const float hue = (atan2f(dy, dx) + kPi) / (2.0f * kPi);
+    const float val = fminf(fmaxf(1.0f - expf(-fmaxf(A.flow_scale, 0.0f) * mag), 0.0f), 1.0f);
+    rgb = hsv_to_rgb(hue, 1.0f, val);
+    t = fminf(fmaxf(A.flow_strength, 0.0f), 1.0f);
+  } else if (A.color_mode == 2u) {
+    const float qx = sx - A.trap_cx, qy = sy - A.trap_cy;
+    float d;
+    if (A.trap_kind == 0u) d = sqrtf(qx * qx + qy * qy);
+    else if (A.trap_kind == 1u) d = fabsf(sqrtf(qx * qx + qy * qy) - A.trap_radius);
+    else d = fabsf(qx * A.trap_nx + qy * A.trap_ny);
+    float tt;
+    if (A.trap_mode == 0u) {
+      tt = fminf(fmaxf(expf(-fmaxf(A.trap_falloff, 0.0f) * d), 0.0f), 1.0f);
+    } else {
+      const float f = d * fmaxf(A.trap_freq, 0.0f);
+      tt = f - floorf(f);
+    }
+    const uint32_t idx = min(f2u_sat(tt * 255.0f), 255u);
+    const float4 c = ((const P3_CONST float4*)A.palette)[idx];
+    rgb = make_float3(c.x, c.y, c.z);
+    t = fminf(fmaxf(A.trap_strength, 0.0f), 1.0f);
+  } else if (A.color_mode == 3u) {
+    const float mag = sqrtf(sx * sx + sy * sy);
+    if (!(mag > 1e-20f)) return;
+    const float hue = (atan2f(sy, sx) + kPi) / (2.0f * kPi);
+    const float f = logf(mag) * fmaxf(A.phase_freq, 0.0f);
+    rgb = hsv_to_rgb(hue, 1.0f, mixf(1.0f, 0.5f, f - floorf(f)));
+    t = fminf(fmaxf(A.phase_strength, 0.0f), 1.0f);
+  } else {
+    return;
+  }
+  pal.x = mixf(pal.x, rgb.x, t);
+  pal.y = mixf(pal.y, rgb.y, t);
+  pal.z = mixf(pal.z, rgb.z, t);
+}
+
 // Per-walker loop state that stays with the walker between turns.
 struct Walker {
   float x, y, z;
@@ -108,6 +171,7 @@ template <bool kSat, class Spec>
 P3_INLINE void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32_t fn,
                                 uint32_t raw, uint32_t wid) {
   const uint32_t i = wk.iter;
+  const float p0x = wk.x, p0y = wk.y;   // position before this turn's chain
   const xrec_t X = (xrec_t)A.xforms;
   const P3_CONST uint32_t* D = (const P3_CONST uint32_t*)A.distrib;
   const xrec_t xf = X + (size_t)fn * kXformFloats;
@@ -208,10 +272,19 @@ P3_INLINE void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32_t fn,
         pal = PAL[pi];
       }
       if (dc_on) { pal.x = dc_rgb.x; pal.y = dc_rgb.y; pal.z = dc_rgb.z; }
+      if (A.color_mode != 0u) colour_mode(A, sx, sy, p0x, p0y, pal);
       const float wgt = opacity * 255.0f;
       P3_GLOBAL uint32_t* b = (P3_GLOBAL uint32_t*)A.hist + ((size_t)yi * A.width + xi) * 4u;
       hist_add4<kSat, Spec::kPairAdds>(b, f2u_sat(pal.x * wgt), f2u_sat(pal.y * wgt),
                                        f2u_sat(pal.z * wgt), f2u_sat(wgt));
+      if (A.idx_sum != nullptr) {   // capture of the colour index (chaos.wgsl:8720-8728)
+        const uint32_t d = f2u_sat(fminf(fmaxf(sz, 0.0f), 1.0f) * wgt);
+        P3_GLOBAL uint32_t* q = (P3_GLOBAL uint32_t*)A.idx_sum + (size_t)yi * A.width + xi;
+        if (d) {
+          const uint32_t o = add_ret(q, d);
+          if (o > 0xffffffffu - d) pin_max(q);
+        }
+      }
     }
   }
 }

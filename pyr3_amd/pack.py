@@ -24,6 +24,7 @@ from __future__ import annotations
 import bisect
 import math
 import struct
+from dataclasses import dataclass
 from typing import List, Tuple
 
 import numpy as np
@@ -268,10 +269,40 @@ def _f32_bits(v: float) -> int:
     return struct.unpack("<I", struct.pack("<f", v))[0]
 
 
+COLOR_MODES = {"palette": 0, "flow": 1, "trap-distance": 2, "phase": 3}
+DEFAULT_FLOW_SCALE = 2.0             # src/chaos.ts:139
+
+
+@dataclass
+class ColorOpts:
+    """Splat-colour options of a dispatch (src/chaos.ts:60-80, 421-450): the
+    colour mode, its parameters, and capture of the colour index."""
+    mode: str = "palette"
+    flow_strength: float = 1.0
+    flow_scale: float = DEFAULT_FLOW_SCALE
+    trap_kind: str = "point"         # point | circle | line
+    trap_mode: str = "glow"          # glow | rings
+    trap_cx: float = 0.0
+    trap_cy: float = 0.0
+    trap_radius: float = 0.5
+    trap_angle: float = 0.0          # degrees (line normal (-sin, cos))
+    trap_falloff: float = 2.0
+    trap_freq: float = 4.0
+    trap_strength: float = 1.0
+    phase_strength: float = 1.0
+    phase_freq: float = 1.0
+    capture_index: bool = False
+
+
 def chaos_uniforms(g: Genome, width_s: int, height_s: int, oversample: int, iters: int,
                    fuse: int, seed: int, walkers: int, jitter: float = DEFAULT_WALKER_JITTER,
-                   trace: bool = False) -> np.ndarray:
-    """The 128-byte chaos uniform block (palette colour mode, capture off)."""
+                   trace: bool = False, color: "ColorOpts" = None) -> np.ndarray:
+    """The 128-byte chaos uniform block (src/chaos.ts:381-456)."""
+    c = color or ColorOpts()
+    if c.mode not in COLOR_MODES:
+        raise ValueError(f"color mode must be one of {sorted(COLOR_MODES)}, got {c.mode!r}")
+    if c.trap_kind not in ("point", "circle", "line") or c.trap_mode not in ("glow", "rings"):
+        raise ValueError(f"bad trap kind/mode: {c.trap_kind!r}/{c.trap_mode!r}")
     u = np.zeros(CHAOS_UNIFORM_WORDS, dtype=np.uint32)
     u[0], u[1], u[2], u[3] = width_s, height_s, iters, fuse
     u[4] = _f32_bits(g.scale * oversample)
@@ -284,15 +315,17 @@ def chaos_uniforms(g: Genome, width_s: int, height_s: int, oversample: int, iter
     u[11] = 1 if trace else 0
     u[12] = walkers
     u[13] = _f32_bits(jitter)
-    u[16] = _f32_bits(1.0)          # flow_strength default
-    u[17] = _f32_bits(1.0)          # flow_scale (unused in palette mode)
-    u[22] = _f32_bits(0.5)          # trap radius default
-    u[24] = _f32_bits(1.0)          # trap ny = cos(0)
-    u[25] = _f32_bits(2.0)
-    u[26] = _f32_bits(4.0)
-    u[27] = _f32_bits(1.0)
-    u[28] = _f32_bits(1.0)
-    u[29] = _f32_bits(1.0)
+    u[14] = 1 if c.capture_index else 0
+    u[15] = COLOR_MODES[c.mode]
+    u[16] = _f32_bits(c.flow_strength)
+    u[17] = _f32_bits(c.flow_scale)
+    u[18] = {"point": 0, "circle": 1, "line": 2}[c.trap_kind]
+    u[19] = 1 if c.trap_mode == "rings" else 0
+    ang = c.trap_angle * math.pi / 180.0
+    for i, v in ((20, c.trap_cx), (21, c.trap_cy), (22, c.trap_radius), (23, -math.sin(ang)),
+                 (24, math.cos(ang)), (25, c.trap_falloff), (26, c.trap_freq),
+                 (27, c.trap_strength), (28, c.phase_strength), (29, c.phase_freq)):
+        u[i] = _f32_bits(v)
     u[30] = _f32_bits(min(1.0, max(0.0, g.xform_blend or 0.0)))
     return u
 

@@ -99,19 +99,21 @@ class Renderer:
 
     def chaos_uniforms(self, genome: Genome, seed: int, walkers: int, iters: int,
                        fuse: int = pack.FUSE, walker_jitter: Optional[float] = None,
-                       trace: bool = False) -> np.ndarray:
+                       trace: bool = False, color: Optional[pack.ColorOpts] = None) -> np.ndarray:
         jit = pack.DEFAULT_WALKER_JITTER if walker_jitter is None else walker_jitter
         return pack.chaos_uniforms(genome, self.super_w, self.super_h, self.oversample, iters,
-                                   fuse, seed, walkers, jit, trace)
+                                   fuse, seed, walkers, jit, trace, color)
 
     def iterate(self, genome: Genome, seed: int, walkers: int, iters_per_walker: int,
                 walker_jitter: Optional[float] = None, fuse: int = pack.FUSE,
-                packed: Optional[PackedGenome] = None, isaac_states: Optional[np.ndarray] = None
-                ) -> int:
-        """One chaos dispatch; returns the samples it contributes (walkers * iters)."""
+                packed: Optional[PackedGenome] = None, isaac_states: Optional[np.ndarray] = None,
+                color: Optional[pack.ColorOpts] = None) -> int:
+        """One chaos dispatch; returns the samples it contributes (walkers * iters).
+        `color` selects the splat colour mode and the colour-index capture."""
         pg = packed or pack_genome(genome)
         u = np.ascontiguousarray(self.chaos_uniforms(pg.expanded, seed, walkers,
-                                                     iters_per_walker, fuse, walker_jitter))
+                                                     iters_per_walker, fuse, walker_jitter,
+                                                     color=color))
         n_slots, n_rows = pg.xforms.shape[0], pg.distrib.shape[0]
         if isaac_states is None:
             rc = self._lib.p3_iterate(self._h, _fp(pg.xforms), n_slots, _up(pg.distrib), n_rows,
@@ -190,6 +192,12 @@ class Renderer:
     def write_histogram(self, hist: np.ndarray) -> None:
         h = np.ascontiguousarray(hist, dtype=np.uint32)
         check(self._lib.p3_write_histogram(self._h, _up(h), h.size), self._h)
+
+    def read_index(self):
+        """(H_s, W_s) u32 colour-index sums of the capturing dispatches."""
+        out = np.zeros((self.super_h, self.super_w), dtype=np.uint32)
+        check(self._lib.p3_read_index(self._h, _up(out), out.size), self._h)
+        return out
 
     def read_filtered(self):
         out = np.empty((self.super_h, self.super_w, 4), dtype=np.float32)
