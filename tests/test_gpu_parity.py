@@ -100,6 +100,22 @@ def test_pick_paths_bit_exact(orc, R, case):
     assert np.array_equal(hg, ho)
 
 
+def test_xform_blend_bit_exact(orc, R):
+    """Blended turns (gate, fallback-row pick, second chain, mix draw) on
+    linear maps: draws, picks, trajectories and histogram bit-exact."""
+    g = genome(sierpinski(), scale=10.0, cx=0.5, cy=0.5, xform_blend=0.5)
+    W = H = 32
+    walkers, iters, fuse = 512, 30, 5
+    _, hg, tg = _render_hist_gpu(R, g, W, H, iters=iters, fuse=fuse, seed=0xB1E4D,
+                                 walkers=walkers, trace=(walkers, iters + fuse))
+    xf, ds, u, pal = blobs(g, W, H, iters=iters, fuse=fuse, seed=0xB1E4D, walkers=walkers)
+    ho, to = orc.chaos(xf, ds, u, pal, trace_shape=(walkers, iters + fuse))
+    assert np.array_equal(tg[..., 10].view(np.uint32), to[..., 10].view(np.uint32))
+    assert np.array_equal(tg[..., 1], to[..., 1])
+    assert np.array_equal(tg[..., 12:14], to[..., 12:14])
+    assert np.array_equal(hg, ho)
+
+
 def test_isaac_explicit_states_match_device_seeding(orc, R):
     g = genome(sierpinski(), scale=10.0, cx=0.5, cy=0.5)
     W = H = 16

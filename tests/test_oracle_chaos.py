@@ -10,6 +10,8 @@ these pin it by what the algorithm fixes:
     opacity gate when the final's opacity is not 1); a bad turn adds the two
     reseed draws and retries up to four times without advancing i (except at
     i = 0), the fifth bad turn is accepted (chaos.wgsl:8391-8455);
+  * with xform blend on, each turn adds its gate draw, and when the gate
+    passes a fallback-row pick and the mix draw (chaos.wgsl:8372-8381);
   * saturation pins at 2^32 - 1 (chaos.wgsl:8120-8135).
 """
 import numpy as np
@@ -87,6 +89,21 @@ def test_draw_contract_final_opacity_gate(orc):
     xf, ds, u, pal = blobs(g, 16, 16, iters=30, fuse=5, walkers=4)
     _, tr = orc.chaos(xf, ds, u, pal, trace_shape=(4, 35))
     assert np.all(_draws_per_turn(tr[0]) == 3)
+
+
+def test_draw_contract_xform_blend(orc):
+    # blend gate draw, then (when it passes) a fallback-row pick draw, the
+    # second chain's draws (none: linear maps) and the mix draw
+    # (chaos.wgsl:8372-8381)
+    g = genome(sierpinski(), scale=10.0, xform_blend=1.0)
+    xf, ds, u, pal = blobs(g, 16, 16, iters=30, fuse=5, walkers=4)
+    _, tr = orc.chaos(xf, ds, u, pal, trace_shape=(4, 35))
+    assert np.all(_draws_per_turn(tr[0]) == 6)
+    g.xform_blend = 0.5
+    xf, ds, u, pal = blobs(g, 16, 16, iters=30, fuse=5, walkers=4)
+    _, tr = orc.chaos(xf, ds, u, pal, trace_shape=(4, 35))
+    d = np.concatenate([_draws_per_turn(tr[w]) for w in range(4)])
+    assert set(np.unique(d)) == {4, 6}
 
 
 def test_retry_window_all_bad(orc):
