@@ -464,9 +464,14 @@ def parse_flame(text: str) -> FlameImport:
 
 
 def load_genome(path: str, text: Optional[str] = None) -> Tuple[Genome, str, Optional[ImportReport]]:
-    """Genome from a `.flam3` / `.flame` / `.pyr3.json` file (src/loader.ts:32-40
-    sniffing: suffix, then a leading '<' means XML). Returns (genome, kind, report)."""
+    """Genome from a `.flam3` / `.flame` / `.pyr3.json` file or a `.png` saved
+    with its genome (src/loader.ts:32-66 sniffing: suffix, then a leading '<'
+    means XML). Returns (genome, kind, report)."""
     from .genome import genome_from_text
+    if path.lower().endswith(".png"):
+        from .png import genome_from_png
+        with open(path, "rb") as fh:
+            return genome_from_png(fh.read()), "pyr3-png", None
     if text is None:
         with open(path, encoding="utf-8") as fh:
             text = fh.read()

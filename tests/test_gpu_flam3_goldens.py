@@ -54,3 +54,22 @@ def test_flam3_golden_within_reference_threshold(fid):
     print(f"[flam3 golden] {fid}: R={R:.4f} (reference expectedR {meta['expectedR']}, "
           f"threshold {meta['thresholdR']})", flush=True)
     assert R <= meta["thresholdR"], (fid, R, meta["thresholdR"])
+
+
+def test_render_cli_writes_png_with_genome(tmp_path):
+    """python -m pyr3_amd.render on one fixture: the PNG decodes to a frame
+    within the fixture's threshold and carries the genome it was made from."""
+    import sys
+    sys.path.insert(0, str(GOLDEN))
+    from png8 import read_png
+    from pyr3_amd.genome import genome_to_json
+    from pyr3_amd.render import main
+    fid = "244.57686"
+    d = GOLDEN / "flam3" / fid
+    out = tmp_path / "out.png"
+    assert main([str(d / f"{fid}.flam3"), str(out), "--seed", "7"]) == 0
+    img = read_png(out)
+    meta = json.loads((d / "meta.json").read_text())
+    assert mean_abs_diff_rgba(img, read_png(d / "golden.png")) <= meta["thresholdR"]
+    g_png = load_genome(str(out))[0]
+    assert genome_to_json(g_png) == genome_to_json(load_genome(str(d / f"{fid}.flam3"))[0])
