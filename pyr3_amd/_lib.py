@@ -7,6 +7,7 @@ so a host without the HIP extension stops instead of silently degrading.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libpyr3hip.so"
@@ -62,6 +63,9 @@ def load() -> C.CDLL:
         raise ImportError(
             f"pyr3_amd: {LIB_PATH} is missing; build it with `python -m pyr3_amd.build` "
             "(there is no CPU fallback for the render path)")
+    # Compiled per-shape chaos kernels persist in the package's kcache/ unless
+    # P3_KERNEL_CACHE names another directory ("" turns the cache off).
+    os.environ.setdefault("P3_KERNEL_CACHE", str(LIB_PATH.parent / "kcache"))
     lib = C.CDLL(str(LIB_PATH))
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)
