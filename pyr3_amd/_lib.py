@@ -30,6 +30,7 @@ SIGNATURES = {
     "p3_present": (_I, [_P, _UP, _UP]),
     "p3_read_histogram": (_I, [_P, _UP, _SZ]),
     "p3_read_index": (_I, [_P, _UP, _SZ]),
+    "p3_set_kernel_policy": (_I, [_P, _I]),
     "p3_read_filtered": (_I, [_P, _FP, _SZ]),
     "p3_read_image": (_I, [_P, _BP, _SZ]),
     "p3_read_image_f32": (_I, [_P, _FP, _SZ]),
@@ -46,6 +47,7 @@ SIGNATURES = {
     "p3_sync": (_I, [_P]),
     "p3_dims": (_I, [_P, C.POINTER(_I)]),
     "p3_chaos_prepare": (_I, [_FP, _I, _I, _I]),
+    "p3_chaos_ready": (_I, [_FP, _I, _I, _I]),
     "p3_chaos_source": (_I, [_FP, _I, _I, _I, C.c_char_p, _SZ, C.POINTER(_SZ)]),
     "p3_variation_supported": (_I, [_I]),
     "p3_last_error": (C.c_char_p, [_P]),

@@ -111,9 +111,29 @@ def precompile(genomes: Sequence[Genome], workers: Optional[int] = None,
     return len(shapes)
 
 
+def compile_pool(workers: Optional[int] = None) -> ProcessPoolExecutor:
+    """A pool of compile worker processes, every worker started before it
+    returns. Create it before this process first touches the GPU: the
+    workers are spawned (fresh interpreters), never forked from a process
+    holding a GPU context."""
+    workers = workers or max(1, min(16, (os.cpu_count() or 2) - 1))
+    ex = ProcessPoolExecutor(max_workers=workers, mp_context=get_context("spawn"))
+    for f in [ex.submit(os.getpid) for _ in range(4 * workers)]:
+        f.result()
+    return ex
+
+
 class BatchRenderer:
     def __init__(self, n_streams: int = 2, compile_workers: Optional[int] = None,
-                 cache_dir: Optional[str] = None):
+                 cache_dir: Optional[str] = None, pool: Optional[ProcessPoolExecutor] = None,
+                 kernels: str = "specialised"):
+        """kernels='specialised': compile every shape (worker processes) and
+        render each job as soon as its kernel is ready; 'generic': render
+        every job at once with the library's generic kernel (no compiles —
+        the choice for batches of one-off shapes)."""
+        if kernels not in ("specialised", "generic"):
+            raise ValueError("kernels must be 'specialised' or 'generic'")
+        self.kernels = kernels
         from .renderer import Renderer
         self._Renderer = Renderer
         self.cache_dir = cache_dir or os.environ.get("P3_KERNEL_CACHE") or os.path.join(
@@ -121,6 +141,7 @@ class BatchRenderer:
         os.environ["P3_KERNEL_CACHE"] = self.cache_dir
         self.compile_workers = compile_workers or max(1, min(16, (os.cpu_count() or 2) - 1))
         self.n_streams = max(1, n_streams)
+        self._workers = pool     # caller-owned compile pool (see compile_pool), or None
         self._pool: List = []
         self.timings = {}
 
@@ -130,12 +151,17 @@ class BatchRenderer:
         r = self._pool[slot]
         if r is None:
             r = self._Renderer(w, h, os_)
+            if self.kernels == "generic":
+                r.set_kernel_policy("generic")
             self._pool[slot] = r
         else:
             r.resize(w, h, os_)
         return r
 
-    def render(self, jobs: Sequence[Job], read_images: bool = False) -> List[JobResult]:
+    def render(self, jobs: Sequence[Job], read_images: bool = False,
+               progress=None) -> List[JobResult]:
+        """Render every job; `progress(k)` (optional) is called after every
+        64th job."""
         t0 = time.perf_counter()
         # distinct shapes -> compile jobs (worker processes, no GPU)
         packed = []
@@ -149,8 +175,22 @@ class BatchRenderer:
         by_key = {}
         for i, j in enumerate(jobs):
             by_key.setdefault(_shape_key(j.genome), []).append(i)
-        ctx = get_context("spawn")
-        with ProcessPoolExecutor(max_workers=self.compile_workers, mp_context=ctx) as ex:
+        if self.kernels == "generic":
+            results = []
+            for i, j in enumerate(jobs):
+                results.append(self._render_one(j, packed[i], i, read_images))
+                if progress and (i + 1) % 64 == 0:
+                    progress(i + 1)
+            for r in self._pool:
+                if r is not None:
+                    r.sync()
+            self.timings = {"total_s": time.perf_counter() - t0, "pack_s": 0.0,
+                            "shapes": len(shapes), "jobs": len(jobs)}
+            return results
+        own = self._workers is None
+        ex = (ProcessPoolExecutor(max_workers=self.compile_workers, mp_context=get_context("spawn"))
+              if own else self._workers)
+        try:
             futs = {}
             for key, pg in shapes.items():
                 e = pg.expanded
@@ -167,6 +207,11 @@ class BatchRenderer:
                     results[i] = self._render_one(jobs[i], packed[i], n_done, read_images)
                     order.append(i)
                     n_done += 1
+                    if progress and n_done % 64 == 0:
+                        progress(n_done)
+        finally:
+            if own:
+                ex.shutdown()
         for r in self._pool:
             if r is not None:
                 r.sync()

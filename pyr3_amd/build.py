@@ -17,7 +17,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libpyr3hip.so"
-SOURCES = ["p3_api.cpp", "p3_rtc.cpp", "p3_image.hip", "p3_denoise.hip"]
+SOURCES = ["p3_api.cpp", "p3_rtc.cpp", "p3_image.hip", "p3_denoise.hip", "p3_chaos_generic.hip"]
 # Device headers embedded as text for the run-time chaos specialisation, in
 # dependency order (their #include lines are dropped).
 RTC_HEADERS = ["p3_args.h", "p3_device.h", "p3_variations.h", "p3_variations_ext.h",
@@ -50,21 +50,39 @@ def hipcc() -> str:
     return str(Path(rocm) / "bin" / "hipcc")
 
 
-def command(out: Path = LIB):
-    return [
-        hipcc(), "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
-        "-ffp-contract=off",
-        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-        "-Wall", "-Wno-unused-function",
-        "-I", str(PKG.parent / "include"),
-        "-o", str(out),
-    ] + [str(CSRC / s) for s in SOURCES] + ["-lhiprtc"]
+FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-Wall", "-Wno-unused-function"]
+
+
+def compile_command(src: str, obj: Path):
+    return [hipcc()] + FLAGS + ["-I", str(PKG.parent / "include"), "-c", "-o", str(obj),
+                                str(CSRC / src)]
+
+
+def link_command(objs, out: Path = LIB):
+    return [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(out)] + \
+        [str(o) for o in objs] + ["-lhiprtc"]
 
 
 def build(verbose: bool = True) -> Path:
+    """Compile every source to an object (in parallel: the generic chaos
+    kernel alone takes about a minute), then link the shared library."""
     write_rtc_source()
+    objdir = PKG / "_objs"
+    objdir.mkdir(exist_ok=True)
+    objs = [objdir / (Path(s).stem + ".o") for s in SOURCES]
+    procs = []
+    for s, o in zip(SOURCES, objs):
+        cmd = compile_command(s, o)
+        if verbose:
+            print("[pyr3_amd.build]", " ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd))
+    failed = [s for s, p in zip(SOURCES, procs) if p.wait() != 0]
+    if failed:
+        raise RuntimeError(f"pyr3_amd.build: compile failed for {', '.join(failed)}")
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = command(tmp)
+    cmd = link_command(objs, tmp)
     if verbose:
         print("[pyr3_amd.build]", " ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

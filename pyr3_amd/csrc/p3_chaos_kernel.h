@@ -94,6 +94,12 @@ P3_INLINE void hist_add4(P3_GLOBAL uint32_t* b, uint32_t d0, uint32_t d1, uint32
   }
 }
 
+// Palette entry i (four floats read through the constant address space).
+P3_INLINE float4 palette_at(const ChaosArgs& A, uint32_t i) {
+  const P3_CONST float* p = (const P3_CONST float*)A.palette + 4u * i;
+  return make_float4(p[0], p[1], p[2], p[3]);
+}
+
 // chaos.wgsl:1662-1666: k = (1, 2/3, 1/3), p = |fract(h + k) * 6 - 3|,
 // rgb = v * mix(1, clamp(p - 1, 0, 1), s); fract(x) = x - floor(x).
 P3_INLINE float3 hsv_to_rgb(float h, float s, float v) {
@@ -139,7 +145,7 @@ P3_INLINE void colour_mode(const ChaosArgs& A, float sx, float sy, float p0x, fl
       tt = f - floorf(f);
     }
     const uint32_t idx = min(f2u_sat(tt * 255.0f), 255u);
-    const float4 c = ((const P3_CONST float4*)A.palette)[idx];
+    const float4 c = palette_at(A, idx);
     rgb = make_float3(c.x, c.y, c.z);
     t = fminf(fmaxf(A.trap_strength, 0.0f), 1.0f);
   } else if (A.color_mode == 3u) {
@@ -222,17 +228,18 @@ P3_INLINE void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32_t fn,
   if (wk.consec >= 5u) wk.consec = 0u;
   wk.prev = (int)fn;
   float sx = ppx, sy = ppy, sz = ppz;
-  if (Spec::kHasFinal) {
-    const xrec_t fx = X + (size_t)Spec::kFinalSlot * kXformFloats;
+  if (Spec::has_final(A)) {
+    const int fslot = Spec::final_slot(A);
+    const xrec_t fx = X + (size_t)fslot * kXformFloats;
     const bool apply = fx[10] == 1.0f ? true : (rng.rand01() < fx[10]);
     // During the fuse the final's output feeds neither the walker nor a
     // splat, so a final that draws nothing is left out there (its opacity
     // gate draw above is kept).
-    const bool needed = !Spec::kFinalDrawFree || i >= A.fuse || tr != nullptr;
+    const bool needed = !Spec::final_draw_free(A) || i >= A.fuse || tr != nullptr;
     if (apply && needed) {
       const float2 fpa = make_float2(fx[0] * pv.x + fx[1] * pv.y + fx[2],
                                      fx[4] * pv.x + fx[5] * pv.y + fx[6]);
-      const float2 fpv = Spec::final_sum(X, fpa, rng, dc_on, dc_rgb);
+      const float2 fpv = Spec::final_sum(X, fslot, fpa, rng, dc_on, dc_rgb);
       const float fz = mixf(nz, fx[8], fx[9]);
       if (!is_bad(fpv)) { sx = fpv.x; sy = fpv.y; sz = fz; }
     }
@@ -257,19 +264,18 @@ P3_INLINE void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32_t fn,
     // floor(p) in [0, W) <=> p in [0, W) for finite p; NaN fails both tests.
     if (px >= 0.0f && px < (float)A.width && py >= 0.0f && py < (float)A.height) {
       const uint32_t xi = (uint32_t)floorf(px), yi = (uint32_t)floorf(py);
-      const P3_CONST float4* PAL = (const P3_CONST float4*)A.palette;
       const float cxf = fminf(fmaxf(sz, 0.0f), 1.0f) * 255.0f;
       float4 pal;
       if (A.palette_mode == 1u) {
         uint32_t i0 = f2u_sat(cxf);
         float frac = cxf - (float)i0;
         if (i0 >= 255u) { i0 = 254u; frac = 1.0f; }
-        const float4 p0 = PAL[i0], p1 = PAL[i0 + 1u];
+        const float4 p0 = palette_at(A, i0), p1 = palette_at(A, i0 + 1u);
         pal = make_float4(mixf(p0.x, p1.x, frac), mixf(p0.y, p1.y, frac),
                           mixf(p0.z, p1.z, frac), mixf(p0.w, p1.w, frac));
       } else {
         const uint32_t pi = min(f2u_sat(cxf), 255u);
-        pal = PAL[pi];
+        pal = palette_at(A, pi);
       }
       if (dc_on) { pal.x = dc_rgb.x; pal.y = dc_rgb.y; pal.z = dc_rgb.z; }
       if (A.color_mode != 0u) colour_mode(A, sx, sy, p0x, p0y, pal);
@@ -306,14 +312,19 @@ P3_INLINE uint32_t draw_pick(const ChaosArgs& A, const BlockLds<kB>& L, Rng& rng
   const uint32_t row = prev >= 0 ? (uint32_t)prev : A.fallback_row;
   raw = rng.irand();
   const uint32_t col = raw & (kGrain - 1);
-  if (Spec::kRegular < kPickRows && A.pick_bounds != nullptr) {
+  // kRegular < 0: the xform count is only known at run time; boundaries past
+  // the last xform are 16384, above every column, so all 15 can be counted
+  constexpr int kCmp = Spec::kRegular < 0 ? kPickCols - 1 : Spec::kRegular - 1;
+  const bool lds = Spec::kRegular < 0 ? A.fallback_row < (uint32_t)kPickRows
+                                      : Spec::kRegular < kPickRows;
+  if (lds && A.pick_bounds != nullptr) {
     const uint4* b = (const uint4*)(L.pick_b + row * kPickCols);
     const uint4 q0 = b[0];
-    const uint4 q1 = Spec::kRegular > 9 ? b[1] : make_uint4(0u, 0u, 0u, 0u);
+    const uint4 q1 = kCmp > 8 ? b[1] : make_uint4(0u, 0u, 0u, 0u);
     const uint32_t w[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
     uint32_t f = 0u;
 #pragma unroll
-    for (int k = 0; k < Spec::kRegular - 1; ++k)
+    for (int k = 0; k < kCmp; ++k)
       f += col >= ((w[k >> 1] >> (16 * (k & 1))) & 0xffffu) ? 1u : 0u;
     return f;
   }
@@ -340,7 +351,8 @@ P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L, uint32_t block)
   const uint32_t idle = A.fallback_row + 1u;   // bucket of finished walkers
 
   for (int k = t; k < 2 * (kMaxXforms + 2); k += kB) (&L.bucket[0][0])[k] = 0u;
-  if (Spec::kRegular < kPickRows && A.pick_bounds != nullptr)
+  if ((Spec::kRegular < 0 ? A.fallback_row < (uint32_t)kPickRows : Spec::kRegular < kPickRows) &&
+      A.pick_bounds != nullptr)
     for (uint32_t k = t; k < (A.fallback_row + 1u) * kPickCols; k += kB)
       L.pick_b[k] = ((const P3_CONST uint16_t*)A.pick_bounds)[k];
   __syncthreads();

@@ -22,6 +22,7 @@
 #include <mutex>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -40,6 +41,17 @@ struct Module {
 std::mutex g_mu;
 std::unordered_map<std::string, std::vector<char>> g_code;  // key: shape
 std::unordered_map<std::string, Module> g_mods;             // key: device + shape
+
+// Background compiles (chaos_compile_async). Declared after the maps above,
+// so at unload it is destroyed first and joins its threads while the maps
+// and the mutex are still alive.
+struct CompileThreads {
+  std::vector<std::thread> threads;
+  ~CompileThreads() {
+    for (auto& t : threads)
+      if (t.joinable()) t.join();
+  }
+} g_threads;
 
 uint32_t slot_index(const float* xf, int k) { return (uint32_t)xf[20 + 4 * k]; }
 int slot_count(const float* xf) {
@@ -102,10 +114,12 @@ std::string generate(const float* xforms, int n_regular, int final_slot) {
   s << "namespace p3 {\nstruct Spec {\n"
     << "  static constexpr bool kPairAdds = " << (p3::pair_adds() ? "true" : "false") << ";\n"
     << "  static constexpr int kRegular = " << n_regular << ";\n"
-    << "  static constexpr bool kHasFinal = " << (final_slot >= 0 ? "true" : "false") << ";\n"
-    << "  static constexpr int kFinalSlot = " << (final_slot >= 0 ? final_slot : 0) << ";\n"
-    << "  static constexpr bool kFinalDrawFree = " << (final_draw_free(xforms, final_slot) ? "true" : "false")
-    << ";\n"
+    << "  static P3_INLINE bool has_final(const ChaosArgs&) { return "
+    << (final_slot >= 0 ? "true" : "false") << "; }\n"
+    << "  static P3_INLINE int final_slot(const ChaosArgs&) { return "
+    << (final_slot >= 0 ? final_slot : 0) << "; }\n"
+    << "  static P3_INLINE bool final_draw_free(const ChaosArgs&) { return "
+    << (final_draw_free(xforms, final_slot) ? "true" : "false") << "; }\n"
     << "  static P3_INLINE Chain chain(uint32_t fn, xrec_t X, float x,\n"
     << "      float y, float z, Rng& rng, bool& dc_on, float3& dc_rgb) {\n"
     << "    (void)dc_on; (void)dc_rgb;\n"
@@ -120,7 +134,7 @@ std::string generate(const float* xforms, int n_regular, int final_slot) {
     << "    }\n"
     << "    return c;\n"
     << "  }\n"
-    << "  static P3_INLINE float2 final_sum(xrec_t X, float2 pa,\n"
+    << "  static P3_INLINE float2 final_sum(xrec_t X, int, float2 pa,\n"
     << "      Rng& rng, bool& dc_on, float3& dc_rgb) {\n"
     << "    (void)dc_on; (void)dc_rgb;\n";
   if (final_slot >= 0) {
@@ -282,19 +296,61 @@ std::string chaos_source(const float* xforms, int n_regular, int final_slot) {
   return std::string(kRtcSource) + generate(xforms, n_regular, final_slot);
 }
 
-// Code object for this shape, compiled on first use (no device needed).
+// Code object for this shape, compiled on first use (no device needed). The
+// compile runs outside the lock, so launches of other shapes are not held up;
+// entries of g_code are never erased, so the returned pointer stays valid.
 bool chaos_code(const float* host_xforms, int n_regular, int final_slot,
                 const std::vector<char>** out, std::string& err) {
   const std::string key = chaos_shape_key(host_xforms, n_regular, final_slot);
-  std::lock_guard<std::mutex> lock(g_mu);
-  auto it = g_code.find(key);
-  if (it == g_code.end()) {
-    std::vector<char> code;
-    if (!compile(chaos_source(host_xforms, n_regular, final_slot), code, err)) return false;
-    it = g_code.emplace(key, std::move(code)).first;
+  {
+    std::lock_guard<std::mutex> lock(g_mu);
+    auto it = g_code.find(key);
+    if (it != g_code.end()) {
+      *out = &it->second;
+      return true;
+    }
   }
-  *out = &it->second;
+  std::vector<char> code;
+  if (!compile(chaos_source(host_xforms, n_regular, final_slot), code, err)) return false;
+  std::lock_guard<std::mutex> lock(g_mu);
+  *out = &g_code.emplace(key, std::move(code)).first->second;  // first insert wins
   return true;
+}
+
+bool chaos_code_ready(const float* host_xforms, int n_regular, int final_slot) {
+  const std::string key = chaos_shape_key(host_xforms, n_regular, final_slot);
+  {
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (g_code.count(key)) return true;
+  }
+  std::vector<char> code;
+  if (!read_cached(cache_path(chaos_source(host_xforms, n_regular, final_slot), kOpts, kNumOpts),
+                   code))
+    return false;
+  std::lock_guard<std::mutex> lock(g_mu);
+  g_code.emplace(key, std::move(code));
+  return true;
+}
+
+void chaos_compile_async(const float* host_xforms, int n_regular, int final_slot) {
+  const std::string key = chaos_shape_key(host_xforms, n_regular, final_slot);
+  static std::unordered_map<std::string, bool> in_flight;
+  {
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (g_code.count(key) || in_flight.count(key)) return;
+    in_flight[key] = true;
+  }
+  std::string src = chaos_source(host_xforms, n_regular, final_slot);
+  std::thread t([key, src] {
+    std::vector<char> code;
+    std::string err;
+    const bool ok = compile(src, code, err);
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (ok) g_code.emplace(key, std::move(code));
+    in_flight.erase(key);   // a failed compile is retried by the next synchronous use
+  });
+  std::lock_guard<std::mutex> lock(g_mu);
+  g_threads.threads.push_back(std::move(t));
 }
 
 // Loaded module of this shape on device `dev` (loaded once per device).

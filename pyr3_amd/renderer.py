@@ -193,6 +193,25 @@ class Renderer:
         h = np.ascontiguousarray(hist, dtype=np.uint32)
         check(self._lib.p3_write_histogram(self._h, _up(h), h.size), self._h)
 
+    KERNEL_POLICIES = {"specialised": 0, "generic": 1, "auto": 2}
+
+    def set_kernel_policy(self, policy: str) -> None:
+        """'specialised' (default: per-shape kernel, compiled on first use),
+        'generic' (the library's kernel for any shape, no compile) or 'auto'
+        (specialised once compiled, generic meanwhile). Same histograms."""
+        if policy not in self.KERNEL_POLICIES:
+            raise ValueError(f"kernel policy must be one of {sorted(self.KERNEL_POLICIES)}")
+        check(self._lib.p3_set_kernel_policy(self._h, self.KERNEL_POLICIES[policy]), self._h)
+
+    def kernel_ready(self, genome: Genome, packed: Optional[PackedGenome] = None) -> bool:
+        """Whether this genome's specialised chaos kernel is already compiled."""
+        pg = packed or pack_genome(genome)
+        rc = self._lib.p3_chaos_ready(_fp(pg.xforms), pg.xforms.shape[0], pg.distrib.shape[0],
+                                      pack.final_xform_slot(pg.expanded))
+        if rc < 0:
+            check(rc, self._h)
+        return rc == 1
+
     def read_index(self):
         """(H_s, W_s) u32 colour-index sums of the capturing dispatches."""
         out = np.zeros((self.super_h, self.super_w), dtype=np.uint32)
