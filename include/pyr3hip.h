@@ -104,18 +104,6 @@ int p3_read_histogram(p3_renderer* r, uint32_t* out, size_t n);  /* readHistogra
  * by dispatches whose uniform slot 14 (capture_index) is 1; cleared by
  * p3_reset once allocated. P3_E_STATE before the first capturing dispatch. */
 int p3_read_index(p3_renderer* r, uint32_t* out, size_t n);
-
-/* Which chaos kernel a dispatch runs (no counterpart in the reference, whose
- * WGSL kernel is generic). 0 (default): the kernel specialised for the
- * genome's shape, compiled with hipRTC on first use (1-35 s) and cached;
- * 1: the generic kernel built into the library (slower per turn, no
- * compile); 2: the specialised kernel when it is already compiled (this
- * process or the code-object cache), else the generic one while the
- * specialised one compiles in the background. All give the same histogram. */
-#define P3_KERNELS_SPECIALISED 0
-#define P3_KERNELS_GENERIC 1
-#define P3_KERNELS_AUTO 2
-int p3_set_kernel_policy(p3_renderer* r, int policy);
 int p3_read_filtered(p3_renderer* r, float* out, size_t n);      /* DE output grid */
 int p3_read_image(p3_renderer* r, uint8_t* out, size_t n);       /* W x H x 4 RGBA8 */
 int p3_read_image_f32(p3_renderer* r, float* out, size_t n);     /* pre-quantisation RGBA */

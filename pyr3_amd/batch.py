@@ -125,15 +125,9 @@ def compile_pool(workers: Optional[int] = None) -> ProcessPoolExecutor:
 
 class BatchRenderer:
     def __init__(self, n_streams: int = 2, compile_workers: Optional[int] = None,
-                 cache_dir: Optional[str] = None, pool: Optional[ProcessPoolExecutor] = None,
-                 kernels: str = "specialised"):
-        """kernels='specialised': compile every shape (worker processes) and
-        render each job as soon as its kernel is ready; 'generic': render
-        every job at once with the library's generic kernel (no compiles —
-        the choice for batches of one-off shapes)."""
-        if kernels not in ("specialised", "generic"):
-            raise ValueError("kernels must be 'specialised' or 'generic'")
-        self.kernels = kernels
+                 cache_dir: Optional[str] = None, pool: Optional[ProcessPoolExecutor] = None):
+        """Compiles every distinct shape in worker processes and renders each
+        job as soon as its kernel is ready."""
         from .renderer import Renderer
         self._Renderer = Renderer
         self.cache_dir = cache_dir or os.environ.get("P3_KERNEL_CACHE") or os.path.join(
@@ -151,8 +145,6 @@ class BatchRenderer:
         r = self._pool[slot]
         if r is None:
             r = self._Renderer(w, h, os_)
-            if self.kernels == "generic":
-                r.set_kernel_policy("generic")
             self._pool[slot] = r
         else:
             r.resize(w, h, os_)
@@ -175,18 +167,6 @@ class BatchRenderer:
         by_key = {}
         for i, j in enumerate(jobs):
             by_key.setdefault(_shape_key(j.genome), []).append(i)
-        if self.kernels == "generic":
-            results = []
-            for i, j in enumerate(jobs):
-                results.append(self._render_one(j, packed[i], i, read_images))
-                if progress and (i + 1) % 64 == 0:
-                    progress(i + 1)
-            for r in self._pool:
-                if r is not None:
-                    r.sync()
-            self.timings = {"total_s": time.perf_counter() - t0, "pack_s": 0.0,
-                            "shapes": len(shapes), "jobs": len(jobs)}
-            return results
         own = self._workers is None
         ex = (ProcessPoolExecutor(max_workers=self.compile_workers, mp_context=get_context("spawn"))
               if own else self._workers)

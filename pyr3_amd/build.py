@@ -17,7 +17,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libpyr3hip.so"
-SOURCES = ["p3_api.cpp", "p3_rtc.cpp", "p3_image.hip", "p3_denoise.hip", "p3_chaos_generic.hip"]
+SOURCES = ["p3_api.cpp", "p3_rtc.cpp", "p3_image.hip", "p3_denoise.hip"]
 # Device headers embedded as text for the run-time chaos specialisation, in
 # dependency order (their #include lines are dropped).
 RTC_HEADERS = ["p3_args.h", "p3_device.h", "p3_variations.h", "p3_variations_ext.h",
@@ -66,8 +66,8 @@ def link_command(objs, out: Path = LIB):
 
 
 def build(verbose: bool = True) -> Path:
-    """Compile every source to an object (in parallel: the generic chaos
-    kernel alone takes about a minute), then link the shared library."""
+    """Compile every source to an object (in parallel), then link the
+    shared library."""
     write_rtc_source()
     objdir = PKG / "_objs"
     objdir.mkdir(exist_ok=True)

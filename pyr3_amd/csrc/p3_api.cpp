@@ -107,7 +107,6 @@ struct p3_renderer {
   unsigned long long hist_bound = 0;  // upper bound on any histogram word
   int trace_walkers = 0, trace_iters = 0;
   bool timing = false;
-  int kernel_policy = 0;   // P3_KERNELS_SPECIALISED / _GENERIC / _AUTO
   bool want_image_f32 = false;
   bool have_image_f32 = false;
   std::vector<float> last_xforms;       // host copies of the last uploads
@@ -305,19 +304,6 @@ p3::ChaosArgs dispatch_args(p3_renderer* r, const uint32_t* u, int n_regular, in
   return a;
 }
 
-// Whether this dispatch runs the library's generic chaos kernel rather than
-// the shape's specialised one (compiled on first use).
-bool use_generic(p3_renderer* r, const float* xforms, int n_regular, int32_t final_idx) {
-  if (p3::chaos_block() != p3::kChaosBlock) return false;   // generic blocks are 256 walkers
-  if (r->kernel_policy == 1) return true;
-  if (r->kernel_policy == 2) {
-    if (p3::chaos_code_ready(xforms, n_regular, final_idx)) return false;
-    p3::chaos_compile_async(xforms, n_regular, final_idx);
-    return true;
-  }
-  return false;
-}
-
 // Capture buffer of the colour index (allocated and cleared on first use,
 // cleared by every reset afterwards) when the uniforms ask for it.
 int attach_capture(p3_renderer* r, p3::ChaosArgs& a, const uint32_t* u) {
@@ -407,16 +393,14 @@ int iterate_impl(p3_renderer* r, const float* xforms, int n_slots, const uint32_
   r->hist_bound += 255ULL * (unsigned long long)walkers * (unsigned long long)a.iters;
   const bool sat = r->hist_bound >= 0xffffffffULL;
   std::string rtc_err;
-  const bool generic = use_generic(r, xforms, n_regular, final_idx);
-  if (!generic) {  // compile (or fetch) the kernel for this shape before the timed launch
+  {  // compile (or fetch) the kernel for this shape before the timed launch
     const std::vector<char>* code = nullptr;
     if (!p3::chaos_code(xforms, n_regular, final_idx, &code, rtc_err))
       return r->fail(P3_E_HIP, "chaos kernel specialisation: %s", rtc_err.c_str());
   }
   r->tic(T_CHAOS);
   const hipError_t le =
-      generic ? p3::launch_chaos_generic(a, sat, r->stream)
-              : p3::launch_chaos_rtc(a, xforms, n_regular, final_idx, sat, r->stream, rtc_err);
+      p3::launch_chaos_rtc(a, xforms, n_regular, final_idx, sat, r->stream, rtc_err);
   if (le != hipSuccess)
     return r->fail(P3_E_HIP, "chaos launch: %s %s", hipGetErrorString(le), rtc_err.c_str());
   r->toc(T_CHAOS);
@@ -504,17 +488,14 @@ int iterate_multi_impl(p3_renderer* r, int n_sub, const float* const* xforms, in
   r->hist_bound += bound;
   const bool sat = r->hist_bound >= 0xffffffffULL;
   std::string rtc_err;
-  const bool generic = use_generic(r, xforms[0], n_regular, final_idx);
-  if (!generic) {
+  {
     const std::vector<char>* code = nullptr;
     if (!p3::chaos_code(xforms[0], n_regular, final_idx, &code, rtc_err))
       return r->fail(P3_E_HIP, "chaos kernel specialisation: %s", rtc_err.c_str());
   }
   r->tic(T_CHAOS);
   const hipError_t le =
-      generic ? p3::launch_chaos_generic_multi(M, sat, r->stream)
-              : p3::launch_chaos_rtc_multi(M, xforms[0], n_regular, final_idx, sat, r->stream,
-                                           rtc_err);
+      p3::launch_chaos_rtc_multi(M, xforms[0], n_regular, final_idx, sat, r->stream, rtc_err);
   if (le != hipSuccess)
     return r->fail(P3_E_HIP, "batched chaos launch: %s %s", hipGetErrorString(le),
                    rtc_err.c_str());
@@ -679,15 +660,6 @@ int p3_read_histogram(p3_renderer* r, uint32_t* out, size_t n) {
   if (!r) return P3_E_ARG;
   if (!out || n != r->hist.n) return r->fail(P3_E_ARG, "histogram needs %zu words", r->hist.n);
   return read_dev(r, out, r->hist.p, n * 4, "read histogram");
-}
-
-int p3_set_kernel_policy(p3_renderer* r, int policy) {
-  if (!r) return P3_E_ARG;
-  if (policy < 0 || policy > 2)
-    return r->fail(P3_E_ARG, "kernel policy %d is not 0 (specialised), 1 (generic) or 2 (auto)",
-                   policy);
-  r->kernel_policy = policy;
-  return P3_OK;
 }
 
 int p3_read_index(p3_renderer* r, uint32_t* out, size_t n) {

@@ -49,13 +49,6 @@ bool chaos_code(const float* host_xforms, int n_regular, int final_slot,
 // True when this shape's specialised kernel is compiled (in this process or
 // in the on-disk code-object cache); never compiles.
 bool chaos_code_ready(const float* host_xforms, int n_regular, int final_slot);
-// Starts compiling this shape's specialised kernel on a background thread
-// (at most one per shape); later chaos_code_ready calls see the result.
-void chaos_compile_async(const float* host_xforms, int n_regular, int final_slot);
-// Chaos kernels for any shape, compiled with the library (p3_chaos_generic.hip);
-// 256-walker blocks.
-hipError_t launch_chaos_generic(const ChaosArgs& a, bool saturating, hipStream_t stream);
-hipError_t launch_chaos_generic_multi(const MultiArgs& m, bool saturating, hipStream_t stream);
 std::string chaos_source(const float* host_xforms, int n_regular, int final_slot);
 std::string chaos_shape_key(const float* host_xforms, int n_regular, int final_slot);
 size_t chaos_compiled_shapes();

@@ -22,7 +22,6 @@
 #include <mutex>
 #include <sstream>
 #include <string>
-#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -41,17 +40,6 @@ struct Module {
 std::mutex g_mu;
 std::unordered_map<std::string, std::vector<char>> g_code;  // key: shape
 std::unordered_map<std::string, Module> g_mods;             // key: device + shape
-
-// Background compiles (chaos_compile_async). Declared after the maps above,
-// so at unload it is destroyed first and joins its threads while the maps
-// and the mutex are still alive.
-struct CompileThreads {
-  std::vector<std::thread> threads;
-  ~CompileThreads() {
-    for (auto& t : threads)
-      if (t.joinable()) t.join();
-  }
-} g_threads;
 
 uint32_t slot_index(const float* xf, int k) { return (uint32_t)xf[20 + 4 * k]; }
 int slot_count(const float* xf) {
@@ -330,27 +318,6 @@ bool chaos_code_ready(const float* host_xforms, int n_regular, int final_slot) {
   std::lock_guard<std::mutex> lock(g_mu);
   g_code.emplace(key, std::move(code));
   return true;
-}
-
-void chaos_compile_async(const float* host_xforms, int n_regular, int final_slot) {
-  const std::string key = chaos_shape_key(host_xforms, n_regular, final_slot);
-  static std::unordered_map<std::string, bool> in_flight;
-  {
-    std::lock_guard<std::mutex> lock(g_mu);
-    if (g_code.count(key) || in_flight.count(key)) return;
-    in_flight[key] = true;
-  }
-  std::string src = chaos_source(host_xforms, n_regular, final_slot);
-  std::thread t([key, src] {
-    std::vector<char> code;
-    std::string err;
-    const bool ok = compile(src, code, err);
-    std::lock_guard<std::mutex> lock(g_mu);
-    if (ok) g_code.emplace(key, std::move(code));
-    in_flight.erase(key);   // a failed compile is retried by the next synchronous use
-  });
-  std::lock_guard<std::mutex> lock(g_mu);
-  g_threads.threads.push_back(std::move(t));
 }
 
 // Loaded module of this shape on device `dev` (loaded once per device).

@@ -8,7 +8,6 @@ RGBA8 PNG carrying the genome in its `pyr3` tEXt chunk.
 
     python -m pyr3_amd.render in.flam3 out.png [--quality Q] [--seed S]
         [--walkers N] [--oversample N] [--no-de] [--transparent]
-        [--kernels auto|generic|specialised]
 
 Walkers default to 131,072 (the reference's `--walkers` override; its own
 default of ~1,024 long walkers fills only a few CUs of this GPU).
@@ -36,9 +35,6 @@ def main(argv=None) -> int:
     ap.add_argument("--oversample", type=int, default=None)
     ap.add_argument("--no-de", action="store_true", help="skip density estimation")
     ap.add_argument("--transparent", action="store_true")
-    ap.add_argument("--kernels", choices=("auto", "generic", "specialised"), default="auto",
-                    help="auto: the genome's specialised kernel if already compiled, else the "
-                         "generic one (no compile wait); specialised: compile it (and cache it)")
     a = ap.parse_args(argv)
     if a.walkers < 1:
         ap.error("--walkers must be positive")
@@ -60,10 +56,9 @@ def main(argv=None) -> int:
 
     from .renderer import Renderer   # loads the HIP library; fails loudly without it
     r = Renderer(size["width"], size["height"], os_, filter_radius=radius)
-    policy = a.kernels
-    if policy == "auto":
-        policy = "specialised" if r.kernel_ready(g) else "generic"
-    r.set_kernel_policy(policy)
+    if not r.kernel_ready(g):
+        print("[pyr3-render] compiling the chaos kernel for this genome's shape "
+              "(cached for later renders)", flush=True)
     seed = a.seed if a.seed is not None else random.getrandbits(32)
     t0 = time.perf_counter()
     samples = r.render(g, seed=seed, force_de_off=a.no_de, transparent=a.transparent,
@@ -73,8 +68,7 @@ def main(argv=None) -> int:
     r.destroy()
     with open(out, "wb") as fh:
         fh.write(encode_render_png(rgba, g))
-    print(f"[pyr3-render] {samples} samples in {dt:.3f}s ({policy} kernel, seed {seed}) -> {out}",
-          flush=True)
+    print(f"[pyr3-render] {samples} samples in {dt:.3f}s (seed {seed}) -> {out}", flush=True)
     return 0
 
 

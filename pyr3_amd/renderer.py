@@ -193,16 +193,6 @@ class Renderer:
         h = np.ascontiguousarray(hist, dtype=np.uint32)
         check(self._lib.p3_write_histogram(self._h, _up(h), h.size), self._h)
 
-    KERNEL_POLICIES = {"specialised": 0, "generic": 1, "auto": 2}
-
-    def set_kernel_policy(self, policy: str) -> None:
-        """'specialised' (default: per-shape kernel, compiled on first use),
-        'generic' (the library's kernel for any shape, no compile) or 'auto'
-        (specialised once compiled, generic meanwhile). Same histograms."""
-        if policy not in self.KERNEL_POLICIES:
-            raise ValueError(f"kernel policy must be one of {sorted(self.KERNEL_POLICIES)}")
-        check(self._lib.p3_set_kernel_policy(self._h, self.KERNEL_POLICIES[policy]), self._h)
-
     def kernel_ready(self, genome: Genome, packed: Optional[PackedGenome] = None) -> bool:
         """Whether this genome's specialised chaos kernel is already compiled."""
         pg = packed or pack_genome(genome)
