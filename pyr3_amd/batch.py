@@ -61,10 +61,10 @@ class JobResult:
 def apportion(total: int, weights: Sequence[float]) -> List[int]:
     """Largest-remainder split of `total` walkers over `weights`
     (src/animate-render.ts:135-140, apportionWalkers)."""
-    s = float(sum(weights))
+    s = float(sum(max(0.0, w) for w in weights))
     if total <= 0 or s <= 0:
         return [0] * len(weights)
-    quotas = [total * w / s for w in weights]
+    quotas = [total * max(0.0, w) / s for w in weights]
     base = [int(math.floor(q)) for q in quotas]
     rest = total - sum(base)
     order = sorted(range(len(weights)), key=lambda i: (-(quotas[i] - base[i]), i))

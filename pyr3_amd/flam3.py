@@ -12,7 +12,10 @@ the returned report instead of being dropped silently:
   order) with its `<name>_<param>` attributes (missing ones take the
   variation's defaults, :420-443); legacy aliases `Re_A`.. -> mobius,
   `oscope_*` -> oscilloscope (:43-67); unknown names are reported as
-  dropped; an xform left with no variation gets linear 1 (:593-596).
+  dropped; an xform left with no variation gets linear 1 (:593-596);
+  `motion_frequency`, `motion_function` (none/sin/triangle/hill), `animate`
+  and `<motion>` children (partial xforms of deltas: missing coefs are zero,
+  no linear fallback; :642-697), consumed by `pyr3_amd.animate`.
 * `<flame>`: size, center, scale (default 50), rotate, quality, supersample
   (-> oversample), filter (+ shape), estimator_radius/minimum/curve,
   gamma/vibrancy/highlight_power/brightness/gamma_threshold (partial sets
@@ -26,8 +29,8 @@ the returned report instead of being dropped silently:
 
 Not honoured (reported): flam3's numbered palette library (a `palette="N"`
 reference falls back to the default palette), spatial filter shapes other
-than gaussian, `<motion>` overlays and per-xform animation attributes, and
-the genome fields the reference lists as ignored (:835-846).
+than gaussian, and the genome fields the reference lists as ignored
+(:835-846).
 """
 from __future__ import annotations
 
@@ -160,7 +163,25 @@ def _variation(name: str, weight: float, attrs: Dict[str, str]) -> Variation:
     return v
 
 
-def _xform(el: ET.Element, index: int, final: bool, parity: int, rep: ImportReport) -> Xform:
+def _motion_func(text: str) -> Optional[int]:
+    """flam3 motion_function names (src/flame-import.ts:700-707)."""
+    t = text.strip().lower()
+    if t in ("", "none", "0"):
+        return 0
+    if t in ("sin", "sine", "1"):
+        return 1
+    if t in ("triangle", "2"):
+        return 2
+    if t in ("hill", "3"):
+        return 3
+    return None
+
+
+def _xform(el: ET.Element, index: int, final: bool, parity: int, rep: ImportReport,
+           in_motion: bool = False) -> Xform:
+    """One <xform>/<finalxform> (src/flame-import.ts:452-697). A <motion>
+    child (in_motion) is a partial xform of additive deltas: missing coefs
+    are zero deltas, and no linear fallback is added."""
     where = "finalxform" if final else f"xform[{index}]"
 
     def number(attr: str, fallback: float) -> float:
@@ -178,8 +199,11 @@ def _xform(el: ET.Element, index: int, final: bool, parity: int, rep: ImportRepo
     else:
         speed = 0.5
     if el.get("coefs") is None:
-        raise GenomeError(f"pyr3: {where} missing coefs")
-    affine = coefs_to_affine(el.get("coefs"))
+        if not in_motion:
+            raise GenomeError(f"pyr3: {where} missing coefs")
+        affine = Affine(0.0, 0.0, 0.0, 0.0, 0.0, 0.0)
+    else:
+        affine = coefs_to_affine(el.get("coefs"))
 
     attrs: Dict[str, str] = {}
     for k, val in el.attrib.items():
@@ -201,7 +225,7 @@ def _xform(el: ET.Element, index: int, final: bool, parity: int, rep: ImportRepo
         if final:
             drop["final"] = True
         rep.dropped_variations.append(drop)
-    if not variations:
+    if not variations and not in_motion:
         variations.append(Variation(VARIATION_INDEX["linear"], 1.0))
 
     x = Xform(weight=weight, color=color, color_speed=speed, affine=affine,
@@ -212,15 +236,38 @@ def _xform(el: ET.Element, index: int, final: bool, parity: int, rep: ImportRepo
             x.post = post
     if el.get("opacity") is not None:
         x.opacity = _finite(el.get("opacity"), f"opacity@{where}")
+        x.opacity_given = True
     if el.get("chaos") is not None:
         row = [_finite(t, f"chaos@{where}[{i}]") for i, t in enumerate(el.get("chaos").split())]
         x.xaos = row or None
-    for attr in ("motion_frequency", "motion_function", "animate"):
-        if el.get(attr) is not None:
-            rep.ignored_fields.append({"field": f"{attr}@{where}", "value": el.get(attr)})
-    if el.find("motion") is not None:
-        rep.ignored_fields.append({"field": f"motion@{where}",
-                                   "value": f"{len(el.findall('motion'))} element(s)"})
+    # keyframe-animation fields (src/flame-import.ts:642-690)
+    if el.get("motion_frequency") is not None:
+        v = number("motion_frequency", 0.0)
+        if v != 0:
+            x.motion_freq = v
+    if el.get("motion_function") is not None:
+        code = _motion_func(el.get("motion_function"))
+        if code is None:
+            rep.ignored_fields.append({"field": f"motion_function@{where}",
+                                       "value": el.get("motion_function")})
+        elif code != 0:
+            x.motion_func = code
+    if el.get("animate") is not None:
+        v = number("animate", 1.0)
+        if v != 1:
+            x.animate = v
+    if not in_motion:
+        motion = []
+        for mi, me in enumerate(c for c in el if c.tag == "motion"):
+            sub_rep = ImportReport()
+            motion.append(_xform(me, index, False, 0, sub_rep, in_motion=True))
+            for d in sub_rep.dropped_variations:
+                rep.dropped_variations.append(dict(d, name=f"motion[{mi}].{d['name']}"))
+            for ig in sub_rep.ignored_fields:
+                rep.ignored_fields.append({"field": f"motion[{mi}].{ig['field']}",
+                                           "value": ig["value"]})
+        if motion:
+            x.motion = motion
     return x
 
 
@@ -413,6 +460,8 @@ def _flame(flame: ET.Element, n_flames: int, rep: ImportReport) -> Tuple[Genome,
     t, ok = _number_or(flame.get("time"), 0.0)
     if not ok:
         rep.defaulted_fields.append({"field": "time", "value": flame.get("time"), "used": "0"})
+    if t != 0:
+        g.time = t
     return g, t
 
 

@@ -12,7 +12,7 @@ import copy
 import json
 import math
 from dataclasses import dataclass, field
-from typing import Dict, List, Optional
+from typing import Dict, List, Optional, Tuple
 
 from .catalog import VARIATION_INDEX, VARIATION_NAMES, VARIATION_PARAMS
 
@@ -60,6 +60,16 @@ class Xform:
     xaos: Optional[List[float]] = None
     post: Optional[Affine] = None
     active: bool = True
+    # keyframe-animation fields (src/genome.ts:61-77): flam3 motion_frequency /
+    # motion_function (0 none, 1 sin, 2 triangle, 3 hill), the per-xform
+    # motion overlays (partial xforms of additive deltas) and `animate`
+    # (1 rotating, 0 stationary; None = 1)
+    motion_freq: Optional[float] = None
+    motion_func: Optional[int] = None
+    motion: Optional[List["Xform"]] = None
+    animate: Optional[float] = None
+    # the source gave an opacity (a motion overlay adds its opacity only then)
+    opacity_given: bool = field(default=False, compare=False, repr=False)
 
 
 @dataclass
@@ -115,6 +125,13 @@ class Genome:
     filter_radius: Optional[float] = None
     background: Optional[List[float]] = None
     palette_mode: str = "step"
+    nick: Optional[str] = None
+    # post-tonemap grading (src/genome.ts:160-163): channel curves per channel
+    # ('composite', 'r', 'g', 'b', 'luma' -> [(x, y), ...]) and HSV adjust
+    # {'hue': deg, 'sat': %, 'light': %}
+    channel_curves: Optional[Dict[str, List[Tuple[float, float]]]] = None
+    hsl_adjust: Optional[Dict[str, float]] = None
+    time: Optional[float] = None    # keyframe time (src/genome.ts:167)
 
     def clone(self) -> "Genome":
         return copy.deepcopy(self)
@@ -197,13 +214,71 @@ def _xform(o, path, is_final) -> Xform:
     )
     if o.get("opacity") is not None:
         x.opacity = _num(o["opacity"], f"{path}.opacity")
+        x.opacity_given = True
     if not is_final and o.get("xaos") is not None:
         x.xaos = [_num(v, f"{path}.xaos[{i}]") for i, v in enumerate(_arr(o["xaos"], f"{path}.xaos"))]
     if o.get("post") is not None:
         x.post = _affine(o["post"], f"{path}.post")
     if o.get("active") is False:
         x.active = False
+    # src/serialize.ts:1341-1367
+    if o.get("motion_freq") is not None:
+        x.motion_freq = _num(o["motion_freq"], f"{path}.motion_freq")
+    if o.get("motion_func") is not None:
+        mf = _num(o["motion_func"], f"{path}.motion_func")
+        if mf not in (0, 1, 2, 3):
+            raise GenomeError(f"pyr3: {path}.motion_func must be 0..3, got: {o['motion_func']}")
+        x.motion_func = int(mf)
+    if o.get("animate") is not None:
+        x.animate = _num(o["animate"], f"{path}.animate")
+    if o.get("motion") is not None:
+        x.motion = [_xform(m, f"{path}.motion[{i}]", False)
+                    for i, m in enumerate(_arr(o["motion"], f"{path}.motion"))]
     return x
+
+
+CURVE_CHANNELS = ("composite", "r", "g", "b", "luma")
+
+
+def _channel_curves(j, path):
+    """src/serialize.ts:1271-1297 parseChannelCurves."""
+    o = _obj(j, path)
+    out = {}
+    for ch in CURVE_CHANNELS:
+        arr = _arr(o.get(ch), f"{path}.{ch}")
+        if len(arr) < 2 or len(arr) > 8:
+            raise GenomeError(f"pyr3: {path}.{ch} must have 2..8 points, got {len(arr)}")
+        pts = []
+        for i, pt in enumerate(arr):
+            po = _obj(pt, f"{path}.{ch}[{i}]")
+            x = _num(po.get("x"), f"{path}.{ch}[{i}].x")
+            y = _num(po.get("y"), f"{path}.{ch}[{i}].y")
+            if x < 0 or x > 1 or y < 0 or y > 1:
+                raise GenomeError(f"pyr3: {path}.{ch}[{i}] out of [0,1]: x={x:g} y={y:g}")
+            pts.append((x, y))
+        for i in range(1, len(pts)):
+            if pts[i][0] <= pts[i - 1][0]:
+                raise GenomeError(f"pyr3: {path}.{ch}[{i}].x not strictly monotonic")
+        out[ch] = pts
+    return out
+
+
+def _hsl_adjust(j, path):
+    """src/serialize.ts:1239-1257: hue wrapped into [-180, 180], sat clamped
+    to [0, 200], light to [-100, 100]; missing or non-numeric fields take
+    the identity values."""
+    o = _obj(j, path)
+
+    def num(v, dflt):
+        if isinstance(v, bool) or not isinstance(v, (int, float)) or not math.isfinite(v):
+            return dflt
+        return float(v)
+    raw_hue = num(o.get("hue"), 0.0)
+    hue_mod = math.fmod(math.fmod(raw_hue, 360.0) + 360.0, 360.0)
+    hue = hue_mod - 360.0 if hue_mod > 180.0 else hue_mod
+    sat = min(200.0, max(0.0, num(o.get("sat"), 100.0)))
+    light = min(100.0, max(-100.0, num(o.get("light"), 0.0)))
+    return {"hue": hue, "sat": sat, "light": light}
 
 
 def genome_from_json(j) -> Genome:
@@ -290,6 +365,16 @@ def genome_from_json(j) -> Genome:
         g.background = [_num(v, f"background[{i}]") for i, v in enumerate(bg)]
     if root.get("paletteMode") is not None:
         g.palette_mode = _str(root["paletteMode"], "paletteMode")
+    if root.get("channelCurves") is not None:
+        g.channel_curves = _channel_curves(root["channelCurves"], "channelCurves")
+    if root.get("hslAdjust") is not None:
+        g.hsl_adjust = _hsl_adjust(root["hslAdjust"], "hslAdjust")
+    if root.get("nick") is not None:
+        g.nick = _str(root["nick"], "nick")
+    if root.get("time") is not None:
+        t = _num(root["time"], "time")
+        if t != 0:
+            g.time = t
     return g
 
 
@@ -326,6 +411,14 @@ def _xform_json(x: Xform, is_final: bool):
         o["post"] = _affine_json(x.post)
     if not x.active:
         o["active"] = False
+    if x.motion_freq:
+        o["motion_freq"] = x.motion_freq
+    if x.motion_func:
+        o["motion_func"] = x.motion_func
+    if x.animate is not None and x.animate != 1:
+        o["animate"] = x.animate
+    if x.motion:
+        o["motion"] = [_xform_json(m, False) for m in x.motion]
     return o
 
 
@@ -371,6 +464,18 @@ def genome_to_json(g: Genome) -> dict:
         o["background"] = list(g.background)
     if g.palette_mode != "step":
         o["paletteMode"] = g.palette_mode
+    if g.channel_curves is not None and any(
+            [tuple(p) for p in g.channel_curves[ch]] != [(0.0, 0.0), (1.0, 1.0)]
+            for ch in CURVE_CHANNELS):
+        o["channelCurves"] = {ch: [{"x": x, "y": y} for x, y in g.channel_curves[ch]]
+                              for ch in CURVE_CHANNELS}
+    h = g.hsl_adjust
+    if h is not None and (h["hue"] != 0 or h["sat"] != 100 or h["light"] != 0):
+        o["hslAdjust"] = {"hue": h["hue"], "sat": h["sat"], "light": h["light"]}
+    if g.nick:
+        o["nick"] = g.nick
+    if g.time:
+        o["time"] = g.time
     return o
 
 

@@ -143,6 +143,10 @@ class Renderer:
         n_slots, n_rows = pgs[0].xforms.shape[0], pgs[0].distrib.shape[0]
         if any(pg.xforms.shape[0] != n_slots or pg.distrib.shape[0] != n_rows for pg in pgs):
             raise ValueError("sub-dispatches differ in xform count")
+        from .animate import packed_shape   # the batched kernel runs the first one's shape
+        if len({packed_shape(pg) for pg in pgs}) != 1:
+            raise ValueError("sub-dispatches differ in kernel shape (variation slots, "
+                             "post-affine presence or final xform)")
         xfp = (C.POINTER(C.c_float) * n)(*[_fp(pg.xforms) for pg in pgs])
         dsp = (C.POINTER(C.c_uint32) * n)(*[_up(pg.distrib) for pg in pgs])
         ufp = (C.POINTER(C.c_uint32) * n)(*[_up(u) for u in us])
