@@ -104,6 +104,14 @@ int p3_read_histogram(p3_renderer* r, uint32_t* out, size_t n);  /* readHistogra
  * by dispatches whose uniform slot 14 (capture_index) is 1; cleared by
  * p3_reset once allocated. P3_E_STATE before the first capturing dispatch. */
 int p3_read_index(p3_renderer* r, uint32_t* out, size_t n);
+/* Channel-curves LUT for p3_present (src/visualize.ts:170-175 uploads the
+ * bakeCurves output, src/channel-curves.ts:115-133): 5 x 256 floats in [0, 1],
+ * channels composite, R, G, B, luma. Read only when the tonemap uniforms' word
+ * 16 (curvesActive bit mask) is non-zero; p3_present fails with P3_E_STATE if
+ * that mask is set and no LUT was uploaded. lut = NULL or n = 0 clears it.
+ * Words 17..20 are hslActive, hue (degrees), saturation factor and lightness
+ * offset (visualize.ts:186-193, applied as visualize_f32.wgsl:240-266). */
+int p3_set_curves(p3_renderer* r, const float* lut, size_t n);
 int p3_read_filtered(p3_renderer* r, float* out, size_t n);      /* DE output grid */
 int p3_read_image(p3_renderer* r, uint8_t* out, size_t n);       /* W x H x 4 RGBA8 */
 int p3_read_image_f32(p3_renderer* r, float* out, size_t n);     /* pre-quantisation RGBA */

@@ -31,7 +31,7 @@ _SIGS = {
     "orc_chaos_mt": (None, [_F, _U, C.c_int, _U, _F, _U, C.c_int]),
     "orc_kernel_norm_lut": (None, [_F]),
     "orc_density": (None, [_U, _U, _F, _F, C.c_int]),
-    "orc_tonemap": (None, [_U, _F, _F, _U, _F, _B, C.c_int]),
+    "orc_tonemap": (None, [_U, _F, _F, _U, _F, _F, _B, C.c_int]),
     "orc_spatial_kernel": (C.c_int, [C.c_double, C.c_int, _F]),
     "orc_calibration": (None, [C.c_double, C.c_double, C.c_double, C.c_int, _D, _D]),
     "orc_build_distrib": (C.c_int, [C.c_int, _D, _D, _U]),
@@ -180,8 +180,13 @@ def density(hist, de_uniforms, threads: int = 8) -> np.ndarray:
     return out
 
 
-def tonemap(viz_uniforms, taps, filtered=None, hist=None, threads: int = 8):
+def tonemap(viz_uniforms, taps, filtered=None, hist=None, threads: int = 8, curves=None):
     u = np.ascontiguousarray(viz_uniforms, dtype=np.uint32)
+    if u[16] != 0 and curves is None:
+        raise ValueError("curves mask set without a curves LUT")
+    cl = np.ascontiguousarray(curves, dtype=np.float32).reshape(-1) if curves is not None else None
+    if cl is not None and cl.size != 5 * 256:
+        raise ValueError("curves LUT must hold 5 x 256 floats")
     t = np.ascontiguousarray(taps, dtype=np.float32)
     W, H = int(u[0]), int(u[1])
     of = np.zeros((H, W, 4), dtype=np.float32)
@@ -189,7 +194,8 @@ def tonemap(viz_uniforms, taps, filtered=None, hist=None, threads: int = 8):
     f = np.ascontiguousarray(filtered, dtype=np.float32) if filtered is not None else None
     h = np.ascontiguousarray(hist, dtype=np.uint32) if hist is not None else None
     load().orc_tonemap(p(u, _U), p(t, _F), p(f, _F) if f is not None else None,
-                       p(h, _U) if h is not None else None, p(of, _F), p(o8, _B), threads)
+                       p(h, _U) if h is not None else None, p(cl, _F) if cl is not None else None,
+                       p(of, _F), p(o8, _B), threads)
     return of, o8
 
 

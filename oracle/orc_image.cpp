@@ -72,6 +72,8 @@ struct VizU {
   uint32_t os, fwidth;
   float bg[3];
   uint32_t transparent;
+  uint32_t curves_mask, hsl_on;   // grading: words 16..20
+  float hsl_hue, hsl_sat, hsl_light;
 };
 VizU read_viz(const uint32_t* u) {
   VizU v;
@@ -81,6 +83,8 @@ VizU read_viz(const uint32_t* u) {
   v.os = u[8]; v.fwidth = u[9];
   v.bg[0] = fget(u, 12); v.bg[1] = fget(u, 13); v.bg[2] = fget(u, 14);
   v.transparent = u[21];
+  v.curves_mask = u[16]; v.hsl_on = u[17];
+  v.hsl_hue = fget(u, 18); v.hsl_sat = fget(u, 19); v.hsl_light = fget(u, 20);
   return v;
 }
 
@@ -156,10 +160,50 @@ void calc_newrgb(const float c[3], float ls, float highpow, float o[3]) {
   }
 }
 
+// visualize_f32.wgsl:63-68 lut(): linear lookup in channel `ch` of the
+// 5 x 256 curves LUT.
+float curve_lut(const float* L, uint32_t ch, float x) {
+  const float idx = clampf(x, 0.0f, 1.0f) * 255.0f;
+  const uint32_t i0 = (uint32_t)std::floor(idx);
+  const uint32_t i1 = std::min(i0 + 1u, 255u);
+  const float t = idx - (float)i0;
+  return L[ch * 256u + i0] * (1.0f - t) + L[ch * 256u + i1] * t;
+}
+
+// visualize_f32.wgsl:240-266: channel curves (luma, then R, G, B, then the
+// composite curve on all three), then the HSV adjust (hue offset wrapped
+// into [0, 360), saturation scaled, value offset, both clamped to [0, 1]).
+void grade(const VizU& u, const float* curves, float o[3]) {
+  if (u.curves_mask != 0u) {
+    if (u.curves_mask & 16u) {
+      const float l = o[0] * 0.299f + o[1] * 0.587f + o[2] * 0.114f;
+      const float ln = curve_lut(curves, 4u, l);
+      if (l > 0.0f) {
+        const float s = ln / l;
+        o[0] = o[0] * s; o[1] = o[1] * s; o[2] = o[2] * s;
+      }
+    }
+    if (u.curves_mask & 2u) o[0] = curve_lut(curves, 1u, o[0]);
+    if (u.curves_mask & 4u) o[1] = curve_lut(curves, 2u, o[1]);
+    if (u.curves_mask & 8u) o[2] = curve_lut(curves, 3u, o[2]);
+    if (u.curves_mask & 1u)
+      for (int i = 0; i < 3; ++i) o[i] = curve_lut(curves, 0u, o[i]);
+  }
+  if (u.hsl_on != 0u) {
+    float hsv[3];
+    rgb2hsv(o, hsv);
+    hsv[0] = hsv[0] + u.hsl_hue;
+    hsv[0] = hsv[0] - 360.0f * std::floor(hsv[0] / 360.0f);
+    hsv[1] = clampf(hsv[1] * u.hsl_sat, 0.0f, 1.0f);
+    hsv[2] = clampf(hsv[2] + u.hsl_light, 0.0f, 1.0f);
+    hsv2rgb(hsv, o);
+  }
+}
+
 // One output pixel. `src_f` (DE on, filtered f32) or `src_u` (DE off, raw
 // u32 histogram with the per-super-pixel log-density inside the loop).
 void tonemap_px(const VizU& u, const float* taps, const float* src_f, const uint32_t* src_u,
-                uint32_t xi, uint32_t yi, float out[4]) {
+                const float* curves, uint32_t xi, uint32_t yi, float out[4]) {
   const uint32_t N = u.os, fw = u.fwidth, half = fw / 2u;
   const int SW = (int)(u.W * N), SH = (int)(u.H * N);
   const uint32_t cx = xi * N + N / 2u, cy = yi * N + N / 2u;
@@ -205,6 +249,8 @@ void tonemap_px(const VizU& u, const float* taps, const float* src_f, const uint
     float composed = nrgb[i] + perch + bgt;
     out[i] = clampf(composed / 256.0f, 0.0f, 1.0f);
   }
+  grade(u, curves, out);
+  for (int i = 0; i < 3; ++i) out[i] = clampf(out[i], 0.0f, 1.0f);
   out[3] = u.transparent != 0u ? alpha : 1.0f;
 }
 
@@ -244,9 +290,11 @@ void orc_density(const uint32_t* hist, const uint32_t* de_uniforms, const float*
 }
 
 // viz_uniforms: the 96-byte VizUniforms block. Exactly one of filtered/hist
-// is non-null. Writes f32 RGBA (pre-quantisation) and RGBA8 (round to nearest).
+// is non-null; `curves` is the 5 x 256 LUT (read only when word 16 is non-zero).
+// Writes f32 RGBA (pre-quantisation) and RGBA8 (round to nearest).
 void orc_tonemap(const uint32_t* viz_uniforms, const float* taps, const float* filtered,
-                 const uint32_t* hist, float* out_f32, uint8_t* out_u8, int threads) {
+                 const uint32_t* hist, const float* curves, float* out_f32, uint8_t* out_u8,
+                 int threads) {
   VizU u = read_viz(viz_uniforms);
   if (threads < 1) threads = 1;
   std::vector<std::thread> pool;
@@ -257,7 +305,7 @@ void orc_tonemap(const uint32_t* viz_uniforms, const float* taps, const float* f
       for (uint32_t y = y0; y < y1; ++y)
         for (uint32_t x = 0; x < u.W; ++x) {
           float px[4];
-          tonemap_px(u, taps, filtered, hist, x, y, px);
+          tonemap_px(u, taps, filtered, hist, curves, x, y, px);
           size_t o = ((size_t)y * u.W + x) * 4;
           for (int c = 0; c < 4; ++c) {
             if (out_f32) out_f32[o + c] = px[c];

@@ -90,6 +90,9 @@ struct p3_renderer {
   DevBuf<uint8_t> rad;
   DevBuf<uint32_t> tile_rmax, image8;
   DevBuf<float> taps, lut, palette, xforms;
+  DevBuf<float> curves;         // 5 x 256 channel-curve LUT (p3_set_curves)
+  std::vector<float> curves_host;
+  bool curves_set = false;
   DevBuf<uint32_t> distrib, isaac;
   DevBuf<uint16_t> bounds;      // step boundaries of the pick table, when it has them
   bool bounds_ok = false;
@@ -648,11 +651,40 @@ int p3_present(p3_renderer* r, const uint32_t* de, const uint32_t* viz) {
   t.vibrancy = bits_f(viz[5]); t.highpow = bits_f(viz[6]); t.linrange = bits_f(viz[7]);
   t.bg0 = bits_f(viz[12]); t.bg1 = bits_f(viz[13]); t.bg2 = bits_f(viz[14]);
   t.transparent = viz[21];
+  if (viz[16] > 31u) return r->fail(P3_E_ARG, "curves mask %u has bits above 4", viz[16]);
+  if (viz[16] != 0u && !r->curves_set)
+    return r->fail(P3_E_STATE, "curves mask %u set but no curves LUT uploaded (p3_set_curves)",
+                   viz[16]);
+  t.curves = r->curves.p;
+  t.curves_mask = viz[16];
+  t.hsl_on = viz[17];
+  t.hsl_hue = bits_f(viz[18]); t.hsl_sat = bits_f(viz[19]); t.hsl_light = bits_f(viz[20]);
   r->tic(T_TM);
   if ((rc = r->hip(p3::launch_tonemap(t, de != nullptr, r->stream), "tonemap launch"))) return rc;
   r->toc(T_TM);
   r->have_image = true;
   r->have_image_f32 = r->want_image_f32;
+  return P3_OK;
+}
+
+int p3_set_curves(p3_renderer* r, const float* lut, size_t n) {
+  if (!r) return P3_E_ARG;
+  if (!lut || n == 0) {
+    r->curves_set = false;
+    return P3_OK;
+  }
+  if (n != 5 * 256) return r->fail(P3_E_ARG, "curves LUT needs 1280 floats (5 x 256), got %zu", n);
+  for (size_t i = 0; i < n; ++i)
+    if (!(lut[i] >= 0.0f && lut[i] <= 1.0f))
+      return r->fail(P3_E_ARG, "curves LUT entry %zu is not in [0, 1]", i);
+  int rc;
+  if ((rc = r->hip(r->curves.ensure(n), "hipMalloc(curves)"))) return rc;
+  r->curves_host.assign(lut, lut + n);   // stays alive while the async copy runs
+  if ((rc = r->hip(hipMemcpyAsync(r->curves.p, r->curves_host.data(), n * sizeof(float),
+                                  hipMemcpyHostToDevice, r->stream),
+                   "upload curves")))
+    return rc;
+  r->curves_set = true;
   return P3_OK;
 }
 

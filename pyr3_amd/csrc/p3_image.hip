@@ -19,7 +19,8 @@
 // The tonemap follows fs (src/shaders/visualize_f32.wgsl:162-271 with DE,
 // src/shaders/visualize_u32.wgsl:148-248 without): separable collapse of
 // fwidth^2 super-pixels (clamp to edge), flam3 alpha/vibrancy/highlight
-// composite, background blend, RGBA8 store with round-to-nearest.
+// composite, background blend, optional channel curves and HSV adjust
+// (:240-266), RGBA8 store with round-to-nearest.
 #include "p3_device.h"
 #include "p3_launch.h"
 
@@ -233,6 +234,44 @@ __device__ inline float3 calc_newrgb(float3 c, float ls, float highpow) {
   return make_float3(m * cw.x, m * cw.y, m * cw.z);
 }
 
+// visualize_f32.wgsl:63-68: linear lookup in channel `ch` of the curves LUT.
+__device__ inline float curve_lut(const float* L, uint32_t ch, float x) {
+  const float idx = fminf(fmaxf(x, 0.0f), 1.0f) * 255.0f;
+  const uint32_t i0 = (uint32_t)floorf(idx);
+  const uint32_t i1 = min(i0 + 1u, 255u);
+  return mixf(L[ch * 256u + i0], L[ch * 256u + i1], idx - (float)i0);
+}
+
+// visualize_f32.wgsl:240-266: channel curves (luma, R, G, B, composite),
+// then the HSV adjust.
+__device__ inline float3 grade(const TmArgs& A, float3 o) {
+  if (A.curves_mask != 0u) {
+    if (A.curves_mask & 16u) {
+      const float l = o.x * 0.299f + o.y * 0.587f + o.z * 0.114f;
+      const float ln = curve_lut(A.curves, 4u, l);
+      if (l > 0.0f) {
+        const float s = ln / l;
+        o = make_float3(o.x * s, o.y * s, o.z * s);
+      }
+    }
+    if (A.curves_mask & 2u) o.x = curve_lut(A.curves, 1u, o.x);
+    if (A.curves_mask & 4u) o.y = curve_lut(A.curves, 2u, o.y);
+    if (A.curves_mask & 8u) o.z = curve_lut(A.curves, 3u, o.z);
+    if (A.curves_mask & 1u)
+      o = make_float3(curve_lut(A.curves, 0u, o.x), curve_lut(A.curves, 0u, o.y),
+                      curve_lut(A.curves, 0u, o.z));
+  }
+  if (A.hsl_on != 0u) {
+    float3 hsv = rgb2hsv(o);
+    hsv.x = hsv.x + A.hsl_hue;
+    hsv.x = hsv.x - 360.0f * floorf(hsv.x / 360.0f);
+    hsv.y = fminf(fmaxf(hsv.y * A.hsl_sat, 0.0f), 1.0f);
+    hsv.z = fminf(fmaxf(hsv.z + A.hsl_light, 0.0f), 1.0f);
+    o = hsv2rgb(hsv);
+  }
+  return o;
+}
+
 __device__ inline uint32_t to_u8(float v) {
   return (uint32_t)(fminf(fmaxf(v, 0.0f), 1.0f) * 255.0f + 0.5f);
 }
@@ -291,7 +330,9 @@ __global__ void __launch_bounds__(256) tonemap_kernel(TmArgs A) {
       const float composed = nn[i] + perch + bgt;
       res[i] = fminf(fmaxf(composed / 256.0f, 0.0f), 1.0f);
     }
-    o = make_float4(res[0], res[1], res[2], A.transparent ? alpha : 1.0f);
+    const float3 gr = grade(A, make_float3(res[0], res[1], res[2]));
+    o = make_float4(fminf(fmaxf(gr.x, 0.0f), 1.0f), fminf(fmaxf(gr.y, 0.0f), 1.0f),
+                    fminf(fmaxf(gr.z, 0.0f), 1.0f), A.transparent ? alpha : 1.0f);
   }
   const size_t oi = (size_t)yi * A.width + xi;
   A.out8[oi] = to_u8(o.x) | (to_u8(o.y) << 8) | (to_u8(o.z) << 16) | (to_u8(o.w) << 24);

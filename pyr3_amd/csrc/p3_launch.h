@@ -34,6 +34,10 @@ struct TmArgs {
   float k1, k2, gamma, vibrancy, highpow, linrange;
   float bg0, bg1, bg2;
   uint32_t transparent;
+  const float* curves;      // 5 x 256 channel-curve LUT (read when curves_mask != 0)
+  uint32_t curves_mask;     // bit 0 composite, 1 R, 2 G, 3 B, 4 luma
+  uint32_t hsl_on;
+  float hsl_hue, hsl_sat, hsl_light;   // degrees, factor, offset
 };
 
 // Chaos kernel specialised for the shape of `host_xforms` (p3_rtc.cpp).

@@ -344,9 +344,18 @@ def de_uniforms(width_s: int, height_s: int, max_rad: float, min_rad: float, cur
 
 
 def viz_uniforms(width: int, height: int, k1: float, k2: float, tm: Tonemap, oversample: int,
-                 fwidth: int, background=(0.0, 0.0, 0.0), transparent: bool = False) -> np.ndarray:
-    """The 96-byte VizUniforms block (curves and HSL off)."""
+                 fwidth: int, background=(0.0, 0.0, 0.0), transparent: bool = False,
+                 curves_mask: int = 0, hsl_adjust=None) -> np.ndarray:
+    """The 96-byte VizUniforms block (src/visualize.ts:177-201): word 16 the
+    channel-curves bit mask, 17..20 HSL on, hue (degrees), sat / 100 and
+    light / 100 when `hsl_adjust` ({'hue', 'sat', 'light'}) is given."""
     u = np.zeros(VIZ_UNIFORM_WORDS, dtype=np.uint32)
+    u[16] = curves_mask
+    if hsl_adjust is not None:
+        u[17] = 1
+        u[18] = _f32_bits(hsl_adjust["hue"])
+        u[19] = _f32_bits(hsl_adjust["sat"] / 100.0)
+        u[20] = _f32_bits(hsl_adjust["light"] / 100.0)
     u[0], u[1] = width, height
     for i, v in enumerate((k1, k2, tm.gamma, tm.vibrancy, tm.highlight_power, tm.gamma_threshold)):
         u[2 + i] = _f32_bits(v)

@@ -18,7 +18,7 @@ from typing import Optional
 
 import numpy as np
 
-from . import pack
+from . import curves, pack
 from ._lib import check, load
 from .genome import Genome, Tonemap
 
@@ -164,9 +164,12 @@ class Renderer:
                                                        d.min_rad, d.curve, k1, k2,
                                                        self.oversample))
         bg = genome.background or (0.0, 0.0, 0.0)
+        mask = curves.active_mask(genome.channel_curves)
+        if mask:
+            self.set_curves(curves.bake_curves(genome.channel_curves))
         viz = np.ascontiguousarray(pack.viz_uniforms(self.width, self.height, k1, k2, tm,
                                                      self.oversample, len(self.taps), bg,
-                                                     transparent))
+                                                     transparent, mask, genome.hsl_adjust))
         check(self._lib.p3_present(self._h, _up(de) if de is not None else None, _up(viz)),
               self._h)
 
@@ -192,6 +195,14 @@ class Renderer:
         out = np.empty((self.super_h, self.super_w, 4), dtype=np.uint32)
         check(self._lib.p3_read_histogram(self._h, _up(out), out.size), self._h)
         return out
+
+    def set_curves(self, lut: Optional[np.ndarray]) -> None:
+        """Upload the (5, 256) channel-curves LUT (None clears it)."""
+        if lut is None:
+            check(self._lib.p3_set_curves(self._h, None, 0), self._h)
+            return
+        arr = np.ascontiguousarray(lut, dtype=np.float32).reshape(-1)
+        check(self._lib.p3_set_curves(self._h, _fp(arr), arr.size), self._h)
 
     def write_histogram(self, hist: np.ndarray) -> None:
         h = np.ascontiguousarray(hist, dtype=np.uint32)
