@@ -5,7 +5,7 @@ The cases restate the reference's own unit tests with their expected
 numbers (src/interpolate.test.ts, src/motion.test.ts, src/easing.test.ts,
 src/temporal-filter.test.ts, src/channel-curves.test.ts), plus closed forms
 that a dropped term or a swapped operand would break: a pure rotation keeps
-unit magnitude, Catmull-Rom weights reproduce a cubic exactly, permutations
+unit magnitude, Catmull-Rom weights reproduce quadratics exactly, permutations
 gather in the documented direction.
 """
 import math
