@@ -112,6 +112,13 @@ int p3_read_index(p3_renderer* r, uint32_t* out, size_t n);
  * Words 17..20 are hslActive, hue (degrees), saturation factor and lightness
  * offset (visualize.ts:186-193, applied as visualize_f32.wgsl:240-266). */
 int p3_set_curves(p3_renderer* r, const float* lut, size_t n);
+/* histogramToLinearRgba (src/export-linear.ts:24-47) on the device, for the
+ * scene-linear EXR export (bin/pyr3-render.ts:520-537): W x H x 4 f32, each
+ * output pixel the sum of its os x os histogram cells times (k1 k2) / os^2
+ * (f64 arithmetic, as the reference), alpha clamped to [0, 1]. k1, k2 from
+ * the calibration of the render (src/calibration.ts:37-43). Reads the
+ * histogram in place: no histogram readback. */
+int p3_read_linear(p3_renderer* r, double k1, double k2, float* out, size_t n);
 int p3_read_filtered(p3_renderer* r, float* out, size_t n);      /* DE output grid */
 int p3_read_image(p3_renderer* r, uint8_t* out, size_t n);       /* W x H x 4 RGBA8 */
 int p3_read_image_f32(p3_renderer* r, float* out, size_t n);     /* pre-quantisation RGBA */

@@ -257,3 +257,16 @@ def dc_color(idx, x, y, params=(0.0, 0.0, 0.0)):
 
 def variation_supported(idx: int) -> bool:
     return bool(load().orc_variation_supported(idx))
+
+
+def histogram_to_linear_rgba(hist, width: int, height: int, oversample: int, k1: float,
+                             k2: float) -> np.ndarray:
+    """src/export-linear.ts:24-47: box sum of each os x os block of the raw
+    histogram (exact integers), times (k1 k2) / os^2 in f64, stored as f32;
+    alpha = clamp(count sum x scale, 0, 1)."""
+    h = np.asarray(hist, dtype=np.uint64).reshape(height, oversample, width, oversample, 4)
+    sums = h.sum(axis=(1, 3)).astype(np.float64)          # exact below 2^53
+    scale = (k1 * k2) / (oversample * oversample)
+    out = sums * scale
+    out[..., 3] = np.clip(out[..., 3], 0.0, 1.0)
+    return out.astype(np.float32)

@@ -90,6 +90,7 @@ struct p3_renderer {
   DevBuf<uint8_t> rad;
   DevBuf<uint32_t> tile_rmax, image8;
   DevBuf<float> taps, lut, palette, xforms;
+  DevBuf<float4> linear;        // p3_read_linear output (allocated on first use)
   DevBuf<float> curves;         // 5 x 256 channel-curve LUT (p3_set_curves)
   std::vector<float> curves_host;
   bool curves_set = false;
@@ -665,6 +666,23 @@ int p3_present(p3_renderer* r, const uint32_t* de, const uint32_t* viz) {
   r->have_image = true;
   r->have_image_f32 = r->want_image_f32;
   return P3_OK;
+}
+
+int p3_read_linear(p3_renderer* r, double k1, double k2, float* out, size_t n) {
+  if (!r) return P3_E_ARG;
+  const size_t want = (size_t)r->width * r->height * 4;
+  if (!out || n != want) return r->fail(P3_E_ARG, "linear export needs %zu floats", want);
+  if (!std::isfinite(k1) || !std::isfinite(k2))
+    return r->fail(P3_E_ARG, "linear export calibration must be finite");
+  int rc;
+  if ((rc = r->hip(r->linear.ensure((size_t)r->width * r->height), "hipMalloc(linear)"))) return rc;
+  p3::LinArgs a{};
+  a.hist = (const uint4*)r->hist.p;
+  a.out = r->linear.p;
+  a.width = (uint32_t)r->width; a.height = (uint32_t)r->height; a.os = (uint32_t)r->os;
+  a.scale = (k1 * k2) / (double)((size_t)r->os * r->os);
+  if ((rc = r->hip(p3::launch_linear_export(a, r->stream), "linear export launch"))) return rc;
+  return read_dev(r, out, r->linear.p, want * sizeof(float), "read linear export");
 }
 
 int p3_set_curves(p3_renderer* r, const float* lut, size_t n) {

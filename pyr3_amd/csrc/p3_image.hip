@@ -339,6 +339,38 @@ __global__ void __launch_bounds__(256) tonemap_kernel(TmArgs A) {
   if (A.outf) A.outf[oi] = o;
 }
 
+// ------------------------------------------------------- linear export --
+
+// histogramToLinearRgba (src/export-linear.ts:24-47) on the device: each
+// output pixel sums its os x os super-pixels (u64, exact), scales by
+// (k1 k2) / os^2 in f64 and stores f32 (round to nearest), alpha clamped to
+// [0, 1] -- the reference's f64 arithmetic, so the result is bit-identical.
+// HBM-bound: 16 B read per super-pixel, 16 B written per output pixel.
+__global__ void __launch_bounds__(256) linear_export(LinArgs A) {
+  const uint32_t x = blockIdx.x * 16 + threadIdx.x;
+  const uint32_t y = blockIdx.y * 16 + threadIdx.y;
+  if (x >= A.width || y >= A.height) return;
+  const uint32_t N = A.os;
+  const size_t SW = (size_t)A.width * N;
+  uint64_t r = 0, g = 0, b = 0, c = 0;
+  for (uint32_t sy = 0; sy < N; ++sy)
+    for (uint32_t sx = 0; sx < N; ++sx) {
+      const uint4 h = A.hist[((size_t)y * N + sy) * SW + (size_t)x * N + sx];
+      r += h.x; g += h.y; b += h.z; c += h.w;
+    }
+  const double s = A.scale;
+  A.out[(size_t)y * A.width + x] =
+      make_float4((float)((double)r * s), (float)((double)g * s), (float)((double)b * s),
+                  (float)fmin(1.0, fmax(0.0, (double)c * s)));
+}
+
+hipError_t launch_linear_export(const LinArgs& a, hipStream_t stream) {
+  const dim3 block(16, 16);
+  const dim3 grid((a.width + 15) / 16, (a.height + 15) / 16);
+  linear_export<<<grid, block, 0, stream>>>(a);
+  return hipGetLastError();
+}
+
 hipError_t launch_tonemap(const TmArgs& a, bool de, hipStream_t stream) {
   const dim3 block(16, 16);
   const dim3 grid((a.width + 15) / 16, (a.height + 15) / 16);

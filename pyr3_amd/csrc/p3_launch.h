@@ -40,6 +40,14 @@ struct TmArgs {
   float hsl_hue, hsl_sat, hsl_light;   // degrees, factor, offset
 };
 
+struct LinArgs {
+  const uint4* hist;        // W*os x H*os x 4 u32
+  float4* out;              // W x H RGBA f32
+  uint32_t width, height, os;
+  double scale;             // (k1 * k2) / os^2
+};
+hipError_t launch_linear_export(const LinArgs& a, hipStream_t stream);
+
 // Chaos kernel specialised for the shape of `host_xforms` (p3_rtc.cpp).
 hipError_t launch_chaos_rtc(const ChaosArgs& a, const float* host_xforms, int n_regular,
                             int final_slot, bool saturating, hipStream_t stream,

@@ -1,6 +1,6 @@
 """PNG output with the genome embedded, and the tEXt reader behind `.png` loads.
 
-* `encode_png(rgba)`: 8-bit RGBA, non-interlaced; each row takes the filter
+* `encode_png(rgba)`: 8- or 16-bit RGBA, non-interlaced; each row takes the filter
   (None, Sub or Up) with the smallest sum of absolute filtered bytes, then
   zlib.
 * `inject_text_chunk` / `read_text_chunks`: tEXt chunks as
@@ -49,14 +49,21 @@ def _filtered_rows(img: np.ndarray) -> bytes:
 
 
 def encode_png(rgba: np.ndarray, level: int = 6) -> bytes:
-    """(H, W, 4) uint8 RGBA -> PNG bytes."""
-    img = np.ascontiguousarray(rgba, dtype=np.uint8)
+    """(H, W, 4) RGBA -> PNG bytes: uint8 gives 8-bit samples, uint16 16-bit
+    samples stored big-endian (src/png16-encode.ts:39-82)."""
+    img = np.asarray(rgba)
     if img.ndim != 3 or img.shape[2] != 4:
         raise ValueError(f"expected (H, W, 4) RGBA, got {img.shape}")
     h, w, _ = img.shape
-    ihdr = struct.pack(">IIBBBBB", w, h, 8, 6, 0, 0, 0)
+    if img.dtype == np.uint16:
+        depth = 16
+        raw = np.ascontiguousarray(img, dtype=">u2").view(np.uint8).reshape(h, w, 8)
+    else:
+        depth = 8
+        raw = np.ascontiguousarray(img, dtype=np.uint8)
+    ihdr = struct.pack(">IIBBBBB", w, h, depth, 6, 0, 0, 0)
     return (SIGNATURE + _chunk(b"IHDR", ihdr)
-            + _chunk(b"IDAT", zlib.compress(_filtered_rows(img), level)) + _chunk(b"IEND", b""))
+            + _chunk(b"IDAT", zlib.compress(_filtered_rows(raw), level)) + _chunk(b"IEND", b""))
 
 
 def _ascii_safe(value: str) -> str:
@@ -115,7 +122,8 @@ def read_text_chunks(png: bytes) -> Dict[str, str]:
 
 
 def encode_render_png(rgba: np.ndarray, genome: Genome) -> bytes:
-    """The rendered image with the genome under tEXt key `pyr3`."""
+    """The rendered image (RGBA8, or RGBA16 as uint16) with the genome under
+    tEXt key `pyr3`."""
     text = json.dumps(genome_to_json(genome), separators=(",", ":"), ensure_ascii=False)
     return inject_text_chunk(encode_png(rgba), "pyr3", text)
 

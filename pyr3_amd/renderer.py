@@ -196,6 +196,14 @@ class Renderer:
         check(self._lib.p3_read_histogram(self._h, _up(out), out.size), self._h)
         return out
 
+    def read_linear(self, k1: float, k2: float) -> np.ndarray:
+        """(H, W, 4) f32 scene-linear RGBA from the histogram on the device
+        (histogramToLinearRgba); k1, k2 as `pack.calibration` gives them."""
+        out = np.empty((self.height, self.width, 4), dtype=np.float32)
+        check(self._lib.p3_read_linear(self._h, float(k1), float(k2), _fp(out), out.size),
+              self._h)
+        return out
+
     def set_curves(self, lut: Optional[np.ndarray]) -> None:
         """Upload the (5, 256) channel-curves LUT (None clears it)."""
         if lut is None:
