@@ -90,6 +90,17 @@ def _compile_worker(xforms: np.ndarray, n_rows: int, final_slot: int, cache: str
                                 final_slot)
 
 
+def _ready(pg) -> bool:
+    """Whether the shape's kernel is compiled (in this process or the cache)."""
+    from . import _lib
+    xf = np.ascontiguousarray(pg.xforms, dtype=np.float32)
+    rc = _lib.load().p3_chaos_ready(xf.ctypes.data_as(C.POINTER(C.c_float)), xf.shape[0],
+                                    pg.distrib.shape[0], pack.final_xform_slot(pg.expanded))
+    if rc < 0:
+        raise ValueError(f"invalid genome shape for the chaos kernel (code {rc})")
+    return rc == 1
+
+
 def precompile(genomes: Sequence[Genome], workers: Optional[int] = None,
                cache_dir: Optional[str] = None) -> int:
     """Compile the chaos kernels of every distinct shape among `genomes` into
@@ -167,31 +178,41 @@ class BatchRenderer:
         by_key = {}
         for i, j in enumerate(jobs):
             by_key.setdefault(_shape_key(j.genome), []).append(i)
-        own = self._workers is None
-        ex = (ProcessPoolExecutor(max_workers=self.compile_workers, mp_context=get_context("spawn"))
-              if own else self._workers)
-        try:
-            futs = {}
-            for key, pg in shapes.items():
-                e = pg.expanded
-                futs[ex.submit(_compile_worker, pg.xforms, pg.distrib.shape[0],
-                               pack.final_xform_slot(e), self.cache_dir)] = key
-            t_sub = time.perf_counter()
-            results: List[Optional[JobResult]] = [None] * len(jobs)
-            n_done = 0
-            for f in as_completed(futs):
-                rc = f.result()
-                if rc != 0:
-                    raise RuntimeError(f"kernel specialisation failed for shape {futs[f]}")
-                for i in by_key[futs[f]]:
-                    results[i] = self._render_one(jobs[i], packed[i], n_done, read_images)
-                    order.append(i)
-                    n_done += 1
-                    if progress and n_done % 64 == 0:
-                        progress(n_done)
-        finally:
-            if own:
-                ex.shutdown()
+        results: List[Optional[JobResult]] = [None] * len(jobs)
+        n_done = 0
+
+        def render_shape(key):
+            nonlocal n_done
+            for i in by_key[key]:
+                results[i] = self._render_one(jobs[i], packed[i], n_done, read_images)
+                order.append(i)
+                n_done += 1
+                if progress and n_done % 64 == 0:
+                    progress(n_done)
+
+        # shapes already compiled (this process or the code-object cache)
+        # render at once; the others go to the compile workers
+        pending = {key: pg for key, pg in shapes.items() if not _ready(pg)}
+        t_sub = time.perf_counter()
+        for key in shapes:
+            if key not in pending:
+                render_shape(key)
+        if pending:
+            own = self._workers is None
+            ex = (ProcessPoolExecutor(max_workers=self.compile_workers,
+                                      mp_context=get_context("spawn")) if own else self._workers)
+            try:
+                futs = {}
+                for key, pg in pending.items():
+                    futs[ex.submit(_compile_worker, pg.xforms, pg.distrib.shape[0],
+                                   pack.final_xform_slot(pg.expanded), self.cache_dir)] = key
+                for f in as_completed(futs):
+                    if f.result() != 0:
+                        raise RuntimeError(f"kernel specialisation failed for shape {futs[f]}")
+                    render_shape(futs[f])
+            finally:
+                if own:
+                    ex.shutdown()
         for r in self._pool:
             if r is not None:
                 r.sync()

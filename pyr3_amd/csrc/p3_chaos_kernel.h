@@ -312,12 +312,8 @@ P3_INLINE uint32_t draw_pick(const ChaosArgs& A, const BlockLds<kB>& L, Rng& rng
   const uint32_t row = prev >= 0 ? (uint32_t)prev : A.fallback_row;
   raw = rng.irand();
   const uint32_t col = raw & (kGrain - 1);
-  // kRegular < 0: the xform count is only known at run time; boundaries past
-  // the last xform are 16384, above every column, so all 15 can be counted
-  constexpr int kCmp = Spec::kRegular < 0 ? kPickCols - 1 : Spec::kRegular - 1;
-  const bool lds = Spec::kRegular < 0 ? A.fallback_row < (uint32_t)kPickRows
-                                      : Spec::kRegular < kPickRows;
-  if (lds && A.pick_bounds != nullptr) {
+  constexpr int kCmp = Spec::kRegular - 1;   // boundaries between the n xforms
+  if (Spec::kRegular < kPickRows && A.pick_bounds != nullptr) {
     const uint4* b = (const uint4*)(L.pick_b + row * kPickCols);
     const uint4 q0 = b[0];
     const uint4 q1 = kCmp > 8 ? b[1] : make_uint4(0u, 0u, 0u, 0u);
@@ -351,8 +347,7 @@ P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L, uint32_t block)
   const uint32_t idle = A.fallback_row + 1u;   // bucket of finished walkers
 
   for (int k = t; k < 2 * (kMaxXforms + 2); k += kB) (&L.bucket[0][0])[k] = 0u;
-  if ((Spec::kRegular < 0 ? A.fallback_row < (uint32_t)kPickRows : Spec::kRegular < kPickRows) &&
-      A.pick_bounds != nullptr)
+  if (Spec::kRegular < kPickRows && A.pick_bounds != nullptr)
     for (uint32_t k = t; k < (A.fallback_row + 1u) * kPickCols; k += kB)
       L.pick_b[k] = ((const P3_CONST uint16_t*)A.pick_bounds)[k];
   __syncthreads();
