@@ -239,23 +239,35 @@ def learned_path(r: Renderer, g, a, ref_img, walkers: int):
     for i in range(a.warmup):
         fl.render(ql, 77 + i)
     r.sync()
+    # timed: frames back to back (the host queues frame i+1 while the GPU
+    # renders frame i), one synchronize after the last
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        fl.render(ql, 700 + i)
+    r.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / a.steps
+    # the same frames again with HIP events around each kernel, one
+    # synchronize per frame (kernel breakdown and the serialised frame time)
     r.set_timing(True)
-    dn = 0.0
+    dn = chaos = 0.0
     t0 = time.perf_counter()
     for i in range(a.steps):
         fl.render(ql, 700 + i)
         r.sync()
-        dn += r.read_timings()["denoise"]
-    wall = time.perf_counter() - t0
+        t = r.read_timings()
+        dn += t["denoise"]
+        chaos += t["chaos"]
+    serial = (time.perf_counter() - t0) * 1e3 / a.steps
     r.set_timing(False)
     r.set_image_f32(False)
     den.destroy()
-    ms = wall * 1e3 / a.steps
-    log(f"learned path: q={ql} {scores[ql]:.2f} dB, {ms:.3f} ms/frame (denoise {dn / a.steps:.3f} ms)")
+    log(f"learned path: q={ql} {scores[ql]:.2f} dB, {ms:.3f} ms/frame ({serial:.3f} with a "
+        f"sync per frame; chaos {chaos / a.steps:.3f} ms, denoise {dn / a.steps:.3f} ms)")
     from pyr3_amd.denoise.model import macs_per_pixel
     flops = 2.0 * macs_per_pixel() * r.width * r.height
     dms = dn / a.steps
     return {"reached": True, "ms_per_frame": round(ms, 3), "quality": ql,
+            "ms_per_frame_serial": round(serial, 3), "chaos_ms": round(chaos / a.steps, 4),
             "psnr_db": round(scores[ql], 2), "denoise_ms": round(dms, 4),
             "denoise_roofline": {"bound": "mfma", "achieved": round(flops / (dms * 1e-3) / 1e12, 2),
                                  "peak": 2500.0, "unit": "TFLOP/s",
@@ -319,18 +331,27 @@ def main():
     for i in range(a.warmup):
         frame.render(q, 77 + i)
     r.sync()
-    r.set_timing(True)
-    chaos_ms = de_ms = tm_ms = reset_ms = 0.0
+    # timed: K frames back to back (the host queues frame i+1 while the GPU
+    # renders frame i), bracketed by synchronizes
     t0 = time.perf_counter()
     walkers = iters = samples = 0
     for i in range(a.steps):
         walkers, iters, samples = frame.render(q, 500 + i)
-        r.sync()
-        t = r.read_timings()
-        chaos_ms += t["chaos"]; de_ms += t["density"]; tm_ms += t["tonemap"]; reset_ms += t["reset"]
     r.sync()
     wall = time.perf_counter() - t0
     ms = wall * 1e3 / a.steps
+    # the same frames again with HIP events around each kernel and one
+    # synchronize per frame: the kernel breakdown and the serialised frame time
+    r.set_timing(True)
+    chaos_ms = de_ms = tm_ms = reset_ms = 0.0
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        frame.render(q, 500 + i)
+        r.sync()
+        t = r.read_timings()
+        chaos_ms += t["chaos"]; de_ms += t["density"]; tm_ms += t["tonemap"]; reset_ms += t["reset"]
+    serial_ms = (time.perf_counter() - t0) * 1e3 / a.steps
+    r.set_timing(False)
     chaos_ms /= a.steps; de_ms /= a.steps; tm_ms /= a.steps; reset_ms /= a.steps
     giters = walkers * (iters + a.fuse) / (chaos_ms * 1e-3) / 1e9
     gsamples = samples / (chaos_ms * 1e-3) / 1e9
@@ -352,6 +373,7 @@ def main():
         "steps": a.steps,
         "warmup": a.warmup,
         "ms_per_step": round(ms, 3),
+        "ms_per_frame_serial": round(serial_ms, 3),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
