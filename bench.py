@@ -239,13 +239,30 @@ def learned_path(r: Renderer, g, a, ref_img, walkers: int):
     for i in range(a.warmup):
         fl.render(ql, 77 + i)
     r.sync()
-    # timed: frames back to back (the host queues frame i+1 while the GPU
-    # renders frame i), one synchronize after the last
+    # timed: frames back to back, alternating over `a.streams` renderers (each
+    # its own HIP stream, buffers and denoiser activations), so one frame's
+    # DE / tonemap / denoiser overlap the next frame's chaos kernel; one
+    # synchronize of every stream after the last frame
+    extra = [Renderer(r.width, r.height, r.oversample) for _ in range(a.streams - 1)]
+    dens = [Denoiser.from_npz(WEIGHTS) for _ in extra]
+    frames = [fl] + [Frame(rx, g, a.fuse, walkers=walkers, denoiser=dx)
+                     for rx, dx in zip(extra, dens)]
+    for rx in extra:
+        rx.set_image_f32(True)
+    for f in frames[1:]:
+        f.render(ql, 76)
+    for f in frames:
+        f.r.sync()
     t0 = time.perf_counter()
     for i in range(a.steps):
-        fl.render(ql, 700 + i)
-    r.sync()
+        frames[i % len(frames)].render(ql, 700 + i)
+    for f in frames:
+        f.r.sync()
     ms = (time.perf_counter() - t0) * 1e3 / a.steps
+    for dx in dens:
+        dx.destroy()
+    for rx in extra:
+        rx.destroy()
     # the same frames again with HIP events around each kernel, one
     # synchronize per frame (kernel breakdown and the serialised frame time)
     r.set_timing(True)
@@ -278,13 +295,15 @@ def learned_path(r: Renderer, g, a, ref_img, walkers: int):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--ref-quality", type=float, default=4000)
     ap.add_argument("--target-db", type=float, default=38.0)
     ap.add_argument("--fuse", type=int, default=pack.FUSE)
+    ap.add_argument("--streams", type=int, default=2,
+                    help="renderers (HIP streams) the timed frames alternate over")
     ap.add_argument("--walkers", type=int, default=None,
                     help="skip the walker autotune and use this count")
     ap.add_argument("--quality", type=float, default=None,
@@ -331,13 +350,22 @@ def main():
     for i in range(a.warmup):
         frame.render(q, 77 + i)
     r.sync()
-    # timed: K frames back to back (the host queues frame i+1 while the GPU
-    # renders frame i), bracketed by synchronizes
+    # timed: K frames back to back, alternating over `a.streams` renderers
+    # (each its own HIP stream and buffers): the host queues frame i+1 while
+    # the GPU renders frame i, and one frame's DE / tonemap overlap the next
+    # frame's chaos kernel; synchronizes of every stream before and after
+    extra = [Renderer(W, H, 1) for _ in range(a.streams - 1)]
+    frames = [frame] + [Frame(rx, g, a.fuse, walkers=frame.walkers) for rx in extra]
+    for f in frames[1:]:
+        f.render(q, 76)
+    for f in frames:
+        f.r.sync()
     t0 = time.perf_counter()
     walkers = iters = samples = 0
     for i in range(a.steps):
-        walkers, iters, samples = frame.render(q, 500 + i)
-    r.sync()
+        walkers, iters, samples = frames[i % len(frames)].render(q, 500 + i)
+    for f in frames:
+        f.r.sync()
     wall = time.perf_counter() - t0
     ms = wall * 1e3 / a.steps
     # the same frames again with HIP events around each kernel and one
@@ -352,6 +380,8 @@ def main():
         chaos_ms += t["chaos"]; de_ms += t["density"]; tm_ms += t["tonemap"]; reset_ms += t["reset"]
     serial_ms = (time.perf_counter() - t0) * 1e3 / a.steps
     r.set_timing(False)
+    for rx in extra:
+        rx.destroy()
     chaos_ms /= a.steps; de_ms /= a.steps; tm_ms /= a.steps; reset_ms /= a.steps
     giters = walkers * (iters + a.fuse) / (chaos_ms * 1e-3) / 1e9
     gsamples = samples / (chaos_ms * 1e-3) / 1e9
@@ -381,7 +411,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"plain-{W}x{H}-os1", "genome_seed": k, "quality": q,
                    "target_db": a.target_db, "ref_quality": a.ref_quality,
-                   "walkers": walkers, "iters": iters, "fuse": a.fuse,
+                   "walkers": walkers, "iters": iters, "fuse": a.fuse, "streams": a.streams,
                    "walker_autotune_ms": {str(k): v for k, v in tune.items()},
                    "psnr_ladder": {str(kq): round(v, 2) for kq, v in sorted(scores.items())}},
         "giters_per_s": round(giters, 3),
