@@ -236,6 +236,19 @@ def learned_path(r: Renderer, g, a, ref_img, walkers: int):
     if ql is None:
         log(f"learned path: no q up to {max(scores)} reached {a.target_db} dB")
         return {"reached": False, "psnr_ladder": {str(k): round(v, 2) for k, v in sorted(scores.items())}}
+    # walker count for this quality (at low q the 200 unsplatted fuse turns
+    # per walker dominate, so fewer, longer walkers can be faster); kept only
+    # if the frame still reaches the target PSNR with it
+    tuned, tune_ms = tune_walkers(fl, ql)
+    if tuned != walkers:
+        fl.walkers = tuned
+        fl.render(ql, 1000 + ql)
+        db = psnr(fl.r.read_image(), ref_img)
+        log(f"learned path: {tuned} walkers at q={ql}: {db:.2f} dB")
+        if db >= a.target_db:
+            walkers, scores[ql] = tuned, db
+        else:
+            fl.walkers = walkers
     for i in range(a.warmup):
         fl.render(ql, 77 + i)
     r.sync()
@@ -283,7 +296,8 @@ def learned_path(r: Renderer, g, a, ref_img, walkers: int):
     from pyr3_amd.denoise.model import macs_per_pixel
     flops = 2.0 * macs_per_pixel() * r.width * r.height
     dms = dn / a.steps
-    return {"reached": True, "ms_per_frame": round(ms, 3), "quality": ql,
+    return {"reached": True, "ms_per_frame": round(ms, 3), "quality": ql, "walkers": walkers,
+            "walker_autotune_ms": {str(k): v for k, v in tune_ms.items()},
             "ms_per_frame_serial": round(serial, 3), "chaos_ms": round(chaos / a.steps, 4),
             "psnr_db": round(scores[ql], 2), "denoise_ms": round(dms, 4),
             "denoise_roofline": {"bound": "mfma", "achieved": round(flops / (dms * 1e-3) / 1e12, 2),
