@@ -396,6 +396,9 @@ def main():
     r.set_timing(False)
     for rx in extra:
         rx.destroy()
+    # the device's rate of the splat's atomics (uniform scatter over the same
+    # histogram size); the next render resets the histogram it overwrites
+    probe_rate, per_splat = r.probe_atomics()
     chaos_ms /= a.steps; de_ms /= a.steps; tm_ms /= a.steps; reset_ms /= a.steps
     giters = walkers * (iters + a.fuse) / (chaos_ms * 1e-3) / 1e9
     gsamples = samples / (chaos_ms * 1e-3) / 1e9
@@ -432,15 +435,25 @@ def main():
         "gsamples_per_s": round(gsamples, 3),
         "kernels_ms": {"chaos": round(chaos_ms, 4), "density": round(de_ms, 4),
                        "tonemap": round(tm_ms, 4), "reset": round(reset_ms, 4)},
+        "atomic_roofline": {"kernel": "chaos_kernel", "requests_per_splat": per_splat,
+                            "achieved_g_requests": round(per_splat * samples / (chaos_ms * 1e-3)
+                                                         / 1e9, 2),
+                            "probe_peak_g_requests": round(probe_rate / 1e9, 2),
+                            "frac": round(per_splat * samples / (chaos_ms * 1e-3) / probe_rate, 4),
+                            "note": "atomic requests per second: requests_per_splat per "
+                                    "post-fuse sample (upper bound: off-frame samples do not "
+                                    "splat) against p3_probe_atomics, a uniform scatter of "
+                                    "16-byte cells in the same request form over the same "
+                                    "histogram size"},
         "roofline": {"bound": "hbm", "kernel": "chaos_kernel", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": round(traffic["bytes"]) if traffic else None,
                      "traffic_source": traffic["source"] if traffic else None,
-                     "note": "achieved = 16 B of u32 atomic scatter per post-fuse sample; "
+                     "note": "achieved = 16 B of atomic scatter per post-fuse sample; "
                              "traffic = rocprofv3 FETCH_SIZE + WRITE_SIZE of the same launch "
-                             "(atomics leave the L2 at ~21 B per 4-B add); the kernel is "
-                             "VALU/latency bound, not HBM bound"},
+                             "(the atomics leave the L2); the kernel is bound by the rate of "
+                             "atomic requests (atomic_roofline), not by HBM bandwidth"},
         "de_roofline": {"achieved": round(de_bytes / (de_ms * 1e-3) / 1e9, 1) if de_ms else None,
                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(de_bytes / (de_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)

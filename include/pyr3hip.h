@@ -119,6 +119,17 @@ int p3_set_curves(p3_renderer* r, const float* lut, size_t n);
  * the calibration of the render (src/calibration.ts:37-43). Reads the
  * histogram in place: no histogram readback. */
 int p3_read_linear(p3_renderer* r, double k1, double k2, float* out, size_t n);
+
+/* Measurement only (no counterpart in the reference): the device's rate of
+ * the histogram splat's atomic requests, for the chaos kernel's roofline.
+ * blocks x 256 threads each add 1 to the four u32 words of per_thread 16-byte
+ * cells drawn uniformly over the handle's histogram (which it overwrites:
+ * reset before the next render), in the splat's request form (two 64-bit
+ * adds per cell, or four 32-bit ones with P3_HIST_ADD=quads). Writes the
+ * atomic requests per second (HIP events on the handle's stream) and the
+ * requests per splat. */
+int p3_probe_atomics(p3_renderer* r, uint32_t blocks, uint32_t per_thread,
+                     double* requests_per_s, int* requests_per_splat);
 int p3_read_filtered(p3_renderer* r, float* out, size_t n);      /* DE output grid */
 int p3_read_image(p3_renderer* r, uint8_t* out, size_t n);       /* W x H x 4 RGBA8 */
 int p3_read_image_f32(p3_renderer* r, float* out, size_t n);     /* pre-quantisation RGBA */

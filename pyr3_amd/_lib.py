@@ -32,6 +32,8 @@ SIGNATURES = {
     "p3_read_index": (_I, [_P, _UP, _SZ]),
     "p3_set_curves": (_I, [_P, _FP, _SZ]),
     "p3_read_linear": (_I, [_P, C.c_double, C.c_double, _FP, _SZ]),
+    "p3_probe_atomics": (_I, [_P, C.c_uint32, C.c_uint32, C.POINTER(C.c_double),
+                              C.POINTER(C.c_int)]),
     "p3_read_filtered": (_I, [_P, _FP, _SZ]),
     "p3_read_image": (_I, [_P, _BP, _SZ]),
     "p3_read_image_f32": (_I, [_P, _FP, _SZ]),

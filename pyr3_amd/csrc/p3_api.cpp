@@ -109,6 +109,11 @@ struct p3_renderer {
   bool have_image = false;
   bool have_filtered = false;
   unsigned long long hist_bound = 0;  // upper bound on any histogram word
+  // event counts of the saturating pair adds (4 u32 per bin): all zero when
+  // sat_ev_clean; sat_dirty when a dispatch may have counted events that
+  // hist_fixup has not yet applied to the histogram
+  DevBuf<uint32_t> sat_ev;
+  bool sat_ev_clean = false, sat_dirty = false;
   int trace_walkers = 0, trace_iters = 0;
   bool timing = false;
   bool want_image_f32 = false;
@@ -212,6 +217,7 @@ int build(p3_renderer* r, int width, int height, int os, const float* taps, int 
   r->palette_set = false;
   r->have_image = r->have_filtered = r->have_image_f32 = false;
   r->hist_bound = 0;
+  r->sat_ev_clean = r->sat_dirty = false;
   r->last_xforms.clear();
   r->last_distrib.clear();
   return P3_OK;
@@ -276,6 +282,35 @@ int validate_dispatch(p3_renderer* r, const float* xforms, int n_slots, const ui
 }
 
 // Kernel arguments of one dispatch whose records and table are at dxf / dds.
+// Event buffer of the saturating pair adds, sized to the grid and zeroed.
+int prepare_sat_events(p3_renderer* r) {
+  int rc;
+  if ((rc = r->hip(r->sat_ev.ensure(r->hist.n), "hipMalloc(saturation events)"))) return rc;
+  if (!r->sat_ev_clean) {
+    if ((rc = r->hip(p3::launch_fill(r->sat_ev.p, r->hist.n * 4, 0u, r->stream),
+                     "clear saturation events")))
+      return rc;
+    r->sat_ev_clean = true;
+  }
+  return P3_OK;
+}
+
+// Applies pending saturation events: every reader of the histogram calls it.
+int settle_hist(p3_renderer* r) {
+  if (!r->sat_dirty) return P3_OK;
+  r->sat_dirty = false;
+  return r->hip(p3::launch_hist_fixup(r->hist.p, r->sat_ev.p, r->hist.n / 4, r->stream),
+                "saturation fix-up");
+}
+
+// Drops pending events when the histogram is overwritten wholesale.
+int drop_sat_events(p3_renderer* r) {
+  if (!r->sat_dirty) return P3_OK;
+  r->sat_dirty = false;
+  return r->hip(p3::launch_fill(r->sat_ev.p, r->hist.n * 4, 0u, r->stream),
+                "clear saturation events");
+}
+
 p3::ChaosArgs dispatch_args(p3_renderer* r, const uint32_t* u, int n_regular, int32_t final_idx,
                             const float* dxf, const uint32_t* dds) {
   p3::ChaosArgs a{};
@@ -305,6 +340,7 @@ p3::ChaosArgs dispatch_args(p3_renderer* r, const uint32_t* u, int n_regular, in
   a.trap_freq = bits_f(u[26]); a.trap_strength = bits_f(u[27]);
   a.phase_strength = bits_f(u[28]); a.phase_freq = bits_f(u[29]);
   a.idx_sum = nullptr;
+  a.sat_events = nullptr;
   return a;
 }
 
@@ -396,6 +432,10 @@ int iterate_impl(p3_renderer* r, const float* xforms, int n_slots, const uint32_
   // Each walker splats at most `iters` times, each adding <= 255 per word.
   r->hist_bound += 255ULL * (unsigned long long)walkers * (unsigned long long)a.iters;
   const bool sat = r->hist_bound >= 0xffffffffULL;
+  if (sat && p3::pair_adds()) {
+    if ((rc = prepare_sat_events(r))) return rc;
+    a.sat_events = r->sat_ev.p;
+  }
   std::string rtc_err;
   {  // compile (or fetch) the kernel for this shape before the timed launch
     const std::vector<char>* code = nullptr;
@@ -407,6 +447,7 @@ int iterate_impl(p3_renderer* r, const float* xforms, int n_slots, const uint32_
       p3::launch_chaos_rtc(a, xforms, n_regular, final_idx, sat, r->stream, rtc_err);
   if (le != hipSuccess)
     return r->fail(P3_E_HIP, "chaos launch: %s %s", hipGetErrorString(le), rtc_err.c_str());
+  if (a.sat_events) r->sat_dirty = true;
   r->toc(T_CHAOS);
   r->have_image = false;
   return P3_OK;
@@ -481,6 +522,11 @@ int iterate_multi_impl(p3_renderer* r, int n_sub, const float* const* xforms, in
     bound += 255ULL * (unsigned long long)args[k].walkers * (unsigned long long)args[k].iters;
   }
   if (M.start[n_sub] == 0) return P3_OK;
+  const bool sat = r->hist_bound + bound >= 0xffffffffULL;
+  if (sat && p3::pair_adds()) {
+    if ((rc = prepare_sat_events(r))) return rc;
+    for (int k = 0; k < n_sub; ++k) args[k].sat_events = r->sat_ev.p;
+  }
   if ((rc = r->hip(hipMemcpyAsync(r->mbounds.p, bounds.data(), bounds.size() * sizeof(uint16_t),
                                   hipMemcpyHostToDevice, r->stream),
                    "upload batched pick bounds")))
@@ -490,7 +536,6 @@ int iterate_multi_impl(p3_renderer* r, int n_sub, const float* const* xforms, in
                    "upload batched args")))
     return rc;
   r->hist_bound += bound;
-  const bool sat = r->hist_bound >= 0xffffffffULL;
   std::string rtc_err;
   {
     const std::vector<char>* code = nullptr;
@@ -503,6 +548,7 @@ int iterate_multi_impl(p3_renderer* r, int n_sub, const float* const* xforms, in
   if (le != hipSuccess)
     return r->fail(P3_E_HIP, "batched chaos launch: %s %s", hipGetErrorString(le),
                    rtc_err.c_str());
+  if (args[0].sat_events) r->sat_dirty = true;
   r->toc(T_CHAOS);
   r->have_image = false;
   return P3_OK;
@@ -579,6 +625,7 @@ int p3_reset(p3_renderer* r, const float* palette_rgba) {
   r->tic(T_RESET);
   if ((rc = r->hip(p3::launch_fill(r->hist.p, r->hist.n * 4, 0u, r->stream), "clear histogram")))
     return rc;
+  if ((rc = drop_sat_events(r))) return rc;
   if (r->idx.p &&
       (rc = r->hip(p3::launch_fill(r->idx.p, r->idx.n * 4, 0u, r->stream), "clear capture")))
     return rc;
@@ -615,6 +662,7 @@ int p3_present(p3_renderer* r, const uint32_t* de, const uint32_t* viz) {
   if ((int)viz[9] != r->n_taps)
     return r->fail(P3_E_ARG, "tonemap fwidth %u does not match %d taps", viz[9], r->n_taps);
   int rc;
+  if ((rc = settle_hist(r))) return rc;
   if (de) {
     if ((int)de[0] != r->sw || (int)de[1] != r->sh)
       return r->fail(P3_E_ARG, "DE uniforms %ux%u do not match super-res grid %dx%d", de[0], de[1],
@@ -668,6 +716,35 @@ int p3_present(p3_renderer* r, const uint32_t* de, const uint32_t* viz) {
   return P3_OK;
 }
 
+int p3_probe_atomics(p3_renderer* r, uint32_t blocks, uint32_t per_thread,
+                     double* requests_per_s, int* requests_per_splat) {
+  if (!r || !requests_per_s || !requests_per_splat) return P3_E_ARG;
+  if (blocks == 0 || blocks > 65535 || per_thread == 0 || per_thread > 4096)
+    return r->fail(P3_E_ARG, "probe size out of range (blocks 1..65535, per_thread 1..4096)");
+  const uint32_t cells = (uint32_t)((size_t)r->sw * r->sh);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc;
+  if ((rc = drop_sat_events(r))) return rc;
+  if ((rc = r->hip(hipEventCreate(&e0), "hipEventCreate")) ||
+      (rc = r->hip(hipEventCreate(&e1), "hipEventCreate")))
+    return rc;
+  float ms = 0.0f;
+  rc = r->hip(hipEventRecord(e0, r->stream), "hipEventRecord");
+  const bool pairs = p3::pair_adds();
+  if (!rc) rc = r->hip(p3::launch_atomic_probe(r->hist.p, cells, blocks, per_thread, 12345u,
+                                                pairs, r->stream), "atomic probe launch");
+  if (!rc) rc = r->hip(hipEventRecord(e1, r->stream), "hipEventRecord");
+  if (!rc) rc = r->hip(hipEventSynchronize(e1), "hipEventSynchronize");
+  if (!rc) rc = r->hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc) return rc;
+  r->have_image = false;   // the histogram now holds probe counts
+  *requests_per_splat = pairs ? 2 : 4;
+  *requests_per_s = *requests_per_splat * 256.0 * blocks * per_thread / (ms * 1e-3);
+  return P3_OK;
+}
+
 int p3_read_linear(p3_renderer* r, double k1, double k2, float* out, size_t n) {
   if (!r) return P3_E_ARG;
   const size_t want = (size_t)r->width * r->height * 4;
@@ -675,6 +752,7 @@ int p3_read_linear(p3_renderer* r, double k1, double k2, float* out, size_t n) {
   if (!std::isfinite(k1) || !std::isfinite(k2))
     return r->fail(P3_E_ARG, "linear export calibration must be finite");
   int rc;
+  if ((rc = settle_hist(r))) return rc;
   if ((rc = r->hip(r->linear.ensure((size_t)r->width * r->height), "hipMalloc(linear)"))) return rc;
   p3::LinArgs a{};
   a.hist = (const uint4*)r->hist.p;
@@ -709,6 +787,8 @@ int p3_set_curves(p3_renderer* r, const float* lut, size_t n) {
 int p3_read_histogram(p3_renderer* r, uint32_t* out, size_t n) {
   if (!r) return P3_E_ARG;
   if (!out || n != r->hist.n) return r->fail(P3_E_ARG, "histogram needs %zu words", r->hist.n);
+  const int rc = settle_hist(r);
+  if (rc) return rc;
   return read_dev(r, out, r->hist.p, n * 4, "read histogram");
 }
 
@@ -747,9 +827,11 @@ int p3_read_image_f32(p3_renderer* r, float* out, size_t n) {
 int p3_write_histogram(p3_renderer* r, const uint32_t* data, size_t n) {
   if (!r) return P3_E_ARG;
   if (!data || n != r->hist.n) return r->fail(P3_E_ARG, "histogram needs %zu words", r->hist.n);
-  int rc = r->hip(hipMemcpyAsync(r->hist.p, data, n * 4, hipMemcpyHostToDevice, r->stream),
-                  "write histogram");
+  int rc = drop_sat_events(r);
   if (rc) return rc;
+  if ((rc = r->hip(hipMemcpyAsync(r->hist.p, data, n * 4, hipMemcpyHostToDevice, r->stream),
+                   "write histogram")))
+    return rc;
   unsigned long long mx = 0;
   for (size_t i = 0; i < n; ++i) mx = std::max<unsigned long long>(mx, data[i]);
   r->hist_bound = mx;

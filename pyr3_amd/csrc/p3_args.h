@@ -43,6 +43,9 @@ struct ChaosArgs {
   float trap_cx, trap_cy, trap_radius, trap_nx, trap_ny, trap_falloff, trap_freq, trap_strength;
   float phase_strength, phase_freq;
   uint32_t* idx_sum;
+  // saturating pair adds: four event counts per bin (low-word carries and
+  // 64-bit wraps of the (R, G) and (B, count) pairs); null otherwise
+  uint32_t* sat_events;
 };
 
 // Batched launch: up to kMaxSub sub-dispatches of one genome shape in one

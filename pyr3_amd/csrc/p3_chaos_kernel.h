@@ -71,10 +71,31 @@ P3_INLINE void add_pair(P3_GLOBAL uint32_t* p, uint32_t lo, uint32_t hi) {
     (void)__hip_atomic_fetch_add((P3_GLOBAL uint64_t*)p, v, __ATOMIC_RELAXED,
                                  __HIP_MEMORY_SCOPE_AGENT);
 }
+// Saturating adds of a word pair (lo, hi) as one 64-bit add. The returned
+// old value is the pair's exact 64-bit prefix, so the thread whose add
+// carries out of the low word, or wraps the 64 bits, knows it: it counts the
+// event in ev[0] (carries) or ev[1] (wraps). Every add crosses at most one
+// multiple of 2^32 in the low word and wraps at most once, so the counts are
+// exact, and hist_fixup (p3_image.hip) turns them into min(sum, 2^32 - 1) per
+// word before the histogram is read.
+P3_INLINE void add_pair_sat(P3_GLOBAL uint32_t* p, P3_GLOBAL uint32_t* ev, uint32_t lo,
+                            uint32_t hi) {
+  const uint64_t v = (uint64_t)lo | ((uint64_t)hi << 32);
+  if (!v) return;
+  const uint64_t o = __hip_atomic_fetch_add((P3_GLOBAL uint64_t*)p, v, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+  if ((uint64_t)(uint32_t)o + lo > 0xffffffffull) (void)add_ret(ev + 0, 1u);
+  if (o + v < o) (void)add_ret(ev + 1, 1u);
+}
+// Four channel adds (R, G, B, count) at b; ev = the bin's four event counts
+// (read only by the saturating pair form).
 template <bool kSat, bool kPairs>
-P3_INLINE void hist_add4(P3_GLOBAL uint32_t* b, uint32_t d0, uint32_t d1, uint32_t d2,
-                         uint32_t d3) {
-  if (!kSat && kPairs) {
+P3_INLINE void hist_add4(P3_GLOBAL uint32_t* b, P3_GLOBAL uint32_t* ev, uint32_t d0, uint32_t d1,
+                         uint32_t d2, uint32_t d3) {
+  if (kSat && kPairs) {
+    add_pair_sat(b + 0, ev + 0, d0, d1);
+    add_pair_sat(b + 2, ev + 2, d2, d3);
+  } else if (!kSat && kPairs) {
     add_pair(b + 0, d0, d1);
     add_pair(b + 2, d2, d3);
   } else if (kSat) {
@@ -280,8 +301,10 @@ P3_INLINE void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32_t fn,
       if (dc_on) { pal.x = dc_rgb.x; pal.y = dc_rgb.y; pal.z = dc_rgb.z; }
       if (A.color_mode != 0u) colour_mode(A, sx, sy, p0x, p0y, pal);
       const float wgt = opacity * 255.0f;
-      P3_GLOBAL uint32_t* b = (P3_GLOBAL uint32_t*)A.hist + ((size_t)yi * A.width + xi) * 4u;
-      hist_add4<kSat, Spec::kPairAdds>(b, f2u_sat(pal.x * wgt), f2u_sat(pal.y * wgt),
+      const size_t bin = ((size_t)yi * A.width + xi) * 4u;
+      P3_GLOBAL uint32_t* b = (P3_GLOBAL uint32_t*)A.hist + bin;
+      P3_GLOBAL uint32_t* ev = (P3_GLOBAL uint32_t*)A.sat_events + bin;
+      hist_add4<kSat, Spec::kPairAdds>(b, ev, f2u_sat(pal.x * wgt), f2u_sat(pal.y * wgt),
                                        f2u_sat(pal.z * wgt), f2u_sat(wgt));
       if (A.idx_sum != nullptr) {   // capture of the colour index (chaos.wgsl:8720-8728)
         const uint32_t d = f2u_sat(fminf(fmaxf(sz, 0.0f), 1.0f) * wgt);

@@ -259,12 +259,14 @@ int chaos_block() {
   return b;
 }
 
-// Histogram adds in the non-saturating kernel: four u32 atomics per splat,
-// or two u64 atomics of word pairs when P3_HIST_ADD=pairs.
+// Histogram adds: two u64 atomics of word pairs per splat (the device's
+// atomic rate is per request, whatever its width), saturating through the
+// event counts when a word may reach 2^32; P3_HIST_ADD=quads selects four u32
+// atomics (returning, pinned on wrap, in the saturating kernel).
 bool pair_adds() {
   static const bool on = [] {
     const char* e = std::getenv("P3_HIST_ADD");
-    return e && std::strcmp(e, "pairs") == 0;
+    return !(e && std::strcmp(e, "quads") == 0);
   }();
   return on;
 }

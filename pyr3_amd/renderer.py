@@ -196,6 +196,15 @@ class Renderer:
         check(self._lib.p3_read_histogram(self._h, _up(out), out.size), self._h)
         return out
 
+    def probe_atomics(self, blocks: int = 4096, per_thread: int = 256):
+        """(atomic requests per second, requests per splat) of a uniform
+        scatter over this renderer's histogram in the splat's request form
+        (the histogram is overwritten: reset before rendering)."""
+        out, per = C.c_double(), C.c_int()
+        check(self._lib.p3_probe_atomics(self._h, blocks, per_thread, C.byref(out),
+                                         C.byref(per)), self._h)
+        return out.value, per.value
+
     def read_linear(self, k1: float, k2: float) -> np.ndarray:
         """(H, W, 4) f32 scene-linear RGBA from the histogram on the device
         (histogramToLinearRgba); k1, k2 as `pack.calibration` gives them."""

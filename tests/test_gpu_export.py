@@ -96,3 +96,18 @@ def test_auto_exposure_fit_on_a_render():
                 or fit.iters == X.AE_MAX_ITERS)
     else:
         assert abs(target / initial - 1) <= X.AE_DEADBAND
+
+
+def test_atomic_probe_counts_every_add():
+    """The scatter-add probe behind the chaos roofline's atomic peak: every
+    add lands (the four words of a cell stay equal, the total is exact)."""
+    from pyr3_amd.renderer import Renderer
+    r = Renderer(64, 48, 1)
+    r.reset(genome(sierpinski(), scale=30.0))
+    rate, per_splat = r.probe_atomics(blocks=64, per_thread=16)
+    h = r.read_histogram().astype(np.int64)
+    r.destroy()
+    assert rate > 0 and per_splat in (2, 4)
+    assert h.sum() == 4 * 64 * 256 * 16
+    assert np.array_equal(h[..., 0], h[..., 3]) and np.array_equal(h[..., 1], h[..., 2])
+    assert np.array_equal(h[..., 0], h[..., 1])

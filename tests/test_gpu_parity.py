@@ -256,6 +256,43 @@ def test_saturation_on_device(orc, R):
     assert out[H // 2, W // 2, 3] == 0xFFFFFFFF and out[H // 2, W // 2, 0] == 0xFFFFFFFF
 
 
+@pytest.mark.parametrize("case", ["near_max", "low_words_wrap", "mixed"])
+def test_saturating_adds_bit_exact_near_the_limit(orc, R, case):
+    """The saturating splat (64-bit pair adds with carry / wrap counts and the
+    fix-up before reads) against the oracle's per-word saturating sums, on
+    histograms seeded next to 2^32 - 1 where the attractor lands: each word
+    must end at min(seed + sum of adds, 2^32 - 1) exactly, including high
+    words that receive a carry just below the limit. Two dispatches in a row
+    (the second adds to fixed-up words)."""
+    MAX = 0xFFFFFFFF
+    W, H = 24, 20
+    g = genome(sierpinski(), scale=18.0, cx=0.5, cy=0.5)
+    rng = np.random.default_rng({"near_max": 1, "low_words_wrap": 2, "mixed": 3}[case])
+    h0 = np.zeros((H, W, 4), dtype=np.uint64)
+    if case == "near_max":
+        h0[...] = MAX - rng.integers(0, 40000, (H, W, 4))
+    elif case == "low_words_wrap":          # R and B near the limit, G and count low
+        h0[..., 0] = MAX - rng.integers(0, 3000, (H, W))
+        h0[..., 2] = MAX - rng.integers(0, 3000, (H, W))
+        h0[..., 1] = rng.integers(0, 1000, (H, W))
+        h0[..., 3] = MAX - 1 - rng.integers(0, 2, (H, W))   # high word one below the limit
+    else:
+        h0[...] = rng.integers(0, MAX + 1, (H, W, 4), dtype=np.uint64)
+        h0[rng.random((H, W)) < 0.5] = MAX - 300
+    h0 = h0.astype(np.uint32)
+    r = R(W, H, 1)
+    r.reset(g)
+    r.write_histogram(h0)
+    ho = h0.copy()
+    for seed, walkers, iters in ((11, 700, 120), (12, 300, 90)):
+        r.iterate(g, seed, walkers, iters, fuse=10)
+        xf, ds, u, pal = blobs(g, W, H, iters=iters, fuse=10, seed=seed, walkers=walkers)
+        orc.chaos(xf, ds, u, pal, hist=ho)
+        out = r.read_histogram()
+        assert np.array_equal(out, ho), (case, seed)
+    assert (ho == MAX).sum() > 0 and (ho < MAX).sum() > 0
+
+
 def test_retry_path_on_device(orc, R):
     g = genome([xform(1.0, (1, 0, 0, 0, 1, 0), vars=((0, 1e12),))])
     iters = 12
