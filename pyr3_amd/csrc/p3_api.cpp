@@ -739,7 +739,8 @@ int p3_probe_atomics(p3_renderer* r, uint32_t blocks, uint32_t per_thread,
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   if (rc) return rc;
-  r->have_image = false;   // the histogram now holds probe counts
+  r->have_image = false;   // the histogram now holds probe counts:
+  r->hist_bound = 0xffffffffULL;   // unbounded until the next reset
   *requests_per_splat = pairs ? 2 : 4;
   *requests_per_s = *requests_per_splat * 256.0 * blocks * per_thread / (ms * 1e-3);
   return P3_OK;
