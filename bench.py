@@ -203,7 +203,7 @@ def batch_main(a):
     samples = sum(j.quality * j.width * j.height for j in jobs)
     out = {}
     for run in ("first", "timed"):
-        br = batch.BatchRenderer(n_streams=2)
+        br = batch.BatchRenderer(n_streams=a.streams)
         t0 = time.perf_counter()
         br.render(jobs, progress=lambda k: log(f"{run}: {k}/{len(jobs)} genomes"))
         wall = time.perf_counter() - t0
@@ -216,7 +216,8 @@ def batch_main(a):
         "metric": "genomes/s, 512-genome batch (plain path)", "value": out["timed"]["genomes_per_s"],
         "unit": "genomes/s", "n_gpus": 1, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32+u32", "data": "synthetic",
-        "config": {"workload": f"batch-{len(jobs)}", "motion_blur_subframes": 8},
+        "config": {"workload": f"batch-{len(jobs)}", "motion_blur_subframes": 8,
+                   "streams": a.streams},
         "runs": out, "total_gsamples": round(samples / 1e9, 3),
     }
     print(json.dumps(line), flush=True)

@@ -84,6 +84,7 @@ struct p3_renderer {
   int width = 0, height = 0, os = 1, sw = 0, sh = 0;
   int n_taps = 0;
   float taps_host[17] = {0};
+  bool tables_on_device = false;  // taps and kernel-norm table uploaded since allocation
   hipStream_t stream = nullptr;
   DevBuf<uint32_t> hist;
   DevBuf<float4> filtered, pre, imagef;
@@ -162,6 +163,28 @@ uint16_t bf16_bits(float f) {
   return (uint16_t)(u >> 16);
 }
 
+
+// Per-radius kernel sums over the integer disc, sigma = r / 3, in f64
+// (src/density.ts:78-99), stored as f32: ~22 M exp() for radii up to 256,
+// so computed once per process (it does not depend on the frame).
+const float* kernel_norm_lut() {
+  static const std::vector<float> lut = [] {
+    std::vector<float> t(257);
+    t[0] = 1.0f;
+    for (int rr = 1; rr <= 256; ++rr) {
+      const double sigma = rr / 3.0, inv = 1.0 / (2.0 * sigma * sigma), r2 = (double)rr * rr;
+      double s = 0.0;
+      for (int dy = -rr; dy <= rr; ++dy)
+        for (int dx = -rr; dx <= rr; ++dx) {
+          const double d2 = (double)dx * dx + (double)dy * dy;
+          if (d2 <= r2) s += std::exp(-d2 * inv);
+        }
+      t[rr] = (float)s;
+    }
+    return t;
+  }();
+  return lut.data();
+}
 int build(p3_renderer* r, int width, int height, int os, const float* taps, int n_taps) {
   if (width <= 0 || height <= 0 || os < 1 || os > 16)
     return r->fail(P3_E_ARG, "invalid size %dx%d oversample %d", width, height, os);
@@ -171,11 +194,22 @@ int build(p3_renderer* r, int width, int height, int os, const float* taps, int 
   if (sw * sh > (1LL << 31))
     return r->fail(P3_E_ARG, "super-res grid %lldx%lld too large", sw, sh);
   r->width = width; r->height = height; r->os = os; r->sw = (int)sw; r->sh = (int)sh;
+  // The taps upload below reads taps_host asynchronously, so a change of
+  // taps waits for the stream before the copy is overwritten; unchanged taps
+  // (the usual resize between batch jobs) are already on the device.
+  const bool same_taps = r->tables_on_device && n_taps == r->n_taps &&
+                         std::memcmp(r->taps_host, taps, sizeof(float) * n_taps) == 0;
+  int rc;
+  if (r->tables_on_device && !same_taps &&
+      (rc = r->hip(hipStreamSynchronize(r->stream), "sync before new taps")))
+    return rc;
   r->n_taps = n_taps;
-  std::memcpy(r->taps_host, taps, sizeof(float) * n_taps);
+  if (!same_taps) {
+    r->tables_on_device = false;  // until the upload below is queued
+    std::memcpy(r->taps_host, taps, sizeof(float) * n_taps);
+  }
   const size_t bins = (size_t)sw * sh;
   const size_t tiles = (size_t)((sw + 15) / 16) * ((sh + 15) / 16);
-  int rc;
   if ((rc = r->hip(r->hist.ensure(bins * 4), "hipMalloc(histogram)"))) return rc;
   if ((rc = r->hip(r->filtered.ensure(bins), "hipMalloc(filtered)"))) return rc;
   if ((rc = r->hip(r->pre.ensure(bins), "hipMalloc(de pre)"))) return rc;
@@ -192,28 +226,20 @@ int build(p3_renderer* r, int width, int height, int os, const float* taps, int 
   if ((rc = r->hip(r->distrib.ensure((size_t)(P3_MAX_XFORMS + 1) * P3_GRAIN),
                    "hipMalloc(distrib)")))
     return rc;
-  if ((rc = r->hip(hipMemcpyAsync(r->taps.p, taps, sizeof(float) * n_taps, hipMemcpyHostToDevice,
-                                  r->stream),
-                   "upload taps")))
-    return rc;
-  // Per-radius kernel sums over the integer disc, sigma = r / 3, in f64
-  // (src/density.ts:78-99), stored as f32.
-  float lut[257];
-  lut[0] = 1.0f;
-  for (int rr = 1; rr <= 256; ++rr) {
-    const double sigma = rr / 3.0, inv = 1.0 / (2.0 * sigma * sigma), r2 = (double)rr * rr;
-    double s = 0.0;
-    for (int dy = -rr; dy <= rr; ++dy)
-      for (int dx = -rr; dx <= rr; ++dx) {
-        const double d2 = (double)dx * dx + (double)dy * dy;
-        if (d2 <= r2) s += std::exp(-d2 * inv);
-      }
-    lut[rr] = (float)s;
+  // From the handle's copy and the process-wide table, so the host continues
+  // before the uploads run: a resize between batch jobs does not wait for the
+  // previous job.
+  if (!r->tables_on_device) {
+    if ((rc = r->hip(hipMemcpyAsync(r->taps.p, r->taps_host, sizeof(float) * n_taps,
+                                    hipMemcpyHostToDevice, r->stream),
+                     "upload taps")))
+      return rc;
+    if ((rc = r->hip(hipMemcpyAsync(r->lut.p, kernel_norm_lut(), sizeof(float) * 257,
+                                    hipMemcpyHostToDevice, r->stream),
+                     "upload lut")))
+      return rc;
+    r->tables_on_device = true;
   }
-  if ((rc = r->hip(hipMemcpyAsync(r->lut.p, lut, sizeof lut, hipMemcpyHostToDevice, r->stream),
-                   "upload lut")))
-    return rc;
-  if ((rc = r->hip(hipStreamSynchronize(r->stream), "sync after create"))) return rc;
   r->palette_set = false;
   r->have_image = r->have_filtered = r->have_image_f32 = false;
   r->hist_bound = 0;
@@ -225,6 +251,7 @@ int build(p3_renderer* r, int width, int height, int os, const float* taps, int 
 
 void release(p3_renderer* r) {
   r->hist.release(); r->filtered.release(); r->pre.release(); r->rad.release();
+  r->tables_on_device = false;
   r->tile_rmax.release(); r->image8.release(); r->imagef.release(); r->taps.release();
   r->lut.release(); r->palette.release(); r->xforms.release(); r->distrib.release();
   r->bounds.release(); r->mbounds.release();

@@ -22,6 +22,8 @@ sequential f64 sums (numpy's add.accumulate is sequential), and the while-loop
 from __future__ import annotations
 
 import bisect
+import copy
+import dataclasses
 import math
 import struct
 from dataclasses import dataclass
@@ -81,14 +83,20 @@ def symmetry_xforms(sym) -> List[Xform]:
 
 
 def expand_genome_for_gpu(g: Genome) -> Genome:
-    """Zero the weights of inactive xforms/variations, then append symmetry xforms."""
-    w = g.clone()
-    for x in w.xforms:
-        if not x.active:
-            x.weight = 0.0
-        for v in x.variations:
-            if not v.active:
-                v.weight = 0.0
+    """Zero the weights of inactive xforms/variations, then append symmetry
+    xforms. The result shares every object it does not change with `g`
+    (palette, records of active xforms): it is read, never written."""
+    w = copy.copy(g)
+    xs = []
+    for x in g.xforms:
+        if not x.active or not all(v.active for v in x.variations):
+            x = copy.copy(x)
+            if not x.active:
+                x.weight = 0.0
+            x.variations = [v if v.active else dataclasses.replace(v, weight=0.0)
+                            for v in x.variations]
+        xs.append(x)
+    w.xforms = xs
     if w.symmetry:
         extra = symmetry_xforms(w.symmetry)
         if extra:
