@@ -443,9 +443,9 @@ def main():
                             "frac": round(per_splat * samples / (chaos_ms * 1e-3) / probe_rate, 4),
                             "note": "atomic requests per second: requests_per_splat per "
                                     "post-fuse sample (upper bound: off-frame samples do not "
-                                    "splat) against p3_probe_atomics, a uniform scatter of "
-                                    "16-byte cells in the same request form over the same "
-                                    "histogram size"},
+                                    "splat) against p3_probe_atomics, a uniform scatter in "
+                                    "the same request form (one 64-bit add per 8-byte cell "
+                                    "for packed adds) over the same number of bins"},
         "roofline": {"bound": "hbm", "kernel": "chaos_kernel", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5),

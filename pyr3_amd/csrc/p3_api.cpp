@@ -102,6 +102,7 @@ struct p3_renderer {
   std::vector<p3::ChaosArgs> margs_host;
   DevBuf<float> trace;
   DevBuf<uint32_t> idx;         // colour-index capture (one u32 per bin), once requested
+  bool idx_live = false;        // idx sized to the current grid and in use
   DevBuf<float> mxforms;        // batched dispatch: per-sub-dispatch records
   DevBuf<uint32_t> mdistrib;    //   and pick tables
   DevBuf<uint16_t> mbounds;     //   and their step boundaries
@@ -110,10 +111,13 @@ struct p3_renderer {
   bool have_image = false;
   bool have_filtered = false;
   unsigned long long hist_bound = 0;  // upper bound on any histogram word
-  // event counts of the saturating pair adds (4 u32 per bin): all zero when
-  // sat_ev_clean; sat_dirty when a dispatch may have counted events that
-  // hist_fixup has not yet applied to the histogram
+  // Pending adds outside the u32 histogram: the event counts of the
+  // saturating pair adds (4 u32 per bin), or the packed accumulator (1 u64
+  // per bin) and its carry counts (4 u64 per bin). All zero when
+  // sat_ev_clean; sat_dirty when a dispatch may have left adds that
+  // hist_fixup / hist_merge has not yet applied to the histogram.
   DevBuf<uint32_t> sat_ev;
+  DevBuf<uint64_t> acc, carry;
   bool sat_ev_clean = false, sat_dirty = false;
   int trace_walkers = 0, trace_iters = 0;
   bool timing = false;
@@ -244,19 +248,25 @@ int build(p3_renderer* r, int width, int height, int os, const float* taps, int 
   r->have_image = r->have_filtered = r->have_image_f32 = false;
   r->hist_bound = 0;
   r->sat_ev_clean = r->sat_dirty = false;
+  r->idx_live = false;   // capture restarts (and is re-sized) at its next use
   r->last_xforms.clear();
   r->last_distrib.clear();
   return P3_OK;
 }
+
+// Bins of the current super-sampled grid (buffers may be larger: a resize
+// keeps the allocations and grows them only when needed).
+size_t grid_bins(const p3_renderer* r) { return (size_t)r->sw * r->sh; }
 
 void release(p3_renderer* r) {
   r->hist.release(); r->filtered.release(); r->pre.release(); r->rad.release();
   r->tables_on_device = false;
   r->tile_rmax.release(); r->image8.release(); r->imagef.release(); r->taps.release();
   r->lut.release(); r->palette.release(); r->xforms.release(); r->distrib.release();
-  r->bounds.release(); r->mbounds.release();
+  r->bounds.release(); r->mbounds.release(); r->acc.release(); r->carry.release();
   r->isaac.release(); r->trace.release(); r->idx.release();
   r->mxforms.release(); r->mdistrib.release(); r->margs.release();
+  r->sat_ev.release(); r->linear.release(); r->curves.release();
 }
 
 int check_records(p3_renderer* r, const float* xforms, int n_slots) {
@@ -308,36 +318,79 @@ int validate_dispatch(p3_renderer* r, const float* xforms, int n_slots, const ui
   return P3_OK;
 }
 
-// Kernel arguments of one dispatch whose records and table are at dxf / dds.
-// Event buffer of the saturating pair adds, sized to the grid and zeroed.
+// Buffers of the pending adds for the current add mode, sized to the grid
+// and zeroed.
 int prepare_sat_events(p3_renderer* r) {
   int rc;
-  if ((rc = r->hip(r->sat_ev.ensure(r->hist.n), "hipMalloc(saturation events)"))) return rc;
+  const size_t bins = grid_bins(r);
+  const bool packed = p3::hist_add_mode() == 2;
+  if (packed) {
+    if ((rc = r->hip(r->acc.ensure(bins), "hipMalloc(packed accumulator)"))) return rc;
+    if ((rc = r->hip(r->carry.ensure(bins * 4), "hipMalloc(packed carries)"))) return rc;
+  } else if ((rc = r->hip(r->sat_ev.ensure(grid_bins(r) * 4), "hipMalloc(saturation events)"))) {
+    return rc;
+  }
   if (!r->sat_ev_clean) {
-    if ((rc = r->hip(p3::launch_fill(r->sat_ev.p, r->hist.n * 4, 0u, r->stream),
-                     "clear saturation events")))
+    if (packed) {
+      if ((rc = r->hip(p3::launch_fill(r->acc.p, bins * 8, 0u, r->stream),
+                       "clear packed accumulator")) ||
+          (rc = r->hip(p3::launch_fill(r->carry.p, bins * 32, 0u, r->stream),
+                       "clear packed carries")))
+        return rc;
+    } else if ((rc = r->hip(p3::launch_fill(r->sat_ev.p, grid_bins(r) * 16, 0u, r->stream),
+                            "clear saturation events"))) {
       return rc;
+    }
     r->sat_ev_clean = true;
   }
   return P3_OK;
 }
 
-// Applies pending saturation events: every reader of the histogram calls it.
+// Applies pending adds: every reader of the histogram calls it.
 int settle_hist(p3_renderer* r) {
   if (!r->sat_dirty) return P3_OK;
   r->sat_dirty = false;
-  return r->hip(p3::launch_hist_fixup(r->hist.p, r->sat_ev.p, r->hist.n / 4, r->stream),
+  if (p3::hist_add_mode() == 2)
+    return r->hip(p3::launch_hist_merge(r->hist.p, r->acc.p, r->carry.p, grid_bins(r),
+                                        r->stream),
+                  "packed merge");
+  return r->hip(p3::launch_hist_fixup(r->hist.p, r->sat_ev.p, grid_bins(r), r->stream),
                 "saturation fix-up");
 }
 
-// Drops pending events when the histogram is overwritten wholesale.
+// Drops pending adds when the histogram is overwritten wholesale.
 int drop_sat_events(p3_renderer* r) {
   if (!r->sat_dirty) return P3_OK;
   r->sat_dirty = false;
-  return r->hip(p3::launch_fill(r->sat_ev.p, r->hist.n * 4, 0u, r->stream),
+  if (p3::hist_add_mode() == 2) {
+    int rc;
+    if ((rc = r->hip(p3::launch_fill(r->acc.p, grid_bins(r) * 8, 0u, r->stream),
+                     "clear packed accumulator")))
+      return rc;
+    return r->hip(p3::launch_fill(r->carry.p, grid_bins(r) * 32, 0u, r->stream),
+                  "clear packed carries");
+  }
+  return r->hip(p3::launch_fill(r->sat_ev.p, grid_bins(r) * 16, 0u, r->stream),
                 "clear saturation events");
 }
 
+// Attaches the pending-add buffers to a dispatch: always in packed mode, and
+// for the saturating kernel in pair mode.
+int attach_adds(p3_renderer* r, p3::ChaosArgs& a, bool sat) {
+  const int mode = p3::hist_add_mode();
+  if (mode == 0 || (mode == 1 && !sat)) return P3_OK;
+  int rc;
+  if ((rc = prepare_sat_events(r))) return rc;
+  if (mode == 2) {
+    a.acc = r->acc.p;
+    a.carry = r->carry.p;
+  } else {
+    a.sat_events = r->sat_ev.p;
+  }
+  return P3_OK;
+}
+
+// Kernel arguments of one dispatch whose records and table are at dxf / dds.
 p3::ChaosArgs dispatch_args(p3_renderer* r, const uint32_t* u, int n_regular, int32_t final_idx,
                             const float* dxf, const uint32_t* dds) {
   p3::ChaosArgs a{};
@@ -376,11 +429,12 @@ p3::ChaosArgs dispatch_args(p3_renderer* r, const uint32_t* u, int n_regular, in
 int attach_capture(p3_renderer* r, p3::ChaosArgs& a, const uint32_t* u) {
   if (u[14] != 1u) return P3_OK;
   int rc;
-  if (!r->idx.p) {
-    if ((rc = r->hip(r->idx.ensure((size_t)r->sw * r->sh), "hipMalloc(capture)"))) return rc;
-    if ((rc = r->hip(p3::launch_fill(r->idx.p, r->idx.n * 4, 0u, r->stream), "clear capture")))
+  if (!r->idx_live) {
+    if ((rc = r->hip(r->idx.ensure(grid_bins(r)), "hipMalloc(capture)"))) return rc;
+    if ((rc = r->hip(p3::launch_fill(r->idx.p, grid_bins(r) * 4, 0u, r->stream), "clear capture")))
       return rc;
   }
+  r->idx_live = true;
   a.idx_sum = r->idx.p;
   return P3_OK;
 }
@@ -459,10 +513,7 @@ int iterate_impl(p3_renderer* r, const float* xforms, int n_slots, const uint32_
   // Each walker splats at most `iters` times, each adding <= 255 per word.
   r->hist_bound += 255ULL * (unsigned long long)walkers * (unsigned long long)a.iters;
   const bool sat = r->hist_bound >= 0xffffffffULL;
-  if (sat && p3::pair_adds()) {
-    if ((rc = prepare_sat_events(r))) return rc;
-    a.sat_events = r->sat_ev.p;
-  }
+  if ((rc = attach_adds(r, a, sat))) return rc;
   std::string rtc_err;
   {  // compile (or fetch) the kernel for this shape before the timed launch
     const std::vector<char>* code = nullptr;
@@ -474,7 +525,7 @@ int iterate_impl(p3_renderer* r, const float* xforms, int n_slots, const uint32_
       p3::launch_chaos_rtc(a, xforms, n_regular, final_idx, sat, r->stream, rtc_err);
   if (le != hipSuccess)
     return r->fail(P3_E_HIP, "chaos launch: %s %s", hipGetErrorString(le), rtc_err.c_str());
-  if (a.sat_events) r->sat_dirty = true;
+  if (a.sat_events || a.acc) r->sat_dirty = true;
   r->toc(T_CHAOS);
   r->have_image = false;
   return P3_OK;
@@ -550,10 +601,8 @@ int iterate_multi_impl(p3_renderer* r, int n_sub, const float* const* xforms, in
   }
   if (M.start[n_sub] == 0) return P3_OK;
   const bool sat = r->hist_bound + bound >= 0xffffffffULL;
-  if (sat && p3::pair_adds()) {
-    if ((rc = prepare_sat_events(r))) return rc;
-    for (int k = 0; k < n_sub; ++k) args[k].sat_events = r->sat_ev.p;
-  }
+  for (int k = 0; k < n_sub; ++k)
+    if ((rc = attach_adds(r, args[k], sat))) return rc;
   if ((rc = r->hip(hipMemcpyAsync(r->mbounds.p, bounds.data(), bounds.size() * sizeof(uint16_t),
                                   hipMemcpyHostToDevice, r->stream),
                    "upload batched pick bounds")))
@@ -575,7 +624,7 @@ int iterate_multi_impl(p3_renderer* r, int n_sub, const float* const* xforms, in
   if (le != hipSuccess)
     return r->fail(P3_E_HIP, "batched chaos launch: %s %s", hipGetErrorString(le),
                    rtc_err.c_str());
-  if (args[0].sat_events) r->sat_dirty = true;
+  if (args[0].sat_events || args[0].acc) r->sat_dirty = true;
   r->toc(T_CHAOS);
   r->have_image = false;
   return P3_OK;
@@ -624,8 +673,9 @@ int p3_create(int width, int height, int oversample, const float* taps, int n_ta
 int p3_resize(p3_renderer* r, int width, int height, int oversample, const float* taps,
               int n_taps) {
   if (!r) return P3_E_ARG;
-  (void)hipStreamSynchronize(r->stream);
-  release(r);
+  // Keeps the allocations (build grows them when needed; freeing one waits
+  // for the device), so work already queued on the stream stays valid and
+  // the host does not wait for it.
   return build(r, width, height, oversample, taps, n_taps);
 }
 
@@ -650,11 +700,11 @@ int p3_reset(p3_renderer* r, const float* palette_rgba) {
                    "upload palette")))
     return rc;
   r->tic(T_RESET);
-  if ((rc = r->hip(p3::launch_fill(r->hist.p, r->hist.n * 4, 0u, r->stream), "clear histogram")))
+  if ((rc = r->hip(p3::launch_fill(r->hist.p, grid_bins(r) * 16, 0u, r->stream), "clear histogram")))
     return rc;
   if ((rc = drop_sat_events(r))) return rc;
-  if (r->idx.p &&
-      (rc = r->hip(p3::launch_fill(r->idx.p, r->idx.n * 4, 0u, r->stream), "clear capture")))
+  if (r->idx_live &&
+      (rc = r->hip(p3::launch_fill(r->idx.p, grid_bins(r) * 4, 0u, r->stream), "clear capture")))
     return rc;
   r->toc(T_RESET);
   r->palette_set = true;
@@ -757,18 +807,22 @@ int p3_probe_atomics(p3_renderer* r, uint32_t blocks, uint32_t per_thread,
     return rc;
   float ms = 0.0f;
   rc = r->hip(hipEventRecord(e0, r->stream), "hipEventRecord");
-  const bool pairs = p3::pair_adds();
-  if (!rc) rc = r->hip(p3::launch_atomic_probe(r->hist.p, cells, blocks, per_thread, 12345u,
-                                                pairs, r->stream), "atomic probe launch");
+  const int mode = p3::hist_add_mode();
+  // packed mode: the requests land in the accumulator, merged afterwards
+  if (!rc && mode == 2) rc = prepare_sat_events(r);
+  if (!rc) rc = r->hip(p3::launch_atomic_probe(mode == 2 ? (uint32_t*)r->acc.p : r->hist.p,
+                                                cells, blocks, per_thread, 12345u, mode,
+                                                r->stream), "atomic probe launch");
   if (!rc) rc = r->hip(hipEventRecord(e1, r->stream), "hipEventRecord");
   if (!rc) rc = r->hip(hipEventSynchronize(e1), "hipEventSynchronize");
   if (!rc) rc = r->hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   if (rc) return rc;
   r->have_image = false;   // the histogram now holds probe counts:
   r->hist_bound = 0xffffffffULL;   // unbounded until the next reset
-  *requests_per_splat = pairs ? 2 : 4;
+  if (mode == 2) r->sat_dirty = true;   // merged (exact below 2^16 per cell) at the next read
+  *requests_per_splat = mode == 2 ? 1 : mode == 1 ? 2 : 4;
   *requests_per_s = *requests_per_splat * 256.0 * blocks * per_thread / (ms * 1e-3);
   return P3_OK;
 }
@@ -814,7 +868,8 @@ int p3_set_curves(p3_renderer* r, const float* lut, size_t n) {
 
 int p3_read_histogram(p3_renderer* r, uint32_t* out, size_t n) {
   if (!r) return P3_E_ARG;
-  if (!out || n != r->hist.n) return r->fail(P3_E_ARG, "histogram needs %zu words", r->hist.n);
+  if (!out || n != grid_bins(r) * 4)
+    return r->fail(P3_E_ARG, "histogram needs %zu words", grid_bins(r) * 4);
   const int rc = settle_hist(r);
   if (rc) return rc;
   return read_dev(r, out, r->hist.p, n * 4, "read histogram");
@@ -822,15 +877,16 @@ int p3_read_histogram(p3_renderer* r, uint32_t* out, size_t n) {
 
 int p3_read_index(p3_renderer* r, uint32_t* out, size_t n) {
   if (!r) return P3_E_ARG;
-  if (!r->idx.p) return r->fail(P3_E_STATE, "no dispatch has captured the colour index");
-  if (!out || n != r->idx.n) return r->fail(P3_E_ARG, "capture buffer needs %zu words", r->idx.n);
+  if (!r->idx_live) return r->fail(P3_E_STATE, "no dispatch has captured the colour index");
+  if (!out || n != grid_bins(r))
+    return r->fail(P3_E_ARG, "capture buffer needs %zu words", grid_bins(r));
   return read_dev(r, out, r->idx.p, n * 4, "read capture");
 }
 
 int p3_read_filtered(p3_renderer* r, float* out, size_t n) {
   if (!r) return P3_E_ARG;
-  if (!out || n != r->filtered.n * 4)
-    return r->fail(P3_E_ARG, "filtered grid needs %zu floats", r->filtered.n * 4);
+  if (!out || n != grid_bins(r) * 4)
+    return r->fail(P3_E_ARG, "filtered grid needs %zu floats", grid_bins(r) * 4);
   if (!r->have_filtered) return r->fail(P3_E_STATE, "no DE output yet");
   return read_dev(r, out, r->filtered.p, n * 4, "read filtered");
 }
@@ -854,7 +910,8 @@ int p3_read_image_f32(p3_renderer* r, float* out, size_t n) {
 
 int p3_write_histogram(p3_renderer* r, const uint32_t* data, size_t n) {
   if (!r) return P3_E_ARG;
-  if (!data || n != r->hist.n) return r->fail(P3_E_ARG, "histogram needs %zu words", r->hist.n);
+  if (!data || n != grid_bins(r) * 4)
+    return r->fail(P3_E_ARG, "histogram needs %zu words", grid_bins(r) * 4);
   int rc = drop_sat_events(r);
   if (rc) return rc;
   if ((rc = r->hip(hipMemcpyAsync(r->hist.p, data, n * 4, hipMemcpyHostToDevice, r->stream),

@@ -46,6 +46,11 @@ struct ChaosArgs {
   // saturating pair adds: four event counts per bin (low-word carries and
   // 64-bit wraps of the (R, G) and (B, count) pairs); null otherwise
   uint32_t* sat_events;
+  // packed adds: one u64 per bin holding (R, G, B, count) as four 16-bit
+  // fields, and four u64 counts per bin of the carries out of each field
+  // (the last: out of bit 63); null otherwise
+  uint64_t* acc;
+  uint64_t* carry;
 };
 
 // Batched launch: up to kMaxSub sub-dispatches of one genome shape in one

@@ -17,12 +17,18 @@
 //
 // The splat adds u32(pal * opacity * 255) and u32(opacity * 255) to the four
 // channels of bin (y * W + x). The reference's adds saturate at 2^32 - 1; a
-// saturating add is order-independent (the final value is min(sum, 2^32-1)),
-// so it is reproduced with native atomic adds whose returned old values detect
-// a wrap, followed by an exchange to 2^32 - 1: after the last wrap of a word
-// the last operation on it is always that exchange. When the host has proven
-// that no word can reach 2^32 (bound on 255 x samples since the last reset)
-// the adds are issued without a return value.
+// saturating add of non-negative values is order-independent (the final
+// value is min(start + sum, 2^32 - 1)). The device's atomic rate is per
+// request, whatever its width, so the default form (Spec::kHistMode 2) is
+// one 64-bit add per splat into a packed 16-bit accumulator with exact carry
+// counts (add_packed), merged into the u32 histogram before it is read.
+// The other forms: two 64-bit pair adds with carry and wrap counts (mode 1);
+// four u32 adds (mode 0, and a packed-mode splat with a word of 2^16 or
+// more), whose returned old values detect a wrap, followed by an exchange to
+// 2^32 - 1: after the last wrap of a word the last operation on it is always
+// that exchange. When the host has proven that no word can reach 2^32 (bound
+// on 255 x samples since the last reset) the u32 adds are issued without a
+// return value.
 #pragma once
 #include "p3_device.h"
 #include "p3_variation_dispatch.h"
@@ -86,6 +92,30 @@ P3_INLINE void add_pair_sat(P3_GLOBAL uint32_t* p, P3_GLOBAL uint32_t* ev, uint3
                                             __HIP_MEMORY_SCOPE_AGENT);
   if ((uint64_t)(uint32_t)o + lo > 0xffffffffull) (void)add_ret(ev + 0, 1u);
   if (o + v < o) (void)add_ret(ev + 1, 1u);
+}
+// Packed add of one splat: (d0, d1, d2, d3), each below 2^16, as the four
+// 16-bit fields of the bin's u64 accumulator, one 64-bit atomic. The returned
+// old value o and the sum n = o + v give the carry into every bit,
+// o ^ v ^ n; a field's increment (its word plus the carry in) is at most
+// 2^16, so it carries out at most once per add, and counting each carry out
+// (bits 16, 32, 48 and the 64-bit wrap) in the bin's carry[0..3] makes the
+// field sums exact: S_k = field_k + 2^16 carry[k] - carry[k-1]. hist_merge
+// (p3_image.hip) adds them to the u32 histogram, saturating.
+P3_INLINE void add_packed(P3_GLOBAL uint64_t* acc, P3_GLOBAL uint64_t* carry, uint32_t d0,
+                          uint32_t d1, uint32_t d2, uint32_t d3) {
+  const uint64_t v = (uint64_t)d0 | ((uint64_t)d1 << 16) | ((uint64_t)d2 << 32) |
+                     ((uint64_t)d3 << 48);
+  if (!v) return;
+  const uint64_t o = __hip_atomic_fetch_add(acc, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint64_t n = o + v;
+  const uint64_t c = o ^ v ^ n;
+  const bool c0 = (c >> 16) & 1u, c1 = (c >> 32) & 1u, c2 = (c >> 48) & 1u, c3 = n < o;
+  if (c0 | c1 | c2 | c3) {
+    if (c0) (void)__hip_atomic_fetch_add(carry + 0, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (c1) (void)__hip_atomic_fetch_add(carry + 1, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (c2) (void)__hip_atomic_fetch_add(carry + 2, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (c3) (void)__hip_atomic_fetch_add(carry + 3, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 // Four channel adds (R, G, B, count) at b; ev = the bin's four event counts
 // (read only by the saturating pair form).
@@ -304,8 +334,15 @@ P3_INLINE void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32_t fn,
       const size_t bin = ((size_t)yi * A.width + xi) * 4u;
       P3_GLOBAL uint32_t* b = (P3_GLOBAL uint32_t*)A.hist + bin;
       P3_GLOBAL uint32_t* ev = (P3_GLOBAL uint32_t*)A.sat_events + bin;
-      hist_add4<kSat, Spec::kPairAdds>(b, ev, f2u_sat(pal.x * wgt), f2u_sat(pal.y * wgt),
-                                       f2u_sat(pal.z * wgt), f2u_sat(wgt));
+      const uint32_t d0 = f2u_sat(pal.x * wgt), d1 = f2u_sat(pal.y * wgt),
+                     d2 = f2u_sat(pal.z * wgt), d3 = f2u_sat(wgt);
+      if (Spec::kHistMode == 2 && (d0 | d1 | d2 | d3) <= 0xffffu)
+        add_packed((P3_GLOBAL uint64_t*)A.acc + (bin >> 2), (P3_GLOBAL uint64_t*)A.carry + bin,
+                   d0, d1, d2, d3);
+      else if (Spec::kHistMode == 1)
+        hist_add4<kSat, true>(b, ev, d0, d1, d2, d3);
+      else   // four u32 adds; in packed mode, a splat with a word of 2^16 or more
+        hist_add4<kSat, false>(b, ev, d0, d1, d2, d3);
       if (A.idx_sum != nullptr) {   // capture of the colour index (chaos.wgsl:8720-8728)
         const uint32_t d = f2u_sat(fminf(fmaxf(sz, 0.0f), 1.0f) * wgt);
         P3_GLOBAL uint32_t* q = (P3_GLOBAL uint32_t*)A.idx_sum + (size_t)yi * A.width + xi;

@@ -397,6 +397,50 @@ __global__ void __launch_bounds__(256) hist_fixup(uint4* hist, uint4* ev, size_t
   ev[i] = make_uint4(0u, 0u, 0u, 0u);
 }
 
+// After chaos dispatches that used packed adds: bin i's accumulator holds
+// four 16-bit fields f_k = (S_k + c_{k-1}) mod 2^16 of the true sums S_k,
+// with c_k the counted carries out of field k (c_{-1} = 0), so
+// S_k = f_k + 2^16 c_k - c_{k-1} exactly; each u32 word becomes
+// min(word + S_k, 2^32 - 1), which is the saturating sum since all adds are
+// non-negative. The accumulator and counts are cleared.
+__device__ inline uint32_t add_sat64(uint32_t w, uint64_t s) {
+  const uint64_t t = (uint64_t)w + s;
+  return t > 0xffffffffull ? 0xffffffffu : (uint32_t)t;
+}
+
+__global__ void __launch_bounds__(256) hist_merge(uint4* hist, uint64_t* acc, ulonglong2* carry,
+                                                  size_t bins) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= bins) return;
+  const uint64_t a = acc[i];
+  const ulonglong2 c01 = carry[2 * i], c23 = carry[2 * i + 1];
+  const bool events = (c01.x | c01.y | c23.x | c23.y) != 0ull;
+  if (a == 0ull && !events) return;
+  const uint64_t s0 = (a & 0xffffull) + (c01.x << 16);
+  const uint64_t s1 = ((a >> 16) & 0xffffull) + (c01.y << 16) - c01.x;
+  const uint64_t s2 = ((a >> 32) & 0xffffull) + (c23.x << 16) - c01.y;
+  const uint64_t s3 = (a >> 48) + (c23.y << 16) - c23.x;
+  uint4 h = hist[i];
+  h.x = add_sat64(h.x, s0);
+  h.y = add_sat64(h.y, s1);
+  h.z = add_sat64(h.z, s2);
+  h.w = add_sat64(h.w, s3);
+  hist[i] = h;
+  acc[i] = 0ull;
+  if (events) {
+    carry[2 * i] = make_ulonglong2(0ull, 0ull);
+    carry[2 * i + 1] = make_ulonglong2(0ull, 0ull);
+  }
+}
+
+hipError_t launch_hist_merge(uint32_t* hist, uint64_t* acc, uint64_t* carry, size_t bins,
+                             hipStream_t stream) {
+  const size_t grid = (bins + 255) / 256;
+  if (grid == 0) return hipSuccess;
+  hist_merge<<<(unsigned)grid, 256, 0, stream>>>((uint4*)hist, acc, (ulonglong2*)carry, bins);
+  return hipGetLastError();
+}
+
 hipError_t launch_hist_fixup(uint32_t* hist, uint32_t* ev, size_t bins, hipStream_t stream) {
   const size_t grid = (bins + 255) / 256;
   if (grid == 0) return hipSuccess;
@@ -406,24 +450,30 @@ hipError_t launch_hist_fixup(uint32_t* hist, uint32_t* ev, size_t bins, hipStrea
 
 // ------------------------------------------------------- atomic probe --
 
-// Scatter-add rate probe for the chaos kernel's roofline: every thread adds
-// 1 to the four u32 words of `per_thread` 16-byte cells drawn uniformly over
-// `cells` (an LCG per thread), in the histogram splat's request form: two
-// relaxed device-scope 64-bit adds per cell (kPairs) or four 32-bit ones.
-template <bool kPairs>
+// Scatter-add probe: `per_thread` random cells per thread over `cells`
+// cells, each in one request form of the histogram adds: mode 2 one 64-bit
+// add per 8-byte cell (the packed accumulator), mode 1 two 64-bit adds per
+// 16-byte cell, mode 0 four 32-bit adds per 16-byte cell (relaxed,
+// device-scope). Every add puts 1 in each 32-bit word (mode 2: in each
+// 16-bit field) of its cell.
+template <int kMode>
 __global__ void __launch_bounds__(256) atomic_probe(uint32_t* grid, uint32_t cells,
                                                     uint32_t per_thread, uint32_t seed) {
   uint32_t s = (blockIdx.x * 256u + threadIdx.x) * 2654435761u ^ seed;
   for (uint32_t i = 0; i < per_thread; ++i) {
     s = s * 1664525u + 1013904223u;
     const uint32_t c = (uint32_t)(((uint64_t)s * cells) >> 32);
-    uint32_t* p = grid + (size_t)c * 4u;
-    if (kPairs) {
+    if (kMode == 2) {
+      (void)__hip_atomic_fetch_add((uint64_t*)grid + c, 0x0001000100010001ull, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+    } else if (kMode == 1) {
+      uint32_t* p = grid + (size_t)c * 4u;
 #pragma unroll
       for (int k = 0; k < 2; ++k)
         (void)__hip_atomic_fetch_add((uint64_t*)p + k, 0x100000001ull, __ATOMIC_RELAXED,
                                      __HIP_MEMORY_SCOPE_AGENT);
     } else {
+      uint32_t* p = grid + (size_t)c * 4u;
 #pragma unroll
       for (int k = 0; k < 4; ++k)
         (void)__hip_atomic_fetch_add(p + k, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -432,10 +482,11 @@ __global__ void __launch_bounds__(256) atomic_probe(uint32_t* grid, uint32_t cel
 }
 
 hipError_t launch_atomic_probe(uint32_t* grid, uint32_t cells, uint32_t blocks,
-                               uint32_t per_thread, uint32_t seed, bool pairs,
+                               uint32_t per_thread, uint32_t seed, int mode,
                                hipStream_t stream) {
-  if (pairs) atomic_probe<true><<<blocks, 256, 0, stream>>>(grid, cells, per_thread, seed);
-  else atomic_probe<false><<<blocks, 256, 0, stream>>>(grid, cells, per_thread, seed);
+  if (mode == 2) atomic_probe<2><<<blocks, 256, 0, stream>>>(grid, cells, per_thread, seed);
+  else if (mode == 1) atomic_probe<1><<<blocks, 256, 0, stream>>>(grid, cells, per_thread, seed);
+  else atomic_probe<0><<<blocks, 256, 0, stream>>>(grid, cells, per_thread, seed);
   return hipGetLastError();
 }
 
@@ -449,18 +500,26 @@ hipError_t launch_tonemap(const TmArgs& a, bool de, hipStream_t stream) {
 
 // ------------------------------------------------------------------ fill --
 
-__global__ void fill_u32(uint4* p, size_t n4, uint32_t v) {
+// Fills n4 16-byte units at p, then the n_tail (< 4) u32 words after them.
+__global__ void fill_u32(uint4* p, size_t n4, uint32_t n_tail, uint32_t v) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t k = i; k < n4; k += stride) p[k] = make_uint4(v, v, v, v);
+  if (i < n_tail) reinterpret_cast<uint32_t*>(p + n4)[i] = v;
 }
 
+// `bytes` must be a multiple of 4; every word is written, including a tail
+// shorter than 16 bytes (an odd count of 8-byte accumulators, or of u32
+// capture words not divisible by 4).
 hipError_t launch_fill(void* p, size_t bytes, uint32_t v, hipStream_t stream) {
+  if (bytes % 4 != 0) return hipErrorInvalidValue;
   const size_t n4 = bytes / 16;
-  if (n4 == 0) return hipSuccess;
+  const uint32_t n_tail = (uint32_t)((bytes / 4) % 4);
+  if (n4 == 0 && n_tail == 0) return hipSuccess;
   size_t blocks = (n4 + 255) / 256;
   if (blocks > 8192) blocks = 8192;
-  fill_u32<<<(unsigned)blocks, 256, 0, stream>>>((uint4*)p, n4, v);
+  if (blocks == 0) blocks = 1;
+  fill_u32<<<(unsigned)blocks, 256, 0, stream>>>((uint4*)p, n4, n_tail, v);
   return hipGetLastError();
 }
 

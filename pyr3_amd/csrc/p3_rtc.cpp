@@ -100,7 +100,7 @@ void emit_body(std::ostringstream& s, const float* xf, int slot, bool full) {
 std::string generate(const float* xforms, int n_regular, int final_slot) {
   std::ostringstream s;
   s << "namespace p3 {\nstruct Spec {\n"
-    << "  static constexpr bool kPairAdds = " << (p3::pair_adds() ? "true" : "false") << ";\n"
+    << "  static constexpr int kHistMode = " << p3::hist_add_mode() << ";\n"
     << "  static constexpr int kRegular = " << n_regular << ";\n"
     << "  static P3_INLINE bool has_final(const ChaosArgs&) { return "
     << (final_slot >= 0 ? "true" : "false") << "; }\n"
@@ -259,16 +259,20 @@ int chaos_block() {
   return b;
 }
 
-// Histogram adds: two u64 atomics of word pairs per splat (the device's
-// atomic rate is per request, whatever its width), saturating through the
-// event counts when a word may reach 2^32; P3_HIST_ADD=quads selects four u32
-// atomics (returning, pinned on wrap, in the saturating kernel).
-bool pair_adds() {
-  static const bool on = [] {
+// Histogram adds (the device's atomic rate is per request, whatever its
+// width): 2 (default, P3_HIST_ADD=packed) one u64 atomic per splat into a
+// packed 16-bit accumulator with carry counts, merged into the histogram
+// before it is read; 1 (P3_HIST_ADD=pairs) two u64 atomics of word pairs,
+// saturating through event counts; 0 (P3_HIST_ADD=quads) four u32 atomics
+// (returning, pinned on wrap, in the saturating kernel).
+int hist_add_mode() {
+  static const int mode = [] {
     const char* e = std::getenv("P3_HIST_ADD");
-    return !(e && std::strcmp(e, "quads") == 0);
+    if (e && std::strcmp(e, "quads") == 0) return 0;
+    if (e && std::strcmp(e, "pairs") == 0) return 1;
+    return 2;
   }();
-  return on;
+  return mode;
 }
 
 std::string chaos_shape_key(const float* xforms, int n_regular, int final_slot) {

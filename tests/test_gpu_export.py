@@ -107,7 +107,7 @@ def test_atomic_probe_counts_every_add():
     rate, per_splat = r.probe_atomics(blocks=64, per_thread=16)
     h = r.read_histogram().astype(np.int64)
     r.destroy()
-    assert rate > 0 and per_splat in (2, 4)
+    assert rate > 0 and per_splat in (1, 2, 4)
     assert h.sum() == 4 * 64 * 256 * 16
     assert np.array_equal(h[..., 0], h[..., 3]) and np.array_equal(h[..., 1], h[..., 2])
     assert np.array_equal(h[..., 0], h[..., 1])

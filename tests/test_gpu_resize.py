@@ -1,4 +1,5 @@
-"""Renderer resize between jobs, with no read-back in between.
+"""Renderer resize between jobs: allocations kept, tables re-uploaded only
+when they change.
 
 `p3_resize` queues the spatial-filter taps and the DE kernel-norm table on
 the stream without waiting for earlier work, and uploads them only when the
@@ -16,9 +17,11 @@ from helpers import genome, sierpinski
 pytestmark = pytest.mark.gpu
 
 # (width, height, oversample, filter radius): the same taps twice in a row,
-# a new size with the same taps, new taps by radius, new taps by oversample
+# a new size with the same taps, new taps by radius, new taps by oversample,
+# and grids with an odd number of bins (135 x 119, 97 x 61) inside larger
+# allocations: every per-bin buffer is cleared to its last word
 STEPS = [(96, 64, 1, 1.0), (96, 64, 1, 1.0), (128, 80, 1, 1.0), (128, 80, 1, 2.5),
-         (64, 48, 2, 2.5), (96, 64, 1, 1.0)]
+         (64, 48, 2, 2.5), (96, 64, 1, 1.0), (135, 119, 1, 1.0), (97, 61, 1, 1.0)]
 
 
 def _render(r, k):
