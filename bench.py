@@ -407,7 +407,10 @@ def main():
     if a.quality is None and not a.no_learned and WEIGHTS.exists():
         learned = learned_path(r, g, a, ref_img, frame.walkers)
     bins = W * H
-    scatter_bytes = 16.0 * samples
+    # algorithmic scatter per post-fuse sample: the 8-byte packed accumulator
+    # word (one request per splat), else the 16-byte histogram bin
+    scatter_b = 8.0 if per_splat == 1 else 16.0
+    scatter_bytes = scatter_b * samples
     de_bytes = 32.0 * bins
     tm_bytes = 16.0 * bins + 4.0 * W * H
     achieved = scatter_bytes / (chaos_ms * 1e-3) / 1e9
@@ -451,7 +454,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": round(traffic["bytes"]) if traffic else None,
                      "traffic_source": traffic["source"] if traffic else None,
-                     "note": "achieved = 16 B of atomic scatter per post-fuse sample; "
+                     "note": f"achieved = {scatter_b:.0f} B of atomic scatter per post-fuse sample; "
                              "traffic = rocprofv3 FETCH_SIZE + WRITE_SIZE of the same launch "
                              "(the atomics leave the L2); the kernel is bound by the rate of "
                              "atomic requests (atomic_roofline), not by HBM bandwidth"},
