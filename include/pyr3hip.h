@@ -48,7 +48,11 @@ typedef struct p3_renderer p3_renderer;
 int p3_create(int width, int height, int oversample, const float* taps, int n_taps,
               p3_renderer** out);
 
-/* Renderer.resize (src/renderer.ts:277-292): rebuild buffers at a new size. */
+/* Renderer.resize (src/renderer.ts:277-292): rebuild the handle at a new
+ * size. Allocations are kept and grown only when the new grid needs more;
+ * work already queued on the handle's stream is not waited for (new filter
+ * taps wait for it before their host copy is replaced). The histogram is
+ * undefined until the next p3_reset, as after p3_create. */
 int p3_resize(p3_renderer* r, int width, int height, int oversample, const float* taps,
               int n_taps);
 
@@ -122,12 +126,14 @@ int p3_read_linear(p3_renderer* r, double k1, double k2, float* out, size_t n);
 
 /* Measurement only (no counterpart in the reference): the device's rate of
  * the histogram splat's atomic requests, for the chaos kernel's roofline.
- * blocks x 256 threads each add 1 to the four u32 words of per_thread 16-byte
- * cells drawn uniformly over the handle's histogram (which it overwrites:
- * reset before the next render), in the splat's request form (two 64-bit
- * adds per cell, or four 32-bit ones with P3_HIST_ADD=quads). Writes the
- * atomic requests per second (HIP events on the handle's stream) and the
- * requests per splat. */
+ * blocks x 256 threads each add 1 to every word of per_thread cells drawn
+ * uniformly over the handle's grid, in the splat's request form: one 64-bit
+ * add per 8-byte cell of the packed accumulator (default; merged into the
+ * histogram at its next read, exact below 2^16 adds per cell), two 64-bit
+ * adds per 16-byte histogram cell (P3_HIST_ADD=pairs) or four 32-bit ones
+ * (P3_HIST_ADD=quads). It overwrites the histogram's contents: reset before
+ * the next render. Writes the atomic requests per second (HIP events on the
+ * handle's stream) and the requests per splat (1, 2 or 4). */
 int p3_probe_atomics(p3_renderer* r, uint32_t blocks, uint32_t per_thread,
                      double* requests_per_s, int* requests_per_splat);
 int p3_read_filtered(p3_renderer* r, float* out, size_t n);      /* DE output grid */
