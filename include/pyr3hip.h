@@ -52,7 +52,10 @@ int p3_create(int width, int height, int oversample, const float* taps, int n_ta
  * size. Allocations are kept and grown only when the new grid needs more;
  * work already queued on the handle's stream is not waited for (new filter
  * taps wait for it before their host copy is replaced). The histogram is
- * undefined until the next p3_reset, as after p3_create. */
+ * undefined until the next p3_reset, as after p3_create. When growing a
+ * buffer fails (P3_E_HIP), the handle is left without usable buffers: every
+ * later call except p3_resize, p3_dims, p3_sync, p3_last_error and
+ * p3_destroy returns P3_E_STATE until a p3_resize succeeds. */
 int p3_resize(p3_renderer* r, int width, int height, int oversample, const float* taps,
               int n_taps);
 
@@ -126,14 +129,13 @@ int p3_read_linear(p3_renderer* r, double k1, double k2, float* out, size_t n);
 
 /* Measurement only (no counterpart in the reference): the device's rate of
  * the histogram splat's atomic requests, for the chaos kernel's roofline.
- * blocks x 256 threads each add 1 to every word of per_thread cells drawn
- * uniformly over the handle's grid, in the splat's request form: one 64-bit
- * add per 8-byte cell of the packed accumulator (default; merged into the
- * histogram at its next read, exact below 2^16 adds per cell), two 64-bit
- * adds per 16-byte histogram cell (P3_HIST_ADD=pairs) or four 32-bit ones
- * (P3_HIST_ADD=quads). It overwrites the histogram's contents: reset before
- * the next render. Writes the atomic requests per second (HIP events on the
- * handle's stream) and the requests per splat (1, 2 or 4). */
+ * blocks x 256 threads each add 1 to every 16-bit field of per_thread cells
+ * drawn uniformly over the handle's grid, in the splat's request form: one
+ * 64-bit add per 8-byte cell of the packed accumulator (merged into the
+ * histogram at its next read, exact below 2^16 adds per cell). It
+ * overwrites the histogram's contents: reset before the next render. Writes
+ * the atomic requests per second (HIP events on the handle's stream) and
+ * the requests per splat (1). */
 int p3_probe_atomics(p3_renderer* r, uint32_t blocks, uint32_t per_thread,
                      double* requests_per_s, int* requests_per_splat);
 int p3_read_filtered(p3_renderer* r, float* out, size_t n);      /* DE output grid */
@@ -157,7 +159,9 @@ int p3_read_trace(p3_renderer* r, float* out, size_t n);
 
 /* Kernel timing with HIP events on the handle's stream: when enabled, each
  * launch is bracketed and p3_read_timings writes the last durations (ms) of
- * [seed, chaos, density, tonemap, reset, denoise] (6 floats). */
+ * [merge, chaos, density, tonemap, reset, denoise] (6 floats); merge is the
+ * packed accumulator's merge into the histogram, run by the first reader
+ * (present, a readback or the linear export) after a dispatch. */
 int p3_set_timing(p3_renderer* r, int enabled);
 int p3_read_timings(p3_renderer* r, float* out6);
 

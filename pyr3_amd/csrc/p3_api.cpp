@@ -24,8 +24,8 @@
 
 namespace {
 
-constexpr int kTimers = 6;  // seed, chaos, density, tonemap, reset, denoise
-enum { T_SEED = 0, T_CHAOS = 1, T_DE = 2, T_TM = 3, T_RESET = 4, T_DENOISE = 5 };
+constexpr int kTimers = 6;  // merge, chaos, density, tonemap, reset, denoise
+enum { T_MERGE = 0, T_CHAOS = 1, T_DE = 2, T_TM = 3, T_RESET = 4, T_DENOISE = 5 };
 
 std::string g_create_error = "";
 
@@ -110,15 +110,16 @@ struct p3_renderer {
   bool palette_set = false;
   bool have_image = false;
   bool have_filtered = false;
-  unsigned long long hist_bound = 0;  // upper bound on any histogram word
-  // Pending adds outside the u32 histogram: the event counts of the
-  // saturating pair adds (4 u32 per bin), or the packed accumulator (1 u64
-  // per bin) and its carry counts (4 u64 per bin). All zero when
-  // sat_ev_clean; sat_dirty when a dispatch may have left adds that
-  // hist_fixup / hist_merge has not yet applied to the histogram.
-  DevBuf<uint32_t> sat_ev;
+  // Set while build() (re)sizes the buffers and left set when it fails: the
+  // buffers may then be missing or smaller than the grid, so every call that
+  // touches them refuses with P3_E_STATE until a p3_resize succeeds.
+  bool broken = false;
+  // Pending adds outside the u32 histogram: the packed accumulator (1 u64
+  // per bin) and its carry counts (4 u64 per bin). All zero when acc_clean;
+  // acc_dirty when a dispatch may have left adds that hist_merge has not yet
+  // applied to the histogram.
   DevBuf<uint64_t> acc, carry;
-  bool sat_ev_clean = false, sat_dirty = false;
+  bool acc_clean = false, acc_dirty = false;
   int trace_walkers = 0, trace_iters = 0;
   bool timing = false;
   bool want_image_f32 = false;
@@ -197,6 +198,13 @@ int build(p3_renderer* r, int width, int height, int os, const float* taps, int 
   const long long sw = (long long)width * os, sh = (long long)height * os;
   if (sw * sh > (1LL << 31))
     return r->fail(P3_E_ARG, "super-res grid %lldx%lld too large", sw, sh);
+  // Until every buffer is sized for the new grid the handle is unusable: a
+  // failed allocation below leaves it so (calls return P3_E_STATE).
+  r->broken = true;
+  r->palette_set = false;
+  r->have_image = r->have_filtered = r->have_image_f32 = false;
+  r->acc_clean = r->acc_dirty = false;
+  r->idx_live = false;
   r->width = width; r->height = height; r->os = os; r->sw = (int)sw; r->sh = (int)sh;
   // The taps upload below reads taps_host asynchronously, so a change of
   // taps waits for the stream before the copy is overwritten; unchanged taps
@@ -244,13 +252,9 @@ int build(p3_renderer* r, int width, int height, int os, const float* taps, int 
       return rc;
     r->tables_on_device = true;
   }
-  r->palette_set = false;
-  r->have_image = r->have_filtered = r->have_image_f32 = false;
-  r->hist_bound = 0;
-  r->sat_ev_clean = r->sat_dirty = false;
-  r->idx_live = false;   // capture restarts (and is re-sized) at its next use
-  r->last_xforms.clear();
+  r->last_xforms.clear();   // the capture restarts (and is re-sized) at its next use
   r->last_distrib.clear();
+  r->broken = false;
   return P3_OK;
 }
 
@@ -266,7 +270,7 @@ void release(p3_renderer* r) {
   r->bounds.release(); r->mbounds.release(); r->acc.release(); r->carry.release();
   r->isaac.release(); r->trace.release(); r->idx.release();
   r->mxforms.release(); r->mdistrib.release(); r->margs.release();
-  r->sat_ev.release(); r->linear.release(); r->curves.release();
+  r->linear.release(); r->curves.release();
 }
 
 int check_records(p3_renderer* r, const float* xforms, int n_slots) {
@@ -318,76 +322,61 @@ int validate_dispatch(p3_renderer* r, const float* xforms, int n_slots, const ui
   return P3_OK;
 }
 
-// Buffers of the pending adds for the current add mode, sized to the grid
-// and zeroed.
-int prepare_sat_events(p3_renderer* r) {
+// The packed accumulator and its carry counts, sized to the grid and zeroed.
+int prepare_acc(p3_renderer* r) {
   int rc;
   const size_t bins = grid_bins(r);
-  const bool packed = p3::hist_add_mode() == 2;
-  if (packed) {
-    if ((rc = r->hip(r->acc.ensure(bins), "hipMalloc(packed accumulator)"))) return rc;
-    if ((rc = r->hip(r->carry.ensure(bins * 4), "hipMalloc(packed carries)"))) return rc;
-  } else if ((rc = r->hip(r->sat_ev.ensure(grid_bins(r) * 4), "hipMalloc(saturation events)"))) {
-    return rc;
-  }
-  if (!r->sat_ev_clean) {
-    if (packed) {
-      if ((rc = r->hip(p3::launch_fill(r->acc.p, bins * 8, 0u, r->stream),
-                       "clear packed accumulator")) ||
-          (rc = r->hip(p3::launch_fill(r->carry.p, bins * 32, 0u, r->stream),
-                       "clear packed carries")))
-        return rc;
-    } else if ((rc = r->hip(p3::launch_fill(r->sat_ev.p, grid_bins(r) * 16, 0u, r->stream),
-                            "clear saturation events"))) {
+  if ((rc = r->hip(r->acc.ensure(bins), "hipMalloc(packed accumulator)"))) return rc;
+  if ((rc = r->hip(r->carry.ensure(bins * 4), "hipMalloc(packed carries)"))) return rc;
+  if (!r->acc_clean) {
+    if ((rc = r->hip(p3::launch_fill(r->acc.p, bins * 8, 0u, r->stream),
+                     "clear packed accumulator")) ||
+        (rc = r->hip(p3::launch_fill(r->carry.p, bins * 32, 0u, r->stream),
+                     "clear packed carries")))
       return rc;
-    }
-    r->sat_ev_clean = true;
+    r->acc_clean = true;
   }
   return P3_OK;
 }
 
 // Applies pending adds: every reader of the histogram calls it.
 int settle_hist(p3_renderer* r) {
-  if (!r->sat_dirty) return P3_OK;
-  r->sat_dirty = false;
-  if (p3::hist_add_mode() == 2)
-    return r->hip(p3::launch_hist_merge(r->hist.p, r->acc.p, r->carry.p, grid_bins(r),
-                                        r->stream),
-                  "packed merge");
-  return r->hip(p3::launch_hist_fixup(r->hist.p, r->sat_ev.p, grid_bins(r), r->stream),
-                "saturation fix-up");
+  if (!r->acc_dirty) return P3_OK;
+  r->acc_dirty = false;
+  r->tic(T_MERGE);
+  const int rc = r->hip(p3::launch_hist_merge(r->hist.p, r->acc.p, r->carry.p, grid_bins(r),
+                                              r->stream),
+                        "packed merge");
+  r->toc(T_MERGE);
+  return rc;
 }
 
 // Drops pending adds when the histogram is overwritten wholesale.
-int drop_sat_events(p3_renderer* r) {
-  if (!r->sat_dirty) return P3_OK;
-  r->sat_dirty = false;
-  if (p3::hist_add_mode() == 2) {
-    int rc;
-    if ((rc = r->hip(p3::launch_fill(r->acc.p, grid_bins(r) * 8, 0u, r->stream),
-                     "clear packed accumulator")))
-      return rc;
-    return r->hip(p3::launch_fill(r->carry.p, grid_bins(r) * 32, 0u, r->stream),
-                  "clear packed carries");
-  }
-  return r->hip(p3::launch_fill(r->sat_ev.p, grid_bins(r) * 16, 0u, r->stream),
-                "clear saturation events");
+int drop_pending(p3_renderer* r) {
+  if (!r->acc_dirty) return P3_OK;
+  r->acc_dirty = false;
+  int rc;
+  if ((rc = r->hip(p3::launch_fill(r->acc.p, grid_bins(r) * 8, 0u, r->stream),
+                   "clear packed accumulator")))
+    return rc;
+  return r->hip(p3::launch_fill(r->carry.p, grid_bins(r) * 32, 0u, r->stream),
+                "clear packed carries");
 }
 
-// Attaches the pending-add buffers to a dispatch: always in packed mode, and
-// for the saturating kernel in pair mode.
-int attach_adds(p3_renderer* r, p3::ChaosArgs& a, bool sat) {
-  const int mode = p3::hist_add_mode();
-  if (mode == 0 || (mode == 1 && !sat)) return P3_OK;
-  int rc;
-  if ((rc = prepare_sat_events(r))) return rc;
-  if (mode == 2) {
-    a.acc = r->acc.p;
-    a.carry = r->carry.p;
-  } else {
-    a.sat_events = r->sat_ev.p;
-  }
+// Attaches the packed accumulator to a dispatch.
+int attach_adds(p3_renderer* r, p3::ChaosArgs& a) {
+  const int rc = prepare_acc(r);
+  if (rc) return rc;
+  a.acc = r->acc.p;
+  a.carry = r->carry.p;
   return P3_OK;
+}
+
+// P3_E_STATE when a failed resize left the handle without its buffers.
+int usable(p3_renderer* r) {
+  if (!r->broken) return P3_OK;
+  return r->fail(P3_E_STATE, "a failed resize left the renderer without its buffers; "
+                             "p3_resize again or destroy it");
 }
 
 // Kernel arguments of one dispatch whose records and table are at dxf / dds.
@@ -420,7 +409,6 @@ p3::ChaosArgs dispatch_args(p3_renderer* r, const uint32_t* u, int n_regular, in
   a.trap_freq = bits_f(u[26]); a.trap_strength = bits_f(u[27]);
   a.phase_strength = bits_f(u[28]); a.phase_freq = bits_f(u[29]);
   a.idx_sum = nullptr;
-  a.sat_events = nullptr;
   return a;
 }
 
@@ -455,6 +443,7 @@ int attach_trace(p3_renderer* r, p3::ChaosArgs& a) {
 int iterate_impl(p3_renderer* r, const float* xforms, int n_slots, const uint32_t* distrib,
                  int n_rows, const uint32_t* u, const uint32_t* states) {
   if (!r) return P3_E_ARG;
+  if (usable(r)) return P3_E_STATE;
   if (!r->palette_set) return r->fail(P3_E_STATE, "iterate before reset (no palette)");
   if (!xforms || !distrib || !u) return r->fail(P3_E_ARG, "null argument");
   const size_t n_xf = (size_t)P3_XFORM_FLOATS * (n_slots > 0 ? n_slots : 0);
@@ -510,10 +499,7 @@ int iterate_impl(p3_renderer* r, const float* xforms, int n_slots, const uint32_
   }
   if ((rc = attach_trace(r, a))) return rc;
   if ((rc = attach_capture(r, a, u))) return rc;
-  // Each walker splats at most `iters` times, each adding <= 255 per word.
-  r->hist_bound += 255ULL * (unsigned long long)walkers * (unsigned long long)a.iters;
-  const bool sat = r->hist_bound >= 0xffffffffULL;
-  if ((rc = attach_adds(r, a, sat))) return rc;
+  if ((rc = attach_adds(r, a))) return rc;
   std::string rtc_err;
   {  // compile (or fetch) the kernel for this shape before the timed launch
     const std::vector<char>* code = nullptr;
@@ -521,11 +507,10 @@ int iterate_impl(p3_renderer* r, const float* xforms, int n_slots, const uint32_
       return r->fail(P3_E_HIP, "chaos kernel specialisation: %s", rtc_err.c_str());
   }
   r->tic(T_CHAOS);
-  const hipError_t le =
-      p3::launch_chaos_rtc(a, xforms, n_regular, final_idx, sat, r->stream, rtc_err);
+  const hipError_t le = p3::launch_chaos_rtc(a, xforms, n_regular, final_idx, r->stream, rtc_err);
   if (le != hipSuccess)
     return r->fail(P3_E_HIP, "chaos launch: %s %s", hipGetErrorString(le), rtc_err.c_str());
-  if (a.sat_events || a.acc) r->sat_dirty = true;
+  r->acc_dirty = true;
   r->toc(T_CHAOS);
   r->have_image = false;
   return P3_OK;
@@ -538,6 +523,7 @@ int iterate_multi_impl(p3_renderer* r, int n_sub, const float* const* xforms, in
                        const uint32_t* const* distribs, int n_rows,
                        const uint32_t* const* uniforms) {
   if (!r) return P3_E_ARG;
+  if (usable(r)) return P3_E_STATE;
   if (!r->palette_set) return r->fail(P3_E_STATE, "iterate before reset (no palette)");
   if (n_sub < 1 || n_sub > p3::kMaxSub)
     return r->fail(P3_E_ARG, "n_sub must be 1..%d (got %d)", p3::kMaxSub, n_sub);
@@ -575,7 +561,6 @@ int iterate_multi_impl(p3_renderer* r, int n_sub, const float* const* xforms, in
   p3::MultiArgs M{};
   M.args = (const p3::ChaosArgs*)r->margs.p;
   M.n = (uint32_t)n_sub;
-  unsigned long long bound = 0;
   for (int k = 0; k < n_sub; ++k) {
     float* dxf = r->mxforms.p + n_xf * k;
     uint32_t* dds = r->mdistrib.p + table * k;
@@ -597,12 +582,10 @@ int iterate_multi_impl(p3_renderer* r, int n_sub, const float* const* xforms, in
     if ((rc = attach_capture(r, args[k], uniforms[k]))) return rc;
     const uint32_t blk = (uint32_t)p3::chaos_block();
     M.start[k + 1] = M.start[k] + (args[k].walkers + blk - 1) / blk;
-    bound += 255ULL * (unsigned long long)args[k].walkers * (unsigned long long)args[k].iters;
   }
   if (M.start[n_sub] == 0) return P3_OK;
-  const bool sat = r->hist_bound + bound >= 0xffffffffULL;
   for (int k = 0; k < n_sub; ++k)
-    if ((rc = attach_adds(r, args[k], sat))) return rc;
+    if ((rc = attach_adds(r, args[k]))) return rc;
   if ((rc = r->hip(hipMemcpyAsync(r->mbounds.p, bounds.data(), bounds.size() * sizeof(uint16_t),
                                   hipMemcpyHostToDevice, r->stream),
                    "upload batched pick bounds")))
@@ -611,7 +594,6 @@ int iterate_multi_impl(p3_renderer* r, int n_sub, const float* const* xforms, in
                                   hipMemcpyHostToDevice, r->stream),
                    "upload batched args")))
     return rc;
-  r->hist_bound += bound;
   std::string rtc_err;
   {
     const std::vector<char>* code = nullptr;
@@ -620,11 +602,11 @@ int iterate_multi_impl(p3_renderer* r, int n_sub, const float* const* xforms, in
   }
   r->tic(T_CHAOS);
   const hipError_t le =
-      p3::launch_chaos_rtc_multi(M, xforms[0], n_regular, final_idx, sat, r->stream, rtc_err);
+      p3::launch_chaos_rtc_multi(M, xforms[0], n_regular, final_idx, r->stream, rtc_err);
   if (le != hipSuccess)
     return r->fail(P3_E_HIP, "batched chaos launch: %s %s", hipGetErrorString(le),
                    rtc_err.c_str());
-  if (args[0].sat_events || args[0].acc) r->sat_dirty = true;
+  r->acc_dirty = true;
   r->toc(T_CHAOS);
   r->have_image = false;
   return P3_OK;
@@ -693,6 +675,7 @@ void p3_destroy(p3_renderer* r) {
 
 int p3_reset(p3_renderer* r, const float* palette_rgba) {
   if (!r) return P3_E_ARG;
+  if (usable(r)) return P3_E_STATE;
   if (!palette_rgba) return r->fail(P3_E_ARG, "null palette");
   int rc;
   if ((rc = r->hip(hipMemcpyAsync(r->palette.p, palette_rgba, sizeof(float) * 1024,
@@ -702,13 +685,12 @@ int p3_reset(p3_renderer* r, const float* palette_rgba) {
   r->tic(T_RESET);
   if ((rc = r->hip(p3::launch_fill(r->hist.p, grid_bins(r) * 16, 0u, r->stream), "clear histogram")))
     return rc;
-  if ((rc = drop_sat_events(r))) return rc;
+  if ((rc = drop_pending(r))) return rc;
   if (r->idx_live &&
       (rc = r->hip(p3::launch_fill(r->idx.p, grid_bins(r) * 4, 0u, r->stream), "clear capture")))
     return rc;
   r->toc(T_RESET);
   r->palette_set = true;
-  r->hist_bound = 0;
   r->have_image = r->have_filtered = false;
   return P3_OK;
 }
@@ -732,6 +714,7 @@ int p3_iterate_states(p3_renderer* r, const float* xforms, int n_slots, const ui
 
 int p3_present(p3_renderer* r, const uint32_t* de, const uint32_t* viz) {
   if (!r) return P3_E_ARG;
+  if (usable(r)) return P3_E_STATE;
   if (!viz) return r->fail(P3_E_ARG, "null tonemap uniforms");
   if ((int)viz[0] != r->width || (int)viz[1] != r->height || (int)viz[8] != r->os)
     return r->fail(P3_E_ARG, "tonemap uniforms %ux%u os %u do not match renderer %dx%d os %d",
@@ -796,22 +779,20 @@ int p3_present(p3_renderer* r, const uint32_t* de, const uint32_t* viz) {
 int p3_probe_atomics(p3_renderer* r, uint32_t blocks, uint32_t per_thread,
                      double* requests_per_s, int* requests_per_splat) {
   if (!r || !requests_per_s || !requests_per_splat) return P3_E_ARG;
+  if (usable(r)) return P3_E_STATE;
   if (blocks == 0 || blocks > 65535 || per_thread == 0 || per_thread > 4096)
     return r->fail(P3_E_ARG, "probe size out of range (blocks 1..65535, per_thread 1..4096)");
   const uint32_t cells = (uint32_t)((size_t)r->sw * r->sh);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc;
-  if ((rc = drop_sat_events(r))) return rc;
+  if ((rc = drop_pending(r))) return rc;
+  if ((rc = prepare_acc(r))) return rc;   // the requests land in the accumulator
   if ((rc = r->hip(hipEventCreate(&e0), "hipEventCreate")) ||
       (rc = r->hip(hipEventCreate(&e1), "hipEventCreate")))
     return rc;
   float ms = 0.0f;
   rc = r->hip(hipEventRecord(e0, r->stream), "hipEventRecord");
-  const int mode = p3::hist_add_mode();
-  // packed mode: the requests land in the accumulator, merged afterwards
-  if (!rc && mode == 2) rc = prepare_sat_events(r);
-  if (!rc) rc = r->hip(p3::launch_atomic_probe(mode == 2 ? (uint32_t*)r->acc.p : r->hist.p,
-                                                cells, blocks, per_thread, 12345u, mode,
+  if (!rc) rc = r->hip(p3::launch_atomic_probe(r->acc.p, cells, blocks, per_thread, 12345u,
                                                 r->stream), "atomic probe launch");
   if (!rc) rc = r->hip(hipEventRecord(e1, r->stream), "hipEventRecord");
   if (!rc) rc = r->hip(hipEventSynchronize(e1), "hipEventSynchronize");
@@ -819,16 +800,19 @@ int p3_probe_atomics(p3_renderer* r, uint32_t blocks, uint32_t per_thread,
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) return rc;
-  r->have_image = false;   // the histogram now holds probe counts:
-  r->hist_bound = 0xffffffffULL;   // unbounded until the next reset
-  if (mode == 2) r->sat_dirty = true;   // merged (exact below 2^16 per cell) at the next read
-  *requests_per_splat = mode == 2 ? 1 : mode == 1 ? 2 : 4;
-  *requests_per_s = *requests_per_splat * 256.0 * blocks * per_thread / (ms * 1e-3);
+  // The accumulator now holds probe counts (no carries are counted: exact
+  // below 2^16 adds per cell); they are merged into the histogram at its
+  // next read, so reset before the next render.
+  r->have_image = false;
+  r->acc_dirty = true;
+  *requests_per_splat = 1;
+  *requests_per_s = 256.0 * blocks * per_thread / (ms * 1e-3);
   return P3_OK;
 }
 
 int p3_read_linear(p3_renderer* r, double k1, double k2, float* out, size_t n) {
   if (!r) return P3_E_ARG;
+  if (usable(r)) return P3_E_STATE;
   const size_t want = (size_t)r->width * r->height * 4;
   if (!out || n != want) return r->fail(P3_E_ARG, "linear export needs %zu floats", want);
   if (!std::isfinite(k1) || !std::isfinite(k2))
@@ -847,6 +831,7 @@ int p3_read_linear(p3_renderer* r, double k1, double k2, float* out, size_t n) {
 
 int p3_set_curves(p3_renderer* r, const float* lut, size_t n) {
   if (!r) return P3_E_ARG;
+  if (usable(r)) return P3_E_STATE;
   if (!lut || n == 0) {
     r->curves_set = false;
     return P3_OK;
@@ -868,6 +853,7 @@ int p3_set_curves(p3_renderer* r, const float* lut, size_t n) {
 
 int p3_read_histogram(p3_renderer* r, uint32_t* out, size_t n) {
   if (!r) return P3_E_ARG;
+  if (usable(r)) return P3_E_STATE;
   if (!out || n != grid_bins(r) * 4)
     return r->fail(P3_E_ARG, "histogram needs %zu words", grid_bins(r) * 4);
   const int rc = settle_hist(r);
@@ -877,6 +863,7 @@ int p3_read_histogram(p3_renderer* r, uint32_t* out, size_t n) {
 
 int p3_read_index(p3_renderer* r, uint32_t* out, size_t n) {
   if (!r) return P3_E_ARG;
+  if (usable(r)) return P3_E_STATE;
   if (!r->idx_live) return r->fail(P3_E_STATE, "no dispatch has captured the colour index");
   if (!out || n != grid_bins(r))
     return r->fail(P3_E_ARG, "capture buffer needs %zu words", grid_bins(r));
@@ -885,6 +872,7 @@ int p3_read_index(p3_renderer* r, uint32_t* out, size_t n) {
 
 int p3_read_filtered(p3_renderer* r, float* out, size_t n) {
   if (!r) return P3_E_ARG;
+  if (usable(r)) return P3_E_STATE;
   if (!out || n != grid_bins(r) * 4)
     return r->fail(P3_E_ARG, "filtered grid needs %zu floats", grid_bins(r) * 4);
   if (!r->have_filtered) return r->fail(P3_E_STATE, "no DE output yet");
@@ -893,6 +881,7 @@ int p3_read_filtered(p3_renderer* r, float* out, size_t n) {
 
 int p3_read_image(p3_renderer* r, uint8_t* out, size_t n) {
   if (!r) return P3_E_ARG;
+  if (usable(r)) return P3_E_STATE;
   if (!out || n != (size_t)r->width * r->height * 4)
     return r->fail(P3_E_ARG, "image needs %zu bytes", (size_t)r->width * r->height * 4);
   if (!r->have_image) return r->fail(P3_E_STATE, "present has not run since the last change");
@@ -901,6 +890,7 @@ int p3_read_image(p3_renderer* r, uint8_t* out, size_t n) {
 
 int p3_read_image_f32(p3_renderer* r, float* out, size_t n) {
   if (!r) return P3_E_ARG;
+  if (usable(r)) return P3_E_STATE;
   if (!out || n != (size_t)r->width * r->height * 4)
     return r->fail(P3_E_ARG, "image needs %zu floats", (size_t)r->width * r->height * 4);
   if (!r->have_image || !r->have_image_f32)
@@ -910,16 +900,14 @@ int p3_read_image_f32(p3_renderer* r, float* out, size_t n) {
 
 int p3_write_histogram(p3_renderer* r, const uint32_t* data, size_t n) {
   if (!r) return P3_E_ARG;
+  if (usable(r)) return P3_E_STATE;
   if (!data || n != grid_bins(r) * 4)
     return r->fail(P3_E_ARG, "histogram needs %zu words", grid_bins(r) * 4);
-  int rc = drop_sat_events(r);
+  int rc = drop_pending(r);
   if (rc) return rc;
   if ((rc = r->hip(hipMemcpyAsync(r->hist.p, data, n * 4, hipMemcpyHostToDevice, r->stream),
                    "write histogram")))
     return rc;
-  unsigned long long mx = 0;
-  for (size_t i = 0; i < n; ++i) mx = std::max<unsigned long long>(mx, data[i]);
-  r->hist_bound = mx;
   r->have_image = r->have_filtered = false;
   return r->hip(hipStreamSynchronize(r->stream), "write histogram");
 }
@@ -935,6 +923,7 @@ int p3_set_trace(p3_renderer* r, int walkers, int iters) {
 
 int p3_read_trace(p3_renderer* r, float* out, size_t n) {
   if (!r) return P3_E_ARG;
+  if (usable(r)) return P3_E_STATE;
   const size_t need = (size_t)r->trace_walkers * r->trace_iters * P3_TRACE_FLOATS;
   if (!out || n != need || need == 0) return r->fail(P3_E_ARG, "trace needs %zu floats", need);
   if (r->trace.n < need) return r->fail(P3_E_STATE, "no trace recorded");
@@ -1095,6 +1084,7 @@ int p3_denoiser_destroy(p3_denoiser* d) {
 
 int p3_denoise(p3_renderer* r, p3_denoiser* d) {
   if (!r) return P3_E_ARG;
+  if (usable(r)) return P3_E_STATE;
   if (!d) return r->fail(P3_E_ARG, "null denoiser");
   if (r->os != 1) return r->fail(P3_E_STATE, "the denoiser runs at oversample 1");
   if (!r->have_image || !r->have_image_f32)

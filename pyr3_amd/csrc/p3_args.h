@@ -43,12 +43,9 @@ struct ChaosArgs {
   float trap_cx, trap_cy, trap_radius, trap_nx, trap_ny, trap_falloff, trap_freq, trap_strength;
   float phase_strength, phase_freq;
   uint32_t* idx_sum;
-  // saturating pair adds: four event counts per bin (low-word carries and
-  // 64-bit wraps of the (R, G) and (B, count) pairs); null otherwise
-  uint32_t* sat_events;
   // packed adds: one u64 per bin holding (R, G, B, count) as four 16-bit
   // fields, and four u64 counts per bin of the carries out of each field
-  // (the last: out of bit 63); null otherwise
+  // (the last: out of bit 63)
   uint64_t* acc;
   uint64_t* carry;
 };

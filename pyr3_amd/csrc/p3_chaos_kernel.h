@@ -19,16 +19,13 @@
 // channels of bin (y * W + x). The reference's adds saturate at 2^32 - 1; a
 // saturating add of non-negative values is order-independent (the final
 // value is min(start + sum, 2^32 - 1)). The device's atomic rate is per
-// request, whatever its width, so the default form (Spec::kHistMode 2) is
-// one 64-bit add per splat into a packed 16-bit accumulator with exact carry
-// counts (add_packed), merged into the u32 histogram before it is read.
-// The other forms: two 64-bit pair adds with carry and wrap counts (mode 1);
-// four u32 adds (mode 0, and a packed-mode splat with a word of 2^16 or
-// more), whose returned old values detect a wrap, followed by an exchange to
-// 2^32 - 1: after the last wrap of a word the last operation on it is always
-// that exchange. When the host has proven that no word can reach 2^32 (bound
-// on 255 x samples since the last reset) the u32 adds are issued without a
-// return value.
+// request, whatever its width, so a splat is ONE 64-bit add into a packed
+// 16-bit accumulator with exact carry counts (add_packed), merged into the
+// u32 histogram before it is read. A splat with a word of 2^16 or more (a
+// palette or opacity above 1, which the reference's packers never produce)
+// goes straight into the histogram as four u32 adds whose returned old
+// values detect a wrap, followed by an exchange to 2^32 - 1: after the last
+// wrap of a word the last operation on it is always that exchange.
 #pragma once
 #include "p3_device.h"
 #include "p3_variation_dispatch.h"
@@ -60,38 +57,12 @@ P3_INLINE bool is_bad(float2 v) {
   return isnan(v.x) || isnan(v.y) || fabsf(v.x) > 1e10f || fabsf(v.y) > 1e10f;
 }
 
-// Four channel adds at device scope (relaxed ordering: only the final sums
-// matter); with kSat the returned old values are checked after all four are
-// issued.
+// Device-scope relaxed atomics: only the final sums matter.
 P3_INLINE uint32_t add_ret(P3_GLOBAL uint32_t* p, uint32_t d) {
   return __hip_atomic_fetch_add(p, d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 P3_INLINE void pin_max(P3_GLOBAL uint32_t* p) {
   (void)__hip_atomic_exchange(p, 0xffffffffu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-// Two 64-bit adds of word pairs (R,G) and (B,count): exact when no word can
-// reach 2^32 (the non-saturating case), since then no carry crosses words.
-P3_INLINE void add_pair(P3_GLOBAL uint32_t* p, uint32_t lo, uint32_t hi) {
-  const uint64_t v = (uint64_t)lo | ((uint64_t)hi << 32);
-  if (v)
-    (void)__hip_atomic_fetch_add((P3_GLOBAL uint64_t*)p, v, __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_AGENT);
-}
-// Saturating adds of a word pair (lo, hi) as one 64-bit add. The returned
-// old value is the pair's exact 64-bit prefix, so the thread whose add
-// carries out of the low word, or wraps the 64 bits, knows it: it counts the
-// event in ev[0] (carries) or ev[1] (wraps). Every add crosses at most one
-// multiple of 2^32 in the low word and wraps at most once, so the counts are
-// exact, and hist_fixup (p3_image.hip) turns them into min(sum, 2^32 - 1) per
-// word before the histogram is read.
-P3_INLINE void add_pair_sat(P3_GLOBAL uint32_t* p, P3_GLOBAL uint32_t* ev, uint32_t lo,
-                            uint32_t hi) {
-  const uint64_t v = (uint64_t)lo | ((uint64_t)hi << 32);
-  if (!v) return;
-  const uint64_t o = __hip_atomic_fetch_add((P3_GLOBAL uint64_t*)p, v, __ATOMIC_RELAXED,
-                                            __HIP_MEMORY_SCOPE_AGENT);
-  if ((uint64_t)(uint32_t)o + lo > 0xffffffffull) (void)add_ret(ev + 0, 1u);
-  if (o + v < o) (void)add_ret(ev + 1, 1u);
 }
 // Packed add of one splat: (d0, d1, d2, d3), each below 2^16, as the four
 // 16-bit fields of the bin's u64 accumulator, one 64-bit atomic. The returned
@@ -117,32 +88,19 @@ P3_INLINE void add_packed(P3_GLOBAL uint64_t* acc, P3_GLOBAL uint64_t* carry, ui
     if (c3) (void)__hip_atomic_fetch_add(carry + 3, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
-// Four channel adds (R, G, B, count) at b; ev = the bin's four event counts
-// (read only by the saturating pair form).
-template <bool kSat, bool kPairs>
-P3_INLINE void hist_add4(P3_GLOBAL uint32_t* b, P3_GLOBAL uint32_t* ev, uint32_t d0, uint32_t d1,
-                         uint32_t d2, uint32_t d3) {
-  if (kSat && kPairs) {
-    add_pair_sat(b + 0, ev + 0, d0, d1);
-    add_pair_sat(b + 2, ev + 2, d2, d3);
-  } else if (!kSat && kPairs) {
-    add_pair(b + 0, d0, d1);
-    add_pair(b + 2, d2, d3);
-  } else if (kSat) {
-    const uint32_t o0 = d0 ? add_ret(b + 0, d0) : 0u;
-    const uint32_t o1 = d1 ? add_ret(b + 1, d1) : 0u;
-    const uint32_t o2 = d2 ? add_ret(b + 2, d2) : 0u;
-    const uint32_t o3 = d3 ? add_ret(b + 3, d3) : 0u;
-    if (d0 && o0 > 0xffffffffu - d0) pin_max(b + 0);
-    if (d1 && o1 > 0xffffffffu - d1) pin_max(b + 1);
-    if (d2 && o2 > 0xffffffffu - d2) pin_max(b + 2);
-    if (d3 && o3 > 0xffffffffu - d3) pin_max(b + 3);
-  } else {
-    if (d0) (void)add_ret(b + 0, d0);
-    if (d1) (void)add_ret(b + 1, d1);
-    if (d2) (void)add_ret(b + 2, d2);
-    if (d3) (void)add_ret(b + 3, d3);
-  }
+// Four saturating channel adds (R, G, B, count) at b: the returned old
+// values are checked after all four are issued; a word that wrapped is
+// pinned to 2^32 - 1.
+P3_INLINE void hist_add4_sat(P3_GLOBAL uint32_t* b, uint32_t d0, uint32_t d1, uint32_t d2,
+                             uint32_t d3) {
+  const uint32_t o0 = d0 ? add_ret(b + 0, d0) : 0u;
+  const uint32_t o1 = d1 ? add_ret(b + 1, d1) : 0u;
+  const uint32_t o2 = d2 ? add_ret(b + 2, d2) : 0u;
+  const uint32_t o3 = d3 ? add_ret(b + 3, d3) : 0u;
+  if (d0 && o0 > 0xffffffffu - d0) pin_max(b + 0);
+  if (d1 && o1 > 0xffffffffu - d1) pin_max(b + 1);
+  if (d2 && o2 > 0xffffffffu - d2) pin_max(b + 2);
+  if (d3 && o3 > 0xffffffffu - d3) pin_max(b + 3);
 }
 
 // Palette entry i (four floats read through the constant address space).
@@ -224,7 +182,7 @@ struct Walker {
 // One turn of walker `wid` (global id) after its pick `fn` (from draw `raw`)
 // is known: chain, blend, bad check and retry window, final lens, jitter,
 // splat. Advances `wk.iter` unless the turn is retried.
-template <bool kSat, class Spec>
+template <class Spec>
 P3_INLINE void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32_t fn,
                                 uint32_t raw, uint32_t wid) {
   const uint32_t i = wk.iter;
@@ -333,16 +291,13 @@ P3_INLINE void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32_t fn,
       const float wgt = opacity * 255.0f;
       const size_t bin = ((size_t)yi * A.width + xi) * 4u;
       P3_GLOBAL uint32_t* b = (P3_GLOBAL uint32_t*)A.hist + bin;
-      P3_GLOBAL uint32_t* ev = (P3_GLOBAL uint32_t*)A.sat_events + bin;
       const uint32_t d0 = f2u_sat(pal.x * wgt), d1 = f2u_sat(pal.y * wgt),
                      d2 = f2u_sat(pal.z * wgt), d3 = f2u_sat(wgt);
-      if (Spec::kHistMode == 2 && (d0 | d1 | d2 | d3) <= 0xffffu)
+      if ((d0 | d1 | d2 | d3) <= 0xffffu)
         add_packed((P3_GLOBAL uint64_t*)A.acc + (bin >> 2), (P3_GLOBAL uint64_t*)A.carry + bin,
                    d0, d1, d2, d3);
-      else if (Spec::kHistMode == 1)
-        hist_add4<kSat, true>(b, ev, d0, d1, d2, d3);
-      else   // four u32 adds; in packed mode, a splat with a word of 2^16 or more
-        hist_add4<kSat, false>(b, ev, d0, d1, d2, d3);
+      else
+        hist_add4_sat(b, d0, d1, d2, d3);
       if (A.idx_sum != nullptr) {   // capture of the colour index (chaos.wgsl:8720-8728)
         const uint32_t d = f2u_sat(fminf(fmaxf(sz, 0.0f), 1.0f) * wgt);
         P3_GLOBAL uint32_t* q = (P3_GLOBAL uint32_t*)A.idx_sum + (size_t)yi * A.width + xi;
@@ -399,7 +354,7 @@ P3_INLINE uint32_t draw_pick(const ChaosArgs& A, const BlockLds<kB>& L, Rng& rng
 // Each walker keeps its own ISAAC column and loop state; only which lane
 // computes it changes, so the per-walker draw sequence (pick, turn, pick,
 // ...), arithmetic and splats are those of the one-lane-per-walker loop.
-template <bool kSat, int kB, class Spec>
+template <int kB, class Spec>
 P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L, uint32_t block) {
   const int t = threadIdx.x;
   const uint32_t w0 = block * kB;
@@ -485,7 +440,7 @@ P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L, uint32_t block)
       rng.draws = L.draws[wl];
       isaac_top_up(rng.s, rng.m);   // the wave's lanes generate rounds together here
       Walker wk{L.x[wl], L.y[wl], L.z[wl], L.prev[wl], L.consec[wl], L.iter[wl]};
-      run_turn<kSat, Spec>(A, wk, rng, L.fn[wl], L.raw[wl], w0 + wl);
+      run_turn<Spec>(A, wk, rng, L.fn[wl], L.raw[wl], w0 + wl);
       if (wk.iter < total) {
         uint32_t raw;
         f = draw_pick<Spec>(A, L, rng, wk.prev, raw);

@@ -373,30 +373,6 @@ hipError_t launch_linear_export(const LinArgs& a, hipStream_t stream) {
 
 // ----------------------------------------------------- saturation fix-up --
 
-// After chaos dispatches that used saturating pair adds: a pair (lo, hi)
-// whose adds produced c low-word carries and n 64-bit wraps holds
-// lo = S_lo mod 2^32 and hi = (S_hi + c) mod 2^32 for the true sums S, so
-// S_hi = hi + n 2^32 - c and S_lo >= 2^32 iff c > 0. Each word becomes
-// min(S, 2^32 - 1) and the counts are cleared. Bins without events are only
-// read (16 B of counts).
-__device__ inline void fix_pair(uint32_t& lo, uint32_t& hi, uint32_t c, uint32_t n) {
-  const uint64_t s_hi = (uint64_t)hi + ((uint64_t)n << 32) - c;
-  hi = s_hi > 0xffffffffull ? 0xffffffffu : (uint32_t)s_hi;
-  if (c) lo = 0xffffffffu;
-}
-
-__global__ void __launch_bounds__(256) hist_fixup(uint4* hist, uint4* ev, size_t bins) {
-  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= bins) return;
-  const uint4 e = ev[i];
-  if ((e.x | e.y | e.z | e.w) == 0u) return;
-  uint4 h = hist[i];
-  fix_pair(h.x, h.y, e.x, e.y);
-  fix_pair(h.z, h.w, e.z, e.w);
-  hist[i] = h;
-  ev[i] = make_uint4(0u, 0u, 0u, 0u);
-}
-
 // After chaos dispatches that used packed adds: bin i's accumulator holds
 // four 16-bit fields f_k = (S_k + c_{k-1}) mod 2^16 of the true sums S_k,
 // with c_k the counted carries out of field k (c_{-1} = 0), so
@@ -441,52 +417,26 @@ hipError_t launch_hist_merge(uint32_t* hist, uint64_t* acc, uint64_t* carry, siz
   return hipGetLastError();
 }
 
-hipError_t launch_hist_fixup(uint32_t* hist, uint32_t* ev, size_t bins, hipStream_t stream) {
-  const size_t grid = (bins + 255) / 256;
-  if (grid == 0) return hipSuccess;
-  hist_fixup<<<(unsigned)grid, 256, 0, stream>>>((uint4*)hist, (uint4*)ev, bins);
-  return hipGetLastError();
-}
-
 // ------------------------------------------------------- atomic probe --
 
-// Scatter-add probe: `per_thread` random cells per thread over `cells`
-// cells, each in one request form of the histogram adds: mode 2 one 64-bit
-// add per 8-byte cell (the packed accumulator), mode 1 two 64-bit adds per
-// 16-byte cell, mode 0 four 32-bit adds per 16-byte cell (relaxed,
-// device-scope). Every add puts 1 in each 32-bit word (mode 2: in each
-// 16-bit field) of its cell.
-template <int kMode>
-__global__ void __launch_bounds__(256) atomic_probe(uint32_t* grid, uint32_t cells,
+// Scatter-add probe in the splat's request form: `per_thread` random cells
+// per thread over `cells` 8-byte cells, one relaxed device-scope 64-bit add
+// per cell that puts 1 in each of its four 16-bit fields (the packed
+// accumulator's form).
+__global__ void __launch_bounds__(256) atomic_probe(uint64_t* grid, uint32_t cells,
                                                     uint32_t per_thread, uint32_t seed) {
   uint32_t s = (blockIdx.x * 256u + threadIdx.x) * 2654435761u ^ seed;
   for (uint32_t i = 0; i < per_thread; ++i) {
     s = s * 1664525u + 1013904223u;
     const uint32_t c = (uint32_t)(((uint64_t)s * cells) >> 32);
-    if (kMode == 2) {
-      (void)__hip_atomic_fetch_add((uint64_t*)grid + c, 0x0001000100010001ull, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-    } else if (kMode == 1) {
-      uint32_t* p = grid + (size_t)c * 4u;
-#pragma unroll
-      for (int k = 0; k < 2; ++k)
-        (void)__hip_atomic_fetch_add((uint64_t*)p + k, 0x100000001ull, __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      uint32_t* p = grid + (size_t)c * 4u;
-#pragma unroll
-      for (int k = 0; k < 4; ++k)
-        (void)__hip_atomic_fetch_add(p + k, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    (void)__hip_atomic_fetch_add(grid + c, 0x0001000100010001ull, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
-hipError_t launch_atomic_probe(uint32_t* grid, uint32_t cells, uint32_t blocks,
-                               uint32_t per_thread, uint32_t seed, int mode,
-                               hipStream_t stream) {
-  if (mode == 2) atomic_probe<2><<<blocks, 256, 0, stream>>>(grid, cells, per_thread, seed);
-  else if (mode == 1) atomic_probe<1><<<blocks, 256, 0, stream>>>(grid, cells, per_thread, seed);
-  else atomic_probe<0><<<blocks, 256, 0, stream>>>(grid, cells, per_thread, seed);
+hipError_t launch_atomic_probe(uint64_t* grid, uint32_t cells, uint32_t blocks,
+                               uint32_t per_thread, uint32_t seed, hipStream_t stream) {
+  atomic_probe<<<blocks, 256, 0, stream>>>(grid, cells, per_thread, seed);
   return hipGetLastError();
 }
 

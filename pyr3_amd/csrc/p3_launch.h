@@ -47,21 +47,17 @@ struct LinArgs {
   double scale;             // (k1 * k2) / os^2
 };
 hipError_t launch_linear_export(const LinArgs& a, hipStream_t stream);
-hipError_t launch_hist_fixup(uint32_t* hist, uint32_t* ev, size_t bins, hipStream_t stream);
 hipError_t launch_hist_merge(uint32_t* hist, uint64_t* acc, uint64_t* carry, size_t bins,
                              hipStream_t stream);
-hipError_t launch_atomic_probe(uint32_t* grid, uint32_t cells, uint32_t blocks,
-                               uint32_t per_thread, uint32_t seed, int mode,
-                               hipStream_t stream);
+hipError_t launch_atomic_probe(uint64_t* cells_base, uint32_t cells, uint32_t blocks,
+                               uint32_t per_thread, uint32_t seed, hipStream_t stream);
 
 // Chaos kernel specialised for the shape of `host_xforms` (p3_rtc.cpp).
 hipError_t launch_chaos_rtc(const ChaosArgs& a, const float* host_xforms, int n_regular,
-                            int final_slot, bool saturating, hipStream_t stream,
-                            std::string& err);
+                            int final_slot, hipStream_t stream, std::string& err);
 // Batched launch of sub-dispatches sharing the shape of `host_xforms`.
 hipError_t launch_chaos_rtc_multi(const MultiArgs& margs, const float* host_xforms, int n_regular,
-                                  int final_slot, bool saturating, hipStream_t stream,
-                                  std::string& err);
+                                  int final_slot, hipStream_t stream, std::string& err);
 bool chaos_code(const float* host_xforms, int n_regular, int final_slot,
                 const std::vector<char>** out, std::string& err);
 // True when this shape's specialised kernel is compiled (in this process or
@@ -71,7 +67,6 @@ std::string chaos_source(const float* host_xforms, int n_regular, int final_slot
 std::string chaos_shape_key(const float* host_xforms, int n_regular, int final_slot);
 size_t chaos_compiled_shapes();
 int chaos_block();  // walkers per chaos block
-int hist_add_mode();  // 2 packed u64 (default), 1 u64 pairs, 0 four u32 (P3_HIST_ADD)
 hipError_t launch_density(const DeArgs& a, hipStream_t stream);
 hipError_t launch_tonemap(const TmArgs& a, bool de, hipStream_t stream);
 hipError_t launch_fill(void* p, size_t bytes, uint32_t v, hipStream_t stream);

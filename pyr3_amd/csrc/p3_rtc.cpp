@@ -34,7 +34,7 @@ namespace {
 
 struct Module {
   hipModule_t mod = nullptr;
-  hipFunction_t sat = nullptr, nosat = nullptr, msat = nullptr, mnosat = nullptr;
+  hipFunction_t single = nullptr, multi = nullptr;
 };
 
 std::mutex g_mu;
@@ -100,7 +100,6 @@ void emit_body(std::ostringstream& s, const float* xf, int slot, bool full) {
 std::string generate(const float* xforms, int n_regular, int final_slot) {
   std::ostringstream s;
   s << "namespace p3 {\nstruct Spec {\n"
-    << "  static constexpr int kHistMode = " << p3::hist_add_mode() << ";\n"
     << "  static constexpr int kRegular = " << n_regular << ";\n"
     << "  static P3_INLINE bool has_final(const ChaosArgs&) { return "
     << (final_slot >= 0 ? "true" : "false") << "; }\n"
@@ -135,31 +134,19 @@ std::string generate(const float* xforms, int n_regular, int final_slot) {
   s << "  }\n};\n}  // namespace p3\n\n"
     << "constexpr int kBlk = " << p3::chaos_block() << ";\n"
     << "extern \"C\" __global__ void __launch_bounds__(kBlk)\n"
-    << "p3_chaos_sat(p3::ChaosArgs A) {\n"
+    << "p3_chaos(p3::ChaosArgs A) {\n"
     << "  __shared__ p3::BlockLds<kBlk> L;\n"
-    << "  p3::chaos_binned<true, kBlk, p3::Spec>(A, L, blockIdx.x);\n"
-    << "}\n"
-    << "extern \"C\" __global__ void __launch_bounds__(kBlk)\n"
-    << "p3_chaos_nosat(p3::ChaosArgs A) {\n"
-    << "  __shared__ p3::BlockLds<kBlk> L;\n"
-    << "  p3::chaos_binned<false, kBlk, p3::Spec>(A, L, blockIdx.x);\n"
+    << "  p3::chaos_binned<kBlk, p3::Spec>(A, L, blockIdx.x);\n"
     << "}\n"
     // Batched launch: sub-dispatch k (same genome shape, its own records,
     // pick table, seed, camera and walker count) owns blocks
     // [start[k], start[k+1]) of one grid.
     << "extern \"C\" __global__ void __launch_bounds__(kBlk)\n"
-    << "p3_chaos_multi_sat(p3::MultiArgs M) {\n"
+    << "p3_chaos_multi(p3::MultiArgs M) {\n"
     << "  __shared__ p3::BlockLds<kBlk> L;\n"
     << "  const uint32_t k = p3::multi_find(M, blockIdx.x);\n"
     << "  const p3::ChaosArgs A = ((const P3_CONST p3::ChaosArgs*)M.args)[k];\n"
-    << "  p3::chaos_binned<true, kBlk, p3::Spec>(A, L, blockIdx.x - M.start[k]);\n"
-    << "}\n"
-    << "extern \"C\" __global__ void __launch_bounds__(kBlk)\n"
-    << "p3_chaos_multi_nosat(p3::MultiArgs M) {\n"
-    << "  __shared__ p3::BlockLds<kBlk> L;\n"
-    << "  const uint32_t k = p3::multi_find(M, blockIdx.x);\n"
-    << "  const p3::ChaosArgs A = ((const P3_CONST p3::ChaosArgs*)M.args)[k];\n"
-    << "  p3::chaos_binned<false, kBlk, p3::Spec>(A, L, blockIdx.x - M.start[k]);\n"
+    << "  p3::chaos_binned<kBlk, p3::Spec>(A, L, blockIdx.x - M.start[k]);\n"
     << "}\n";
   return s.str();
 }
@@ -259,20 +246,16 @@ int chaos_block() {
   return b;
 }
 
-// Histogram adds (the device's atomic rate is per request, whatever its
-// width): 2 (default, P3_HIST_ADD=packed) one u64 atomic per splat into a
-// packed 16-bit accumulator with carry counts, merged into the histogram
-// before it is read; 1 (P3_HIST_ADD=pairs) two u64 atomics of word pairs,
-// saturating through event counts; 0 (P3_HIST_ADD=quads) four u32 atomics
-// (returning, pinned on wrap, in the saturating kernel).
-int hist_add_mode() {
-  static const int mode = [] {
-    const char* e = std::getenv("P3_HIST_ADD");
-    if (e && std::strcmp(e, "quads") == 0) return 0;
-    if (e && std::strcmp(e, "pairs") == 0) return 1;
-    return 2;
-  }();
-  return mode;
+// When P3_SHAPE_LOG names a file, the key of every shape this process
+// compiles or loads is appended to it (one line each), so a run's shapes can
+// be compiled ahead of the next run (pyr3_amd/kcache.py). Called under g_mu.
+void log_shape(const std::string& key) {
+  const char* path = std::getenv("P3_SHAPE_LOG");
+  if (!path || !*path) return;
+  FILE* f = std::fopen(path, "a");
+  if (!f) return;
+  std::fprintf(f, "%s\n", key.c_str());
+  std::fclose(f);
 }
 
 std::string chaos_shape_key(const float* xforms, int n_regular, int final_slot) {
@@ -307,6 +290,7 @@ bool chaos_code(const float* host_xforms, int n_regular, int final_slot,
   std::vector<char> code;
   if (!compile(chaos_source(host_xforms, n_regular, final_slot), code, err)) return false;
   std::lock_guard<std::mutex> lock(g_mu);
+  log_shape(key);
   *out = &g_code.emplace(key, std::move(code)).first->second;  // first insert wins
   return true;
 }
@@ -322,7 +306,7 @@ bool chaos_code_ready(const float* host_xforms, int n_regular, int final_slot) {
                    code))
     return false;
   std::lock_guard<std::mutex> lock(g_mu);
-  g_code.emplace(key, std::move(code));
+  if (g_code.emplace(key, std::move(code)).second) log_shape(key);
   return true;
 }
 
@@ -339,18 +323,15 @@ hipError_t module_for(int dev, const float* host_xforms, int n_regular, int fina
   }
   hipError_t e;
   if ((e = hipModuleLoadData(&m.mod, code.data())) != hipSuccess) return e;
-  if ((e = hipModuleGetFunction(&m.sat, m.mod, "p3_chaos_sat")) != hipSuccess) return e;
-  if ((e = hipModuleGetFunction(&m.nosat, m.mod, "p3_chaos_nosat")) != hipSuccess) return e;
-  if ((e = hipModuleGetFunction(&m.msat, m.mod, "p3_chaos_multi_sat")) != hipSuccess) return e;
-  if ((e = hipModuleGetFunction(&m.mnosat, m.mod, "p3_chaos_multi_nosat")) != hipSuccess) return e;
+  if ((e = hipModuleGetFunction(&m.single, m.mod, "p3_chaos")) != hipSuccess) return e;
+  if ((e = hipModuleGetFunction(&m.multi, m.mod, "p3_chaos_multi")) != hipSuccess) return e;
   g_mods.emplace(key, m);
   return hipSuccess;
 }
 
 // Compiles (or fetches) the kernel for this shape and launches it on `stream`.
 hipError_t launch_chaos_rtc(const ChaosArgs& a, const float* host_xforms, int n_regular,
-                            int final_slot, bool saturating, hipStream_t stream,
-                            std::string& err) {
+                            int final_slot, hipStream_t stream, std::string& err) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
@@ -363,15 +344,14 @@ hipError_t launch_chaos_rtc(const ChaosArgs& a, const float* host_xforms, int n_
   if (grid == 0) return hipSuccess;
   ChaosArgs args = a;
   void* params[] = {&args};
-  return hipModuleLaunchKernel(saturating ? m.sat : m.nosat, grid, 1, 1, blk, 1, 1,
+  return hipModuleLaunchKernel(m.single, grid, 1, 1, blk, 1, 1,
                                0, stream, params, nullptr);
 }
 
 // Batched launch of n sub-dispatches sharing one genome shape: `dev_args`
 // holds their ChaosArgs in device memory, start[] their first blocks.
 hipError_t launch_chaos_rtc_multi(const MultiArgs& margs, const float* host_xforms, int n_regular,
-                                  int final_slot, bool saturating, hipStream_t stream,
-                                  std::string& err) {
+                                  int final_slot, hipStream_t stream, std::string& err) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
@@ -383,7 +363,7 @@ hipError_t launch_chaos_rtc_multi(const MultiArgs& margs, const float* host_xfor
   if (grid == 0) return hipSuccess;
   MultiArgs args = margs;
   void* params[] = {&args};
-  return hipModuleLaunchKernel(saturating ? m.msat : m.mnosat, grid, 1, 1, chaos_block(), 1, 1, 0,
+  return hipModuleLaunchKernel(m.multi, grid, 1, 1, chaos_block(), 1, 1, 0,
                                stream, params, nullptr);
 }
 

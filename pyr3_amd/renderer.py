@@ -273,10 +273,10 @@ class Renderer:
         check(self._lib.p3_set_timing(self._h, 1 if on else 0), self._h)
 
     def read_timings(self):
-        """Last kernel durations (ms): seed, chaos, density, tonemap, reset, denoise."""
+        """Last kernel durations (ms): merge, chaos, density, tonemap, reset, denoise."""
         out = (C.c_float * 6)()
         check(self._lib.p3_read_timings(self._h, out), self._h)
-        keys = ("seed", "chaos", "density", "tonemap", "reset", "denoise")
+        keys = ("merge", "chaos", "density", "tonemap", "reset", "denoise")
         return dict(zip(keys, list(out)))
 
     def sync(self) -> None:

@@ -258,12 +258,12 @@ def test_saturation_on_device(orc, R):
 
 @pytest.mark.parametrize("case", ["near_max", "low_words_wrap", "mixed"])
 def test_saturating_adds_bit_exact_near_the_limit(orc, R, case):
-    """The saturating splat (64-bit pair adds with carry / wrap counts and the
-    fix-up before reads) against the oracle's per-word saturating sums, on
-    histograms seeded next to 2^32 - 1 where the attractor lands: each word
-    must end at min(seed + sum of adds, 2^32 - 1) exactly, including high
-    words that receive a carry just below the limit. Two dispatches in a row
-    (the second adds to fixed-up words)."""
+    """The splat (one packed 64-bit add per sample with carry counts, merged
+    into the u32 histogram before every read) against the oracle's per-word
+    saturating sums, on histograms seeded next to 2^32 - 1 where the
+    attractor lands: each word must end at min(seed + sum of adds,
+    2^32 - 1) exactly. Two dispatches in a row (the second adds to merged
+    words)."""
     MAX = 0xFFFFFFFF
     W, H = 24, 20
     g = genome(sierpinski(), scale=18.0, cx=0.5, cy=0.5)
@@ -291,6 +291,34 @@ def test_saturating_adds_bit_exact_near_the_limit(orc, R, case):
         out = r.read_histogram()
         assert np.array_equal(out, ho), (case, seed)
     assert (ho == MAX).sum() > 0 and (ho < MAX).sum() > 0
+
+
+def test_wide_splat_words_saturate(orc, R):
+    """A splat with a word of 2^16 or more (here a palette of 300, so the
+    colour words are 76,500) bypasses the packed accumulator: four u32 adds
+    straight into the histogram, pinned at 2^32 - 1 on wrap, while the count
+    word still goes through the packed add. Against the oracle's saturating
+    sums on a histogram seeded next to the limit, two dispatches in a row."""
+    from pyr3_amd.genome import ColorStop, Palette
+    MAX = 0xFFFFFFFF
+    W, H = 24, 20
+    g = genome(sierpinski(), scale=18.0, cx=0.5, cy=0.5)
+    g.palette = Palette("wide", [ColorStop(0, 300.0, 300.0, 300.0),
+                                 ColorStop(1, 300.0, 200.0, 300.0)])
+    rng = np.random.default_rng(5)
+    h0 = (MAX - rng.integers(0, 4_000_000, (H, W, 4), dtype=np.int64)).astype(np.uint32)
+    r = R(W, H, 1)
+    r.reset(g)
+    r.write_histogram(h0)
+    ho = h0.copy()
+    for seed, walkers, iters in ((21, 512, 80), (22, 256, 60)):
+        r.iterate(g, seed, walkers, iters, fuse=10)
+        xf, ds, u, pal = blobs(g, W, H, iters=iters, fuse=10, seed=seed, walkers=walkers)
+        assert pal[:, :3].min() >= 200.0
+        orc.chaos(xf, ds, u, pal, hist=ho)
+        assert np.array_equal(r.read_histogram(), ho), seed
+    r.destroy()
+    assert (ho[..., :3] == MAX).sum() > 0 and (ho[..., 3] < MAX).sum() > 0
 
 
 def test_retry_path_on_device(orc, R):

@@ -18,8 +18,18 @@ from pyr3_amd import pack
 
 NAPI = ROOT / "pyr3_amd" / "napi"
 NODE = shutil.which("node")
-pytestmark = pytest.mark.skipif(NODE is None or not (NAPI / "pyr3hip.node").exists(),
-                                reason="node or the built addon is absent")
+HEADERS = Path("/usr/include/node/node_api.h")
+pytestmark = pytest.mark.skipif(NODE is None or shutil.which("gcc") is None or
+                                not HEADERS.exists(),
+                                reason="node, gcc or the Node-API headers are absent")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def addon():
+    """The addon built from pyr3_napi.c against this tree's C ABI (a few
+    hundred ms of gcc; the library itself is not rebuilt)."""
+    subprocess.run(["sh", str(NAPI / "build.sh")], check=True, timeout=120)
+    return NAPI / "pyr3hip.node"
 
 
 def _node(*args, **kw):

@@ -59,3 +59,50 @@ def test_benchmark_genome_compiles():
 def test_unsupported_entry_is_rejected():
     rc, _, _ = _prepare(genome([xform(1.0, vars=((326, 1.0),))]))
     assert rc != 0
+
+
+def _source(xf, n_rows, final_slot):
+    from pyr3_amd import _lib
+    lib = _lib.load()
+    xf = np.ascontiguousarray(xf, dtype=np.float32)
+    fp = xf.ctypes.data_as(C.POINTER(C.c_float))
+    n = C.c_size_t()
+    assert lib.p3_chaos_source(fp, xf.shape[0], n_rows, final_slot, None, 0, C.byref(n)) == 0
+    buf = C.create_string_buffer(n.value + 1)
+    lib.p3_chaos_source(fp, xf.shape[0], n_rows, final_slot, buf, n.value + 1, None)
+    return buf.value
+
+
+@pytest.mark.parametrize("k", range(3))
+def test_shape_key_rebuilds_the_same_kernel(k):
+    """A record blob rebuilt from a genome's shape key (counts, post flags,
+    slot indices; zero weights) generates the genome's kernel source byte for
+    byte, so compiling the key ahead of time fills the cache entry the
+    genome's dispatch looks up. Genomes with a final xform, post-affines,
+    several slots and pre_blur."""
+    from pyr3_amd import kcache
+    from pyr3_amd.renderer import pack_genome
+    g = [synth.bench_genome(0), synth.config2_genome(1),
+         genome([xform(1.0, vars=((97, 0.2), (4, 0.8)), post=(1, 0, 0, 0, 1, 0)),
+                 xform(0.5, vars=((13, 1.0),))], final=xform(0.0, vars=((2, 1.0),)))][k]
+    pg = pack_genome(g)
+    fs = pack.final_xform_slot(pg.expanded)
+    key = kcache.key_of(pg.xforms, pg.distrib.shape[0], fs)
+    xf, n_rows, fs2 = kcache.blobs_from_key(key)
+    assert (n_rows, fs2) == (pg.distrib.shape[0], fs)
+    assert kcache.key_of(xf, n_rows, fs2) == key
+    assert _source(xf, n_rows, fs2) == _source(pg.xforms, n_rows, fs)
+
+
+def test_manifest_keys_parse():
+    """Every line of the shipped manifest names a valid shape of implemented
+    entries."""
+    from pyr3_amd import kcache
+    keys = kcache.read_manifest([kcache.MANIFEST])
+    assert len(keys) >= 10
+    for key in keys:
+        xf, n_rows, fs = kcache.blobs_from_key(key)
+        assert kcache.key_of(xf, n_rows, fs) == key
+        for rec in xf:
+            for s in range(int(min(rec[7], 8))):
+                assert int(rec[20 + 4 * s]) in SUPPORTED
