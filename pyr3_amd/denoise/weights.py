@@ -1,9 +1,13 @@
 """Flat float32 parameter vector of the denoiser in the C ABI's order."""
 from __future__ import annotations
 
+from pathlib import Path
+
 import numpy as np
 
 from .model import LAYERS
+
+DEFAULT = Path(__file__).resolve().parent / "weights.npz"
 
 
 def flat_params(arrays) -> np.ndarray:
@@ -26,3 +30,17 @@ def from_module(net) -> np.ndarray:
     arrays["out.weight"] = net.out.weight.detach().float().cpu().numpy()
     arrays["out.bias"] = net.out.bias.detach().float().cpu().numpy()
     return flat_params(arrays)
+
+
+def load_module(cls, path=DEFAULT):
+    """An eval-mode network of class `cls` holding the weights of `path`."""
+    import torch
+    arrays = np.load(path, allow_pickle=False)
+    net = cls()
+    with torch.no_grad():
+        for n, m in net.convs.items():
+            m.weight.copy_(torch.from_numpy(np.asarray(arrays[f"{n}.weight"], np.float32)))
+            m.bias.copy_(torch.from_numpy(np.asarray(arrays[f"{n}.bias"], np.float32)))
+        net.out.weight.copy_(torch.from_numpy(np.asarray(arrays["out.weight"], np.float32)))
+        net.out.bias.copy_(torch.from_numpy(np.asarray(arrays["out.bias"], np.float32)))
+    return net.eval()
