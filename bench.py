@@ -1,13 +1,20 @@
 #!/usr/bin/env python3
 """Benchmark of the render hot path on one MI355X.
 
-Workload (default "plain-1080p"): a seeded synthetic flame at 1920x1080,
+Headline workload ("plain-1080p"): a seeded synthetic flame at 1920x1080,
 oversample 1, DE (9, 0, 0.4), gamma 4, vibrancy 1. A plain render at quality
 4000 is the reference image; the plain path's quality is then the smallest q
 (ladder + bisection) whose render reaches >= 38 dB PSNR (8-bit RGB) against
-it. One step = one frame at that q: reset -> chaos -> DE -> tonemap, with
-every input resident in HBM (the genome blobs are a few KB of uploads).
-value = ms/frame (lower is better); Giters/s is reported beside it.
+it. One step = one frame at that q: reset -> chaos -> merge -> DE -> tonemap,
+with every input resident in HBM (the genome blobs are a few KB of uploads).
+value = ms/frame (lower is better); Giters/s is reported beside it, and the
+learned path (low quality + BF16-MFMA denoiser) beside the plain one.
+
+The other benchmark configurations are timed in the same run and reported
+under "configs" (C0 256x256 q50, C1 1024x768 os2 q200, C2 3840x2160 os2
+q200 from the whole catalog), each with its own chaos / DE / tonemap
+rooflines; the CPU baseline is the oracle on this host's cores for C0 in
+full and for a bounded sample of the headline frame.
 
 Prints ONE JSON line. --gpus above 1 exits: this build targets one GPU.
 """
@@ -30,6 +37,26 @@ from pyr3_amd import pack, synth  # noqa: E402
 from pyr3_amd.renderer import Renderer, pack_genome  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+SCATTER_BYTES = 16.0   # algorithmic scatter per post-fuse sample: four 4-byte adds
+CHAOS_KERNEL = "p3_chaos"
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads() -> int:
+    """Threads for the CPU baseline: the cores of this host, capped at the
+    CPU share the process is given (OMP_NUM_THREADS, 16 when unset), so the
+    baseline does not oversubscribe a host it shares with other jobs."""
+    share = int(os.environ.get("OMP_NUM_THREADS") or 16)
+    return max(1, min(os.cpu_count() or 1, share))
 
 
 def profiled_traffic(kernel: str, workload: str, quality, walkers: int):
@@ -149,11 +176,18 @@ def quality_to_target(frame: Frame, ref_img, target_db: float,
     return hi, scores
 
 
-def cpu_baseline(g, width, height, quality, fuse, threads, budget_s):
-    """Oracle (reference-faithful C++ restatement) timed on host cores."""
+def _oracle():
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
     oracle.load()
+    return oracle
+
+
+def cpu_baseline(g, width, height, quality, fuse, threads, budget_s):
+    """The oracle (reference-faithful C++ restatement) timed on host cores on
+    a bounded sample of the headline frame: chaos on a sample sized to the
+    budget and extrapolated linearly in samples, DE + tonemap at full size."""
+    oracle = _oracle()
     e = pack.expand_genome_for_gpu(g)
     xf, ds, pal = pack.pack_xforms(e), pack.pack_xform_distrib(e), pack.pack_palette(e)
     # calibrate the chaos rate on a small probe, then size the sample to the budget
@@ -184,11 +218,131 @@ def cpu_baseline(g, width, height, quality, fuse, threads, budget_s):
     ms = (t_chaos_full + t_img) * 1e3
     return {
         "value": round(ms, 1), "unit": "ms/frame", "cores": threads, "kind": "port",
+        "cpu": cpu_model(),
         "sample": (f"oracle chaos {walkers}x({iters}+{fuse}) turns = {samples_cpu} samples at "
                    f"{width}x{height} in {t_chaos:.2f}s ({rate / 1e6:.1f} Miters/s), extrapolated "
                    f"linearly to q={quality} ({samples_full:.3g} samples); DE+tonemap measured at "
                    f"full size in {t_img:.2f}s"),
     }
+
+
+def cpu_baseline_config0(threads):
+    """C0 in full on the oracle: the reference's own schedule (computeDispatch:
+    800 walkers x 4,096 iterations + 20 fuse), DE and the tonemap, timed."""
+    oracle = _oracle()
+    g = synth.config0_genome(0)
+    W, H = 256, 256
+    walkers, iters, samples = pack.compute_dispatch(g.quality, W, H)
+    e = pack.expand_genome_for_gpu(g)
+    xf, ds, pal = pack.pack_xforms(e), pack.pack_xform_distrib(e), pack.pack_palette(e)
+    u = pack.chaos_uniforms(e, W, H, 1, iters, 20, REF_SEED, walkers)
+    t0 = time.perf_counter()
+    hist = oracle.chaos_mt(xf, ds, u, pal, threads)
+    t_chaos = time.perf_counter() - t0
+    k1, k2 = pack.calibration(g.scale, samples, g.tonemap.brightness, 1)
+    t0 = time.perf_counter()
+    filt = oracle.density(hist, pack.de_uniforms(W, H, 9, 0, 0.4, k1, k2, 1), threads)
+    taps = pack.spatial_kernel(0.5, 1)
+    oracle.tonemap(pack.viz_uniforms(W, H, k1, k2, g.tonemap, 1, len(taps)), taps,
+                   filtered=filt, threads=threads)
+    t_img = time.perf_counter() - t0
+    return {"value": round((t_chaos + t_img) * 1e3, 2), "unit": "ms/frame", "cores": threads,
+            "kind": "port", "cpu": cpu_model(),
+            "sample": f"config 0 in full: {walkers}x({iters}+20) turns ({samples} samples) in "
+                      f"{t_chaos * 1e3:.1f} ms, DE+tonemap {t_img * 1e3:.1f} ms"}
+
+
+def time_frames(frames, q, steps, seed0):
+    """K frames of `frames[0]`'s genome at quality q: issued back to back over
+    the given renderers (one synchronize of every stream before and after),
+    then again serialised on frames[0] with HIP events around each kernel.
+    Returns (ms/frame, serial ms/frame, kernel ms means, walkers, iters, samples)."""
+    for f in frames:
+        f.render(q, seed0 - 1)
+    for f in frames:
+        f.r.sync()
+    t0 = time.perf_counter()
+    walkers = iters = samples = 0
+    for i in range(steps):
+        walkers, iters, samples = frames[i % len(frames)].render(q, seed0 + i)
+    for f in frames:
+        f.r.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    r = frames[0].r
+    r.set_timing(True)
+    acc = dict.fromkeys(("merge", "chaos", "density", "tonemap", "reset", "denoise"), 0.0)
+    t0 = time.perf_counter()
+    for i in range(steps):
+        frames[0].render(q, seed0 + i)
+        r.read_image()   # the readback runs the merge (no-op here: present ran it)
+        t = r.read_timings()
+        for k in acc:
+            acc[k] += t[k]
+    serial = (time.perf_counter() - t0) * 1e3 / steps
+    r.set_timing(False)
+    return ms, serial, {k: v / steps for k, v in acc.items()}, walkers, iters, samples
+
+
+def rooflines(kms, samples, bins, pixels, fuse, walkers, iters):
+    """Chaos: 16 B of scatter per post-fuse sample; DE: 32 B per super-res
+    bin; tonemap: 16 B per bin + 4 B per output pixel (all vs 8 TB/s)."""
+    def gbs(b, ms):
+        return round(b / (ms * 1e-3) / 1e9, 2) if ms > 0 else None
+
+    def frac(b, ms):
+        return round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5) if ms > 0 else None
+    out = {
+        "giters_per_s": round(walkers * (iters + fuse) / (kms["chaos"] * 1e-3) / 1e9, 3),
+        "gsamples_per_s": round(samples / (kms["chaos"] * 1e-3) / 1e9, 3),
+        "chaos": {"achieved": gbs(SCATTER_BYTES * samples, kms["chaos"]), "unit": "GB/s",
+                  "frac": frac(SCATTER_BYTES * samples, kms["chaos"])},
+        "density": {"achieved": gbs(32.0 * bins, kms["density"]), "unit": "GB/s",
+                    "frac": frac(32.0 * bins, kms["density"])},
+        "tonemap": {"achieved": gbs(16.0 * bins + 4.0 * pixels, kms["tonemap"]), "unit": "GB/s",
+                    "frac": frac(16.0 * bins + 4.0 * pixels, kms["tonemap"])},
+    }
+    return out
+
+
+def measure_config(name, g, W, H, os_, quality, steps, streams, fuse=pack.FUSE,
+                   walkers=131072):
+    """One benchmark configuration on the GPU schedule: ms/frame and rooflines."""
+    rs = [Renderer(W, H, os_) for _ in range(streams)]
+    frames = [Frame(r, g, fuse, walkers=walkers) for r in rs]
+    ms, serial, kms, w, it, s = time_frames(frames, quality, steps, 900)
+    for r in rs:
+        r.destroy()
+    out = {"workload": name, "width": W, "height": H, "oversample": os_, "quality": quality,
+           "fuse": fuse, "walkers": w, "iters": it, "samples": s, "steps": steps,
+           "ms_per_frame": round(ms, 3), "ms_per_frame_serial": round(serial, 3),
+           "kernels_ms": {k: round(v, 4) for k, v in kms.items() if k != "denoise"}}
+    out.update(rooflines(kms, s, W * H * os_ * os_, W * H, fuse, w, it))
+    log(f"{name}: {ms:.3f} ms/frame ({serial:.3f} serial), chaos {kms['chaos']:.3f} ms")
+    return out
+
+
+def run_configs(steps, streams):
+    """C0, C1 and C2 at their full sizes and qualities (GPU schedule: 131,072
+    walkers, fewer when a frame has fewer samples), plus C0 on the
+    reference's own walker schedule (800 walkers x 4,096 iterations)."""
+    out = {}
+    g0 = synth.config0_genome(0)
+    out["C0"] = measure_config("preview-small-256x256-os1-q50", g0, 256, 256, 1, 50, steps,
+                               streams, fuse=20)
+    rw, ri, _ = pack.compute_dispatch(50, 256, 256)
+    r = Renderer(256, 256, 1)
+    f = Frame(r, g0, 20, walkers=rw)
+    f.render(50, 1)
+    r.set_timing(True)
+    f.render(50, 2)
+    r.sync()
+    out["C0"]["reference_schedule_chaos_ms"] = round(r.read_timings()["chaos"], 3)
+    r.destroy()
+    out["C1"] = measure_config("preview-1024x768-os2-q200", synth.config1_genome(0), 1024, 768,
+                               2, 200, steps, streams)
+    out["C2"] = measure_config("4k-3840x2160-os2-q200", synth.config2_genome(0), 3840, 2160, 2,
+                               200, max(2, steps // 4), 1)
+    return out
 
 
 def batch_main(a):
@@ -323,9 +477,12 @@ def main():
                     help="skip the walker autotune and use this count")
     ap.add_argument("--quality", type=float, default=None,
                     help="skip the PSNR search and time this quality")
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="CPU-baseline threads (default: host cores, capped at the CPU share)")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the C0 / C1 / C2 configuration measurements")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the low-quality + denoiser (learned) path")
     ap.add_argument("--workload", default="plain-1080p", choices=("plain-1080p", "batch"))
@@ -368,54 +525,24 @@ def main():
     # timed: K frames back to back, alternating over `a.streams` renderers
     # (each its own HIP stream and buffers): the host queues frame i+1 while
     # the GPU renders frame i, and one frame's DE / tonemap overlap the next
-    # frame's chaos kernel; synchronizes of every stream before and after
+    # frame's chaos kernel; synchronizes of every stream before and after.
+    # Then the same frames serialised with HIP events around each kernel.
     extra = [Renderer(W, H, 1) for _ in range(a.streams - 1)]
     frames = [frame] + [Frame(rx, g, a.fuse, walkers=frame.walkers) for rx in extra]
-    for f in frames[1:]:
-        f.render(q, 76)
-    for f in frames:
-        f.r.sync()
-    t0 = time.perf_counter()
-    walkers = iters = samples = 0
-    for i in range(a.steps):
-        walkers, iters, samples = frames[i % len(frames)].render(q, 500 + i)
-    for f in frames:
-        f.r.sync()
-    wall = time.perf_counter() - t0
-    ms = wall * 1e3 / a.steps
-    # the same frames again with HIP events around each kernel and one
-    # synchronize per frame: the kernel breakdown and the serialised frame time
-    r.set_timing(True)
-    chaos_ms = de_ms = tm_ms = reset_ms = 0.0
-    t0 = time.perf_counter()
-    for i in range(a.steps):
-        frame.render(q, 500 + i)
-        r.sync()
-        t = r.read_timings()
-        chaos_ms += t["chaos"]; de_ms += t["density"]; tm_ms += t["tonemap"]; reset_ms += t["reset"]
-    serial_ms = (time.perf_counter() - t0) * 1e3 / a.steps
-    r.set_timing(False)
+    ms, serial_ms, kms, walkers, iters, samples = time_frames(frames, q, a.steps, 500)
     for rx in extra:
         rx.destroy()
     # the device's rate of the splat's atomics (uniform scatter over the same
     # histogram size); the next render resets the histogram it overwrites
     probe_rate, per_splat = r.probe_atomics()
-    chaos_ms /= a.steps; de_ms /= a.steps; tm_ms /= a.steps; reset_ms /= a.steps
-    giters = walkers * (iters + a.fuse) / (chaos_ms * 1e-3) / 1e9
-    gsamples = samples / (chaos_ms * 1e-3) / 1e9
+    roof = rooflines(kms, samples, W * H, W * H, a.fuse, walkers, iters)
     learned = None
     if a.quality is None and not a.no_learned and WEIGHTS.exists():
         learned = learned_path(r, g, a, ref_img, frame.walkers)
-    bins = W * H
-    # algorithmic scatter per post-fuse sample: the 8-byte packed accumulator
-    # word (one request per splat), else the 16-byte histogram bin
-    scatter_b = 8.0 if per_splat == 1 else 16.0
-    scatter_bytes = scatter_b * samples
-    de_bytes = 32.0 * bins
-    tm_bytes = 16.0 * bins + 4.0 * W * H
-    achieved = scatter_bytes / (chaos_ms * 1e-3) / 1e9
-    traffic = profiled_traffic("p3_chaos_sat" if 255.0 * samples >= 2 ** 32 else "p3_chaos_nosat",
-                               f"plain-{W}x{H}-os1", q, walkers)
+    r.destroy()
+    configs = None if a.no_configs else run_configs(max(2, a.steps // 2), a.streams)
+    traffic = profiled_traffic(CHAOS_KERNEL, f"plain-{W}x{H}-os1", q, walkers)
+    chaos_ms = kms["chaos"]
     out = {
         "metric": "ms/frame to >=38 dB vs q4000 @1080p (plain path); Giters/s",
         "value": round(ms, 3),
@@ -435,43 +562,45 @@ def main():
                    "walkers": walkers, "iters": iters, "fuse": a.fuse, "streams": a.streams,
                    "walker_autotune_ms": {str(k): v for k, v in tune.items()},
                    "psnr_ladder": {str(kq): round(v, 2) for kq, v in sorted(scores.items())}},
-        "giters_per_s": round(giters, 3),
-        "gsamples_per_s": round(gsamples, 3),
-        "kernels_ms": {"chaos": round(chaos_ms, 4), "density": round(de_ms, 4),
-                       "tonemap": round(tm_ms, 4), "reset": round(reset_ms, 4)},
-        "atomic_roofline": {"kernel": "chaos_kernel", "requests_per_splat": per_splat,
+        "giters_per_s": roof["giters_per_s"],
+        "gsamples_per_s": roof["gsamples_per_s"],
+        "kernels_ms": {kk: round(v, 4) for kk, v in kms.items() if kk != "denoise"},
+        "roofline": {"bound": "hbm", "kernel": CHAOS_KERNEL,
+                     "achieved": roof["chaos"]["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": roof["chaos"]["frac"],
+                     "traffic": round(traffic["bytes"]) if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None,
+                     "note": "achieved = 16 B of scatter per post-fuse sample (four 4-byte "
+                             "channel adds) / the chaos kernel's HIP-event time; traffic = "
+                             "rocprofv3 FETCH_SIZE + WRITE_SIZE of the same launch. The kernel "
+                             "is bound by its turn's VALU work and its atomic requests "
+                             "(atomic_roofline), not by HBM bandwidth"},
+        "atomic_roofline": {"kernel": CHAOS_KERNEL, "requests_per_splat": per_splat,
                             "achieved_g_requests": round(per_splat * samples / (chaos_ms * 1e-3)
                                                          / 1e9, 2),
                             "probe_peak_g_requests": round(probe_rate / 1e9, 2),
                             "frac": round(per_splat * samples / (chaos_ms * 1e-3) / probe_rate, 4),
-                            "note": "atomic requests per second: requests_per_splat per "
+                            "note": "atomic requests per second: one packed 64-bit add per "
                                     "post-fuse sample (upper bound: off-frame samples do not "
-                                    "splat) against p3_probe_atomics, a uniform scatter in "
-                                    "the same request form (one 64-bit add per 8-byte cell "
-                                    "for packed adds) over the same number of bins"},
-        "roofline": {"bound": "hbm", "kernel": "chaos_kernel", "achieved": round(achieved, 2),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": round(traffic["bytes"]) if traffic else None,
-                     "traffic_source": traffic["source"] if traffic else None,
-                     "note": f"achieved = {scatter_b:.0f} B of atomic scatter per post-fuse sample; "
-                             "traffic = rocprofv3 FETCH_SIZE + WRITE_SIZE of the same launch "
-                             "(the atomics leave the L2); the kernel is bound by the rate of "
-                             "atomic requests (atomic_roofline), not by HBM bandwidth"},
-        "de_roofline": {"achieved": round(de_bytes / (de_ms * 1e-3) / 1e9, 1) if de_ms else None,
-                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(de_bytes / (de_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
-                        if de_ms else None},
-        "tonemap_roofline": {"achieved": round(tm_bytes / (tm_ms * 1e-3) / 1e9, 1) if tm_ms else None,
-                             "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                             "frac": round(tm_bytes / (tm_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
-                             if tm_ms else None},
+                                    "splat) against p3_probe_atomics, a uniform scatter of the "
+                                    "same request form over the same number of bins"},
+        "de_roofline": {"achieved": roof["density"]["achieved"], "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": roof["density"]["frac"]},
+        "tonemap_roofline": {"achieved": roof["tonemap"]["achieved"], "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": roof["tonemap"]["frac"]},
     }
     if learned is not None:
         out["learned"] = learned
+    if configs is not None:
+        out["configs"] = configs
     if not a.no_cpu_baseline:
-        log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(g, W, H, q, a.fuse, a.cpu_threads, a.cpu_budget)
+        threads = a.cpu_threads or host_threads()
+        log(f"cpu baseline on {threads} threads ({cpu_model()}) ...")
+        out["cpu_baseline"] = cpu_baseline(g, W, H, q, a.fuse, threads, a.cpu_budget)
+        if configs is not None:
+            c0 = cpu_baseline_config0(threads)
+            c0["gpu_ms_per_frame"] = configs["C0"]["ms_per_frame"]
+            out["cpu_baseline_configs"] = {"C0": c0}
     print(json.dumps(out), flush=True)
 
 

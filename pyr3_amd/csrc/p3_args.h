@@ -61,7 +61,8 @@ struct MultiArgs {
 
 // LDS image of a chaos block's walkers (structure of arrays, kB walkers):
 // ISAAC words [48][kB] (mem 0..15, two rsl rounds 16..47) and the
-// per-walker scalars: 244 bytes per walker; the pick boundaries (544 B).
+// per-walker scalars: 244 bytes per walker; the pick boundaries (544 B);
+// the splat aggregation's 12 bytes per lane.
 template <int kB>
 struct BlockLds {
   alignas(16) uint16_t pick_b[kPickRows * kPickCols];
@@ -72,6 +73,10 @@ struct BlockLds {
   uint32_t consec[kB], iter[kB], draws[kB], raw[kB];
   uint16_t fn[kB], order[kB];
   uint32_t bucket[2][kMaxXforms + 2];   // per-xform counts, two turns alternating
+  // splat aggregation, 64 entries per wave: hash slot -> lane, and the
+  // packed sum of each group's splats (indexed by the group's leader lane)
+  uint32_t agg_lane[kB];
+  alignas(8) uint64_t agg_sum[kB];
 };
 
 }  // namespace p3

@@ -64,19 +64,16 @@ P3_INLINE uint32_t add_ret(P3_GLOBAL uint32_t* p, uint32_t d) {
 P3_INLINE void pin_max(P3_GLOBAL uint32_t* p) {
   (void)__hip_atomic_exchange(p, 0xffffffffu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// Packed add of one splat: (d0, d1, d2, d3), each below 2^16, as the four
-// 16-bit fields of the bin's u64 accumulator, one 64-bit atomic. The returned
-// old value o and the sum n = o + v give the carry into every bit,
-// o ^ v ^ n; a field's increment (its word plus the carry in) is at most
-// 2^16, so it carries out at most once per add, and counting each carry out
-// (bits 16, 32, 48 and the 64-bit wrap) in the bin's carry[0..3] makes the
-// field sums exact: S_k = field_k + 2^16 carry[k] - carry[k-1]. hist_merge
-// (p3_image.hip) adds them to the u32 histogram, saturating.
-P3_INLINE void add_packed(P3_GLOBAL uint64_t* acc, P3_GLOBAL uint64_t* carry, uint32_t d0,
-                          uint32_t d1, uint32_t d2, uint32_t d3) {
-  const uint64_t v = (uint64_t)d0 | ((uint64_t)d1 << 16) | ((uint64_t)d2 << 32) |
-                     ((uint64_t)d3 << 48);
-  if (!v) return;
+// Packed add of v, whose four 16-bit fields are each an increment below
+// 2^16 (one splat's (d0, d1, d2, d3), or the sum of a group's splats), as ONE
+// 64-bit atomic on the bin's u64 accumulator. The returned old value o and
+// the sum n = o + v give the carry into every bit, o ^ v ^ n; a field's
+// increment (its word plus the carry in) is at most 2^16, so it carries out
+// at most once per add, and counting each carry out (bits 16, 32, 48 and the
+// 64-bit wrap) in the bin's carry[0..3] makes the field sums exact:
+// S_k = field_k + 2^16 carry[k] - carry[k-1]. hist_merge (p3_image.hip) adds
+// them to the u32 histogram, saturating.
+P3_INLINE void add_packed(P3_GLOBAL uint64_t* acc, P3_GLOBAL uint64_t* carry, uint64_t v) {
   const uint64_t o = __hip_atomic_fetch_add(acc, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const uint64_t n = o + v;
   const uint64_t c = o ^ v ^ n;
@@ -87,6 +84,9 @@ P3_INLINE void add_packed(P3_GLOBAL uint64_t* acc, P3_GLOBAL uint64_t* carry, ui
     if (c2) (void)__hip_atomic_fetch_add(carry + 2, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (c3) (void)__hip_atomic_fetch_add(carry + 3, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
+}
+P3_INLINE uint64_t pack_words(uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3) {
+  return (uint64_t)d0 | ((uint64_t)d1 << 16) | ((uint64_t)d2 << 32) | ((uint64_t)d3 << 48);
 }
 // Four saturating channel adds (R, G, B, count) at b: the returned old
 // values are checked after all four are issued; a word that wrapped is
@@ -179,12 +179,24 @@ struct Walker {
   uint32_t consec, iter;
 };
 
+// A turn's splat request for the wave-level aggregation: the bin and its
+// packed words (each at most kAggWord), or bin = kNoSplat.
+constexpr uint32_t kNoSplat = 0xffffffffu;
+constexpr uint32_t kAggWord = 1023u;   // 64 lanes x 1023 < 2^16: group sums stay in their fields
+struct Splat {
+  uint32_t bin;
+  uint64_t v;
+};
+
 // One turn of walker `wid` (global id) after its pick `fn` (from draw `raw`)
 // is known: chain, blend, bad check and retry window, final lens, jitter,
-// splat. Advances `wk.iter` unless the turn is retried.
+// splat. Advances `wk.iter` unless the turn is retried. A splat whose words
+// are all at most kAggWord (every splat of a palette and opacity in [0, 1]:
+// the words are at most 255) is returned in `sp` for splat_wave; a wider one
+// is added here (packed below 2^16, else four saturating u32 adds).
 template <class Spec>
 P3_INLINE void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32_t fn,
-                                uint32_t raw, uint32_t wid) {
+                        uint32_t raw, uint32_t wid, Splat& sp) {
   const uint32_t i = wk.iter;
   const float p0x = wk.x, p0y = wk.y;   // position before this turn's chain
   const xrec_t X = (xrec_t)A.xforms;
@@ -293,11 +305,15 @@ P3_INLINE void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32_t fn,
       P3_GLOBAL uint32_t* b = (P3_GLOBAL uint32_t*)A.hist + bin;
       const uint32_t d0 = f2u_sat(pal.x * wgt), d1 = f2u_sat(pal.y * wgt),
                      d2 = f2u_sat(pal.z * wgt), d3 = f2u_sat(wgt);
-      if ((d0 | d1 | d2 | d3) <= 0xffffu)
+      const uint32_t dmax = d0 | d1 | d2 | d3;
+      if (dmax <= kAggWord) {
+        if (dmax) { sp.bin = (uint32_t)(bin >> 2); sp.v = pack_words(d0, d1, d2, d3); }
+      } else if (dmax <= 0xffffu) {
         add_packed((P3_GLOBAL uint64_t*)A.acc + (bin >> 2), (P3_GLOBAL uint64_t*)A.carry + bin,
-                   d0, d1, d2, d3);
-      else
+                   pack_words(d0, d1, d2, d3));
+      } else {
         hist_add4_sat(b, d0, d1, d2, d3);
+      }
       if (A.idx_sum != nullptr) {   // capture of the colour index (chaos.wgsl:8720-8728)
         const uint32_t d = f2u_sat(fminf(fmaxf(sz, 0.0f), 1.0f) * wgt);
         P3_GLOBAL uint32_t* q = (P3_GLOBAL uint32_t*)A.idx_sum + (size_t)yi * A.width + xi;
@@ -307,6 +323,40 @@ P3_INLINE void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32_t fn,
         }
       }
     }
+  }
+}
+
+// The wave's splats of one turn, aggregated by bin: lanes whose bins agree
+// add their packed words into one sum (in LDS) and one lane issues the
+// global add for the group. Every lane of the wave calls it (the wave is
+// converged). Groups are found through a 64-slot hash of the bin: the last
+// lane to write a slot leads the lanes of the same bin that hashed there;
+// lanes of another bin in a taken slot add on their own. Each splat is
+// added exactly once, and a group's field sums (at most 64 x kAggWord) stay
+// below 2^16, so add_packed stays exact. Hot bins (a compact attractor
+// concentrating its samples on a few bins) then cost one atomic per wave
+// instead of one per lane.
+template <int kB>
+P3_INLINE void splat_wave(const ChaosArgs& A, BlockLds<kB>& L, const Splat& sp, int t) {
+  const int lane = t & 63;
+  P3_LDS uint32_t* tab = (P3_LDS uint32_t*)L.agg_lane + (t & ~63);
+  P3_LDS uint64_t* sum = (P3_LDS uint64_t*)L.agg_sum + (t & ~63);
+  const bool on = sp.bin != kNoSplat;
+  sum[lane] = 0ull;
+  const uint32_t slot = (sp.bin * 2654435761u) >> 26;
+  if (on) tab[slot] = (uint32_t)lane;
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  const uint32_t lead = on ? tab[slot] : (uint32_t)lane;
+  const uint32_t lead_bin = (uint32_t)__shfl((int)sp.bin, (int)lead, 64);
+  const bool grouped = on && lead_bin == sp.bin;
+  if (grouped)
+    (void)__hip_atomic_fetch_add(sum + lead, sp.v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  if (on && (!grouped || lead == (uint32_t)lane)) {
+    const uint64_t v = grouped ? sum[lane] : sp.v;
+    add_packed((P3_GLOBAL uint64_t*)A.acc + sp.bin, (P3_GLOBAL uint64_t*)A.carry + (size_t)4 * sp.bin, v);
   }
 }
 
@@ -432,6 +482,7 @@ P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L, uint32_t block)
     // run the turn of the t-th walker in that order, then its next pick
     wl = L.order[t];
     f = idle;
+    Splat sp{kNoSplat, 0ull};
     if (L.iter[wl] < total) {
       Rng rng;
       rng.m.base = (P3_LDS uint32_t*)(L.isaac + wl);
@@ -440,7 +491,7 @@ P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L, uint32_t block)
       rng.draws = L.draws[wl];
       isaac_top_up(rng.s, rng.m);   // the wave's lanes generate rounds together here
       Walker wk{L.x[wl], L.y[wl], L.z[wl], L.prev[wl], L.consec[wl], L.iter[wl]};
-      run_turn<Spec>(A, wk, rng, L.fn[wl], L.raw[wl], w0 + wl);
+      run_turn<Spec>(A, wk, rng, L.fn[wl], L.raw[wl], w0 + wl, sp);
       if (wk.iter < total) {
         uint32_t raw;
         f = draw_pick<Spec>(A, L, rng, wk.prev, raw);
@@ -452,6 +503,7 @@ P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L, uint32_t block)
       L.prev[wl] = wk.prev; L.consec[wl] = wk.consec; L.iter[wl] = wk.iter;
       L.fn[wl] = (uint16_t)f;
     }
+    splat_wave<kB>(A, L, sp, t);
     pos = atomicAdd(&L.bucket[p ^ 1u][f], 1u);
   }
 }
