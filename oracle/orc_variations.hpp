@@ -14,9 +14,22 @@
 #include <cmath>
 #include <cstdint>
 #include <algorithm>
+#include <type_traits>
 #include <cstring>
 
 namespace orc {
+
+// WGSL max / min (and clamp = min(max(x, lo), hi)): a NaN operand yields the
+// other operand; otherwise max(a, b) = a < b ? b : a and min(a, b) =
+// b < a ? b : a, which also fixes which zero a tie of +0 and -0 returns.
+template <typename T> inline T wmax(T a, T b) {
+  if (std::is_floating_point<T>::value && a != a) return b;
+  return a < b ? b : a;
+}
+template <typename T> inline T wmin(T a, T b) {
+  if (std::is_floating_point<T>::value && a != a) return b;
+  return b < a ? b : a;
+}
 
 template <typename T> struct V2 { T x, y; };
 
@@ -341,7 +354,7 @@ template <typename T, typename R> V2<T> var_cpow(V2<T> p, T w, T pr, T pi, T pw,
   return {m * safe_cos(ang), m * safe_sin(ang)};
 }
 template <typename T> V2<T> var_curve(V2<T> p, T w, T xamp, T yamp, T xlen, T ylen) {
-  T pcx = std::max(xlen * xlen, (T)1e-20), pcy = std::max(ylen * ylen, (T)1e-20);
+  T pcx = wmax(xlen * xlen, (T)1e-20), pcy = wmax(ylen * ylen, (T)1e-20);
   return {w * (p.x + xamp * std::exp((-p.y * p.y) / pcx)),
           w * (p.y + yamp * std::exp((-p.x * p.x) / pcy))};
 }
@@ -484,8 +497,8 @@ template <typename T> V2<T> var_edisc(V2<T> p, T w) {
   T tmp2 = (T)2 * p.x;
   T r1 = std::sqrt(tmp + tmp2), r2 = std::sqrt(tmp - tmp2);
   T xmax = (r1 + r2) * (T)0.5;
-  T a1 = std::log(xmax + std::sqrt(std::max(xmax - (T)1, (T)0)));
-  T a2 = -std::acos(std::min(std::max(p.x / xmax, (T)-1), (T)1));
+  T a1 = std::log(xmax + std::sqrt(wmax(xmax - (T)1, (T)0)));
+  T a2 = -std::acos(wmin(wmax(p.x / xmax, (T)-1), (T)1));
   T wn = w / (T)11.57034632;
   T csv = safe_cos(a1);
   T snv = sel(safe_sin(a1), -safe_sin(a1), p.y > 0);
@@ -497,8 +510,8 @@ template <typename T> V2<T> var_elliptic(V2<T> p, T w) {
   T x2 = (T)2 * p.x;
   T xmax = (T)0.5 * (std::sqrt(tmp + x2) + std::sqrt(tmp - x2));
   T a = p.x / xmax;
-  T b = std::sqrt(std::max((T)1 - a * a, (T)0));
-  T ssx = std::sqrt(std::max(xmax - (T)1, (T)0));
+  T b = std::sqrt(wmax((T)1 - a * a, (T)0));
+  T ssx = std::sqrt(wmax(xmax - (T)1, (T)0));
   T wn = w / (K<T>::PI * (T)0.5);
   T ylog = wn * std::log(xmax + ssx);
   return {wn * std::atan2(a, b), sel(-ylog, ylog, p.y > 0)};

@@ -18,7 +18,7 @@ namespace orc {
 // ---- shared helpers ----------------------------------------------------------
 
 template <typename T> inline T fract(T x) { return x - std::floor(x); }
-template <typename T> inline T clampv(T x, T lo, T hi) { return std::min(std::max(x, lo), hi); }
+template <typename T> inline T clampv(T x, T lo, T hi) { return wmin(wmax(x, lo), hi); }
 template <typename T> inline T mixv(T a, T b, T t) { return a * ((T)1 - t) + b * t; }
 template <typename T> inline T rint_even(T x) { return std::nearbyint(x); }
 template <typename T> inline T signv(T x) { return x > (T)0 ? (T)1 : (x < (T)0 ? (T)-1 : (T)0); }
@@ -42,18 +42,18 @@ template <typename T> inline V2<T> c_mul(V2<T> a, V2<T> b) {
 template <typename T> inline V2<T> c_sqr(V2<T> z) { return {z.x * z.x - z.y * z.y, (T)2 * z.x * z.y}; }
 // |b|^2 floored at the f32 value of 1e-100 (zero: the floor is a no-op in f32).
 template <typename T> inline V2<T> c_div(V2<T> a, V2<T> b) {
-  T m2 = std::max(b.x * b.x + b.y * b.y, (T)(float)1e-100);
+  T m2 = wmax(b.x * b.x + b.y * b.y, (T)(float)1e-100);
   return {(a.x * b.x + a.y * b.y) / m2, (a.y * b.x - a.x * b.y) / m2};
 }
 template <typename T> inline V2<T> c_recip(V2<T> z) {
-  T m2 = std::max(z.x * z.x + z.y * z.y, (T)(float)1e-100);
+  T m2 = wmax(z.x * z.x + z.y * z.y, (T)(float)1e-100);
   return {z.x / m2, -z.y / m2};
 }
 template <typename T> inline V2<T> c_sqrt(V2<T> z) {
   T rad = len2(z);
   T sb = sel((T)1, (T)-1, z.y < (T)0);
-  return {std::sqrt(std::max((T)0.5 * (rad + z.x), (T)0)),
-          sb * std::sqrt(std::max((T)0.5 * (rad - z.x), (T)0))};
+  return {std::sqrt(wmax((T)0.5 * (rad + z.x), (T)0)),
+          sb * std::sqrt(wmax((T)0.5 * (rad - z.x), (T)0))};
 }
 template <typename T> inline V2<T> c_log(V2<T> z) {
   T m2 = (z.x * z.x + z.y * z.y) + (T)1e-20;
@@ -140,7 +140,7 @@ template <typename T> V2<T> var_dc_cylinder_pos(V2<T> p, T w) { return {w * safe
 template <typename T, typename R>
 V2<T> var_cpow2(V2<T> p, T w, T pr, T pa, T divisor, T range, R& rng) {
   T div = sel(divisor, (T)1, divisor == (T)0);
-  T rmax = std::max(range, (T)1);
+  T rmax = wmax(range, (T)1);
   T step = K<T>::TAU / div;
   T c = pr * safe_cos(K<T>::PI * (T)0.5 * pa) / div;
   T d = pr * safe_sin(K<T>::PI * (T)0.5 * pa) / div;
@@ -163,7 +163,7 @@ template <typename T, typename R>
 V2<T> var_cpow3(V2<T> p, T w, T pr, T pd, T divisor, T spread, R& rng) {
   T div = sel(divisor, (T)1, divisor == (T)0);
   T step = K<T>::TAU / div;
-  T dabs = std::max(std::fabs(pd), (T)1e-30);
+  T dabs = wmax(std::fabs(pd), (T)1e-30);
   T slog = sel(std::log(dabs), -std::log(dabs), pd < (T)0);
   T ang = std::atan2(slog * pr, K<T>::TAU);
   T ca = safe_cos(ang), sa = safe_sin(ang);
@@ -176,7 +176,7 @@ V2<T> var_cpow3(V2<T> p, T w, T pr, T pd, T divisor, T spread, R& rng) {
   if (a < (T)0) a = a + K<T>::TAU;
   if (safe_cos(a * (T)0.5) < (u0 * (T)2 - (T)1)) a = a - K<T>::TAU;
   T bsign = sel(-K<T>::TAU, K<T>::TAU, u1 < (T)0.5);
-  a = a + bsign * rint_even(std::log(std::max(u2, (T)1e-30)) * coeff);
+  a = a + bsign * rint_even(std::log(wmax(u2, (T)1e-30)) * coeff);
   T lnr2 = std::log(dot2(p));
   T r = w * std::exp(hc * lnr2 - td * a);
   T th = tc * a + hd * lnr2 + step * std::floor(div * u3);
@@ -198,7 +198,7 @@ template <typename T> V2<T> var_loonie2(V2<T> p, T w, T sides_f, T star, T circl
     T nx = xr * ca - yr * sa;
     yr = xr * sa + yr * ca;
     xr = nx;
-    r2 = std::max(r2, xr * coss + std::fabs(yr) * sins);
+    r2 = wmax(r2, xr * coss + std::fabs(yr) * sins);
   }
   r2 = r2 * cosc + cr * sinc;
   if (sides > 2) r2 = r2 * r2;
@@ -250,7 +250,7 @@ V2<T> var_bwraps(V2<T> p, T w, T cell, T space, T gain, T itwist, T otwist) {
   if (mb > (T)2) mb = (T)1;
   else mb = mb * ((T)1 / ((mb * mb) / (T)4 + (T)1));
   T rr2 = radius * radius;
-  T rfac = radius / std::max(mb, (T)1e-30);
+  T rfac = radius / wmax(mb, (T)1e-30);
   T cx = (std::floor(p.x / cell) + (T)0.5) * cell;
   T cy = (std::floor(p.y / cell) + (T)0.5) * cell;
   T lx = p.x - cx, ly = p.y - cy;
@@ -260,7 +260,7 @@ V2<T> var_bwraps(V2<T> p, T w, T cell, T space, T gain, T itwist, T otwist) {
   T rd = rfac / ((lx * lx + ly * ly) / (T)4 + (T)1);
   lx = lx * rd;
   ly = ly * rd;
-  T rf = (lx * lx + ly * ly) / std::max(rr2, (T)1e-30);
+  T rf = (lx * lx + ly * ly) / wmax(rr2, (T)1e-30);
   T th = itwist * ((T)1 - rf) + otwist * rf;
   T st = safe_sin(th), ct = safe_cos(th);
   return {w * (cx + ct * lx + st * ly), w * (cy - st * lx + ct * ly)};
@@ -322,7 +322,7 @@ V2<T> var_crackle(V2<T> p, T w, T cell, T power, T distort, T scale, R& rng) {
 // chaos.wgsl:2149 — one draw.
 template <typename T, typename R> V2<T> var_juliaq(V2<T> p, T w, T power, T divisor, R& rng) {
   T ps = sel(power, (T)1, std::fabs(power) < (T)1e-30);
-  T ap = std::max((T)1, std::fabs(ps));
+  T ap = wmax((T)1, std::fabs(ps));
   T ip = divisor / ps;
   T hip = (T)0.5 * ip;
   T ip2 = K<T>::TAU / ps;
@@ -416,8 +416,8 @@ V2<T> var_falloff3(V2<T> p, T w, T scatter, T mx, T my, T x0, T y0, T mindist, T
   T rmax = (T)0.04 * scatter;
   T dx = p.x - x0, dy = p.y - y0;
   T radius = std::sqrt(dx * dx + dy * dy);
-  T base = sel(std::max(radius, (T)0), std::max((T)1 - radius, (T)0), inv > (T)0.5);
-  T dist = std::max((base - mindist) * rmax, (T)0);
+  T base = sel(wmax(radius, (T)0), wmax((T)1 - radius, (T)0), inv > (T)0.5);
+  T dist = wmax((base - mindist) * rmax, (T)0);
   T u0 = rng.rand01() - (T)0.5;
   T u1 = rng.rand01() - (T)0.5;
   T u2 = rng.rand01() - (T)0.5;
@@ -484,11 +484,11 @@ template <typename T> V2<T> var_eswirl(V2<T> p, T w, T inp, T outp) {
   T tmp = p.y * p.y + p.x * p.x + (T)1;
   T tmp2 = (T)2 * p.x;
   T a1 = tmp + tmp2, a2 = tmp - tmp2;
-  T s1 = sel((T)0, std::sqrt(std::max(a1, (T)0)), a1 > (T)0);
-  T s2 = sel((T)0, std::sqrt(std::max(a2, (T)0)), a2 > (T)0);
+  T s1 = sel((T)0, std::sqrt(wmax(a1, (T)0)), a1 > (T)0);
+  T s2 = sel((T)0, std::sqrt(wmax(a2, (T)0)), a2 > (T)0);
   T xmax = (s1 + s2) * (T)0.5;
   if (xmax < (T)1) xmax = (T)1;
-  T mu = std::log(xmax + std::sqrt(std::max(xmax * xmax - (T)1, (T)0)));
+  T mu = std::log(xmax + std::sqrt(wmax(xmax * xmax - (T)1, (T)0)));
   T t = p.x / xmax;
   if (t > (T)1) t = (T)1;
   if (t < (T)-1) t = (T)-1;
@@ -608,7 +608,7 @@ template <typename T> V2<T> var_hexes(V2<T> p, T w, T cell, T power, T rotate, T
   T vx = dxo * rc + dyo * rs;
   T vy = -dxo * rs + dyo * rc;
   T L2 = hex_ratio_max(H, vx + H[0].x, vy + H[0].y);
-  T L = std::max(L1, L2);
+  T L = wmax(L1, L2);
   T Rr;
   if (L < (T)0.5) Rr = trg / L1;
   else if (L > (T)0.8) Rr = trg / L2;
@@ -878,7 +878,7 @@ template <typename T> V2<T> var_curl_sp(V2<T> p, T w, T powp, T c1, T c2, T sx, 
   T bb = c1 * y + c2x2 * x * y;
   T im = std::sqrt(bb * bb + sy * sy) * sel((T)-1, (T)1, bb > (T)0);
   T c = std::pow(std::fabs(re * re + im * im), pinv);
-  T r = w / std::max(c, (T)1e-30);
+  T r = w / wmax(c, (T)1e-30);
   return {(x * re + y * im) * r, (y * re - x * im) * r};
 }
 
@@ -895,7 +895,7 @@ template <typename T> V2<T> var_murl2(V2<T> p, T w, T c, T pf) {
   T r1 = std::pow(std::fabs(re0 * re0 + im0 * im0), invp);
   T a2 = std::atan2(im0, re0) * (T)2 * invp;
   T re1 = r1 * safe_cos(a2), im1 = r1 * safe_sin(a2);
-  T rl = vp / std::max(r1 * r1, (T)1e-30);
+  T rl = vp / wmax(r1 * r1, (T)1e-30);
   return {rl * (p.x * re1 + p.y * im1), rl * (p.y * re1 - p.x * im1)};
 }
 
@@ -929,10 +929,10 @@ V2<T> var_spirograph(V2<T> p, T w, T a, T b, T d, T tmin, T tmax, T ymin, T ymax
 template <typename T, typename R>
 V2<T> var_waffle(V2<T> p, T w, T slices_f, T xth, T yth, T rotation, R& rng) {
   (void)p;
-  T slices = std::max((T)1, std::floor(std::fabs(slices_f)));
+  T slices = wmax((T)1, std::floor(std::fabs(slices_f)));
   T inv = (T)1 / slices;
   T vc = w * safe_cos(rotation), vs = w * safe_sin(rotation);
-  const uint32_t mode = f2u(std::min((T)4, std::floor(rng.rand01() * (T)5)));
+  const uint32_t mode = f2u(wmin((T)4, std::floor(rng.rand01() * (T)5)));
   T a, r;
   if (mode == 0u) {
     T k0 = std::floor(rng.rand01() * slices);
@@ -982,7 +982,7 @@ V2<T> var_glynnSim1(V2<T> p, T w, T radius, T radius1, T phi1, T thick, T powp, 
     T phi = K<T>::TAU * rng.rand01();
     return {w * (ri * safe_cos(phi) + x1), w * (ri * safe_sin(phi) + y1)};
   }
-  T alpha = radius / std::max(r, (T)1e-30);
+  T alpha = radius / wmax(r, (T)1e-30);
   T x, y;
   if (rng.rand01() > contrast * std::pow(std::fabs(alpha), ap)) { x = p.x; y = p.y; }
   else { T a2 = alpha * alpha; x = a2 * p.x; y = a2 * p.y; }
@@ -1001,7 +1001,7 @@ V2<T> var_glynnSim2(V2<T> p, T w, T radius, T thick, T contrast, T powp, T phi1,
   T d2r = K<T>::PI / (T)180;
   T f1 = phi1 * d2r, f2 = phi2 * d2r;
   T rt = radius + thick;
-  T gam = thick * ((T)2 * radius + thick) / std::max(rt, (T)1e-30);
+  T gam = thick * ((T)2 * radius + thick) / wmax(rt, (T)1e-30);
   T delta = f2 - f1;
   T ap = std::fabs(powp);
   T r = std::sqrt(dot2(p));
@@ -1010,7 +1010,7 @@ V2<T> var_glynnSim2(V2<T> p, T w, T radius, T thick, T contrast, T powp, T phi1,
     T ph = f1 + delta * rng.rand01();
     return {w * (ri * safe_cos(ph)), w * (ri * safe_sin(ph))};
   }
-  T alpha = radius / std::max(r, (T)1e-30);
+  T alpha = radius / wmax(r, (T)1e-30);
   if (rng.rand01() > contrast * std::pow(std::fabs(alpha), ap)) return {w * p.x, w * p.y};
   T a2 = alpha * alpha;
   return {w * (a2 * p.x), w * (a2 * p.y)};
@@ -1020,8 +1020,8 @@ V2<T> var_glynnSim2(V2<T> p, T w, T radius, T thick, T contrast, T powp, T phi1,
 template <typename T, typename R>
 V2<T> var_glynnSim3(V2<T> p, T w, T radius, T thick, T contrast, T powp, R& rng) {
   T r1 = radius + thick;
-  T r2 = (radius * radius) / std::max(r1, (T)1e-30);
-  T gam = r1 / std::max(r1 + r2, (T)1e-30);
+  T r2 = (radius * radius) / wmax(r1, (T)1e-30);
+  T gam = r1 / wmax(r1 + r2, (T)1e-30);
   T ap = std::fabs(powp);
   T r = std::sqrt(dot2(p));
   if (r < r1) {
@@ -1029,7 +1029,7 @@ V2<T> var_glynnSim3(V2<T> p, T w, T radius, T thick, T contrast, T powp, R& rng)
     T ri = sel(r2, r1, rng.rand01() < gam);
     return {w * (ri * safe_cos(phi)), w * (ri * safe_sin(phi))};
   }
-  T alpha = radius / std::max(r, (T)1e-30);
+  T alpha = radius / wmax(r, (T)1e-30);
   if (rng.rand01() > contrast * std::pow(std::fabs(alpha), ap)) return {w * p.x, w * p.y};
   T a2 = alpha * alpha;
   return {w * (a2 * p.x), w * (a2 * p.y)};
@@ -1043,7 +1043,7 @@ template <typename T> V2<T> var_flipy(V2<T> p, T w) {
 // chaos.wgsl:3571
 template <typename T> V2<T> var_eclipse(V2<T> p, T w, T shift) {
   if (!(std::fabs(p.y) <= w)) return {w * p.x, w * p.y};
-  T c2 = std::sqrt(std::max(w * w - p.y * p.y, (T)0));
+  T c2 = std::sqrt(wmax(w * w - p.y * p.y, (T)0));
   T ox;
   if (std::fabs(p.x) <= c2) {
     T xs = p.x + shift * w;
@@ -1119,8 +1119,8 @@ template <typename T, typename R> V2<T> var_asteria(V2<T> p, T w, T alpha, R& rn
   else keep = !inside;
   if (keep) return {x0, y0};
   T rx = x0 * ca - y0 * sa, ry = x0 * sa + y0 * ca;
-  T den = std::sqrt(std::max((T)1 - ry * ry, (T)1e-30));
-  T in = std::max((T)1 - (-std::fabs(ry) + (T)1) * (-std::fabs(ry) + (T)1), (T)0);
+  T den = std::sqrt(wmax((T)1 - ry * ry, (T)1e-30));
+  T in = wmax((T)1 - (-std::fabs(ry) + (T)1) * (-std::fabs(ry) + (T)1), (T)0);
   T nx = rx / den * ((T)1 - std::sqrt(in));
   return {nx * ca + ry * sa, -nx * sa + ry * ca};
 }
@@ -1136,7 +1136,7 @@ template <typename T>
 V2<T> var_devil_warp(V2<T> p, T w, T a, T b, T effect, T warp, T rmin, T rmax) {
   (void)w;
   T x = p.x, y = p.y;
-  T r2 = (T)1 / std::max(x * x + y * y, (T)1e-30);
+  T r2 = (T)1 / wmax(x * x + y * y, (T)1e-30);
   T ba = x * x + r2 * b * y * y;
   T bb = y * y + r2 * a * x * x;
   T pa = sel((T)0, std::pow(ba, warp), ba > (T)0);
@@ -1198,7 +1198,7 @@ template <typename T> V2<T> var_atan(V2<T> p, T w, T modep, T stretch) {
 template <typename T> V2<T> var_cardioid(V2<T> p, T w, T a) {
   T th = std::atan2(p.y, p.x);
   T rs = p.x * p.x + p.y * p.y + safe_sin(a * th) + (T)1;
-  T r = w * std::sqrt(std::max(rs, (T)0));
+  T r = w * std::sqrt(wmax(rs, (T)0));
   return {r * std::cos(th), r * std::sin(th)};
 }
 
@@ -1218,10 +1218,10 @@ template <typename T, typename R> V2<T> var_chrysanthemum(V2<T> p, T w, R& rng) 
 
 // chaos.wgsl:3928 — bipolar coordinates with angular folding.
 template <typename T> V2<T> var_bcollide(V2<T> p, T w, T nump, T a) {
-  T num = std::max((T)1, nump);
+  T num = wmax((T)1, nump);
   T bnpi = num / K<T>::PI, pibn = K<T>::PI / num, bab = K<T>::PI * a / num;
   T xp = p.x + (T)1, xm = p.x - (T)1, y2 = p.y * p.y;
-  T tau = (T)0.5 * (std::log(std::max(xp * xp + y2, (T)1e-30)) - std::log(std::max(xm * xm + y2, (T)1e-30)));
+  T tau = (T)0.5 * (std::log(wmax(xp * xp + y2, (T)1e-30)) - std::log(wmax(xm * xm + y2, (T)1e-30)));
   T sr = K<T>::PI - std::atan2(p.y, xp) - std::atan2(p.y, (T)1 - p.x);
   const int32_t alt = f2i(sr * bnpi);
   T off = sel(-bab, bab, (alt & 1) == 0);
@@ -1245,7 +1245,7 @@ template <typename T> V2<T> var_bsplit(V2<T> p, T w, T sx, T sy) {
 
 // chaos.wgsl:3990
 template <typename T> V2<T> var_bulge(V2<T> p, T w, T n) {
-  T rs = std::max(std::sqrt(p.x * p.x + p.y * p.y), (T)1e-30);
+  T rs = wmax(std::sqrt(p.x * p.x + p.y * p.y), (T)1e-30);
   T sc = std::pow(rs, n) / rs;
   return {w * p.x * sc, w * p.y * sc};
 }
@@ -1290,10 +1290,10 @@ V2<T> var_corners(V2<T> p, T w, T cx, T cy, T mx, T my, T xp, T yp, T xyp, T log
   T xs = p.x * p.x, ys = p.y * p.y;
   T ex, ey;
   if (logm == (T)0) {
-    ex = std::pow(std::max(xs, (T)0), xp + xyp) * mx;
-    ey = std::pow(std::max(ys, (T)0), yp + xyp) * my;
+    ex = std::pow(wmax(xs, (T)0), xp + xyp) * mx;
+    ey = std::pow(wmax(ys, (T)0), yp + xyp) * my;
   } else {
-    T lb = std::log(std::max(std::fabs(logb), (T)1.000001));
+    T lb = std::log(wmax(std::fabs(logb), (T)1.000001));
     ex = std::pow(std::log((xs * mx) + (T)3) / lb, xp + (T)2.25 + xyp) - (T)1.33;
     ey = std::pow(std::log((ys * my) + (T)3) / lb, yp + (T)2.25 + xyp) - (T)1.33;
   }
@@ -1325,7 +1325,7 @@ template <typename T> V2<T> var_fibonacci2(V2<T> p, T w, T sc, T sc2) {
 // Hyperbolic {p, q} tiling radius (chaos.wgsl:4152-4220); `form` picks the
 // reference's two algebraically equal spellings of r^2.
 template <typename T> T hypertile_r(T pp, T q, bool form1, T& pa) {
-  T pi_ = std::max((T)3, pp), qs = std::max((T)3, q);
+  T pi_ = wmax((T)3, pp), qs = wmax((T)3, q);
   pa = (T)2 * K<T>::PI / pi_;
   T r2;
   if (!form1) {
@@ -1337,7 +1337,7 @@ template <typename T> T hypertile_r(T pp, T q, bool form1, T& pa) {
     T den = cp + std::cos((T)2 * K<T>::PI / qs);
     r2 = sel((T)1, (T)1 - (cp - (T)1) / den, std::fabs(den) > (T)1e-30);
   }
-  return sel((T)1, (T)1 / std::sqrt(std::max(r2, (T)1e-30)), r2 > (T)0);
+  return sel((T)1, (T)1 / std::sqrt(wmax(r2, (T)1e-30)), r2 > (T)0);
 }
 // chaos.wgsl:4152 — Moebius step of the tiling group by the nth rotation.
 template <typename T> V2<T> var_hypertile(V2<T> p, T w, T pp, T q, T n) {
@@ -1347,7 +1347,7 @@ template <typename T> V2<T> var_hypertile(V2<T> p, T w, T pp, T q, T n) {
   T re = r * safe_cos(an), im = r * safe_sin(an);
   T a = p.x + re, b = p.y - im;
   T c = re * p.x - im * p.y + (T)1, d = re * p.y + im * p.x;
-  T vr = w / std::max(c * c + d * d, (T)1e-30);
+  T vr = w / wmax(c * c + d * d, (T)1e-30);
   return {vr * (a * c + b * d), vr * (b * c - a * d)};
 }
 // chaos.wgsl:4176 — one draw picks one of ten rotations.
@@ -1358,7 +1358,7 @@ template <typename T, typename R> V2<T> var_hypertile1(V2<T> p, T w, T pp, T q, 
   T re = r * std::cos(rpa), im = r * std::sin(rpa);
   T a = p.x + re, b = p.y - im;
   T c = re * p.x - im * p.y + (T)1, d = re * p.y + im * p.x;
-  T vr = w / std::max(c * c + d * d, (T)1e-30);
+  T vr = w / wmax(c * c + d * d, (T)1e-30);
   return {vr * (a * c + b * d), vr * (b * c - a * d)};
 }
 // chaos.wgsl:4200 — one draw picks the rotation applied after the step.
@@ -1368,7 +1368,7 @@ template <typename T, typename R> V2<T> var_hypertile2(V2<T> p, T w, T pp, T q, 
   T a = p.x + r, b = p.y;
   T c = r * p.x + (T)1, d = r * p.y;
   T xx = a * c + b * d, yy = b * c - a * d;
-  T vr = w / std::max(c * c + d * d, (T)1e-30);
+  T vr = w / wmax(c * c + d * d, (T)1e-30);
   T rpa = std::floor(rng.rand01() * (T)32767) * pa;
   T ca = safe_cos(rpa), sa = safe_sin(rpa);
   return {vr * (xx * ca + yy * sa), vr * (yy * ca - xx * sa)};
@@ -1385,9 +1385,9 @@ template <typename T> V2<T> var_idisc(V2<T> p, T w) {
 // chaos.wgsl:4243
 template <typename T> V2<T> var_hole(V2<T> p, T w, T a, T insidep) {
   T al = std::atan2(p.y, p.x);
-  T delta = std::pow(std::max(al / K<T>::PI + (T)1, (T)1e-30), a);
+  T delta = std::pow(wmax(al / K<T>::PI + (T)1, (T)1e-30), a);
   T ss = p.x * p.x + p.y * p.y;
-  T r = sel(w * std::sqrt(std::max(ss + delta, (T)0)), w * delta / std::max(ss + delta, (T)1e-30),
+  T r = sel(w * std::sqrt(wmax(ss + delta, (T)0)), w * delta / wmax(ss + delta, (T)1e-30),
             f2i(insidep) != 0);
   return {r * std::cos(al), r * std::sin(al)};
 }
@@ -1412,8 +1412,8 @@ template <typename T> V2<T> var_layered_spiral(V2<T> p, T w, T radius) {
 // chaos.wgsl:4283
 template <typename T> V2<T> var_linear_t(V2<T> p, T w, T px, T py) {
   T sx = sel((T)-1, (T)1, p.x >= (T)0), sy = sel((T)-1, (T)1, p.y >= (T)0);
-  return {w * sx * std::pow(std::max(std::fabs(p.x), (T)1e-30), px),
-          w * sy * std::pow(std::max(std::fabs(p.y), (T)1e-30), py)};
+  return {w * sx * std::pow(wmax(std::fabs(p.x), (T)1e-30), px),
+          w * sy * std::pow(wmax(std::fabs(p.y), (T)1e-30), py)};
 }
 
 // chaos.wgsl:4296 — one draw along a fixed direction.
@@ -1422,7 +1422,7 @@ template <typename T, typename R> V2<T> var_line(V2<T> p, T w, T delta, T phi, R
   T cd = safe_cos(delta * K<T>::PI), sd = safe_sin(delta * K<T>::PI);
   T cp = safe_cos(phi * K<T>::PI), sp = safe_sin(phi * K<T>::PI);
   T ux = cd * cp, uy = sd * cp, uz = sp;
-  T r = std::sqrt(std::max(ux * ux + uy * uy + uz * uz, (T)1e-30));
+  T r = std::sqrt(wmax(ux * ux + uy * uy + uz * uz, (T)1e-30));
   ux = ux / r;
   uy = uy / r;
   T t = rng.rand01() * w;
@@ -1444,7 +1444,7 @@ V2<T> var_phoenix_julia(V2<T> p, T w, T power, T dist, T xd, T yd, R& rng) {
   T px = p.x * (xd + (T)1), py = p.y * (yd + (T)1);
   T k = std::floor(rng.rand01() * (T)32768);
   T a = std::atan2(py, px) * inv_n + k * inv2;
-  T r = w * std::pow(std::max(p.x * p.x + p.y * p.y, (T)1e-30), cn);
+  T r = w * std::pow(wmax(p.x * p.x + p.y * p.y, (T)1e-30), cn);
   return {r * safe_cos(a), r * safe_sin(a)};
 }
 
@@ -1458,7 +1458,7 @@ template <typename T> V2<T> var_unpolar(V2<T> p, T w) {
 // chaos.wgsl:4368
 template <typename T> V2<T> var_shredrad(V2<T> p, T w, T n, T widthp) {
   T width = clampv(widthp, (T)-1, (T)1);
-  T alpha = (T)2 * K<T>::PI / std::max((T)0.001, n);
+  T alpha = (T)2 * K<T>::PI / wmax((T)0.001, n);
   T ang = std::atan2(p.y, p.x);
   T rad = std::sqrt(p.x * p.x + p.y * p.y);
   T xa = (ang + (T)3 * K<T>::PI + alpha * (T)0.5) / alpha;
@@ -1470,7 +1470,7 @@ template <typename T> V2<T> var_shredrad(V2<T> p, T w, T n, T widthp) {
 template <typename T, typename R> V2<T> var_vogel(V2<T> p, T w, T n, T scale, R& rng) {
   const T phi = (T)1.61803398874989;
   T k2 = (T)2 * K<T>::PI / (phi * phi);
-  T i = std::floor(rng.rand01() * std::max((T)1, n)) + (T)1;
+  T i = std::floor(rng.rand01() * wmax((T)1, n)) + (T)1;
   T a = i * k2;
   T r = w * (std::sqrt(p.x * p.x + p.y * p.y) + std::sqrt(i));
   return {r * (safe_cos(a) + scale * p.x), r * (safe_sin(a) + scale * p.y)};
@@ -1495,14 +1495,14 @@ V2<T> var_yin_yang(V2<T> p, T w, T radius, T ang1, T ang2, T dualp, T outp, R& r
     x = nx;
     y = ny;
     if (y > (T)0) {
-      T t = std::sqrt(std::max((T)1 - y * y, (T)0));
-      T k = x / std::max(std::fabs(t), (T)1e-30) * signv(t);
+      T t = std::sqrt(wmax((T)1 - y * y, (T)0));
+      T k = x / wmax(std::fabs(t), (T)1e-30) * signv(t);
       T t1 = (t - (T)0.5) * (T)2;
       T al = ((T)1 - k) * (T)0.5;
       T be = (T)1 - al;
       T dx = al * (RR - (T)1);
       T k1 = al * RR + be * (T)1;
-      return {w * (t1 * k1 + dx) * inv, w * std::sqrt(std::max((T)1 - t1 * t1, (T)0)) * k1 * inv};
+      return {w * (t1 * k1 + dx) * inv, w * std::sqrt(wmax((T)1 - t1 * t1, (T)0)) * k1 * inv};
     }
     return {w * (x * ((T)1 - RR) + RR) * inv, w * (y * ((T)1 - RR)) * inv};
   }
@@ -1512,7 +1512,7 @@ V2<T> var_yin_yang(V2<T> p, T w, T radius, T ang1, T ang2, T dualp, T outp, R& r
 
 // chaos.wgsl:4466 — one draw picks the winding.
 template <typename T, typename R> V2<T> var_squish(V2<T> p, T w, T powp, R& rng) {
-  T power = std::max((T)2, powp);
+  T power = wmax((T)2, powp);
   T ip = (T)1 / power;
   T ax = std::fabs(p.x), ay = std::fabs(p.y);
   T s, pp;
@@ -1532,10 +1532,10 @@ template <typename T> V2<T> var_target(V2<T> p, T w, T even, T odd, T size) {
   T h = (T)0.5 * size;
   T a = std::atan2(p.y, p.x);
   T r = std::sqrt(p.x * p.x + p.y * p.y);
-  T t = std::log(std::max(r, (T)1e-30));
+  T t = std::log(wmax(r, (T)1e-30));
   if (t < (T)0) t = t - h;
   T at = std::fabs(t);
-  T ss = std::max(std::fabs(size), (T)1e-30);
+  T ss = wmax(std::fabs(size), (T)1e-30);
   t = at - std::floor(at / ss) * ss;
   a = a + sel(odd, even, t < h);
   return {r * safe_cos(a) * w, r * safe_sin(a) * w};
@@ -1564,25 +1564,25 @@ template <typename T>
 V2<T> var_hole2(V2<T> p, T w, T a, T b, T c, T d, T insidep, T shapep) {
   T rho = p.x * p.x + p.y * p.y;
   T th = std::atan2(p.y, p.x) * d;
-  T delta = std::pow(std::max(th / K<T>::PI + (T)1, (T)1e-30), a) * c;
+  T delta = std::pow(wmax(th / K<T>::PI + (T)1, (T)1e-30), a) * c;
   T r1;
   switch (f2i(shapep)) {
-    case 0: r1 = std::sqrt(std::max(rho, (T)0)) + delta; break;
-    case 1: r1 = std::sqrt(std::max(rho + delta, (T)0)); break;
-    case 2: r1 = std::sqrt(std::max(rho + safe_sin(b * th) + delta, (T)0)); break;
-    case 3: r1 = std::sqrt(std::max(rho + safe_sin(th) + delta, (T)0)); break;
-    case 4: r1 = std::sqrt(std::max(rho + th * th - delta + (T)1, (T)0)); break;
-    case 5: r1 = std::sqrt(std::max(rho + std::fabs(safe_tan(th)) + delta, (T)0)); break;
-    case 6: r1 = std::sqrt(std::max(rho * ((T)1 + safe_sin(b * th)) + delta, (T)0)); break;
-    case 7: r1 = std::sqrt(std::max(rho + std::fabs(safe_sin((T)0.5 * b * th)) + delta, (T)0)); break;
-    case 8: r1 = std::sqrt(std::max(rho + safe_sin(K<T>::PI * safe_sin(b * th)) + delta, (T)0)); break;
+    case 0: r1 = std::sqrt(wmax(rho, (T)0)) + delta; break;
+    case 1: r1 = std::sqrt(wmax(rho + delta, (T)0)); break;
+    case 2: r1 = std::sqrt(wmax(rho + safe_sin(b * th) + delta, (T)0)); break;
+    case 3: r1 = std::sqrt(wmax(rho + safe_sin(th) + delta, (T)0)); break;
+    case 4: r1 = std::sqrt(wmax(rho + th * th - delta + (T)1, (T)0)); break;
+    case 5: r1 = std::sqrt(wmax(rho + std::fabs(safe_tan(th)) + delta, (T)0)); break;
+    case 6: r1 = std::sqrt(wmax(rho * ((T)1 + safe_sin(b * th)) + delta, (T)0)); break;
+    case 7: r1 = std::sqrt(wmax(rho + std::fabs(safe_sin((T)0.5 * b * th)) + delta, (T)0)); break;
+    case 8: r1 = std::sqrt(wmax(rho + safe_sin(K<T>::PI * safe_sin(b * th)) + delta, (T)0)); break;
     case 9:
-      r1 = std::sqrt(std::max(
+      r1 = std::sqrt(wmax(
           rho + (safe_sin(b * th) + safe_sin((T)2 * b * th + K<T>::PI * (T)0.5)) * (T)0.5 + delta, (T)0));
       break;
     default: r1 = (T)1; break;
   }
-  T rf = sel(w * r1, w / std::max(r1, (T)1e-30), f2i(insidep) != 0);
+  T rf = sel(w * r1, w / wmax(r1, (T)1e-30), f2i(insidep) != 0);
   return {rf * safe_cos(th), rf * safe_sin(th)};
 }
 
@@ -1640,13 +1640,13 @@ V2<T> var_fourth(V2<T> p, T w, T spin, T space, T twist, T offx, T offy) {
   T ww = w * w;
   if (p.x > (T)0 && p.y > (T)0) {
     T th = std::atan2(p.y, p.x);
-    T r = (T)1 / std::max(std::sqrt(p.x * p.x + p.y * p.y), (T)1e-30);
+    T r = (T)1 / wmax(std::sqrt(p.x * p.x + p.y * p.y), (T)1e-30);
     return {w * r * std::cos(th), w * r * std::sin(th)};
   }
   if (p.x > (T)0 && p.y < (T)0) {
     T r2 = p.x * p.x + p.y * p.y;
     if (r2 < ww) {
-      T r = w * std::sqrt(std::max(ww / std::max(r2, (T)1e-30) - (T)1, (T)0));
+      T r = w * std::sqrt(wmax(ww / wmax(r2, (T)1e-30) - (T)1, (T)0));
       return {r * p.x, r * p.y};
     }
     return {w * p.x, w * p.y};
@@ -1659,7 +1659,7 @@ V2<T> var_fourth(V2<T> p, T w, T spin, T space, T twist, T offx, T offy) {
       T r = w * r0;
       return {r * safe_cos(th) + offx, r * safe_sin(th) - offy};
     }
-    T r = w * ((T)1 + space / std::max(r0, (T)1e-30));
+    T r = w * ((T)1 + space / wmax(r0, (T)1e-30));
     return {r * xx + offx, r * yy - offy};
   }
   return {w * p.x, w * p.y};
@@ -1673,7 +1673,7 @@ template <typename T> V2<T> var_pulse(V2<T> p, T w, T fx, T fy, T sx, T sy) {
 // chaos.wgsl:4738
 template <typename T> V2<T> var_rays1(V2<T> p, T w) {
   T t = p.x * p.x + p.y * p.y;
-  T tv = safe_tan(std::sqrt(std::max(t, (T)1e-30)));
+  T tv = safe_tan(std::sqrt(wmax(t, (T)1e-30)));
   T it = (T)1 / sel(tv, (T)1e-30, std::fabs(tv) < (T)1e-30);
   T u = it + w * ((T)2 / K<T>::PI) * ((T)2 / K<T>::PI);
   return {w * u * t / sel(p.x, (T)1e-30, p.x == (T)0), w * u * t / sel(p.y, (T)1e-30, p.y == (T)0)};
@@ -1681,7 +1681,7 @@ template <typename T> V2<T> var_rays1(V2<T> p, T w) {
 // chaos.wgsl:4751
 template <typename T> V2<T> var_rays2(V2<T> p, T w) {
   T t = p.x * p.x + p.y * p.y;
-  T ts = std::max(t, (T)1e-30);
+  T ts = wmax(t, (T)1e-30);
   T in = (ts + (T)1e-6) * safe_tan((T)1 / ts + (T)1e-6);
   T ci = safe_cos(in);
   T u = (T)1 / sel(ci, (T)1e-30, std::fabs(ci) < (T)1e-30);
@@ -1691,10 +1691,10 @@ template <typename T> V2<T> var_rays2(V2<T> p, T w) {
 // chaos.wgsl:4766
 template <typename T> V2<T> var_rays3(V2<T> p, T w) {
   T t = p.x * p.x + p.y * p.y;
-  T ts = std::max(t, (T)1e-30);
+  T ts = wmax(t, (T)1e-30);
   T in = safe_sin(t * t + (T)1e-6) * safe_sin((T)1 / (ts * ts) + (T)1e-6);
-  T den = std::sqrt(std::max(safe_cos(in), (T)1e-30));
-  T u = (T)1 / std::max(den, (T)1e-30);
+  T den = std::sqrt(wmax(safe_cos(in), (T)1e-30));
+  T u = (T)1 / wmax(den, (T)1e-30);
   T co = w / (T)10;
   return {co * u * safe_cos(t) * t / sel(p.x, (T)1e-30, p.x == (T)0),
           co * u * safe_tan(t) * t / sel(p.y, (T)1e-30, p.y == (T)0)};
@@ -1710,7 +1710,7 @@ template <typename T> V2<T> var_tancos(V2<T> p, T w) {
 // chaos.wgsl:4802
 template <typename T> V2<T> var_twoface(V2<T> p, T w) {
   T v = w;
-  if (p.x > (T)0) v = v / std::max(p.x * p.x + p.y * p.y, (T)1e-30);
+  if (p.x > (T)0) v = v / wmax(p.x * p.x + p.y * p.y, (T)1e-30);
   return {v * p.x, v * p.y};
 }
 
@@ -1720,7 +1720,7 @@ template <typename T> V2<T> var_twoface(V2<T> p, T w) {
 // (chaos.wgsl:4814-4936): returns xmax = cosh(mu) (>= 1) and nu in [0, pi].
 template <typename T> T ell_xmax(T x, T r2) {
   T tmp = r2 + (T)1, tmp2 = (T)2 * x;
-  T sa = std::sqrt(std::max(tmp + tmp2, (T)0)), sb = std::sqrt(std::max(tmp - tmp2, (T)0));
+  T sa = std::sqrt(wmax(tmp + tmp2, (T)0)), sb = std::sqrt(wmax(tmp - tmp2, (T)0));
   T xmax = (sa + sb) * (T)0.5;
   return xmax < (T)1 ? (T)1 : xmax;
 }
@@ -1738,7 +1738,7 @@ template <typename T, typename R> V2<T> var_e_julia(V2<T> p, T w, T powp, R& rng
   T r2 = p.y * p.y + p.x * p.x;
   T x = p.x;
   if (!(ps > (T)0)) {
-    r2 = (T)1 / std::max(r2, (T)1e-30);
+    r2 = (T)1 / wmax(r2, (T)1e-30);
     x = p.x * r2;
   }
   T xmax = ell_xmax(x, r2);
@@ -1763,7 +1763,7 @@ template <typename T, typename R> V2<T> var_cannabis_curve_wf(V2<T> p, T w, T fi
 
 // chaos.wgsl:4868
 template <typename T> V2<T> var_e_collide(V2<T> p, T w, T nump, T a) {
-  T num = std::max((T)1, nump);
+  T num = wmax((T)1, nump);
   T knpi = num / K<T>::PI, pikn = K<T>::PI / num, ka = K<T>::PI * a / num;
   T xmax = ell_xmax(p.x, p.y * p.y + p.x * p.x);
   T nu = ell_nu(p.x, xmax);
@@ -1773,7 +1773,7 @@ template <typename T> V2<T> var_e_collide(V2<T> p, T w, T nump, T a) {
   nu = (T)alt * pikn + (arg - std::floor(arg / pikn) * pikn);
   if (p.y <= (T)0) nu = -nu;
   return {w * xmax * std::cos(nu),
-          w * std::sqrt(std::max(xmax - (T)1, (T)0)) * std::sqrt(xmax + (T)1) * std::sin(nu)};
+          w * std::sqrt(wmax(xmax - (T)1, (T)0)) * std::sqrt(xmax + (T)1) * std::sin(nu)};
 }
 
 // chaos.wgsl:4897
@@ -1783,7 +1783,7 @@ template <typename T> V2<T> var_e_mod(V2<T> p, T w, T radius, T distance) {
   T nu = ell_nu(p.x, xmax);
   if (p.y < (T)0) nu = -nu;
   if (mu < radius && -mu < radius) {
-    T rs = std::max(radius, (T)1e-30);
+    T rs = wmax(radius, (T)1e-30);
     T two = (T)2 * rs;
     if (nu > (T)0) mu = wmod(mu + rs + distance * rs, two) - rs;
     else mu = wmod(mu - rs - distance * rs, two) + rs;
@@ -1794,13 +1794,13 @@ template <typename T> V2<T> var_e_mod(V2<T> p, T w, T radius, T distance) {
 // Tiled band shared by intersection / parallel: the coordinate folded into
 // [-m1, m1) with period m2*m1 (chaos.wgsl:4938-5204, `%` spelling).
 template <typename T> T band_fold_mod(T v, T m1, T period) {
-  if (v > m1) return -m1 + wmod(v + m1, std::max(period, (T)1e-30));
-  if (v < -m1) return m1 - wmod(m1 - v, std::max(period, (T)1e-30));
+  if (v > m1) return -m1 + wmod(v + m1, wmax(period, (T)1e-30));
+  if (v < -m1) return m1 - wmod(m1 - v, wmax(period, (T)1e-30));
   return v;
 }
 // The same fold written with floor (parallel).
 template <typename T> T band_fold_floor(T v, T m1, T period) {
-  T rs = std::max(period, (T)1e-30);
+  T rs = wmax(period, (T)1e-30);
   if (v > m1) { T a = v + m1; return -m1 + (a - std::floor(a / rs) * rs); }
   if (v < -m1) { T a = m1 - v; return m1 - (a - std::floor(a / rs) * rs); }
   return v;
@@ -1808,7 +1808,7 @@ template <typename T> T band_fold_floor(T v, T m1, T period) {
 // Random integer step along one axis: three draws (branch already taken).
 template <typename T, typename R> T band_jump(T width, R& rng) {
   T x = sel(-width, width, rng.rand01() < (T)0.5);
-  return rint_even(x * std::log(std::max(rng.rand01(), (T)1e-30)));
+  return rint_even(x * std::log(wmax(rng.rand01(), (T)1e-30)));
 }
 
 // chaos.wgsl:4938 — one branch draw, then two more.
@@ -1831,8 +1831,8 @@ template <typename T> V2<T> var_inv_squircular(V2<T> p, T w) {
   const T sq2 = (T)1.41421356237;
   T u = p.x, v = p.y;
   T ri = u * u + v * v;
-  T r2 = std::sqrt(std::max(ri * (w * w * ri - (T)4 * u * u * v * v) / w, (T)0));
-  T r = std::sqrt(std::max(ri - r2, (T)0)) / sq2;
+  T r2 = std::sqrt(wmax(ri * (w * w * ri - (T)4 * u * u * v * v) / w, (T)0));
+  T r = std::sqrt(wmax(ri - r2, (T)0)) / sq2;
   return {r / sel(u, (T)1e-30, u == (T)0), r / sel(v, (T)1e-30, v == (T)0)};
 }
 
@@ -1844,16 +1844,16 @@ template <typename T> V2<T> var_lozi(V2<T> p, T w, T a, T b, T c) {
 // chaos.wgsl:5014
 template <typename T> V2<T> var_hypershift(V2<T> p, T w, T shift, T stretch) {
   T scale = (T)1 - shift * shift;
-  T r1 = (T)1 / std::max(p.x * p.x + p.y * p.y, (T)1e-30);
+  T r1 = (T)1 / wmax(p.x * p.x + p.y * p.y, (T)1e-30);
   T x = r1 * p.x + shift, y = r1 * p.y;
-  T rad = w * scale / std::max(x * x + y * y, (T)1e-30);
+  T rad = w * scale / wmax(x * x + y * y, (T)1e-30);
   return {rad * x + shift, rad * y * stretch};
 }
 
 // chaos.wgsl:5026 — offset from the nearest hexagon centre (cube rounding).
 template <typename T> V2<T> var_hex_modulus(V2<T> p, T w, T size) {
   const T h32 = (T)0.8660254037844386, sq3 = (T)1.7320508075688772;
-  T hs = h32 / std::max(size, (T)1e-30);
+  T hs = h32 / wmax(size, (T)1e-30);
   T wt = w / h32;
   T X = p.x * hs, Y = p.y * hs;
   T x = (T)0.5773502691896258 * X - Y / (T)3;
@@ -1871,8 +1871,8 @@ template <typename T> V2<T> var_hex_modulus(V2<T> p, T w, T size) {
 
 // chaos.wgsl:5067 — one draw decides whether the point is pushed to a border.
 template <typename T, typename R> V2<T> var_boarders2(V2<T> p, T w, T c, T left, T right, R& rng) {
-  T ac = std::max(std::fabs(c), (T)1e-6);
-  T al = std::max(std::fabs(left), (T)1e-6), ar = std::max(std::fabs(right), (T)1e-6);
+  T ac = wmax(std::fabs(c), (T)1e-6);
+  T al = wmax(std::fabs(left), (T)1e-6), ar = wmax(std::fabs(right), (T)1e-6);
   T cl = ac * al, cr = ac + ac * ar;
   T rx = rint_even(p.x), ry = rint_even(p.y);
   T ox = p.x - rx, oy = p.y - ry;
@@ -1889,7 +1889,7 @@ template <typename T, typename R> V2<T> var_boarders2(V2<T> p, T w, T c, T left,
 // Bipolar coordinates (tau, sigma) of p (chaos.wgsl:5107-5145).
 template <typename T> T bip_tau(V2<T> p) {
   T xp = p.x + (T)1, xm = p.x - (T)1, y2 = p.y * p.y;
-  return (T)0.5 * (std::log(std::max(xp * xp + y2, (T)1e-30)) - std::log(std::max(xm * xm + y2, (T)1e-30)));
+  return (T)0.5 * (std::log(wmax(xp * xp + y2, (T)1e-30)) - std::log(wmax(xm * xm + y2, (T)1e-30)));
 }
 template <typename T> T bip_sigma(V2<T> p) {
   return K<T>::PI - std::atan2(p.y, p.x + (T)1) - std::atan2(p.y, (T)1 - p.x);
@@ -1899,7 +1899,7 @@ template <typename T> V2<T> var_b_mod(V2<T> p, T w, T radius, T distance) {
   T tau = bip_tau(p);
   T sg = bip_sigma(p);
   if (tau < radius && -tau < radius) {
-    T rs = std::max(radius, (T)1e-30);
+    T rs = wmax(radius, (T)1e-30);
     T two = (T)2 * rs;
     T arg = tau + rs + distance * rs;
     tau = arg - std::floor(arg / two) * two - rs;
@@ -1911,7 +1911,7 @@ template <typename T> V2<T> var_b_mod(V2<T> p, T w, T radius, T distance) {
 // chaos.wgsl:5127 — one draw picks the root.
 template <typename T, typename R>
 V2<T> var_b_transform(V2<T> p, T w, T rotate, T powp, T move, T split, R& rng) {
-  T power = std::max((T)1, powp);
+  T power = wmax((T)1, powp);
   T tau = bip_tau(p) / power + move;
   T sg = bip_sigma(p) + rotate;
   T k = std::floor(rng.rand01() * power);
@@ -1968,7 +1968,7 @@ template <typename T> V2<T> var_scry2(V2<T> p, T w, T sides_f, T star, T circle)
     T nx = xr * ca - yr * sa;
     yr = xr * sa + yr * ca;
     xr = nx;
-    r2 = std::max(r2, xr * coss + std::fabs(yr) * sins);
+    r2 = wmax(r2, xr * coss + std::fabs(yr) * sins);
     iters = i + 1;
   }
   r2 = r2 * cosc + cr * sinc;
@@ -1985,7 +1985,7 @@ template <typename T> V2<T> var_scry2(V2<T> p, T w, T sides_f, T star, T circle)
 // chaos.wgsl:5281
 template <typename T> V2<T> var_ennepers2(V2<T> p, T w, T ap, T bp, T cp) {
   T x = p.x, y = p.y;
-  T r2 = (T)1 / std::max(x * x + y * y, (T)1e-30);
+  T r2 = (T)1 / wmax(x * x + y * y, (T)1e-30);
   T dxy = (ap * x) * (ap * x) - (bp * y) * (bp * y);
   return {w * x * (ap * ap - dxy * r2 - cp * std::sqrt(std::fabs(x))),
           w * y * (bp * bp - dxy * r2 - cp * std::sqrt(std::fabs(y)))};
@@ -1995,10 +1995,10 @@ template <typename T> V2<T> var_ennepers2(V2<T> p, T w, T ap, T bp, T cp) {
 template <typename T, typename R> V2<T> var_apollony(V2<T> p, T w, R& rng) {
   const T sq3 = (T)1.7320508075688772;
   T dx = (T)1 + sq3 - p.x;
-  T den = std::max(dx * dx + p.y * p.y, (T)1e-30);
+  T den = wmax(dx * dx + p.y * p.y, (T)1e-30);
   T a0 = (T)3 * dx / den - ((T)1 + sq3) / ((T)2 + sq3);
   T b0 = (T)3 * p.y / den;
-  T ab = std::max(a0 * a0 + b0 * b0, (T)1e-30);
+  T ab = wmax(a0 * a0 + b0 * b0, (T)1e-30);
   T fx = a0 / ab, fy = -b0 / ab;
   const int32_t br = imod(f2i(std::floor((T)4 * rng.rand01())), 3);
   T x, y;
@@ -2223,7 +2223,7 @@ template <typename T> V2<T> var_box_fold(V2<T> p, T w, T limit) {
 template <typename T> T sphere_scale(T r2, T rmin, T rmax) {
   T a = rmin * rmin, b = rmax * rmax;
   if (r2 < a) return b / a;
-  if (r2 < b) return b / std::max(r2, (T)1e-6);
+  if (r2 < b) return b / wmax(r2, (T)1e-6);
   return (T)1;
 }
 // chaos.wgsl:5710
@@ -2242,7 +2242,7 @@ template <typename T> V2<T> var_kifs_fold(V2<T> p, T w, T n, T offset) {
   T r = len2(p);
   if (r < (T)1e-6) return {(T)0, (T)0};
   T a = std::atan2(p.y, p.x) - offset;
-  T th = (T)6.283185307179586 / std::max((T)1, std::fabs(n));
+  T th = (T)6.283185307179586 / wmax((T)1, std::fabs(n));
   a = a - th * std::floor(a / th);
   if (a > th * (T)0.5) a = th - a;
   a = a + offset;
@@ -2281,10 +2281,10 @@ template <typename T> V2<T> polar_w(T w, T r, T th) { return {w * (r * safe_cos(
 // chaos.wgsl:5810
 template <typename T> V2<T> var_superellipse(V2<T> p, T w, T a, T b, T n) {
   T th = std::atan2(p.y, p.x);
-  T sa = std::max(std::fabs(a), (T)1e-4), sb = std::max(std::fabs(b), (T)1e-4);
-  T sn = std::max(n, (T)0.01);
+  T sa = wmax(std::fabs(a), (T)1e-4), sb = wmax(std::fabs(b), (T)1e-4);
+  T sn = wmax(n, (T)0.01);
   T c = std::fabs(safe_cos(th) / sa), s = std::fabs(safe_sin(th) / sb);
-  T r = std::pow(std::max(std::pow(c, sn) + std::pow(s, sn), (T)1e-6), (T)-1 / sn);
+  T r = std::pow(wmax(std::pow(c, sn) + std::pow(s, sn), (T)1e-6), (T)-1 / sn);
   return polar_w(w, r, th);
 }
 // chaos.wgsl:5822
@@ -2300,7 +2300,7 @@ template <typename T> V2<T> var_epicycloid(V2<T> p, T w, T k) {
 }
 // chaos.wgsl:5838
 template <typename T> V2<T> var_catenary(V2<T> p, T w, T a) {
-  T sa = std::max(std::fabs(a), (T)1e-4) * signv(a + (T)1e-8);
+  T sa = wmax(std::fabs(a), (T)1e-4) * signv(a + (T)1e-8);
   return {w * p.x, w * (sa * std::cosh(p.x / sa))};
 }
 // chaos.wgsl:5845
@@ -2328,9 +2328,9 @@ template <typename T> V2<T> var_rossler(V2<T> p, T w, T h, T a) {
 }
 // chaos.wgsl:5883
 template <typename T> V2<T> var_droste(V2<T> p, T w, T s) {
-  T r = std::max(len2(p), (T)1e-6);
+  T r = wmax(len2(p), (T)1e-6);
   T th = std::atan2(p.y, p.x);
-  T k = std::log(std::max(s, (T)1e-4)) / (T)6.283185307179586;
+  T k = std::log(wmax(s, (T)1e-4)) / (T)6.283185307179586;
   T re = std::log(r) - k * th;
   T im = th + k * std::log(r);
   T er = std::exp(re);
@@ -2344,7 +2344,7 @@ template <typename T> V2<T> var_logspiral(V2<T> p, T w, T a, T k) {
 // Angle in (0, 2pi], floored at 1e-6 (chaos.wgsl:5903-5925).
 template <typename T> T pos_angle(V2<T> p) {
   T t = std::atan2(p.y, p.x);
-  return std::max(sel(t, t + (T)6.283185307179586, t < (T)0), (T)1e-6);
+  return wmax(sel(t, t + (T)6.283185307179586, t < (T)0), (T)1e-6);
 }
 // chaos.wgsl:5903
 template <typename T> V2<T> var_fermat_spiral(V2<T> p, T w, T a) {
@@ -2406,7 +2406,7 @@ template <typename T> V2<T> reflect2(T vx, T vy, T nx, T ny) {
 
 // chaos.wgsl:5971 — one step of length `step` inside a circular table.
 template <typename T> V2<T> var_billiard_circle(V2<T> p, T w, T radius, T step, T angle) {
-  T rc = std::max(radius, (T)1.0e-5);
+  T rc = wmax(radius, (T)1.0e-5);
   T px = p.x, py = p.y;
   T r0 = len2(p);
   if (r0 > rc) { px = (px / r0) * rc; py = (py / r0) * rc; r0 = rc; }
@@ -2416,7 +2416,7 @@ template <typename T> V2<T> var_billiard_circle(V2<T> p, T w, T radius, T step, 
   T vx = bx * ca - by * sa, vy = bx * sa + by * ca;
   T dpv = px * vx + py * vy;
   T disc = dpv * dpv + rc * rc - (px * px + py * py);
-  T th = -dpv + std::sqrt(std::max(disc, (T)0));
+  T th = -dpv + std::sqrt(wmax(disc, (T)0));
   T fx = px + step * vx, fy = py + step * vy;
   if (th > (T)0 && th < step) {
     T hx = px + th * vx, hy = py + th * vy;
@@ -2440,7 +2440,7 @@ template <typename T> bool circle_exit(T px, T py, T vx, T vy, T cx, T R, T& t) 
 
 // chaos.wgsl:6014 — stadium: two flat walls and two semicircular caps.
 template <typename T> V2<T> var_billiard_stadium(V2<T> p, T w, T width, T height, T step, T angle) {
-  T R = std::max(height / (T)2, (T)1.0e-5), w2 = std::max(width / (T)2, (T)1.0e-5);
+  T R = wmax(height / (T)2, (T)1.0e-5), w2 = wmax(width / (T)2, (T)1.0e-5);
   T px = p.x, py = p.y;
   if (std::fabs(px) > w2) {
     T cx = sel(-w2, w2, px > (T)0);
@@ -2477,11 +2477,11 @@ template <typename T> V2<T> var_billiard_stadium(V2<T> p, T w, T width, T height
     if (wall == 1) ny = (T)-1;
     else if (wall == 2) ny = (T)1;
     else if (wall == 3) {
-      T rl = std::max(len2(V2<T>{hx - w2, hy}), (T)1.0e-5);
+      T rl = wmax(len2(V2<T>{hx - w2, hy}), (T)1.0e-5);
       nx = -(hx - w2) / rl;
       ny = -hy / rl;
     } else if (wall == 4) {
-      T rl = std::max(len2(V2<T>{hx + w2, hy}), (T)1.0e-5);
+      T rl = wmax(len2(V2<T>{hx + w2, hy}), (T)1.0e-5);
       nx = -(hx + w2) / rl;
       ny = -hy / rl;
     }
@@ -2494,7 +2494,7 @@ template <typename T> V2<T> var_billiard_stadium(V2<T> p, T w, T width, T height
 
 // chaos.wgsl:6121 — square table with a central circular scatterer.
 template <typename T> V2<T> var_billiard_sinai(V2<T> p, T w, T length, T radius, T step, T angle) {
-  T L2 = std::max(length / (T)2, (T)1.0e-5), R = std::max(radius, (T)1.0e-5);
+  T L2 = wmax(length / (T)2, (T)1.0e-5), R = wmax(radius, (T)1.0e-5);
   T px = p.x, py = p.y;
   if (std::fabs(px) > L2) px = sel(-L2, L2, px > (T)0);
   if (std::fabs(py) > L2) py = sel(-L2, L2, py > (T)0);
@@ -2554,12 +2554,12 @@ template <typename T> V2<T> var_billiard_sinai(V2<T> p, T w, T length, T radius,
 template <typename T> V2<T> var_billiard_polygon(V2<T> p, T w, T sides_f, T radius, T step, T angle) {
   const uint32_t n = f2u(clampv(sides_f + (T)0.5, (T)3, (T)12));
   const T two_pi = (T)(2.0 * 3.141592653589793);
-  T R = std::max(radius, (T)1.0e-5);
+  T R = wmax(radius, (T)1.0e-5);
   T inr = R * std::cos((T)3.141592653589793 / (T)n);
   T px = p.x, py = p.y;
   T r0 = std::sqrt(px * px + py * py);
   if (r0 > inr) {
-    T sc = inr / std::max(r0, (T)1.0e-6);
+    T sc = inr / wmax(r0, (T)1.0e-6);
     px = px * sc;
     py = py * sc;
   }
@@ -2587,7 +2587,7 @@ template <typename T> V2<T> var_billiard_polygon(V2<T> p, T w, T sides_f, T radi
     T sx = R * safe_cos(p0), sy = R * safe_sin(p0);
     T ex_ = R * safe_cos(p1), ey_ = R * safe_sin(p1);
     T ex = ex_ - sx, ey = ey_ - sy;
-    T el = std::max(std::sqrt(ex * ex + ey * ey), (T)1.0e-5);
+    T el = wmax(std::sqrt(ex * ex + ey * ey), (T)1.0e-5);
     T nx = -ey / el, ny = ex / el;
     T mx = (T)0.5 * (sx + ex_), my = (T)0.5 * (sy + ey_);
     if (nx * mx + ny * my > (T)0) { nx = -nx; ny = -ny; }
@@ -2610,7 +2610,7 @@ template <typename T> V2<T> var_lorentz_boost(V2<T> p, T w, T rapidity, T angle)
 }
 // chaos.wgsl:6355
 template <typename T> V2<T> var_schwarzschild_lensing(V2<T> p, T w, T mass, T eps) {
-  T b = len2(p) + std::max(eps, (T)1e-6);
+  T b = len2(p) + wmax(eps, (T)1e-6);
   T al = mass / b;
   T ca = safe_cos(al), sa = safe_sin(al);
   return {w * (p.x * ca - p.y * sa), w * (p.x * sa + p.y * ca)};
@@ -2645,8 +2645,8 @@ V2<T> var_magnetic_pendulum_pos(V2<T> p, T w, T magnets, T radius, T strength, T
 // ---- modular forms (chaos.wgsl:462-522, 6435-6498) --------------------------------
 
 template <typename T> V2<T> upper_half(V2<T> p, T im_floor) {
-  T im = std::fabs(p.y) + std::max(im_floor, (T)0.02);
-  return {p.x, std::max(im, (T)0.006435)};
+  T im = std::fabs(p.y) + wmax(im_floor, (T)0.02);
+  return {p.x, wmax(im, (T)0.006435)};
 }
 template <typename T> V2<T> nome(V2<T> tau) { return c_exp(V2<T>{-K<T>::PI * tau.y, K<T>::PI * tau.x}); }
 // Jacobi theta series in the nome (8 terms; powers by recurrence).
@@ -2673,7 +2673,7 @@ template <typename T> V2<T> theta2(V2<T> q) {
 template <typename T> V2<T> log_compress(V2<T> z, T w) {
   V2<T> c{clampv(z.x, (T)-1e18, (T)1e18), clampv(z.y, (T)-1e18, (T)1e18)};
   T mag = len2(c);
-  T sc = std::log((T)1 + mag) / std::max(mag, (T)1e-12);
+  T sc = std::log((T)1 + mag) / wmax(mag, (T)1e-12);
   return {w * sc * c.x, w * sc * c.y};
 }
 // chaos.wgsl:6435
@@ -2717,7 +2717,7 @@ template <typename T> V2<T> var_weierstrass_p(V2<T> p, T w, T o1r, T o1i, T o2r,
 }
 // chaos.wgsl:6500 — fractional part of 1/t, |t| floored at 1e-4.
 template <typename T> T gauss_frac(T t) {
-  T at = std::max(std::fabs(t), (T)1.0e-4);
+  T at = wmax(std::fabs(t), (T)1.0e-4);
   T r = sel((T)1, (T)-1, t < (T)0) / at;
   return r - std::floor(r);
 }
@@ -2818,14 +2818,14 @@ template <typename T> T struve_h1(T xin) {
 // [-extent, extent) (wrapped) and the cell centre of an index.
 inline uint32_t bits_mask(uint32_t bits) { return (1u << (bits & 31u)) - 1u; }
 template <typename T> uint32_t fp_encode(T c, T extent, uint32_t bits) {
-  T s = std::max(extent, (T)1.0e-4);
+  T s = wmax(extent, (T)1.0e-4);
   T levels = (T)(float)(1u << (bits & 31u));
   T norm = (c + s) / ((T)2 * s);
   T folded = norm - std::floor(norm);
   return umin(f2u(folded * levels), bits_mask(bits));
 }
 template <typename T> T fp_decode(uint32_t i, T extent, uint32_t bits) {
-  T s = std::max(extent, (T)1.0e-4);
+  T s = wmax(extent, (T)1.0e-4);
   T levels = (T)(float)(1u << (bits & 31u));
   T unit = ((T)(float)i + (T)0.5) / levels;
   return unit * (T)2 * s - s;
@@ -2855,7 +2855,7 @@ template <typename T> T legendre_P(T x, int32_t n) {
 template <typename T> T assoc_legendre(int32_t l, int32_t m, T u) {
   T pmm = 1;
   if (m > 0) {
-    T s = std::sqrt(std::max((T)1 - u * u, (T)0));
+    T s = std::sqrt(wmax((T)1 - u * u, (T)0));
     T fact = 1;
     for (int32_t i = 1; i <= m; ++i) { pmm = pmm * (-fact) * s; fact = fact + (T)2; }
   }
@@ -2903,7 +2903,7 @@ template <typename T> T assoc_laguerre(int32_t k, T alpha, T x) {
 }
 // chaos.wgsl:6773 — hydrogen radial density rho^(2l) e^-rho (L^(2l+1)_(n-l-1))^2.
 template <typename T> T radial_psi2(int32_t n, int32_t l, T rho) {
-  T rf = std::max(rho, (T)0);
+  T rf = wmax(rho, (T)0);
   T lag = assoc_laguerre(n - l - 1, (T)2 * (T)l + (T)1, rf);
   T rp = 1;
   if (l > 0)
@@ -2913,7 +2913,7 @@ template <typename T> T radial_psi2(int32_t n, int32_t l, T rho) {
 // chaos.wgsl:6796 — Hastings polynomial approximations of E(m), K(m).
 template <typename T> T elliptic_E(T m) {
   T m1 = clampv((T)1 - m, (T)0, (T)1);
-  T l = std::log((T)1 / std::max(m1, (T)1.0e-6));
+  T l = std::log((T)1 / wmax(m1, (T)1.0e-6));
   T a = (T)1 + m1 * ((T)0.4630151 + m1 * (T)0.1077812);
   T b = m1 * ((T)0.2452727 + m1 * (T)0.0412496);
   return a + b * l;
@@ -2931,8 +2931,8 @@ template <typename T> T erfinv_w(T xin) {
   const T a = (T)0.147;
   T l1 = std::log((T)1 - x * x);
   T t1 = (T)2 / (K<T>::PI * a) + (T)0.5 * l1;
-  T in = std::sqrt(std::max(t1 * t1 - l1 / a, (T)0)) - t1;
-  return signv(x) * std::sqrt(std::max(in, (T)0));
+  T in = std::sqrt(wmax(t1 * t1 - l1 / a, (T)0)) - t1;
+  return signv(x) * std::sqrt(wmax(in, (T)0));
 }
 // chaos.wgsl:6825-6870 — base-3 digit codes (Peano curve).
 inline uint32_t pow3u(uint32_t n) {
@@ -2941,14 +2941,14 @@ inline uint32_t pow3u(uint32_t n) {
   return p;
 }
 template <typename T> uint32_t tri_encode(T c, T extent, uint32_t trits) {
-  T s = std::max(extent, (T)1.0e-4);
+  T s = wmax(extent, (T)1.0e-4);
   const uint32_t cells = pow3u(trits);
   T norm = (c + s) / ((T)2 * s);
   T folded = norm - std::floor(norm);
   return umin(f2u(folded * (T)(float)cells), cells - 1u);
 }
 template <typename T> T tri_decode(uint32_t i, T extent, uint32_t trits) {
-  T s = std::max(extent, (T)1.0e-4);
+  T s = wmax(extent, (T)1.0e-4);
   T levels = (T)(float)pow3u(trits);
   return (((T)(float)i + (T)0.5) / levels) * (T)2 * s - s;
 }
@@ -3007,14 +3007,14 @@ template <typename T> V2<T> nbody_pull(V2<T> p, T cx, T cy, T m, T g, T eps) {
 // chaos.wgsl:6917
 template <typename T>
 V2<T> var_nbody_lensing(V2<T> p, T w, T c1x, T c1y, T c2x, T c2y, T m1, T m2, T g, T eps) {
-  T e = std::max(eps, (T)5.0e-3);
+  T e = wmax(eps, (T)5.0e-3);
   V2<T> a = nbody_pull(p, c1x, c1y, m1, g, e), b = nbody_pull(p, c2x, c2y, m2, g, e);
   T dx = a.x + b.x, dy = a.y + b.y;
   return {w * (p.x + dx), w * (p.y + dy)};
 }
 // chaos.wgsl:6926 — displacement along the curl of a value-noise stream function.
 template <typename T> V2<T> var_curl_noise(V2<T> p, T w, T freq, T amp) {
-  T f = std::max(freq, (T)1.0e-3);
+  T f = wmax(freq, (T)1.0e-3);
   T sx = p.x * f, sy = p.y * f;
   const T h = (T)1.0e-2;
   T dx = (value_noise2(sx + h, sy, 0) - value_noise2(sx - h, sy, 0)) / ((T)2 * h);
@@ -3026,11 +3026,11 @@ template <typename T> V2<T> var_curl_noise(V2<T> p, T w, T freq, T amp) {
 template <typename T>
 V2<T> var_lichtenberg(V2<T> p, T w, T freqin, T brin, T radin, T detin, T strin) {
   const T H = (T)1.0e-2, R_EPS = (T)1.0e-4, G_EPS = (T)1.0e-3, MAX_STEP = (T)2;
-  T freq = std::max(freqin, (T)1.0e-3);
+  T freq = wmax(freqin, (T)1.0e-3);
   T br = clampv(rint_even(brin), (T)1, (T)16);
   T rad = clampv(radin, (T)0, (T)1);
   const int32_t oct = f2i(clampv(rint_even(detin), (T)1, (T)4));
-  T str = std::max(strin, (T)0);
+  T str = wmax(strin, (T)0);
   if (len2(p) < R_EPS) return {w * p.x, w * p.y};
   T f0 = lich_field(p, freq, br, rad, oct);
   T fxp = lich_field(V2<T>{p.x + H, p.y + (T)0}, freq, br, rad, oct);
@@ -3038,7 +3038,7 @@ V2<T> var_lichtenberg(V2<T> p, T w, T freqin, T brin, T radin, T detin, T strin)
   T fyp = lich_field(V2<T>{p.x + (T)0, p.y + H}, freq, br, rad, oct);
   T fym = lich_field(V2<T>{p.x - (T)0, p.y - H}, freq, br, rad, oct);
   T gx = (fxp - fxm) / ((T)2 * H), gy = (fyp - fym) / ((T)2 * H);
-  T g2 = std::max(gx * gx + gy * gy, G_EPS);
+  T g2 = wmax(gx * gx + gy * gy, G_EPS);
   T k = -(f0 / g2);
   T stx = k * gx, sty = k * gy;
   T sl = std::sqrt(stx * stx + sty * sty);
@@ -3048,25 +3048,25 @@ V2<T> var_lichtenberg(V2<T> p, T w, T freqin, T brin, T radin, T detin, T strin)
 template <typename T> T cop_sigmoid(T t) { return (T)1 / ((T)1 + std::exp(-t)); }
 // chaos.wgsl:6981 — Gaussian copula: y replaced by its rho-correlated normal.
 template <typename T> V2<T> var_copula_gaussian(V2<T> p, T w, T strength, T rhoin) {
-  T s = std::max(strength, (T)1.0e-3);
+  T s = wmax(strength, (T)1.0e-3);
   const T hi = (T)(1.0 - 1.0e-4);
   T u = clampv(cop_sigmoid(s * p.x), (T)1.0e-4, hi);
   T v = clampv(cop_sigmoid(s * p.y), (T)1.0e-4, hi);
   T zx = (T)1.4142135 * erfinv_w((T)2 * u - (T)1);
   T zy = (T)1.4142135 * erfinv_w((T)2 * v - (T)1);
   T rho = clampv(rhoin, (T)-0.95, (T)0.95);
-  T zyp = rho * zx + std::sqrt(std::max((T)1 - rho * rho, (T)0)) * zy;
+  T zyp = rho * zx + std::sqrt(wmax((T)1 - rho * rho, (T)0)) * zy;
   return {w * p.x, w * (zyp / s)};
 }
 // chaos.wgsl:6998 — Clayton copula conditional quantile.
 template <typename T> V2<T> var_copula_clayton(V2<T> p, T w, T strength, T thin) {
-  T s = std::max(strength, (T)1.0e-3);
+  T s = wmax(strength, (T)1.0e-3);
   const T hi = (T)(1.0 - 1.0e-4);
   T u = clampv(cop_sigmoid(s * p.x), (T)1.0e-4, hi);
   T v = clampv(cop_sigmoid(s * p.y), (T)1.0e-4, hi);
   T th = clampv(thin, (T)0.05, (T)8);
   T ua = std::pow(u, -th), va = std::pow(v, -th);
-  T base = std::max(ua + va - (T)1, (T)1.0e-6);
+  T base = wmax(ua + va - (T)1, (T)1.0e-6);
   T h = std::pow(u, -th - (T)1) * std::pow(base, (T)-1 - (T)1 / th);
   T up = clampv(h, (T)1.0e-4, hi);
   T yp = clampv(std::log(up / ((T)1 - up)), (T)-12, (T)12) / s;
@@ -3074,11 +3074,11 @@ template <typename T> V2<T> var_copula_clayton(V2<T> p, T w, T strength, T thin)
 }
 // chaos.wgsl:7021 — truncated series of the disc-to-polygon map.
 template <typename T> V2<T> var_schwarz_christoffel(V2<T> p, T w, T sides) {
-  T n = std::max(rint_even(sides), (T)3);
+  T n = wmax(rint_even(sides), (T)3);
   const int32_t ni = f2i(n);
   T r = len2(p);
   V2<T> z = p;
-  if (r > (T)0.999) { T k = (T)0.999 / std::max(r, (T)1.0e-12); z = V2<T>{p.x * k, p.y * k}; }
+  if (r > (T)0.999) { T k = (T)0.999 / wmax(r, (T)1.0e-12); z = V2<T>{p.x * k, p.y * k}; }
   T a = (T)2 / n;
   T ax = 0, ay = 0, coef = 1;
   for (int32_t k = 0; k < 10; ++k) {
@@ -3093,7 +3093,7 @@ template <typename T> V2<T> var_schwarz_christoffel(V2<T> p, T w, T sides) {
 }
 // chaos.wgsl:7047 — Doyle-style log-spiral shear with a ripple.
 template <typename T> V2<T> var_doyle(V2<T> p, T w, T dp, T dq) {
-  T pp = std::max(std::fabs(dp), (T)1);
+  T pp = wmax(std::fabs(dp), (T)1);
   V2<T> L = c_log(p);
   T u = L.x, v = L.y;
   T pitch = dq / pp;
@@ -3106,7 +3106,7 @@ template <typename T> V2<T> var_doyle(V2<T> p, T w, T dp, T dq) {
 // chaos.wgsl:7070 — gradient of an n-fold sum of plane waves.
 template <typename T> V2<T> var_quasicrystal(V2<T> p, T w, T symin, T freqin) {
   const int32_t n = f2i(clampv(rint_even(symin), (T)2, (T)12));
-  T freq = std::max(freqin, (T)1.0e-3);
+  T freq = wmax(freqin, (T)1.0e-3);
   T gx = 0, gy = 0;
   for (int32_t k = 0; k < n; ++k) {
     T th = K<T>::PI * (T)k / (T)n;
@@ -3122,7 +3122,7 @@ template <typename T> V2<T> var_quasicrystal(V2<T> p, T w, T symin, T freqin) {
 }
 // chaos.wgsl:7093 — de Bruijn pentagrid cell vertex.
 template <typename T> V2<T> var_penrose(V2<T> p, T w, T scalein, T offset) {
-  T scale = std::max(scalein, (T)1.0e-3);
+  T scale = wmax(scalein, (T)1.0e-3);
   T vx = 0, vy = 0;
   for (int j = 0; j < 5; ++j) {
     T th = K<T>::TAU * (T)j / (T)5;
@@ -3135,7 +3135,7 @@ template <typename T> V2<T> var_penrose(V2<T> p, T w, T scalein, T offset) {
 }
 // chaos.wgsl:7115 — analytic interpolation of the Collatz map.
 template <typename T> V2<T> var_collatz(V2<T> p, T w, T scalein, T shift) {
-  T scale = std::max(std::fabs(scalein), (T)1.0e-3);
+  T scale = wmax(std::fabs(scalein), (T)1.0e-3);
   V2<T> z{p.x * scale + shift, p.y * scale};
   const T hp = K<T>::PI * (T)0.5;
   V2<T> arg{z.x * hp, z.y * hp};
@@ -3147,7 +3147,7 @@ template <typename T> V2<T> var_collatz(V2<T> p, T w, T scalein, T shift) {
 }
 // chaos.wgsl:7137 — digamma by recurrence to z + 6 and the asymptotic series.
 template <typename T> V2<T> var_digamma(V2<T> p, T w, T scalein, T shift) {
-  T scale = std::max(std::fabs(scalein), (T)1.0e-3);
+  T scale = wmax(std::fabs(scalein), (T)1.0e-3);
   V2<T> z{p.x * scale + shift, p.y * scale};
   V2<T> sum{(T)0, (T)0};
   for (int k = 0; k < 6; ++k) {
@@ -3273,26 +3273,26 @@ template <typename T> V2<T> var_snell_refraction(V2<T> p, T w, T nratio, T stren
   T ox, oy;
   if (std::fabs(s2) >= (T)1) { ox = dx; oy = -dy; }
   else {
-    T c2 = std::sqrt(std::max((T)1 - s2 * s2, (T)0));
+    T c2 = std::sqrt(wmax((T)1 - s2 * s2, (T)0));
     ox = s2;
     oy = sel(-c2, c2, dy >= (T)0);
   }
   T s = clampv(strength, (T)0, (T)1);
   T bx = mixv(dx, ox, s), by = mixv(dy, oy, s);
-  T bl = std::max(std::sqrt(bx * bx + by * by), (T)1.0e-6);
+  T bl = wmax(std::sqrt(bx * bx + by * by), (T)1.0e-6);
   bx = bx / bl;
   by = by / bl;
   return {w * r * bx, w * r * by};
 }
 // chaos.wgsl:7345
 template <typename T> V2<T> var_grin_lens(V2<T> p, T w, T focal, T eps) {
-  T f = std::max(std::fabs(focal), (T)0.05), e = std::max(std::fabs(eps), (T)1.0e-3);
+  T f = wmax(std::fabs(focal), (T)0.05), e = wmax(std::fabs(eps), (T)1.0e-3);
   T r = len2(p);
   if (r < (T)1.0e-6) return {w * p.x, w * p.y};
   T soft = (e * e) / (r * r + e * e);
   T pull = (r / f) * soft;
   T dx = p.x / r, dy = p.y / r;
-  T nr = std::max(r - pull, -r);
+  T nr = wmax(r - pull, -r);
   return {w * nr * dx, w * nr * dy};
 }
 // chaos.wgsl:7359
@@ -3347,7 +3347,7 @@ template <typename T> V2<T> var_radial_shell(V2<T> p, T w, T nin, T lin, T shell
   const int32_t n = imin(imax(f2i(rint_even(nin)), 1), 4);
   const int32_t l = imin(imax(f2i(rint_even(lin)), 0), n - 1);
   T r = len2(p);
-  T rho = ((T)2 * r) / ((T)n * std::max(shell, (T)1.0e-3));
+  T rho = ((T)2 * r) / ((T)n * wmax(shell, (T)1.0e-3));
   return shell_out(p, w, r, radial_psi2(n, l, rho));
 }
 // chaos.wgsl:7444
@@ -3356,7 +3356,7 @@ template <typename T> V2<T> var_hydrogen_orbital(V2<T> p, T w, T nin, T lin, T m
   const int32_t l = imin(imax(f2i(rint_even(lin)), 0), n - 1);
   const int32_t m = imin(imax(f2i(rint_even(min_)), -l), l);
   T r = len2(p);
-  T rho = ((T)2 * r) / ((T)n * std::max(shell, (T)1.0e-3));
+  T rho = ((T)2 * r) / ((T)n * wmax(shell, (T)1.0e-3));
   T rad = radial_psi2(n, l, rho);
   T yl = sph_harmonic(std::atan2(p.y, p.x), l, m);
   T amp = mixv(rad, rad * (yl * yl), clampv(lobe, (T)0, (T)1));
@@ -3372,14 +3372,14 @@ template <typename T> T cdf_u(T r2) {
 // w * rp along p / |p| ((0, 0) when |p| < 1e-9).
 template <typename T> V2<T> along(V2<T> p, T w, T r, T rp) {
   if (r < (T)1.0e-9) return {w * rp * (T)0, w * rp * (T)0};
-  T m = std::max(r, (T)1.0e-12);
+  T m = wmax(r, (T)1.0e-12);
   return {w * rp * (p.x / m), w * rp * (p.y / m)};
 }
 // chaos.wgsl:7467
 template <typename T> V2<T> var_weibull_cdf(V2<T> p, T w, T lambda, T k) {
   T r2 = dot2(p), r = std::sqrt(r2);
   T u = cdf_u(r2);
-  T kk = std::max(std::fabs(k), (T)0.05);
+  T kk = wmax(std::fabs(k), (T)0.05);
   return along(p, w, r, lambda * std::pow(-std::log((T)1 - u), (T)1 / kk));
 }
 // chaos.wgsl:7477
@@ -3387,45 +3387,45 @@ template <typename T> V2<T> var_logistic_cdf(V2<T> p, T w, T mu, T s) {
   T r2 = dot2(p), r = std::sqrt(r2);
   T u = cdf_u(r2);
   T lg = clampv(std::log(u / ((T)1 - u)), (T)-12, (T)12);
-  return along(p, w, r, std::max(mu + s * lg, (T)0));
+  return along(p, w, r, wmax(mu + s * lg, (T)0));
 }
 // chaos.wgsl:7490
 template <typename T> V2<T> var_cauchy_cdf(V2<T> p, T w, T gamma, T tail) {
   T r2 = dot2(p), r = std::sqrt(r2);
   T u = cdf_u(r2);
   T raw = gamma * safe_tan(K<T>::PI * (u - (T)0.5));
-  T cap = std::max(tail, (T)0.01);
+  T cap = wmax(tail, (T)0.01);
   return along(p, w, r, clampv(raw, -cap, cap));
 }
 // chaos.wgsl:7502
 template <typename T> V2<T> var_pareto_cdf(V2<T> p, T w, T xm, T alpha, T tail) {
   T r2 = dot2(p), r = std::sqrt(r2);
   T u = cdf_u(r2);
-  T a = std::max(std::fabs(alpha), (T)0.05);
+  T a = wmax(std::fabs(alpha), (T)0.05);
   T raw = xm * std::pow((T)1 - u, (T)-1 / a);
-  return along(p, w, r, std::min(raw, std::max(tail, (T)0.01)));
+  return along(p, w, r, wmin(raw, wmax(tail, (T)0.01)));
 }
 // Radial scale (1 + amp psi) applied to p.
 template <typename T> V2<T> scaled(V2<T> p, T w, T sc) { return {w * sc * p.x, w * sc * p.y}; }
 // chaos.wgsl:7517
 template <typename T> V2<T> var_morlet(V2<T> p, T w, T freq, T sigma, T amp) {
   T r = len2(p);
-  T t = r / std::max(sigma, (T)0.05);
-  T psi = std::exp((T)-0.5 * std::min(t * t, (T)80)) * safe_cos(freq * r);
+  T t = r / wmax(sigma, (T)0.05);
+  T psi = std::exp((T)-0.5 * wmin(t * t, (T)80)) * safe_cos(freq * r);
   return scaled(p, w, (T)1 + amp * psi);
 }
 // chaos.wgsl:7527
 template <typename T> V2<T> var_mexican_hat(V2<T> p, T w, T sigma, T amp) {
   T r = len2(p);
-  T t = r / std::max(sigma, (T)0.05);
-  T t2 = std::min(t * t, (T)80);
+  T t = r / wmax(sigma, (T)0.05);
+  T t2 = wmin(t * t, (T)80);
   T psi = ((T)1 - t2) * std::exp((T)-0.5 * t2);
   return scaled(p, w, (T)1 + amp * psi);
 }
 // chaos.wgsl:7537
 template <typename T> V2<T> var_chirp(V2<T> p, T w, T alpha, T amp, T decay) {
   T r2 = dot2(p);
-  T psi = std::exp(-decay * std::min(r2, (T)160)) * safe_sin(alpha * r2);
+  T psi = std::exp(-decay * wmin(r2, (T)160)) * safe_sin(alpha * r2);
   return scaled(p, w, (T)1 + amp * psi);
 }
 // chaos.wgsl:7548 — two Newton steps of Kepler's equation.
@@ -3436,9 +3436,9 @@ template <typename T> V2<T> var_kepler_orbit(V2<T> p, T w, T ein, T scale) {
   T E = M;
   for (int i = 0; i < 2; ++i) {
     T s = safe_sin(E), c = safe_cos(E);
-    E = E - (E - e * s - M) / std::max((T)1 - e * c, (T)0.05);
+    E = E - (E - e * s - M) / wmax((T)1 - e * c, (T)0.05);
   }
-  T b = std::sqrt(std::max((T)1 - e * e, (T)0));
+  T b = std::sqrt(wmax((T)1 - e * e, (T)0));
   return {w * (a * (safe_cos(E) - e)), w * (a * b * safe_sin(E))};
 }
 // chaos.wgsl:7567 — Euler step in the rotating-frame restricted 3-body field.
@@ -3467,14 +3467,14 @@ template <typename T> V2<T> var_hill_epicyclic(V2<T> p, T w, T kappa, T phase, T
 // chaos.wgsl:7598
 template <typename T> V2<T> var_torus_knot(V2<T> p, T w, T pp, T qq, T radius, T tube) {
   T phi = std::atan2(p.y, p.x);
-  T pf = std::max(rint_even(pp), (T)1), qf = std::max(rint_even(qq), (T)1);
+  T pf = wmax(rint_even(pp), (T)1), qf = wmax(rint_even(qq), (T)1);
   T amp = tube * clampv(len2(p), (T)0, (T)1);
   T rr = radius + amp * safe_cos(qf * phi);
   return {w * (rr * safe_cos(pf * phi)), w * (rr * safe_sin(pf * phi))};
 }
 // chaos.wgsl:7612 — alternate angular lanes twisted in opposite senses.
 template <typename T> V2<T> var_braid_warp(V2<T> p, T w, T strands, T twist, T crossings) {
-  T n = std::max(rint_even(strands), (T)2), cr = std::max(rint_even(crossings), (T)1);
+  T n = wmax(rint_even(strands), (T)2), cr = wmax(rint_even(crossings), (T)1);
   T r = len2(p);
   T th = std::atan2(p.y, p.x);
   T a = (th + K<T>::PI) / K<T>::TAU;
@@ -3498,27 +3498,27 @@ template <typename T> V2<T> var_elliptic_E(V2<T> p, T w, T scale) {
   T e = elliptic_E(r2 / ((T)1 + r2));
   T k = scale * e;
   if (r < (T)1.0e-9) return {w * k * (T)0, w * k * (T)0};
-  T m = std::max(r, (T)1.0e-12);
+  T m = wmax(r, (T)1.0e-12);
   return {w * k * (p.x / m), w * k * (p.y / m)};
 }
 // chaos.wgsl:7661
 template <typename T> V2<T> var_elliptic_K(V2<T> p, T w, T scale, T tail) {
   T r2 = dot2(p), r = std::sqrt(r2);
   T k = elliptic_K(r2 / ((T)1 + r2));
-  return along(p, w, r, std::min(scale * k, std::max(tail, (T)0.01)));
+  return along(p, w, r, wmin(scale * k, wmax(tail, (T)0.01)));
 }
 // chaos.wgsl:7676
 template <typename T> V2<T> var_gaussian_cdf(V2<T> p, T w, T mu, T sigma) {
   T r2 = dot2(p), r = std::sqrt(r2);
   T q = erfinv_w((T)2 * cdf_u(r2) - (T)1);
-  return along(p, w, r, std::max(mu + sigma * (T)1.4142135 * q, (T)0));
+  return along(p, w, r, wmax(mu + sigma * (T)1.4142135 * q, (T)0));
 }
 // chaos.wgsl:7689
 template <typename T> V2<T> var_levy_cdf(V2<T> p, T w, T c, T tail) {
   T r2 = dot2(p), r = std::sqrt(r2);
   T e = erfinv_w((T)1 - cdf_u(r2));
-  T raw = std::max(c, (T)1.0e-4) / std::max((T)2 * e * e, (T)1.0e-6);
-  return along(p, w, r, std::min(raw, std::max(tail, (T)0.01)));
+  T raw = wmax(c, (T)1.0e-4) / wmax((T)2 * e * e, (T)1.0e-6);
+  return along(p, w, r, wmin(raw, wmax(tail, (T)0.01)));
 }
 // chaos.wgsl:7707
 template <typename T> V2<T> var_peano(V2<T> p, T w, T extent, T trits_f) {
@@ -3793,12 +3793,12 @@ inline RGB dc_linear_color(V2<float> p) {
 }
 // chaos.wgsl:1721
 inline RGB dc_perlin_color(V2<float> p, float scale, float octaves, float seed) {
-  float n = perlin_fbm(p.x, p.y, octaves, std::max(scale, 1e-6f));
+  float n = perlin_fbm(p.x, p.y, octaves, wmax(scale, 1e-6f));
   return hsl_rgb(fract(0.5f + 0.5f * n + seed), 1.0f, 0.55f);
 }
 // chaos.wgsl:1732 — hashed colour per grid cell.
 inline RGB dc_gridout_color(V2<float> p, float cells) {
-  float n = std::max(cells, 1.0f);
+  float n = wmax(cells, 1.0f);
   const int32_t cx = f2i(std::floor(p.x * n)), cy = f2i(std::floor(p.y * n));
   const uint32_t mixed = (((uint32_t)cx * 73856093u) ^ ((uint32_t)cy * 19349663u)) + 0x9e3779b9u;
   return {hash01(mixed), hash01(mixed ^ 0xdeadbeefu), hash01(mixed ^ 0x13579bdfu)};
@@ -3850,8 +3850,8 @@ inline RGB escape_color(V2<float> p, int idx, float cx, float cy, float relax) {
     it = i + 1;
     if (z.x * z.x + z.y * z.y > 256.0f) break;
   }
-  float r = std::max(len2(z), 1.0001f);
-  float mu = (float)it + 1.0f - std::log2(std::max(std::log(r), 1.0e-6f));
+  float r = wmax(len2(z), 1.0001f);
+  float mu = (float)it + 1.0f - std::log2(wmax(std::log(r), 1.0e-6f));
   return hsl_rgb(fract(mu * 0.12f), 1.0f, 0.55f);
 }
 // chaos.wgsl:8192 — colour override of one slot with parameters p0, p1, p2.

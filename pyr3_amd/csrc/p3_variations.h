@@ -15,9 +15,10 @@ namespace p3 {
 
 P3_INLINE float dot2(float2 p) { return p.x * p.x + p.y * p.y; }
 P3_INLINE float len2(float2 p) { return sqrtf(p.x * p.x + p.y * p.y); }
-// C++ std::max/std::min semantics (a NaN first argument propagates).
-P3_INLINE float smax(float a, float b) { return a < b ? b : a; }
-P3_INLINE float smin(float a, float b) { return b < a ? b : a; }
+// WGSL max / min (clamp = min(max(x, lo), hi)): a NaN operand yields the other
+// operand; a tie of +0 and -0 returns the first (max) or first (min) argument.
+P3_INLINE float smax(float a, float b) { return isnan(a) ? b : (a < b ? b : a); }
+P3_INLINE float smin(float a, float b) { return isnan(a) ? b : (b < a ? b : a); }
 P3_INLINE float sel(float f, float t, bool c) { return c ? t : f; }  // WGSL select
 P3_INLINE float wmod(float x, float y) { return x - y * truncf(x / y); }  // WGSL f32 %
 P3_INLINE int32_t f2i(float f) {  // WGSL i32(f): saturating, NaN -> 0

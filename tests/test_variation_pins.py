@@ -1,0 +1,1761 @@
+"""Value pins of catalog entries V99..V325 from the reference's own tests (CPU).
+
+The reference checks these entries in its per-variation GPU tests
+(src/issue*.gpu.test.ts, src/post-flam3-batch*.gpu.test.ts, ...): the WGSL
+function runs on a handful of points with explicit parameters and its f32
+output is compared with an f64 oracle written in the test file, at the
+test's toBeCloseTo precision (|a - b| < 10^-d / 2). Each pin below restates
+one such test oracle in Python (reference file:line cited), evaluates it at
+the test's points with the test's weight and parameters, and holds
+
+  * the oracle's f64 instantiation to the restated test oracle within
+    1e-9 relative (two transcriptions of one formula, from different
+    sources: a slip in either shows), and
+  * the oracle's f32 instantiation, the arithmetic the kernels run, to the
+    test's own precision (the reference's GPU bar).
+
+Parameters go in the WGSL function's argument order after the weight, the
+order the oracle's apply_variation passes a record's parameter words in.
+Where the reference test of an entry is a finiteness smoke test only (most
+randomised entries), the pin restates that test (finite output at its points
+and parameters for any RNG words) and adds the special cases the mathematics
+fixes.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from pyr3_amd import catalog as cat
+
+V = cat.VARIATION_INDEX
+PI = math.pi
+# TEST_POINTS shared by most per-variation files (e.g.
+# src/post-flam3-batch1.gpu.test.ts:52-61)
+P8 = [(0.5, 0.3), (-0.5, 0.7), (1.2, -0.4), (-1.0, -1.0), (0.1, 0.05), (2.0, 0.0),
+      (0.0, 2.0), (-2.5, 1.7)]
+# src/post-flam3-batch2b-b.gpu.test.ts:44-53 (off-axis variant)
+P8B = [(0.5, 0.3), (-0.5, 0.7), (1.2, -0.4), (-1.0, -1.0), (0.1, 0.05), (2.0, 0.3),
+       (0.3, 2.0), (-2.5, 1.7)]
+WORDS = [int(v) for v in np.random.default_rng(77).integers(0, 2 ** 32, 256, dtype=np.uint64)]
+
+
+def f32(v):
+    return float(np.float32(v))
+
+
+def run(orc, name, x, y, w, params, dtype=np.float64, words=()):
+    return orc.variation(V[name], x, y, w, list(params), rng_words=words, dtype=dtype)
+
+
+def check_pin(orc, name, oracle, w, params, points, digits, skip=None, f64_rel=1e-9, f32=True):
+    """The test's bar on the f32 oracle and a tight f64 agreement (f64 only
+    with f32=False: points where the f32 result is ill-conditioned)."""
+    tol = 0.5 * 10.0 ** -digits
+    n = 0
+    for x, y in points:
+        if skip is not None and skip(x, y):
+            continue
+        want = oracle(x, y, w, *params)
+        g64 = run(orc, name, x, y, w, params)
+        for k in range(2):
+            assert abs(g64[k] - want[k]) <= f64_rel * max(1.0, abs(want[k])), \
+                (name, x, y, k, g64[k], want[k])
+        if f32:
+            g32 = run(orc, name, np.float32(x).item(), np.float32(y).item(), w, params,
+                      dtype=np.float32)
+            for k in range(2):
+                assert abs(g32[k] - want[k]) < tol, (name, "f32", x, y, k, g32[k], want[k])
+        n += 1
+    assert n > 0
+    return n
+
+
+def check_finite(orc, name, w, params, points, min_finite=None):
+    """The reference's smoke bar: finite output at its points for its
+    parameters, here for several RNG streams."""
+    for s in range(4):
+        words = WORDS[s * 32:(s + 1) * 32]
+        fin = 0
+        for x, y in points:
+            o = run(orc, name, f32(x), f32(y), w, params, dtype=np.float32, words=words)
+            fin += int(np.isfinite(o[0])) + int(np.isfinite(o[1]))
+        need = 2 * len(points) if min_finite is None else min_finite
+        assert fin >= need, (name, s, fin)
+
+
+# ---------------------------------------------------------------- bubble2 --
+def bubble2_ref(x, y, w, xs, ys):   # src/issue120-b2-bubble2.gpu.test.ts:32-35
+    r = w / (0.25 * (x * x + y * y) + 1.0)
+    return x * r * xs, y * r * ys
+
+
+def test_bubble2(orc):
+    # :87-101 default (1, 1), :103-117 anisotropic (2.0, 0.5), toBeCloseTo 5
+    check_pin(orc, "bubble2", bubble2_ref, 1.0, (1.0, 1.0), P8, 5)
+    check_pin(orc, "bubble2", bubble2_ref, 1.0, (2.0, 0.5), P8, 5)
+    # :119-136 finite at (-1.5, 1.7); (1, 1) equals bubble (V20)
+    check_finite(orc, "bubble2", 1.0, (-1.5, 1.7), P8)
+    for x, y in P8:
+        assert run(orc, "bubble2", x, y, 1.0, (1.0, 1.0))[:2] == \
+            pytest.approx(orc.variation(V["bubble"], x, y, 1.0)[:2], rel=1e-12)
+
+
+# -------------------------------------------------------- post-flam3 batch 1
+def loonie2_ref(x, y, w, sides_f, star, circle):   # src/post-flam3-batch1.gpu.test.ts:132-163
+    sides = max(1, min(16, int(math.trunc(sides_f))))
+    a = 2 * PI / sides
+    sa, ca = math.sin(a), math.cos(a)
+    ss, cs = math.sin(-star * PI * 0.5), math.cos(star * PI * 0.5)
+    sc, cc = math.sin(circle * PI * 0.5), math.cos(circle * PI * 0.5)
+    vv = w * w
+    xr, yr = x, y
+    r2 = xr * cs + abs(yr) * ss
+    cr = math.sqrt(x * x + y * y)
+    for _ in range(min(sides - 1, 16)):
+        xr, yr = xr * ca - yr * sa, xr * sa + yr * ca
+        r2 = max(r2, xr * cs + abs(yr) * ss)
+    r2 = r2 * cc + cr * sc
+    r2 = r2 * r2 if sides > 2 else abs(r2) * r2
+    if 0 < r2 < vv:
+        r = w * math.sqrt(abs(vv / r2 - 1))
+        return r * x, r * y
+    if r2 < 0:
+        r = w / math.sqrt(abs(vv / r2) - 1)
+        return r * x, r * y
+    return w * x, w * y
+
+
+def test_loonie2(orc):
+    check_pin(orc, "loonie2", loonie2_ref, 1.0, (4.0, 0.15, 0.25), P8, 3)   # :166-182
+    check_finite(orc, "loonie2", 0.8, (2.0, 0.2, 0.3), P8)                  # :184-196
+
+
+def epispiral_ref(x, y, w, n, holes):   # src/post-flam3-batch1.gpu.test.ts:203-209
+    th = math.atan2(y, x)
+    d = math.cos(n * th)
+    t = -holes + 1 / d
+    return w * t * math.cos(th), w * t * math.sin(th)
+
+
+def test_epispiral(orc):
+    # thickness 0 draws nothing (:212-232, toBeCloseTo 2, near-pole points skipped)
+    def pole(x, y):
+        return abs(math.cos(6.0 * math.atan2(y, x))) < 1e-3
+
+    params = (6.0, 0.0, 1.0)
+    for x, y in P8:
+        if not pole(x, y):
+            want = epispiral_ref(x, y, 1.0, 6.0, 1.0)
+            g = run(orc, "epispiral", x, y, 1.0, params, words=WORDS)
+            assert g[:2] == pytest.approx(want, rel=1e-9, abs=1e-9)
+            g32 = run(orc, "epispiral", f32(x), f32(y), 1.0, params, np.float32, WORDS)
+            assert abs(g32[0] - want[0]) < 5e-3 and abs(g32[1] - want[1]) < 5e-3
+    check_finite(orc, "epispiral", 1.0, (4.0, 0.5, 1.0), P8, min_finite=len(P8) + 1)   # :234-250
+
+
+def test_cpow2_cpow3_smoke_and_identity(orc):
+    # src/post-flam3-batch1.gpu.test.ts:257-292: finite at the defaults
+    check_finite(orc, "cpow2", 1.0, (1.0, 0.0, 1.0, 1.0), P8, min_finite=2 * len(P8) - 4)
+    check_finite(orc, "cpow3", 1.0, (1.0, 1.0, 1.0, 1.0), P8, min_finite=2 * len(P8) - 4)
+
+
+# ------------------------------------------------------- post-flam3 batch 2b-a
+def loonie3_ref(x, y, w):   # src/post-flam3-batch2b-a.gpu.test.ts:132-146
+    vv = w * w
+    r2 = 2 * vv
+    if x > 1e-30:
+        q = (x * x + y * y) / x
+        r2 = q * q
+    if r2 < vv:
+        r = w * math.sqrt(vv / r2 - 1)
+        return r * x, r * y
+    return w * x, w * y
+
+
+def test_loonie3(orc):
+    check_pin(orc, "loonie3", loonie3_ref, 1.0, (), P8, 3)   # :148-165
+    check_finite(orc, "loonie3", 0.7, (), P8)                # :167-179
+
+
+def test_batch2b_a_rng_smoke(orc):
+    # src/post-flam3-batch2b-a.gpu.test.ts:186-317 (finite-output smoke)
+    check_finite(orc, "juliaq", 1.0, (3.0, 2.0), P8, min_finite=2 * len(P8) - 4)
+    check_finite(orc, "glynnia", 1.0, (), P8)
+    check_finite(orc, "falloff", 1.0, (1.0, 0.5, 1.0, 1.0, 0.0, 0.0), P8)
+    for typ in (0.0, 1.0, 2.0):
+        check_finite(orc, "falloff2", 1.0, (1.0, typ, 1.0, 1.0, 0.0, 0.0, 0.5), P8)
+    for inv in (0.0, 1.0):
+        check_finite(orc, "falloff3", 1.0, (1.0, 1.0, 1.0, 0.0, 0.0, 0.5, inv), P8)
+
+
+# ------------------------------------------------------- post-flam3 batch 2b-b
+def collideoscope_ref(x, y, w, a_param, num_raw):   # src/post-flam3-batch2b-b.gpu.test.ts:111-135
+    num = max(1, int(math.trunc(num_raw)))
+    kn_pi, pi_kn = num / PI, PI / num
+    ka_kn = PI * a_param / num
+    a = math.atan2(y, x)
+    r = w * math.sqrt(x * x + y * y)
+    fm = math.fmod   # JS % keeps the dividend's sign, as fmod
+    if a >= 0.0:
+        alt = int(math.trunc(a * kn_pi))
+        a = alt * pi_kn + (fm(ka_kn + a, pi_kn) if alt % 2 == 0 else fm(-ka_kn + a, pi_kn))
+    else:
+        alt = int(math.trunc(-a * kn_pi))
+        a = -(alt * pi_kn + (fm(-ka_kn - a, pi_kn) if alt % 2 != 0 else fm(ka_kn - a, pi_kn)))
+    return r * math.cos(a), r * math.sin(a)
+
+
+def _circ_perimeter(x, y):
+    ax, ay = abs(x), abs(y)
+    if ax >= ay:
+        return (ax + y if x >= ay else 5.0 * ax - y), ax
+    return (3.0 * ay - x if y >= ax else 7.0 * ay + x), ay
+
+
+def circlize_ref(x, y, w, hole):   # :175-200
+    per, side = _circ_perimeter(x, y)
+    if side == 0:
+        return 0.0, 0.0
+    r = w / (PI / 4) * side + hole
+    a = (PI / 4) * per / side - PI / 4
+    return r * math.cos(a), r * math.sin(a)
+
+
+def circlize2_ref(x, y, w, hole):   # :244-268
+    per, side = _circ_perimeter(x, y)
+    if side == 0:
+        return 0.0, 0.0
+    r = w * (side + hole)
+    a = (PI / 4) * per / side - PI / 4
+    return r * math.cos(a), r * math.sin(a)
+
+
+def eswirl_ref(x, y, w, in_p, out_p):   # :314-332
+    tmp, tmp2 = y * y + x * x + 1.0, 2.0 * x
+    r1 = math.sqrt(tmp + tmp2) if tmp + tmp2 > 0 else 0.0
+    r2 = math.sqrt(tmp - tmp2) if tmp - tmp2 > 0 else 0.0
+    xmax = max((r1 + r2) * 0.5, 1.0)
+    mu = math.acosh(xmax)
+    t = min(max(x / xmax, -1.0), 1.0)
+    nu = math.acos(t)
+    if y < 0:
+        nu = -nu
+    nw = nu + mu * out_p + in_p / (mu if mu != 0 else 1e-30)
+    return w * math.cosh(mu) * math.cos(nw), w * math.sinh(mu) * math.sin(nw)
+
+
+def petal_ref(x, y, w):   # :382-389
+    a = math.cos(x)
+    bx = (math.cos(x) * math.cos(y)) ** 3
+    by = (math.sin(x) * math.cos(y)) ** 3
+    return w * a * bx, w * a * by
+
+
+def test_batch2b_b(orc):
+    src = "post-flam3-batch2b-b"
+    check_pin(orc, "collideoscope", collideoscope_ref, 1.0, (0.20, 1.0), P8B, 2)   # :138-154
+    check_finite(orc, "collideoscope", 1.0, (0.5, 5.0), P8B)                        # :156-168
+    check_pin(orc, "circlize", circlize_ref, 1.0, (0.40,), P8B, 3)                 # :203-219
+    check_pin(orc, "circlize", circlize_ref, 0.8, (0.0,), P8B, 3)                  # :221-237
+    check_pin(orc, "circlize2", circlize2_ref, 1.0, (0.0,), P8B, 3)                # :271-287
+    check_pin(orc, "circlize2", circlize2_ref, 1.0, (0.5,), P8B, 3)                # :289-305
+    # :335-361: parity where x^2 + y^2 > 0.5 (clear of the xmax = 1 cusp), 1e-2
+    check_pin(orc, "eswirl", eswirl_ref, 1.0, (1.2, 0.2), P8B, 2,
+              skip=lambda x, y: x * x + y * y <= 0.5)
+    check_finite(orc, "eswirl", 1.0, (1.2, 0.0), P8B)                              # :363-375
+    check_pin(orc, "petal", petal_ref, 1.0, (), P8B, 3)                            # :392-408
+    check_finite(orc, "petal", 0.5, (), P8B)                                       # :410-422
+    assert src
+
+
+# ------------------------------------------------------- post-flam3 batch 2b-c
+HALF_WORD = 134217727   # rand01 of this draw is 0.5 to 2e-9 (the test stubs rand01 = 0.5)
+
+
+def bcircle_ref(x, y, w, scale):   # src/post-flam3-batch2b-c.gpu.test.ts:125-132
+    if x == 0 and y == 0:
+        return 0.0, 0.0
+    xs, ys = x * scale, y * scale
+    return (w * xs, w * ys) if math.sqrt(xs * xs + ys * ys) <= 1.0 else (0.0, 0.0)
+
+
+def curl2_ref(x, y, w, c1, c2, c3):   # :196-208
+    re = c3 * x ** 3 - 3 * c3 * x * y * y + c2 * x * x - c2 * y * y + c1 * x + 1.0
+    im = 3 * c3 * x * x * y - c3 * y ** 3 + 2 * c2 * x * y + c1 * y
+    r = w / (re * re + im * im)
+    return (x * re + y * im) * r, (y * re - x * im) * r
+
+
+def murl_ref(x, y, w, c_in, power_in):   # :258-271
+    power = math.trunc(power_in)
+    c = c_in / (power - 1) if power != 1 else c_in
+    vp = w * (c + 1)
+    a = math.atan2(y, x) * power
+    r = c * (x * x + y * y) ** (power / 2.0)
+    re, im = r * math.cos(a) + 1, r * math.sin(a)
+    r1 = vp / (re * re + im * im + 1e-29)
+    return r1 * (x * re + y * im), r1 * (y * re - x * im)
+
+
+def stwins_ref(x, y, w, distort):   # :318-331
+    sx, sy = x * w * 0.05, y * w * 0.05
+    num = (sx * sx - sy * sy) * math.sin(2 * PI * distort * (sx + sy))
+    den = sx * sx + sy * sy
+    res = num / (den if den != 0 else 1.0)
+    return w * x + res, w * y + res
+
+
+def hexes_ref(x, y, w, cell, power, rotate, scale):   # :378-452
+    if cell == 0:
+        return 0.0, 0.0
+    s3 = math.sqrt(3.0)
+    ah, bh, ch, dh = 1 / 3, s3 / 3, -1 / 3, s3 / 3
+    ac, bc, cc_, dc = 1.5, -1.5, s3 / 2, s3 / 2
+    rs, rc = math.sin(rotate * 2 * PI), math.cos(rotate * 2 * PI)
+    hx0 = math.floor((ah * x + bh * y) / cell)
+    hy0 = math.floor((ch * x + dh * y) / cell)
+    best, q = math.inf, 0
+    for di in (-1, 0, 1):
+        for dj in (-1, 0, 1):
+            cx = (ac * (hx0 + di) + bc * (hy0 + dj)) * cell
+            cy = (cc_ * (hx0 + di) + dc * (hy0 + dj)) * cell
+            d2 = (cx - x) ** 2 + (cy - y) ** 2
+            if d2 < best:
+                best, q = d2, (di + 1) * 3 + (dj + 1)
+    hx, hy = hx0 + q // 3 - 1, hy0 + q % 3 - 1
+
+    def ctr(i, j):
+        return (ac * i + bc * j) * cell, (cc_ * i + dc * j) * cell
+    p0 = ctr(hx, hy)
+    nbrs = [ctr(hx, hy + 1), ctr(hx + 1, hy + 1), ctr(hx + 1, hy), ctr(hx, hy - 1),
+            ctr(hx - 1, hy - 1), ctr(hx - 1, hy)]
+
+    def vor(ux, uy):
+        m = -1.0e20
+        for px, py in nbrs:
+            dx, dy = px - p0[0], py - p0[1]
+            if dx == 0 and dy == 0:
+                m = max(m, 1.0)
+                continue
+            m = max(m, 2.0 * ((ux - p0[0]) * dx + (uy - p0[1]) * dy) / (dx * dx + dy * dy))
+        return m
+    l1 = vor(x, y)
+    dxo, dyo = x - p0[0], y - p0[1]
+    trg = (l1 + 1e-30) ** power * scale
+    vx0, vy0 = dxo * rc + dyo * rs, -dxo * rs + dyo * rc
+    l2 = vor(vx0 + p0[0], vy0 + p0[1])
+    L = max(l1, l2)
+    if L < 0.5:
+        R = trg / l1
+    elif L > 0.8:
+        R = trg / l2
+    else:
+        R = ((trg / l1) * (0.8 - L) + (trg / l2) * (L - 0.5)) / 0.3
+    return w * (vx0 * R + p0[0]), w * (vy0 * R + p0[1])
+
+
+def test_batch2b_c(orc):
+    half = [HALF_WORD] * 8
+
+    def bc(x, y, w, scale, border):
+        return bcircle_ref(x, y, w, scale)
+    # :135-172 borderwidth 0 (disk path), scale 1 and 0.5, toBeCloseTo 3
+    for scale in (1.0, 0.5):
+        for x, y in P8B:
+            want = bcircle_ref(x, y, 1.0, scale)
+            got = run(orc, "bcircle", f32(x), f32(y), 1.0, (scale, 0.0), np.float32, half)
+            assert abs(got[0] - want[0]) < 5e-4 and abs(got[1] - want[1]) < 5e-4, (x, y, scale)
+    check_finite(orc, "bcircle", 1.0, (1.0, 0.5), P8B)                            # :174-189
+    check_pin(orc, "curl2", curl2_ref, 1.0, (1.0, 0.0, 0.0), P8B, 3)               # :211-229
+    check_pin(orc, "curl2", curl2_ref, 1.0, (0.5, 0.2, 0.1), P8B, 2)               # :231-251
+    check_pin(orc, "murl", murl_ref, 1.0, (0.1, 1.0), P8B, 3)                      # :274-290
+    check_pin(orc, "murl", murl_ref, 1.0, (0.5, 3.0), P8B, 1)                      # :292-311
+    check_pin(orc, "stwins", stwins_ref, 1.0, (1.0,), P8B, 3)                      # :334-350
+    check_pin(orc, "stwins", stwins_ref, 1.0, (0.5,), P8B, 3)                      # :352-368
+    check_pin(orc, "hexes", hexes_ref, 1.0, (1.0, 1.0, 0.166, 1.0), P8B, 2)        # :455-480
+    check_finite(orc, "hexes", 1.0, (0.5, 2.0, 0.166, 1.0), P8B)                   # :482-498
+
+
+# ------------------------------------------------------- post-flam3 batch 2b-d
+def xheart_ref(x, y, w, angle, ratio):   # src/post-flam3-batch2b-d.gpu.test.ts:121-135
+    ang = PI / 4 + 0.5 * (PI / 4) * angle
+    ca, sa = math.cos(ang), math.sin(ang)
+    rat = 6 + 2 * ratio
+    r2 = x * x + y * y + 4
+    if r2 == 0:
+        r2 = 1
+    bx, by = 4 / r2, rat / r2
+    xr = ca * (bx * x) - sa * (by * y)
+    yr = sa * (bx * x) + ca * (by * y)
+    return (w * xr, w * yr) if xr > 0 else (w * xr, -w * yr)
+
+
+def xhyperbol_ref(x, y, w, m00, m01, m10, m11, m20, m21):   # :180-195
+    r = 1 / (x * x + y * y + 1e-10)
+    xi, yi = x * r, y * r
+    re, im = m00 * xi + m01 * yi + m20, m10 * xi + m11 * yi + m21
+    al = math.atan2(im, re) + 2 * PI
+    rsq = re * re + im * im
+    xo, yo = rsq * math.cos(al), rsq * math.sin(al)
+    ri = w / (xo * xo + yo * yo + 1e-10)
+    return xo * ri, yo * ri
+
+
+def xcurl2_ref(x, y, w, c1, c2, c3):   # :245-254
+    re = 1 + c1 * x + c2 * (x * x - y * y) + c3 * (x ** 3 - 3 * x)
+    im = c1 * y + c2 * (2 * x * y) + c3 * (3 * x * y - 1)
+    r = w / (re * re + im * im)
+    return (x * re + y * im) * r, (y * re + x * im) * r
+
+
+def xyrus_gridout_ref(x, y, w):   # :342-361
+    rx = math.floor(x + 0.5) if x >= 0 else math.ceil(x - 0.5)
+    ry = math.floor(y + 0.5) if y >= 0 else math.ceil(y - 0.5)
+    dx = dy = 0
+    if ry <= 0:
+        if rx > 0:
+            if -ry >= rx:
+                dx = 1
+            else:
+                dy = 1
+        elif ry <= rx:
+            dx = 1
+        else:
+            dy = -1
+    elif rx > 0:
+        if ry >= rx:
+            dx = -1
+        else:
+            dy = 1
+    elif ry > -rx:
+        dx = -1
+    else:
+        dy = -1
+    return w * (x + dx), w * (y + dy)
+
+
+def test_batch2b_d(orc):
+    check_pin(orc, "xheart", xheart_ref, 1.0, (0.0, 0.0), P8B, 3)                  # :138-154
+    check_pin(orc, "xheart", xheart_ref, 1.0, (1.0, 0.5), P8B, 3)                  # :156-172
+    check_pin(orc, "xhyperbol", xhyperbol_ref, 1.0, (1, 0, 0, 1, 0, 0), P8B, 2)    # :198-216
+    check_pin(orc, "xhyperbol", xhyperbol_ref, 1.0, (0.6, -0.8, 0.8, 0.6, 0, 0), P8B, 2)   # :218-236
+    check_pin(orc, "xcurl2", xcurl2_ref, 1.0, (1.0, 0.0, 0.0), P8B, 3)             # :257-273
+    check_pin(orc, "xcurl2", xcurl2_ref, 1.0, (0.3, 0.2, 0.1), P8B, 2)             # :275-294
+    for prm in ((2.0, 1.0, 1.0, 0.5, 1.0, 1.0), (3.0, 1.0, 1.0, 0.3, 1.0, 1.0)):   # :304-335
+        for s in range(4):
+            for x, y in P8B:
+                o = run(orc, "xtrb", f32(x), f32(y), 1.0, prm, np.float32, WORDS[32 * s:])
+                assert not (math.isnan(o[0]) or math.isnan(o[1])), (prm, x, y)
+    # :364-390: exact up to a whole-cell offset at the rounding boundary
+    for x, y in P8B:
+        want = xyrus_gridout_ref(x, y, 1.0)
+        assert run(orc, "gridout", x, y, 1.0, ())[:2] == pytest.approx(want, abs=1e-12)
+        g = run(orc, "gridout", f32(x), f32(y), 1.0, (), np.float32)
+        for k in range(2):
+            d = abs(g[k] - want[k])
+            assert d < 1e-3 or abs(d - 1.0) < 1e-3
+    check_finite(orc, "blur_circle", 1.0, (0.0,), P8B)                             # :397-418
+    check_finite(orc, "blur_circle", 1.0, (0.3,), P8B)                             # :420-433
+
+
+# ------------------------------------------------------------ issue 120 ----
+def bipolar2_ref(x, y, w, shift, a, b, c, d, e, f1, g1, h):   # src/issue120-b1-bipolar2.gpu.test.ts:35-56
+    hp = PI * 0.5
+    x2y2 = (x * x + y * y) * g1
+    t = x2y2 + a
+    x2 = b * x
+    yv = c * math.atan2(e * y, x2y2 - d) - hp * shift
+    if yv > hp:
+        yv = -hp + math.fmod(yv + hp, PI)
+    elif yv < -hp:
+        yv = hp - math.fmod(hp - yv, PI)
+    fn, gn = t + x2, t - x2
+    if gn == 0 or fn / gn <= 0:
+        return 0.0, 0.0
+    return w * f1 * (2 / PI) * math.log(fn / gn), w * (2 / PI) * yv * h
+
+
+def test_bipolar2(orc):
+    prm = (0.0, 1.0, 2.0, 0.5, 1.0, 2.0, 0.25, 1.0, 1.0)
+    check_pin(orc, "bipolar2", bipolar2_ref, 1.0, prm, P8, 3)                       # :106-134
+    assert run(orc, "bipolar2", 1.0, 0.1, 1.0, (0.0, -3.0) + prm[2:])[:2] == (0.0, 0.0)   # :136-157
+    check_finite(orc, "bipolar2", 1.0, (0.5, 0.7, 1.5, 0.3, 0.8, 1.2, 0.4, 1.1, 0.9), P8)   # :159-173
+
+
+def test_inverse_hyperbolics(orc):
+    # src/issue120-b3-inverse-hyperbolic.gpu.test.ts:141-161: arcsinh at (0.5, 0.3)
+    x, y = 0.5, 0.3
+    zr, zi = x * x - y * y + 1.0, 2 * x * y
+    rad = math.hypot(zr, zi)
+    sr = math.sqrt(max(0.5 * (rad + zr), 0.0))
+    si = (-1 if zi < 0 else 1) * math.sqrt(max(0.5 * (rad - zr), 0.0))
+    ur, ui = x + sr, y + si
+    want = (0.5 * math.log(ur * ur + ui * ui + 1e-20) * 2 / PI, math.atan2(ui, ur) * 2 / PI)
+    assert run(orc, "arcsinh", x, y, 1.0, ())[:2] == pytest.approx(want, rel=1e-9)
+    g = run(orc, "arcsinh", x, y, 1.0, (), np.float32)
+    assert abs(g[0] - want[0]) < 5e-4 and abs(g[1] - want[1]) < 5e-4
+    # :135-189 finite output of all six on the points (acosh / acosech draw)
+    for name in ("acosh", "arcsinh", "arctanh", "acoth", "acosech", "arcsech2"):
+        check_finite(orc, name, 1.0, (), P8)
+
+
+def cell2_ref(x, y, w, size, a, snx, sny, ssx, ssy):   # src/issue120-b35-cell2.gpu.test.ts:32-52
+    s = 1e-30 if abs(size) < 1e-30 else size
+    inv = a / s
+    cx, cy = math.floor(x * inv), math.floor(y * inv)
+    dx, dy = x - cx * s, y - cy * s
+    if cy >= 0:
+        sx, sy = cx * snx, cy * sny
+    else:
+        sx, sy = cx * ssx, -ssy * cy
+    return w * (dx + sx * s), -w * (dy + sy * s)
+
+
+def test_cell2(orc):
+    check_pin(orc, "cell2", cell2_ref, 1.0, (0.6, 1.0, 2.0, 2.0, 2.0, 2.0), P8, 4)   # :102-123
+    check_pin(orc, "cell2", cell2_ref, 1.0, (0.5, 1.2, 1.7, 3.0, 0.8, 1.2), P8, 4)   # :125-146
+    check_finite(orc, "cell2", 1.0, (0.0, 1.0, 2.0, 2.0, 2.0, 2.0), P8)              # :148-164
+
+
+def curl_sp_ref(x, y, w, pw, c1, c2, sx, sy):   # src/issue120-b4-xyrus-lukout.gpu.test.ts:129-148 (pow 1)
+    xp = abs(x) ** pw * math.copysign(1.0, x) if x != 0 else 0.0
+    yp = abs(y) ** pw * math.copysign(1.0, y) if y != 0 else 0.0
+    d = xp * xp - yp * yp
+    s1 = c1 * xp + c2 * d
+    re = abs(s1) * (1 if s1 > 0 else -1) + 1.0
+    s2 = c1 * yp + 2.0 * c2 * xp * yp
+    im = abs(s2) * (1 if s2 > 0 else -1)
+    r = 1.0 / max(abs(re * re + im * im), 1e-30)
+    return (xp * re + yp * im) * r * w, (yp * re - xp * im) * r * w
+
+
+def murl2_ref(x, y, w, c, pw):   # :155-175
+    p2, invp = pw * 0.5, 1.0 / pw
+    cp1 = c + 1.0
+    vp = abs(cp1) ** (2.0 * invp) * (1 if cp1 >= 0 else -1)
+    a1 = math.atan2(y, x) * pw
+    r0 = c * abs(x * x + y * y) ** p2
+    re0, im0 = r0 * math.cos(a1) + 1.0, r0 * math.sin(a1)
+    r1 = abs(re0 * re0 + im0 * im0) ** invp
+    a2 = math.atan2(im0, re0) * 2.0 * invp
+    re1, im1 = r1 * math.cos(a2), r1 * math.sin(a2)
+    rl = vp / max(r1 * r1, 1e-30)
+    return w * rl * (x * re1 + y * im1), w * rl * (y * re1 - x * im1)
+
+
+def test_xyrus_lukout(orc):
+    check_pin(orc, "curl_sp", curl_sp_ref, 1.0, (1.0, -0.01, 0.03, 0.0, 0.0), P8, 3)   # :125-149
+    check_pin(orc, "murl2", murl2_ref, 1.0, (0.1, 3.0), P8, 3)                           # :151-176
+    check_finite(orc, "lissajous", 1.0, (-3.14159, 3.14159, 3.0, 2.0, 0.0, 0.0, 0.0), P8)   # :178-183
+    check_finite(orc, "spirograph", 1.0, (3.0, 2.0, 0.0, -1.0, 1.0, -1.0, 1.0, 0.0, 0.0), P8)   # :185-190
+    check_finite(orc, "waffle", 1.0, (6.0, 0.5, 0.5, 0.0), P8)                           # :192-197
+
+
+def test_glynnsim_smoke(orc):
+    # src/issue120-b5-glynnsim.gpu.test.ts:122-144 (finite inside and outside the radius)
+    check_finite(orc, "glynnSim1", 1.0, (1.0, 0.1, 110.0, 0.1, 1.5, 0.5), P8)
+    check_finite(orc, "glynnSim2", 1.0, (1.0, 0.1, 0.5, 1.5, 110.0, 150.0), P8)
+    check_finite(orc, "glynnSim3", 1.0, (1.0, 0.1, 0.5, 1.5), P8)
+
+
+# src/issue120-b6-faber-novelties.gpu.test.ts:19-27 (P8 without (2, 0))
+P7 = [(0.5, 0.3), (-0.5, 0.7), (1.2, -0.4), (-1.0, -1.0), (0.1, 0.05), (0.0, 2.0), (-2.5, 1.7)]
+
+
+def eclipse_ref(x, y, w, shift):   # src/issue120-b6-faber-novelties.gpu.test.ts:113-128
+    if abs(y) <= w:
+        c2 = math.sqrt(max(w * w - y * y, 0.0))
+        if abs(x) <= c2:
+            xs = x + shift * w
+            return (-w * x if abs(xs) >= c2 else w * xs), w * y
+    return w * x, w * y
+
+
+def barycentroid_ref(x, y, w, a, b, c, d):   # :148-164
+    d00, d01, d02 = a * a + b * b, a * c + b * d, a * x + b * y
+    d11, d12 = c * c + d * d, c * x + d * y
+    den = d00 * d11 - d01 * d01
+    if abs(den) < 1e-30:
+        return x * w, y * w
+    u = (d11 * d02 - d01 * d12) / den
+    v = (d00 * d12 - d01 * d02) / den
+    sg = lambda t: (t > 0) - (t < 0)   # noqa: E731
+    return w * math.sqrt(u * u + x * x) * sg(u), w * math.sqrt(v * v + y * y) * sg(v)
+
+
+def test_faber_novelties(orc):
+    check_pin(orc, "flipy", lambda x, y, w: (w * x, w * y * (-1 if x > 0 else 1)), 1.0, (), P7, 5)   # :78-97
+    check_pin(orc, "eclipse", eclipse_ref, 1.0, (0.0,), P7, 5)                             # :99-132
+    check_pin(orc, "barycentroid", barycentroid_ref, 1.0, (1.0, 0.0, 0.0, 1.0), P7, 4)     # :134-168
+    # :170-196 chunk mode 0 (a = c = 1, f = -1): passthrough inside the unit circle, else 0
+    check_pin(orc, "chunk", lambda x, y, w, *q: (x, y) if x * x + y * y - 1.0 <= 0 else (0.0, 0.0),
+              1.0, (1.0, 0.0, 1.0, 0.0, 0.0, -1.0, 0.0), P7, 5)
+
+
+def test_folds_against_the_mandelbox_definitions(orc):
+    # src/issue129-folds.gpu.test.ts:101-127 (box_fold limit 1, sphere_fold (0.5, 1),
+    # mandelbox_step (2, 0.5, 1, 0, 0), kifs_fold (3, 0)); Tom Lowe's Mandelbox:
+    # box fold reflects a coordinate about +-L, sphere fold scales by R^2/r_min^2
+    # inside r_min and by R^2/r^2 inside R, a step is c + s * sphere(box(z)).
+    def box(t, lim):
+        return 2 * lim - t if t > lim else (-2 * lim - t if t < -lim else t)
+
+    def sphere(x, y, rmin, rmax):
+        r2 = x * x + y * y
+        s = rmax ** 2 / rmin ** 2 if r2 < rmin ** 2 else (rmax ** 2 / r2 if r2 < rmax ** 2 else 1.0)
+        return x * s, y * s
+    pts = [(0.5, 0.5), (1.5, -1.5), (0.75, 0.0), (2.0, 0.0), (0.2, -0.1), (-1.3, 0.4), (2, 2),
+           (-0.5, -0.5)]
+    check_pin(orc, "box_fold", lambda x, y, w, L: (w * box(x, L), w * box(y, L)), 1.0, (1.0,),
+              pts, 5)
+    check_pin(orc, "sphere_fold", lambda x, y, w, a, b: tuple(w * v for v in sphere(x, y, a, b)),
+              1.0, (0.5, 1.0), pts, 5)
+
+    def mstep(x, y, w, s, rmin, rmax, cx, cy):
+        u, v = sphere(box(x, 1.0), box(y, 1.0), rmin, rmax)
+        return w * (s * u + cx), w * (s * v + cy)
+    check_pin(orc, "mandelbox_step", mstep, 1.0, (2.0, 0.5, 1.0, 0.0, 0.0), pts, 4)
+    check_pin(orc, "mandelbox_step", mstep, 0.7, (-1.5, 0.3, 1.2, 0.2, -0.1), pts, 4)
+    # kaleidoscopic fold: |z| kept, angle folded into [offset, offset + pi/n]
+    for n, off in ((3.0, 0.0), (5.0, 0.4)):
+        for x, y in pts + P8:
+            o = run(orc, "kifs_fold", x, y, 1.0, (n, off))
+            r = math.hypot(x, y)
+            assert math.hypot(o[0], o[1]) == pytest.approx(r, rel=1e-12, abs=1e-12)
+            a = (math.atan2(o[1], o[0]) - off) % (2 * PI)
+            assert a <= PI / n + 1e-9 or r < 1e-6
+
+
+def test_attractors_against_their_definitions(orc):
+    # src/issue130-attractors.gpu.test.ts:99-112 (standard_map k = 1, de_jong
+    # (-2.24, 0.43, -0.65, -2.43), ikeda u = 0.9; finite at (0,0), (1,1), (-0.5,-0.5)):
+    # Chirikov's standard map, Peter de Jong's attractor and the Ikeda map.
+    pts = [(0.0, 0.0), (1.0, 1.0), (-0.5, -0.5)] + P8
+    check_pin(orc, "standard_map", lambda x, y, w, k: (w * (x + k * math.sin(y)),
+                                                       w * (y + x + k * math.sin(y))),
+              1.0, (1.0,), pts, 5)
+    check_pin(orc, "de_jong", lambda x, y, w, a, b, c, d: (w * (math.sin(a * y) - math.cos(b * x)),
+                                                           w * (math.sin(c * x) - math.cos(d * y))),
+              1.0, (-2.24, 0.43, -0.65, -2.43), pts, 5)
+
+    def ikeda(x, y, w, u):
+        t = 0.4 - 6.0 / (1.0 + x * x + y * y)
+        return w * (1 + u * (x * math.cos(t) - y * math.sin(t))), w * u * (x * math.sin(t) + y * math.cos(t))
+    check_pin(orc, "ikeda", ikeda, 1.0, (0.9,), pts, 5)
+
+
+# ------------------------------------------------------------ issue 131 ----
+def test_modular_reference_expectations(orc):
+    # src/issue131-modular.gpu.test.ts:136-142: jacobi_theta -> (1, 0) for im_floor 5 (q -> 0)
+    o = run(orc, "jacobi_theta", f32(0.0), f32(0.0), 1.0, (5.0,), np.float32)
+    assert abs(o[0] - 1.0) < 5e-5 and abs(o[1]) < 5e-5
+    # :144-150 lambda(i) = 1/2; :152-162 real and in (0, 1) on the imaginary axis
+    o = run(orc, "modular_lambda", 0.0, f32(0.98), 1.0, (f32(0.02),), np.float32)
+    assert abs(o[0] - 0.5) < 5e-4 and abs(o[1]) < 5e-4
+    for y in (0.5, 1.5, 2.5):
+        o = run(orc, "modular_lambda", 0.0, y, 1.0, (0.1,), np.float32)
+        assert 0 < o[0] < 1 and abs(o[1]) < 1e-3
+    # :165-172 j(i) = 1728, compressed to log(1729) on the real axis
+    o = run(orc, "klein_j", 0.0, f32(0.98), 1.0, (f32(0.02),), np.float32)
+    assert abs(o[0] - math.log(1729)) < 5e-3 and abs(o[1]) < 5e-3
+    for x, y, fl in ((0.0, 0.0, 0.05), (0.9, 0.1, 0.05), (-0.5, 0.5, 0.3), (0.3, -0.7, 0.1)):   # :174-179
+        assert np.isfinite(run(orc, "klein_j", x, y, 1.0, (fl,), np.float32)[:2]).all()
+    # :183-200 gauss_map = frac(1 / x) per axis, in [0, 1) and finite at the origin
+    o = run(orc, "gauss_map", f32(0.3), f32(-0.4), 1.0, (), np.float32)
+    assert abs(o[0] - (1 / 0.3 - math.floor(1 / 0.3))) < 5e-5 and abs(o[1] - 0.5) < 5e-5
+    for x, y in ((0.0, 0.0), (1.7, -2.3), (0.001, 0.05)):
+        o = run(orc, "gauss_map", f32(x), f32(y), 1.0, (), np.float32)
+        assert all(np.isfinite(v) and 0 <= v < 1 for v in o[:2])
+    # :204-221 weierstrass_p even on the square lattice, finite near a pole
+    lat = (1.0, 0.0, 0.0, 1.0)
+    a = run(orc, "weierstrass_p", f32(0.37), f32(0.21), 1.0, lat, np.float32)
+    b = run(orc, "weierstrass_p", f32(-0.37), f32(-0.21), 1.0, lat, np.float32)
+    assert abs(a[0] - b[0]) < 5e-5 and abs(a[1] - b[1]) < 5e-5
+    for x, y in ((0.5, 0.5), (0.99, 0.01), (0.01, 0.01), (1.3, 0.7)):
+        assert np.isfinite(run(orc, "weierstrass_p", x, y, 1.0, lat, np.float32)[:2]).all()
+
+
+# ------------------------------------------------------------ issue 132 ----
+def nbody_ref(x, y, w, c1x, c1y, c2x, c2y, m1, m2, g, eps):   # src/issue132-exotic.gpu.test.ts:108-114
+    e = max(eps, 5e-3)
+
+    def pull(cx, cy, m):
+        dx, dy = cx - x, cy - y
+        r2 = dx * dx + dy * dy + e
+        inv = g * m / (r2 * math.sqrt(r2))
+        return inv * dx, inv * dy
+    a, b = pull(c1x, c1y, m1), pull(c2x, c2y, m2)
+    return w * (x + a[0] + b[0]), w * (y + a[1] + b[1])
+
+
+def _hash01(n):   # :115
+    h = n & 0xFFFFFFFF
+    h ^= h >> 17
+    h = (h * 0xED5AD4BB) & 0xFFFFFFFF
+    h ^= h >> 11
+    h = (h * 0xAC4C1B51) & 0xFFFFFFFF
+    h ^= h >> 15
+    return h / 4294967296.0
+
+
+def _corner(ix, iy):   # :116
+    ux = ((ix & 0xFFFFFFFF) * 0x9E3779B1) & 0xFFFFFFFF
+    uy = ((iy & 0xFFFFFFFF) * 0x85EBCA77) & 0xFFFFFFFF
+    return _hash01(ux ^ uy) * 2 - 1
+
+
+def _vnoise(px, py):   # :117
+    fx, fy = math.floor(px), math.floor(py)
+    ix, iy = int(fx), int(fy)
+    tx, ty = px - fx, py - fy
+    ux, uy = tx * tx * (3 - 2 * tx), ty * ty * (3 - 2 * ty)
+    c00, c10, c01, c11 = _corner(ix, iy), _corner(ix + 1, iy), _corner(ix, iy + 1), _corner(ix + 1, iy + 1)
+    bot, top = c00 + (c10 - c00) * ux, c01 + (c11 - c01) * ux
+    return bot + (top - bot) * uy
+
+
+def curl_noise_ref(x, y, w, freq, amp):   # :118-125
+    f = max(freq, 1e-3)
+    sx, sy, h = x * f, y * f, 1e-2
+    ddx = (_vnoise(sx + h, sy) - _vnoise(sx - h, sy)) / (2 * h)
+    ddy = (_vnoise(sx, sy + h) - _vnoise(sx, sy - h)) / (2 * h)
+    dx, dy = min(8.0, max(-8.0, ddy)), min(8.0, max(-8.0, -ddx))
+    return w * (x + amp * dx), w * (y + amp * dy)
+
+
+def test_exotic(orc):
+    P = (-0.4, 0.0, 0.4, 0.0, 0.06, 0.06, 1.0, 0.05)
+    check_pin(orc, "nbody_lensing", nbody_ref, 1.0, P,
+              [(0.5, 0.3), (-0.7, 0.4), (1.0, -0.5), (0.2, 0.6), (0.0, 0.5)], 4)   # :127-145
+    assert abs(run(orc, "nbody_lensing", 0.0, 0.5, 1.0, P, np.float32)[0]) < 1e-4  # :140-145
+    check_pin(orc, "nbody_lensing", nbody_ref, 1.0, P[:4] + (0.0, 0.0) + P[6:],
+              [(0.3, 0.4), (-0.6, 0.2)], 5)                                       # :146-155
+    assert np.isfinite(run(orc, "nbody_lensing", -0.4, 0.0, 1.0,
+                           (-0.4, 0.0, 0.4, 0.0, 1.0, 1.0, 4.0, 0.005), np.float32)[:2]).all()
+    # curl of value noise (the noise hash mirrored for negative cells), :166-188
+    check_pin(orc, "curl_noise", curl_noise_ref, 1.0, (2.5, 0.3),
+              [(0.5, 0.3), (-0.7, 0.4), (-1.2, -0.9), (0.9, -0.5)], 3, f64_rel=1e-7)
+    check_pin(orc, "curl_noise", curl_noise_ref, 1.0, (2.5, 0.0), [(0.5, 0.3), (-0.7, 0.4)], 6)
+    for k in range(13):   # :189-201 bounded displacement
+        x, y = math.cos(0.5 * k), math.sin(0.5 * k)
+        o = run(orc, "curl_noise", x, y, 1.0, (2.5, 0.3), np.float32)
+        assert math.hypot(o[0] - x, o[1] - y) < 3.4
+
+
+# ---------------------------------------------------- literal expectations --
+# (entry, (x, y), params, (expected x, expected y), toBeCloseTo digits for x
+# and y, reference file:line). The reference feeds (x, y, p0, p1) rows to
+# var_<entry>(p, 1.0, p0, p1) (e.g. src/issue133-blaschke.gpu.test.ts:53).
+E = math.e
+SQRT_PI = math.sqrt(PI)
+LITERALS = [
+    ("blaschke", (0.0, 0.0), (0.5, 0.0), (0.0, 0.0), 5, 5, "issue133-blaschke.gpu.test.ts:87-96"),
+    ("blaschke", (0.5, 0.0), (0.5, 0.0), (0.0, 0.0), 5, 5, "issue133-blaschke.gpu.test.ts:87-96"),
+    ("blaschke", (1.0, 0.0), (0.5, 0.0), (1.0, 0.0), 4, 4, "issue133-blaschke.gpu.test.ts:98-105"),
+    ("cayley", (0.0, 0.0), (1.0,), (-1.0, 0.0), 4, 4, "issue133-cayley.gpu.test.ts:84-96"),
+    ("cayley", (1.0, 0.0), (1.0,), (0.0, -1.0), 4, 4, "issue133-cayley.gpu.test.ts:84-96"),
+    ("cayley", (-1.0, 0.0), (1.0,), (0.0, 1.0), 4, 4, "issue133-cayley.gpu.test.ts:84-96"),
+    ("cayley", (0.0, 1.0), (1.0,), (0.0, 0.0), 5, 5, "issue133-cayley.gpu.test.ts:98-105"),
+    ("cayley", (0.0, 0.0), (2.0,), (-1.0, 0.0), 4, 4, "issue133-cayley.gpu.test.ts:116-126"),
+    ("cayley", (0.0, 0.0), (0.5,), (-1.0, 0.0), 4, 4, "issue133-cayley.gpu.test.ts:116-126"),
+    ("complex_gamma", (1.0, 0.0), (1.0,), (1.0, 0.0), 2, 2, "issue133-complex-gamma.gpu.test.ts:109-118"),
+    ("complex_gamma", (2.0, 0.0), (1.0,), (1.0, 0.0), 2, 2, "issue133-complex-gamma.gpu.test.ts:109-118"),
+    ("complex_gamma", (3.0, 0.0), (1.0,), (2.0, 0.0), 2, 2, "issue133-complex-gamma.gpu.test.ts:120-129"),
+    ("complex_gamma", (4.0, 0.0), (1.0,), (6.0, 0.0), 1, 1, "issue133-complex-gamma.gpu.test.ts:120-129"),
+    ("complex_gamma", (0.5, 0.0), (1.0,), (SQRT_PI, 0.0), 1, 2, "issue133-complex-gamma.gpu.test.ts:131-137"),
+    ("complex_gamma", (1.0, 1.0), (1.0,), (0.4980, -0.1549), 2, 2, "issue133-complex-gamma.gpu.test.ts:139-145"),
+    ("complex_gamma", (-0.5, 0.0), (1.0,), (-2 * SQRT_PI, 0.0), 0, 0, "issue133-complex-gamma.gpu.test.ts:147-154"),
+    ("complex_gamma", (1.0, 0.0), (0.3,), (0.3, None), 2, 2, "issue133-complex-gamma.gpu.test.ts:156-162"),
+    ("lambert_w", (0.0, 0.0), (3,), (0.0, 0.0), 3, 3, "issue133-lambert-w.gpu.test.ts:101-107"),
+    ("lambert_w", (1.0, 0.0), (3,), (0.5671, 0.0), 2, 3, "issue133-lambert-w.gpu.test.ts:109-115"),
+    ("lambert_w", (E, 0.0), (3,), (1.0, 0.0), 2, 3, "issue133-lambert-w.gpu.test.ts:117-123"),
+    ("lambert_w", (-0.3, 0.0), (3,), (-0.4894, 0.0), 2, 3, "issue133-lambert-w.gpu.test.ts:125-131"),
+    ("lambert_w", (1.0, 1.0), (4,), (0.6569, 0.3254), 2, 2, "issue133-lambert-w.gpu.test.ts:133-139"),
+    ("lambert_w", (100.0, 0.0), (4,), (3.385, 0.0), 1, 1, "issue133-lambert-w.gpu.test.ts:141-147"),
+    ("newton", (1.0, 0.0), (3,), (1.0, 0.0), 4, 4, "issue133-newton.gpu.test.ts:167-176"),
+    ("newton", (2.0, 0.0), (3,), (17 / 12, 0.0), 3, 4, "issue133-newton.gpu.test.ts:178-186"),
+    ("newton", (1.0, 0.0), (4,), (1.0, 0.0), 4, 4, "issue133-newton.gpu.test.ts:196-202"),
+]
+# (entry, points, params, reference file:line): finite output required
+FINITE = [
+    ("blaschke", [(2.0, 0.0), (1.95, 0.01)], (0.5, 0.0), "issue133-blaschke.gpu.test.ts:118-125"),
+    ("blaschke", [(0.3, 0.4)], (0.5, 0.3), "issue133-blaschke.gpu.test.ts:127-135"),
+    ("blaschke", [(-0.6, 0.2)], (0.1, -0.4), "issue133-blaschke.gpu.test.ts:127-135"),
+    ("blaschke", [(0.7, -0.5)], (0.8, 0.0), "issue133-blaschke.gpu.test.ts:127-135"),
+    ("blaschke", [(0.2, 0.9)], (-0.3, 0.5), "issue133-blaschke.gpu.test.ts:127-135"),
+    ("cayley", [(0.0, -1.0), (0.01, -0.99)], (1.0,), "issue133-cayley.gpu.test.ts:107-114"),
+    ("complex_gamma", [(0.5, 0.5), (-0.3, 0.7), (1.2, -0.4), (0.1, 0.05)], (0.3,),
+     "issue133-complex-gamma.gpu.test.ts:164-172"),
+    ("lambert_w", P8[:6], (2,), "issue133-lambert-w.gpu.test.ts:149-159"),
+    ("newton", [(0.0, 0.0), (1e-9, 0.0)], (3,), "issue133-newton.gpu.test.ts:188-194"),
+    ("newton", [(0.5, 0.5), (-0.3, 0.7)], (3,), "issue133-newton.gpu.test.ts:204-211"),
+    ("newton", [(1.2, -0.4)], (5,), "issue133-newton.gpu.test.ts:204-211"),
+]
+
+
+@pytest.mark.parametrize("case", LITERALS, ids=lambda c: f"{c[0]}@{c[6].split(':')[1]}")
+def test_literal_expectation(orc, case):
+    name, (x, y), params, want, dx, dy, src = case
+    got = run(orc, name, f32(x), f32(y), 1.0, params, np.float32)
+    for k, (w, d) in enumerate(zip(want, (dx, dy))):
+        if w is not None:
+            assert abs(got[k] - w) < 0.5 * 10.0 ** -d, (src, k, got[k], w)
+
+
+@pytest.mark.parametrize("case", FINITE, ids=lambda c: f"{c[0]}@{c[3].split(':')[1]}")
+def test_finite_expectation(orc, case):
+    name, pts, params, src = case
+    for x, y in pts:
+        assert np.isfinite(run(orc, name, f32(x), f32(y), 1.0, params, np.float32)[:2]).all(), src
+
+
+def test_reference_properties_issue133(orc):
+    # src/issue133-blaschke.gpu.test.ts:107-116: |B(z)| = 1 on the unit circle
+    for (x, y), a in (((0.0, 1.0), (0.5, 0.0)), ((-1.0, 0.0), (0.3, 0.2)), ((0.6, 0.8), (0.4, 0.1))):
+        o = run(orc, "blaschke", x, y, 1.0, a, np.float32)
+        assert abs(math.hypot(o[0], o[1]) - 1.0) < 5e-3
+    # src/issue133-cayley.gpu.test.ts:128-139: upper half-plane into the unit disk
+    for x, y in ((0.5, 1.0), (-0.3, 2.0), (1.0, 3.0)):
+        o = run(orc, "cayley", x, y, 1.0, (1.0,), np.float32)
+        assert math.hypot(o[0], o[1]) < 1.0
+    # src/issue133-lambert-w.gpu.test.ts:161-183: w e^w = z after the iteration
+    for x, y in ((2.0, 0.0), (0.7, 0.4)):
+        wr, wi = run(orc, "lambert_w", x, y, 1.0, (4,), np.float32)[:2]
+        ez = complex(wr, wi) * np.exp(complex(wr, wi))
+        assert abs(ez.real - x) < 0.05 and abs(ez.imag - y) < 0.05
+    # src/issue133-newton.gpu.test.ts:220-292: basin colours of z^n - 1
+    c0 = orc.dc_color(V["newton"], 0.95, 0.0, (3.0,))
+    c1 = orc.dc_color(V["newton"], -0.5, 0.85, (3.0,))
+    c2 = orc.dc_color(V["newton"], -0.5, -0.85, (3.0,))
+    assert c0[0] > c0[1] and c0[0] > c0[2] and c0[0] > 0.9
+    assert c1[1] > c1[0] and c1[1] > c1[2]
+    assert c2[2] > c2[0] and c2[2] > c2[1]
+
+    def distinct(a, b):
+        return any(abs(u - v) > 0.1 for u, v in zip(a, b))
+    assert distinct(c0, c1) and distinct(c1, c2) and distinct(c0, c2)
+    c4 = [orc.dc_color(V["newton"], x, y, (4.0,)) for x, y in ((0.95, 0), (0, 0.95), (-0.95, 0), (0, -0.95))]
+    assert all(distinct(c4[i], c4[j]) for i in range(4) for j in range(i + 1, 4))
+
+
+# ---------------------------------------------- issues 134, 135: textbook --
+def test_map_projections_against_textbook(orc):
+    """src/issue134-cartographic.gpu.test.ts:99-123 only asks for finite
+    output at (0,0), (1,1), (-0.5,-0.5), (0,1.5), (0,pi/2); the entries are the
+    textbook projections of (longitude x, latitude y) with a 1e-6 guard in
+    each denominator, so they are held to the textbook formulas."""
+    from scipy.optimize import brentq
+    pts = [(0.0, 0.0), (1.0, 1.0), (-0.5, -0.5), (0.3, 1.2), (-1.1, 0.4), (0.7, -0.9)] + P8[:5]
+
+    def mercator(x, y, w):
+        return w * x, w * math.log(math.tan(PI / 4 + min(max(y, -1.5), 1.5) / 2))
+
+    def lambert(x, y, w):   # azimuthal equal-area, centred on (0, 0)
+        k = math.sqrt(2 / (1 + math.cos(y) * math.cos(x)))
+        return w * k * math.cos(y) * math.sin(x), w * k * math.sin(y)
+
+    def hammer(x, y, w):
+        z = math.sqrt(1 + math.cos(y) * math.cos(x / 2))
+        return w * 2 * math.sqrt(2) * math.cos(y) * math.sin(x / 2) / z, w * math.sqrt(2) * math.sin(y) / z
+
+    def stereo(x, y, w):
+        k = 2 / (1 + math.cos(y) * math.cos(x))
+        return w * k * math.cos(y) * math.sin(x), w * k * math.sin(y)
+
+    def mollweide(x, y, w):   # 2t + sin 2t = pi sin(lat)
+        t = brentq(lambda t: 2 * t + math.sin(2 * t) - PI * math.sin(y), -PI / 2, PI / 2) \
+            if abs(y) > 0 else 0.0
+        return w * 2 * math.sqrt(2) / PI * x * math.cos(t), w * math.sqrt(2) * math.sin(t)
+    for name, ref in (("mercator", mercator), ("lambert", lambert), ("hammer", hammer),
+                      ("stereographic", stereo), ("mollweide", mollweide)):
+        check_pin(orc, name, ref, 1.0, (), pts, 4, f64_rel=2e-5)
+    for x, y in ((0.0, 1.5), (0.0, PI / 2)):
+        assert np.isfinite(run(orc, "mercator", x, y, 1.0, (), np.float32)[:2]).all()
+        assert np.isfinite(run(orc, "mollweide", x, y, 1.0, (), np.float32)[:2]).all()
+
+
+def test_plane_curves_against_textbook(orc):
+    """src/issue135-plane-curves.gpu.test.ts:106-140 (superellipse (1, 1, 2)
+    maps (1,0), (0,1) to themselves; limacon (1, 0.5) at theta 0 -> 1.5;
+    epicycloid k = 3 at theta 0 -> 3; catenary a = 1 at 0 -> (0, 1); tractrix
+    finite at (1, 1)), and the curves' definitions at the angle of the input:
+    the Lame curve |cos/a|^n + |sin/b|^n = r^-n, the limacon r = b + a cos t,
+    the epicycloid ((k+1) cos t - cos (k+1)t, (k+1) sin t - sin (k+1)t), the
+    catenary y = a cosh(x / a), the tractrix (x - tanh x, sech x)."""
+    def superellipse(x, y, w, a, b, n):
+        t = math.atan2(y, x)
+        r = (abs(math.cos(t) / a) ** n + abs(math.sin(t) / b) ** n) ** (-1 / n)
+        return w * r * math.cos(t), w * r * math.sin(t)
+
+    def limacon(x, y, w, a, b):
+        t = math.atan2(y, x)
+        r = b + a * math.cos(t)
+        return w * r * math.cos(t), w * r * math.sin(t)
+
+    def epicycloid(x, y, w, k):
+        t = math.atan2(y, x)
+        return (w * ((k + 1) * math.cos(t) - math.cos((k + 1) * t)),
+                w * ((k + 1) * math.sin(t) - math.sin((k + 1) * t)))
+    pts = [(1.0, 0.0), (0.0, 1.0)] + P8
+    check_pin(orc, "superellipse", superellipse, 1.0, (1.0, 1.0, 2.0), pts, 4, f64_rel=1e-6)
+    check_pin(orc, "superellipse", superellipse, 1.0, (1.5, 0.7, 3.5), pts, 4, f64_rel=1e-6)
+    check_pin(orc, "limacon", limacon, 1.0, (1.0, 0.5), pts, 5)
+    check_pin(orc, "epicycloid", epicycloid, 1.0, (3.0,), pts, 5)
+    check_pin(orc, "catenary", lambda x, y, w, a: (w * x, w * a * math.cosh(x / a)), 1.0, (1.0,),
+              [(0.0, 0.0)] + P8, 4)
+    check_pin(orc, "tractrix", lambda x, y, w: (w * (x - math.tanh(x)), w / math.cosh(x)), 1.0, (),
+              [(1.0, 1.0)] + P8, 5)
+    assert run(orc, "limacon", 1.0, 0.0, 1.0, (1.0, 0.5))[:2] == pytest.approx((1.5, 0.0))
+    assert run(orc, "epicycloid", 1.0, 0.0, 1.0, (3.0,))[:2] == pytest.approx((3.0, 0.0))
+
+
+# ------------------------------------------------------------ issue 137 ----
+def _j0_ref(x):   # src/issue137-special-functions.gpu.test.ts:113-123 (A&S 9.4.1 / 9.4.3 in (3/x)^2)
+    ax = abs(x)
+    if ax < 3.0:
+        t = x * x / 9.0
+        return 1.0 + t * (-2.2499997 + t * (1.2656208 + t * (-0.3163866 + t * (0.0444479 + t * (
+            -0.0039444 + t * 0.0002100)))))
+    t = (3.0 / ax) ** 2
+    amp = 0.79788456 + t * (-0.00000077 + t * (-0.00552740 + t * (-0.00009512 + t * (
+        0.00137237 + t * (-0.00072805 + t * 0.00014476)))))
+    ph = ax - 0.78539816 + t * (-0.04166397 + t * (-0.00003954 + t * (0.00262573 + t * (
+        -0.00054125 + t * (-0.00029333 + t * 0.00013558)))))
+    return amp / math.sqrt(ax) * math.cos(ph)
+
+
+def _airy_ref(x):   # :124-131
+    if x > 4.0:
+        xi = 2 / 3 * x ** 1.5
+        return math.exp(-xi) / (2 * 1.7724539 * x ** 0.25)
+    if x < -5.0:
+        xi = 2 / 3 * (-x) ** 1.5
+        return math.sin(xi + 0.78539816) / (1.7724539 * (-x) ** 0.25)
+    x3 = x ** 3
+    f = tf = 1.0
+    g = tg = x
+    for k in range(1, 12):
+        tf *= x3 / ((3 * k - 1) * (3 * k))
+        f += tf
+        tg *= x3 / ((3 * k) * (3 * k + 1))
+        g += tg
+    return 0.355028053887817 * f - 0.258819403792807 * g
+
+
+def _struve_h1_ref(x):   # :132-138
+    h = 0.5 * min(10.0, max(-10.0, x))
+    term = h * h / (0.8862269 * 1.3293404)
+    s = term
+    for m in range(1, 16):
+        term = term * -(h * h) / ((m + 0.5) * (m + 1.5))
+        s += term
+    return s
+
+
+def _cornu_ref(x, y, w, freq):   # :214-222 (rational Fresnel fit)
+    t = freq * x
+    s, sg = abs(t), (-1 if t < 0 else 1)
+    f = (1 + 0.926 * s) / (2 + 1.792 * s + 3.104 * s * s)
+    g = 1 / (2 + 4.142 * s + 3.492 * s * s + 6.670 * s ** 3)
+    a = PI * s * s / 2
+    c = 0.5 + f * math.sin(a) - g * math.cos(a)
+    sn = 0.5 - f * math.cos(a) - g * math.sin(a)
+    return w * sg * c, w * (sg * sn + 0.25 * y)
+
+
+def test_special_function_entries_against_the_reference_oracles(orc):
+    P4 = [(0.5, 0.3), (-0.7, 0.4), (1.0, -0.5), (0.0, 0.6)]
+    sweep = [(0.1 + 0.2 * k, 0.0) for k in range(15)]
+
+    def radial(fn, gain):
+        return lambda x, y, w, *q: tuple(w * gain(*q) * fn(math.hypot(x, y), *q) * v for v in (x, y))
+    # bessel_j0, :140-170: J0(freq r) p; origin fixed; collapse near the first zero
+    check_pin(orc, "bessel_j0", lambda x, y, w, f: (w * _j0_ref(f * math.hypot(x, y)) * x,
+                                                    w * _j0_ref(f * math.hypot(x, y)) * y),
+              1.0, (4.0,), P4 + sweep, 4)
+    assert math.hypot(*run(orc, "bessel_j0", 2.4048 / 4, 0.0, 1.0, (4.0,), np.float32)[:2]) < 0.02
+    # airy_radial, :172-207: 3 Ai(scale (r - shift)) p, bounded by 2 |p|
+    check_pin(orc, "airy_radial", lambda x, y, w, sc, sh: tuple(
+        w * 3 * _airy_ref(sc * (math.hypot(x, y) - sh)) * v for v in (x, y)), 1.0, (3.0, 1.5),
+        P4 + sweep, 4)
+    assert abs(_airy_ref(0) - 0.3550280) < 5e-6 and abs(_airy_ref(8)) < 1e-4
+    for x, y in sweep:
+        o = run(orc, "airy_radial", x, y, 1.0, (3.0, 1.5), np.float32)
+        assert math.hypot(o[0], o[1]) < 2 * x + 1e-3
+    # cornu_spiral, :209-248
+    check_pin(orc, "cornu_spiral", _cornu_ref, 1.0, (1.6,), P4 + [(0.0, 0.8), (12.5, 0.0)], 4)
+    o = run(orc, "cornu_spiral", 12.5, 0.0, 1.0, (1.6,), np.float32)
+    assert abs(o[0] - 0.5) < 0.05 and abs(o[1] - 0.5) < 0.05
+    # struve_h1, :250-287: 0.6 H1(freq r) p, factor below 0.7
+    check_pin(orc, "struve_h1", lambda x, y, w, f: tuple(
+        w * 0.6 * _struve_h1_ref(f * math.hypot(x, y)) * v for v in (x, y)), 1.0, (2.5,),
+        P4 + sweep + [(0.0, 0.0)], 4)
+    for x, y in sweep:
+        o = run(orc, "struve_h1", x, y, 1.0, (2.5,), np.float32)
+        assert math.hypot(o[0], o[1]) / x < 0.7
+    assert radial
+
+
+# ------------------------------------------------------------ issue 138 ----
+def lorentz_ref(x, y, w, rap, ang):   # src/variations.ts:2844-2858
+    c, s = math.cos(ang), math.sin(ang)
+    rx, ry = x * c + y * s, -x * s + y * c
+    xp, yp = rx * math.cosh(rap) + ry * math.sinh(rap), rx * math.sinh(rap) + ry * math.cosh(rap)
+    return w * (xp * c - yp * s), w * (xp * s + yp * c)
+
+
+def schwarzschild_ref(x, y, w, mass, eps):   # src/variations.ts:2861-2871
+    a = mass / (math.hypot(x, y) + max(eps, 1e-6))
+    return w * (x * math.cos(a) - y * math.sin(a)), w * (x * math.sin(a) + y * math.cos(a))
+
+
+def dipole_ref(x, y, w, q, sep, step, ang):   # src/variations.ts:2874-2895
+    d = sep * 0.5
+    px, py = d * math.cos(ang), d * math.sin(ang)
+    rp, rn = (x - px, y - py), (x + px, y + py)
+    dp3 = (rp[0] ** 2 + rp[1] ** 2 + 1e-4) ** 1.5
+    dn3 = (rn[0] ** 2 + rn[1] ** 2 + 1e-4) ** 1.5
+    ex, ey = q * (rp[0] / dp3 - rn[0] / dn3), q * (rp[1] / dp3 - rn[1] / dn3)
+    return w * (x + step * ex), w * (y + step * ey)
+
+
+def pendulum_ref(x, y, w, magnets, radius, strength, damping):   # src/variations.ts:2898-2921
+    n = max(3, min(6, int(math.trunc(magnets))))
+    fx = fy = 0.0
+    for k in range(n):
+        th = 2.0 * PI * k / n
+        rx, ry = radius * math.cos(th) - x, radius * math.sin(th) - y
+        den = (rx * rx + ry * ry + 1e-4) ** 1.5
+        fx += rx / den
+        fy += ry / den
+    return w * (x + strength * fx - damping * x), w * (y + strength * fy - damping * y)
+
+
+def test_physics_entries_against_the_reference_oracles(orc):
+    # src/issue138-physics.gpu.test.ts (toBeCloseTo 4; identity / radius properties at 5 / 4)
+    P4a = [(0.0, 0.0), (0.5, 0.0), (0.3, 0.4), (-1.2, 0.7)]
+    check_pin(orc, "lorentz_boost", lorentz_ref, 1.0, (0.5, 0.3), P4a + P8, 4)          # :111-131
+    check_pin(orc, "lorentz_boost", lorentz_ref, 1.0, (0.0, 0.5),
+              [(0.4, 0.6), (-0.3, 0.2), (1.1, -0.8)], 5)                                  # :133-143
+    check_pin(orc, "schwarzschild_lensing", schwarzschild_ref, 1.0, (0.5, 0.05), P4a + P8, 4)   # :145-167
+    for x, y in ((0.4, 0.6), (-0.3, 0.2), (1.1, -0.8)):                                   # :169-182
+        o = run(orc, "schwarzschild_lensing", x, y, 1.0, (0.7, 0.05), np.float32)
+        assert abs(math.hypot(o[0], o[1]) - math.hypot(x, y)) < 5e-5
+    for x, y in ((1e-7, 0.0), (0.0, -1e-7), (3e-7, 4e-7)):                               # :184-196
+        assert np.isfinite(run(orc, "schwarzschild_lensing", x, y, 1.0, (2.0, 0.0), np.float32)[:2]).all()
+    check_pin(orc, "field_dipole", dipole_ref, 1.0, (1.0, 0.5, 0.2, 0.0),
+              [(0.5, 0.3), (-0.7, 0.4), (1.0, -0.5), (0.0, 0.6)] + P8, 4, f64_rel=1e-8)   # :198-221
+    for x, y in ((0.3, 0.05), (-0.3, -0.05)):                                             # :223-234
+        assert np.isfinite(run(orc, "field_dipole", x, y, 1.0, (1.0, 0.5, 0.2, 0.0), np.float32)[:2]).all()
+    check_pin(orc, "magnetic_pendulum", pendulum_ref, 1.0, (3.0, 1.0, 0.5, 0.1),
+              [(0.0, 0.0), (0.5, 0.3), (-0.7, 0.8), (1.2, -0.5)] + P8, 4, f64_rel=1e-8)   # :236-257
+    for x, y in ((0.4, 0.2), (-0.6, 0.7), (0.1, -1.0)):                                   # :259-267
+        assert np.isfinite(run(orc, "magnetic_pendulum", x, y, 1.0, (6.0, 1.0, 0.5, 0.1), np.float32)[:2]).all()
+    for hi, ref in ((15.0, 6.0), (1.0, 3.0)):                                             # :269-283
+        a = run(orc, "magnetic_pendulum", 0.3, 0.4, 1.0, (hi, 1.0, 0.5, 0.1))
+        b = run(orc, "magnetic_pendulum", 0.3, 0.4, 1.0, (ref, 1.0, 0.5, 0.1))
+        assert a[:2] == b[:2]
+
+
+# ------------------------------------------------------- issues 139 - 141 --
+LITERALS += [
+    ("tinkerbell", (0.5, 0.5), (0.9, -0.6, 2.0, 0.5), (0.15, 1.75), 4, 4, "issue139-flows.gpu.test.ts:101-109"),
+    ("duffing", (1.0, 1.0), (0.1, 0.1, 0.1, 1.0), (1.1, 1.0 + 0.1 * (-0.1 + 0.1 * math.cos(1.0))), 4, 4,
+     "issue139-flows.gpu.test.ts:111-120"),
+    ("vanderpol", (0.5, 0.5), (0.1, 1.0), (0.55, 0.4875), 4, 4, "issue139-flows.gpu.test.ts:122-130"),
+    ("rossler", (1.0, 0.0), (0.1, 0.2), (0.9, 0.1), 4, 4, "issue139-flows.gpu.test.ts:132-141"),
+    ("bakers_map", (0.25, 0.25), (), (0.0, 0.375), 2, 2, "issue140-toral.gpu.test.ts:98-106"),
+    ("tent_map", (0.0, 0.0), (), (0.5, 0.5), 2, 2, "issue140-toral.gpu.test.ts:108-113"),
+    ("logistic_map", (0.0, 0.0), (3.9,), (0.475, 0.475), 2, 2, "issue140-toral.gpu.test.ts:115-121"),
+]
+
+
+def test_flows_and_toral_maps(orc):
+    # the Tinkerbell map (src/issue139-flows.gpu.test.ts:101-109 states it)
+    check_pin(orc, "tinkerbell", lambda x, y, w, a, b, c, d: (w * (x * x - y * y + a * x + b * y),
+                                                              w * (2 * x * y + c * x + d * y)),
+              1.0, (0.9, -0.6, 2.0, 0.5), P8, 4)
+    # src/issue140-toral.gpu.test.ts:90-96: arnold_cat lands in [-0.5, 0.5]^2
+    for x, y in [(10.5, -2.1), (0.1, 0.1)] + P8:
+        o = run(orc, "arnold_cat", f32(x), f32(y), 1.0, (), np.float32)
+        assert -0.5 <= o[0] <= 0.5 and -0.5 <= o[1] <= 0.5
+
+
+def _scramble_ref(bits, extent, perm):   # src/issue141-digit-scramble.gpu.test.ts:121-126
+    levels = 2 ** bits
+    s = max(extent, 1e-4)
+    mask = levels - 1
+
+    def enc(c):
+        nrm = (c + s) / (2 * s)
+        return min(int((nrm - math.floor(nrm)) * levels), levels - 1)
+
+    def dec(i):
+        return (i + 0.5) / levels * 2 * s - s
+    return lambda x, y, w, *q: (w * dec(perm(enc(x)) & mask), w * dec(perm(enc(y)) & mask))
+
+
+def test_digit_scrambles(orc):
+    def rev(bits):   # :127
+        return lambda v: int(format(v, f"0{bits}b")[::-1], 2)
+    pts3 = [(0.5, 0.3), (-0.7, 0.4), (0.9, -0.5)]
+    check_pin(orc, "radical_inverse", _scramble_ref(12, 1.0, rev(12)), 1.0, (1.0, 12.0),
+              pts3 + [(0.13, 0.77)], 4)                                          # :141-152
+    check_pin(orc, "gray_code", _scramble_ref(12, 1.0, lambda v: v ^ (v >> 1)), 1.0,
+              (1.0, 12.0), pts3, 4)                                              # :172-184
+
+    def morton(x, y, w, extent, bits_f):   # :196-212
+        bits = int(bits_f)
+        levels, s = 2 ** bits, max(extent, 1e-4)
+
+        def enc(c):
+            nrm = (c + s) / (2 * s)
+            return min(int((nrm - math.floor(nrm)) * levels), levels - 1)
+
+        def part(v):
+            return sum(((v >> i) & 1) << (2 * i) for i in range(bits))
+
+        def dec(i):
+            return (i + 0.5) / levels * 2 * s - s
+        code = part(enc(x)) | (part(enc(y)) << 1)
+        return w * dec(code & (levels - 1)), w * dec((code >> bits) & (levels - 1))
+    check_pin(orc, "morton_zorder", morton, 1.0, (1.0, 8.0), pts3, 4)
+    for name, prm in (("radical_inverse", (1.0, 12.0)), ("morton_zorder", (1.0, 8.0))):   # :153-161, :214-221
+        for k in range(13):
+            v = -3 + 0.5 * k
+            o = run(orc, name, f32(v), f32(-v), 1.0, prm, np.float32)
+            assert all(np.isfinite(t) and abs(t) < 1.0 for t in o[:2])
+
+
+# ------------------------------------------------------- issues 142, 143 --
+GRID142 = [(0.5, 0.7), (-1.2, 0.4), (0.3, -2.1), (2.5, 1.1), (-3, 0.5), (1.7, -0.8), (0.05, 0.05),
+           (4, 1.5)]   # src/issue142-number-theoretic.gpu.test.ts:85-87
+GRID143 = [(0, 0), (0.5, 0.7), (-1.2, 0.4), (0.3, -2.1), (3, 3), (-3, -3), (8, -8), (0.001, 0.001)]
+LITERALS += [
+    ("collatz", (2.0, 0.0), (1.0, 0.0), (1.0, 0.0), 3, 3, "issue142-number-theoretic.gpu.test.ts:89-95"),
+    ("collatz", (3.0, 0.0), (1.0, 0.0), (5.0, 0.0), 3, 3, "issue142-number-theoretic.gpu.test.ts:96-102"),
+    ("digamma", (1.0, 0.0), (1.0, 0.0), (-0.5772157, 0.0), 3, 3, "issue142-number-theoretic.gpu.test.ts:116-122"),
+    ("digamma", (2.0, 0.0), (1.0, 0.0), (0.4227843, 0.0), 3, 3, "issue142-number-theoretic.gpu.test.ts:123-129"),
+    ("quasicrystal", (0.0, 0.0), (5.0, 3.0), (0.0, 0.0), 5, 5, "issue143-aperiodic.gpu.test.ts:84-91"),
+]
+FINITE += [
+    ("collatz", GRID142, (1.0, 0.0), "issue142-number-theoretic.gpu.test.ts:103-113"),
+    ("digamma", GRID142, (1.0, 0.0), "issue142-number-theoretic.gpu.test.ts:130-139"),
+    ("quasicrystal", GRID143, (5.0, 3.0), "issue143-aperiodic.gpu.test.ts:92-103"),
+    ("penrose", GRID143, (1.0, 0.2), "issue143-aperiodic.gpu.test.ts:115-126"),
+]
+
+
+def test_number_theoretic_and_aperiodic_properties(orc):
+    def moved(name, p, a, b, tol):
+        u = run(orc, name, *p, 1.0, a, np.float32)
+        v = run(orc, name, *p, 1.0, b, np.float32)
+        return math.hypot(u[0] - v[0], u[1] - v[1]) > tol
+    # src/issue142-number-theoretic.gpu.test.ts:109-112, :135-138: scale / shift have effect
+    assert moved("collatz", (0.6, 0.4), (1.0, 0.0), (1.5, 0.3), 1e-3)
+    assert moved("digamma", (0.6, 0.4), (1.0, 0.0), (1.5, 0.3), 1e-3)
+    # digamma on the real axis against scipy's psi (recurrence + asymptotic series)
+    from scipy.special import digamma as psi
+    for x in (0.7, 1.5, 3.2, 6.0):
+        assert run(orc, "digamma", x, 0.0, 1.0, (1.0, 0.0))[0] == pytest.approx(psi(x), abs=1e-4)
+    # src/issue143-aperiodic.gpu.test.ts:92-112: displacement at most 0.4; n = 5 and 7 differ
+    agg = 0.0
+    for x, y in GRID143:
+        a = run(orc, "quasicrystal", f32(x), f32(y), 1.0, (5.0, 3.0), np.float32)
+        b = run(orc, "quasicrystal", f32(x), f32(y), 1.0, (7.0, 3.0), np.float32)
+        assert math.hypot(a[0] - x, a[1] - y) <= 0.4001
+        agg += math.hypot(a[0] - b[0], a[1] - b[1])
+    assert agg > 1e-3
+    # :115-143 penrose: |out| <= 5 |in| + 5, lattice snap, scale has effect
+    for x, y in GRID143:
+        o = run(orc, "penrose", f32(x), f32(y), 1.0, (1.0, 0.2), np.float32)
+        assert math.hypot(o[0], o[1]) <= 5 * math.hypot(x, y) + 5
+    a = run(orc, "penrose", f32(0.30), f32(0.30), 1.0, (1.0, 0.2), np.float32)
+    b = run(orc, "penrose", f32(0.3001), f32(0.3001), 1.0, (1.0, 0.2), np.float32)
+    assert abs(a[0] - b[0]) < 5e-6 and abs(a[1] - b[1]) < 5e-6
+    assert moved("penrose", (1.3, -0.7), (1.0, 0.2), (2.0, 0.2), 1e-4)
+
+
+# ------------------------------------------------------------ issue 144 ----
+def _cheb_t(v, n):   # src/issue144-polynomials.gpu.test.ts:112-118 (argument clamped to [-1, 1])
+    c = max(-1.0, min(1.0, v))
+    if n <= 0:
+        return 1.0
+    a, b, t = 1.0, c, c
+    for _ in range(2, n + 1):
+        t = 2 * c * b - a
+        a, b = b, t
+    return t
+
+
+def _leg_p(v, n):   # :119-125
+    c = max(-1.0, min(1.0, v))
+    if n <= 0:
+        return 1.0
+    a, b, p = 1.0, c, c
+    for k in range(1, n):
+        p = ((2 * k + 1) * c * b - k * a) / (k + 1)
+        a, b = b, p
+    return p
+
+
+def test_polynomial_warps(orc):
+    check_pin(orc, "chebyshev", lambda x, y, w, nx, ny: (w * _cheb_t(x, int(nx)), w * _cheb_t(y, int(ny))),
+              1.0, (4.0, 3.0), [(0.5, 0.3), (-0.7, 0.4), (0.9, -0.5), (1.4, -1.8)], 4)   # :127-137
+    assert abs(run(orc, "chebyshev", f32(math.cos(0.4)), 0.0, 1.0, (3.0, 0.0), np.float32)[0]
+               - math.cos(1.2)) < 5e-5                                                     # :138-143
+    p = run(orc, "chebyshev", 0.6, 0.6, 1.0, (2.0, 3.0))
+    n = run(orc, "chebyshev", -0.6, -0.6, 1.0, (2.0, 3.0))
+    assert n[0] == pytest.approx(p[0]) and n[1] == pytest.approx(-p[1])                   # :144-151
+    sweep = [(-1 + 0.25 * k, -1 + 0.25 * k) for k in range(9)]
+    for x, y in sweep:                                                                     # :152-159
+        o = run(orc, "chebyshev", f32(x), f32(y), 1.0, (12.0, 11.0), np.float32)
+        assert abs(o[0]) <= 1 + 1e-5 and abs(o[1]) <= 1 + 1e-5
+        o = run(orc, "legendre", f32(x), f32(y), 1.0, (9.0, 8.0), np.float32)           # :185-192
+        assert abs(o[0]) <= 1 + 1e-5 and abs(o[1]) <= 1 + 1e-5
+    check_pin(orc, "legendre", lambda x, y, w, nx, ny: (w * _leg_p(x, int(nx)), w * _leg_p(y, int(ny))),
+              1.0, (5.0, 4.0), [(0.5, 0.3), (-0.7, 0.4), (0.9, -0.5)], 4)                  # :162-172
+    assert abs(run(orc, "legendre", 0.6, 0.0, 1.0, (2.0, 0.0), np.float32)[0] - 0.04) < 5e-5   # :173-178
+    assert abs(run(orc, "legendre", 1.0, 0.0, 1.0, (5.0, 0.0), np.float32)[0] - 1.0) < 5e-5    # :179-184
+    pts3 = [(0.5, 0.3), (-0.7, 0.4), (0.9, -0.5)]
+    check_pin(orc, "spherical_harmonic", lambda x, y, w, *q: (w * x, w * y), 1.0, (0.0, 0.0, 1.0),
+              pts3, 4, f64_rel=1e-6)                                                       # :196-205
+    for x, y in pts3:                                                                      # :206-215
+        o = run(orc, "spherical_harmonic", f32(x), f32(y), 1.0, (3.0, 2.0, 1.0), np.float32)
+        assert abs(o[0] * y - o[1] * x) < 1e-4
+    assert run(orc, "spherical_harmonic", 0.0, 0.0, 1.0, (3.0, 2.0, 1.0), np.float32)[:2] == (0.0, 0.0)   # :216-222
+    for k in range(5):                                                                     # :223-236
+        r = 0.2 + 0.3 * k
+        o = run(orc, "spherical_harmonic", f32(0.6 * r), f32(0.8 * r), 1.0, (6.0, 6.0, 2.0), np.float32)
+        assert np.isfinite(o[:2]).all() and math.hypot(o[0], o[1]) <= 3 * r + 1e-4
+    check_pin(orc, "fourier_warp", lambda x, y, w, *q: (w * x, w * y), 1.0, (4.0, 0.0, 1.0), pts3, 4)   # :239-248
+    for k in range(16):                                                                    # :249-261
+        x, y = math.cos(0.4 * k), math.sin(0.4 * k)
+        o = run(orc, "fourier_warp", f32(x), f32(y), 1.0, (4.0, 0.4, 1.0), np.float32)
+        assert abs(o[0] * y - o[1] * x) < 1e-4 and o[0] * x + o[1] * y >= 0
+    pts = [(0.7, 0.4), (-0.5, 0.6), (0.3, -0.9)]                                           # :262-271
+    assert any(abs(a - b) > 1e-4 for x, y in pts
+               for a, b in zip(run(orc, "fourier_warp", x, y, 1.0, (4.0, 0.4, 1.0))[:2],
+                               run(orc, "fourier_warp", x, y, 1.0, (4.0, 0.4, 7.0))[:2]))
+
+
+# ------------------------------------------------------------ issue 145 ----
+# escape-time single steps: var_<entry>(z, 1.0, c_re, c_im[, relax])
+LITERALS += [
+    ("burning_ship", (0.3, 0.4), (-0.5, -0.5), (-0.57, -0.26), 4, 4, "issue145-escape-time.gpu.test.ts:88-93"),
+    ("magnet1", (1.0, 0.0), (0.0, 0.0), (1.0, 0.0), 4, 4, "issue145-escape-time.gpu.test.ts:102-107"),
+    ("magnet1", (0.5, 0.0), (0.0, 0.0), (0.5625, 0.0), 4, 4, "issue145-escape-time.gpu.test.ts:108-114"),
+    ("nova", (1.0, 0.0), (0.0, 0.0, 1.0), (1.0, 0.0), 4, 4, "issue145-escape-time.gpu.test.ts:115-119"),
+    ("nova", (0.0, 0.0), (0.0, 0.0, 1.0), (0.0, 0.0), 6, 6, "issue145-escape-time.gpu.test.ts:120-124"),
+    ("halley", (1.0, 0.0), (0.0, 0.0), (1.0, 0.0), 4, 4, "issue145-escape-time.gpu.test.ts:125-129"),
+]
+_GRID145 = [(-1.2 + 0.3 * i, -1.2 + 0.3 * j) for i in range(9) for j in range(9)]
+FINITE += [
+    ("burning_ship", _GRID145, (-0.5, -0.5), "issue145-escape-time.gpu.test.ts:130-144"),
+    ("magnet1", _GRID145, (0.5, 0.3), "issue145-escape-time.gpu.test.ts:130-144"),
+    ("nova", _GRID145, (0.0, 0.0, 1.0), "issue145-escape-time.gpu.test.ts:130-144"),
+    ("halley", _GRID145, (0.0, 0.0), "issue145-escape-time.gpu.test.ts:130-144"),
+]
+
+
+def test_escape_time_steps(orc):
+    # one burning-ship step (|Re z| + i |Im z|)^2 + c (the escape-time test's closed form)
+    def bship(x, y, w, cr, ci):
+        a, b = abs(x), abs(y)
+        return w * (a * a - b * b + cr), w * (2 * a * b + ci)
+    check_pin(orc, "burning_ship", bship, 1.0, (-0.5, -0.5), P8 + [(0.3, 0.4)], 5)
+    a = run(orc, "burning_ship", -0.3, 0.4, 1.0, (0.0, 0.0))                     # :94-100
+    b = run(orc, "burning_ship", 0.3, -0.4, 1.0, (0.0, 0.0))
+    assert a[:2] == pytest.approx(b[:2], abs=1e-12)
+    # src/issue145-escape-color.gpu.test.ts: in-gamut, finite band colours; a fast
+    # escaper and a converger get different hues
+    for idx, c, relax in ((V["burning_ship"], (-0.5, -0.5), 0.0), (V["magnet1"], (0.5, 0.3), 0.0),
+                          (V["nova"], (0.0, 0.0), 1.0), (V["halley"], (0.0, 0.0), 0.0)):
+        rgb = orc.dc_color(idx, 0.4, -0.3, (c[0], c[1], relax))
+        assert rgb is not None and all(np.isfinite(v) and 0.0 <= v <= 1.0 for v in rgb)
+    fast = orc.dc_color(V["burning_ship"], 3.0, 3.0, (-0.5, -0.5, 0.0))
+    slow = orc.dc_color(V["burning_ship"], 0.0, 0.0, (-0.5, -0.5, 0.0))
+    assert sum(abs(u - v) for u, v in zip(fast, slow)) > 0.05
+    assert all(0.0 <= v <= 1.0 for v in orc.dc_color(V["burning_ship"], 0.3, 0.4, (-0.5, -0.5, 0.0)))
+
+
+# ------------------------------------------------------- issues 146, 147 --
+def r32(orc, name, x, y, params):
+    return run(orc, name, f32(x), f32(y), 1.0, params, np.float32)
+
+
+def test_optics_entries(orc):
+    # src/issue146-optics.gpu.test.ts V284 snell_refraction: |out| = |p|; n 1 keeps p;
+    # total internal reflection flips the normal component
+    for x, y in ((0.5, 0.3), (-0.7, 0.4), (0.9, -0.5), (0.2, 0.95)):
+        o = r32(orc, "snell_refraction", x, y, (0.67, 1.0))
+        assert abs(math.hypot(o[0], o[1]) - math.hypot(x, y)) < 5e-5
+    for x, y in ((0.5, 0.3), (0.9, -0.4)):
+        o = r32(orc, "snell_refraction", x, y, (1.0, 1.0))
+        assert abs(o[0] - x) < 5e-5 and abs(o[1] - y) < 5e-5
+    o = r32(orc, "snell_refraction", 0.95, 0.31, (2.0, 1.0))
+    assert o[1] < 0 and abs(math.hypot(o[0], o[1]) - math.hypot(0.95, 0.31)) < 5e-5
+    # V285 grin_lens: collinear with the input, origin fixed, bounded near the
+    # focus, thin lens at large r
+    for x, y in ((0.5, 0.3), (-0.7, 0.4), (0.9, -0.5)):
+        o = r32(orc, "grin_lens", x, y, (0.6, 0.15))
+        assert abs(o[0] * y - o[1] * x) < 1e-4
+    assert r32(orc, "grin_lens", 0.0, 0.0, (0.6, 0.15))[:2] == (0.0, 0.0)
+    o = r32(orc, "grin_lens", 0.001, 0.0, (0.6, 0.15))
+    assert np.isfinite(o[0]) and abs(o[0]) < 1.0
+    assert abs(r32(orc, "grin_lens", 2.0, 0.0, (0.6, 0.15))[0] - 2.0) < 0.02
+    # V286 caustic_fold: displacement <= amp sqrt 2, amp 0 identity, separable,
+    # x-displacement -amp sin(freq x) at freq 8 (safe_sin native branch)
+    for x, y in ((0.5, 0.3), (-0.7, 0.4), (0.9, -0.5), (1.2, 0.8)):
+        o = r32(orc, "caustic_fold", x, y, (2.5, 0.35, 0.0))
+        assert math.hypot(o[0] - x, o[1] - y) <= 0.35 * math.sqrt(2) + 1e-4
+    for x, y in ((0.5, 0.3), (-0.7, 0.4)):
+        o = r32(orc, "caustic_fold", x, y, (2.5, 0.0, 0.0))
+        assert abs(o[0] - x) < 5e-6 and abs(o[1] - y) < 5e-6
+    a, b = r32(orc, "caustic_fold", 0.5, 0.3, (2.5, 0.35, 0.0)), r32(orc, "caustic_fold", 0.5, -0.9, (2.5, 0.35, 0.0))
+    assert abs((a[0] - 0.5) - (b[0] - 0.5)) < 5e-6
+    assert abs(r32(orc, "caustic_fold", 5.0, 0.0, (8.0, 0.35, 0.0))[0] - (5.0 - 0.35 * math.sin(40.0))) < 5e-4
+
+
+def _chladni_ref(x, y, w, n, m, step):   # src/issue147-waves.gpu.test.ts jsChladni
+    an, am = PI * n, PI * m
+    f = math.cos(an * x) * math.cos(am * y) - math.cos(am * x) * math.cos(an * y)
+    gx = -an * math.sin(an * x) * math.cos(am * y) + am * math.sin(am * x) * math.cos(an * y)
+    gy = -am * math.cos(an * x) * math.sin(am * y) + an * math.cos(am * x) * math.sin(an * y)
+    s = -1 if f < 0 else 1
+    gx, gy = gx * s, gy * s
+    inv = 1 / (math.hypot(gx, gy) + 1e-4)
+    return w * (x - step * gx * inv), w * (y - step * gy * inv)
+
+
+def _moire_ref(x, y, w, freq, beat, angle, amp):   # jsMoire
+    ca, sa = math.cos(angle), math.sin(angle)
+    u, v = x * ca + y * sa, -x * sa + y * ca
+    gx = math.sin(freq * PI * x) * math.sin((freq + beat) * PI * u)
+    gy = math.sin(freq * PI * y) * math.sin((freq + beat) * PI * v)
+    return w * (x + amp * gx), w * (y + amp * gy)
+
+
+def test_wave_entries(orc):
+
+    def node(x, y):
+        return abs(math.cos(3 * PI * x) * math.cos(5 * PI * y) - math.cos(5 * PI * x) * math.cos(3 * PI * y)) < 0.15
+    check_pin(orc, "chladni", lambda x, y, w, n, m, st: _chladni_ref(x, y, w, int(n), int(m), st), 1.0,
+              (3.0, 5.0, 0.18), [(0.37, 0.62), (-0.41, 0.23), (0.55, -0.71), (0.18, 0.44)], 4, skip=node)
+    for x, y in ((0.4, 0.7), (-0.6, 0.2)):   # (n, m) and (m, n) descend alike
+        a, b = r32(orc, "chladni", x, y, (3.0, 5.0, 0.18)), r32(orc, "chladni", x, y, (5.0, 3.0, 0.18))
+        assert abs(a[0] - b[0]) < 5e-5 and abs(a[1] - b[1]) < 5e-5
+    for i in range(7):
+        for j in range(4):
+            x, y = -0.9 + 0.3 * i, -0.9 + 0.6 * j
+            o = r32(orc, "chladni", x, y, (3.0, 5.0, 0.18))
+            assert math.hypot(o[0] - x, o[1] - y) < 0.18 + 1e-3
+    assert r32(orc, "chladni", 0.5, 0.3, (3.4, 5.0, 0.18))[:2] == \
+        pytest.approx(r32(orc, "chladni", 0.5, 0.3, (2.6, 5.0, 0.18))[:2], abs=5e-6)
+    # V288 standing_wave: amp 0 identity, origin node, modes gate, bounded by amp / (1 - decay)
+    for x, y in ((0.5, 0.3), (-0.7, 0.4)):
+        o = r32(orc, "standing_wave", x, y, (3.0, 1.0, 0.0, 0.6))
+        assert abs(o[0] - x) < 5e-6 and abs(o[1] - y) < 5e-6
+    assert r32(orc, "standing_wave", 0.0, 0.0, (3.0, 1.0, 0.35, 0.6))[:2] == (0.0, 0.0)
+    m1, m4 = r32(orc, "standing_wave", 0.6, 0.4, (1.0, 1.0, 0.35, 0.6)), r32(orc, "standing_wave", 0.6, 0.4, (4.0, 1.0, 0.35, 0.6))
+    assert abs(m1[0] - m4[0]) + abs(m1[1] - m4[1]) > 1e-4
+    for i in range(5):
+        for j in range(3):
+            x, y = -0.9 + 0.45 * i, -0.9 + 0.9 * j
+            o = r32(orc, "standing_wave", x, y, (3.0, 1.0, 0.35, 0.6))
+            assert abs(o[0] - x) < 0.35 / 0.4 + 1e-3 and abs(o[1] - y) < 0.35 / 0.4 + 1e-3
+            o = r32(orc, "moire", x, y, (4.0, 0.6, 0.4, 0.25))
+            assert abs(o[0] - x) <= 0.25 + 1e-4 and abs(o[1] - y) <= 0.25 + 1e-4
+    check_pin(orc, "moire", _moire_ref, 1.0, (4.0, 0.6, 0.4, 0.25),
+              [(0.5, 0.3), (-0.7, 0.4), (0.9, -0.5)] + P8, 4)
+    check_pin(orc, "moire", _moire_ref, 1.0, (4.0, 0.6, 0.4, 0.0), [(0.5, 0.3)], 5)
+    assert r32(orc, "moire", 0.0, 0.0, (4.0, 0.6, 0.4, 0.25))[:2] == (0.0, 0.0)
+
+
+# ------------------------------------------------- issues 148, 149, 150 ----
+LITERALS += [   # var_<entry>(p, 1.0, params) with src/issue149-spirals.gpu.test.ts:55-60's params
+    ("droste", (1.0, 0.0), (2.0,), (1.0, 0.0), 2, 2, "issue149-spirals.gpu.test.ts:99-105"),
+    ("droste", (0.0, 1.0), (2.0,), (0.0, 0.8408964), 2, 5, "issue149-spirals.gpu.test.ts:99-105"),
+    ("logspiral", (1.0, 1.0), (2.0, 0.5), (2.0944111, 2.0944111), 5, 5, "issue149-spirals.gpu.test.ts:106-110"),
+    ("fermat_spiral", (0.0, 1.0), (2.0,), (0.0, 2.5066282), 5, 5, "issue149-spirals.gpu.test.ts:111-119"),
+    ("fermat_spiral", (0.0, -1.0), (2.0,), (0.0, -4.3416075), 5, 5, "issue149-spirals.gpu.test.ts:111-119"),
+    ("fermat_spiral", (1.0, -1.0), (2.0,), (3.3159577, -3.3159577), 5, 5, "issue149-spirals.gpu.test.ts:111-119"),
+    ("lituus", (0.0, 1.0), (2.0,), (0.0, 1.5957691), 5, 5, "issue149-spirals.gpu.test.ts:120-128"),
+    ("lituus", (0.0, -1.0), (2.0,), (0.0, -0.9213177), 5, 5, "issue149-spirals.gpu.test.ts:120-128"),
+    ("lituus", (1.0, -1.0), (2.0,), (0.6031440, -0.6031440), 5, 5, "issue149-spirals.gpu.test.ts:120-128"),
+    ("hyperbolic_spiral", (0.0, 1.0), (2.0,), (0.0, 1.2732395), 5, 5, "issue149-spirals.gpu.test.ts:129-137"),
+    ("hyperbolic_spiral", (0.0, -1.0), (2.0,), (0.0, -0.4244132), 5, 5, "issue149-spirals.gpu.test.ts:129-137"),
+    ("hyperbolic_spiral", (1.0, -1.0), (2.0,), (0.2572333, -0.2572333), 5, 5, "issue149-spirals.gpu.test.ts:129-137"),
+]
+
+
+def test_orbital_entries(orc):
+    # src/issue148-orbitals.gpu.test.ts: 1s monotone, 2s node at r = 2, bounded
+    # by 1.5, angle kept; hydrogen_orbital with lobe_mix 0 is radial_shell, the
+    # p orbital's node on the y axis
+    def rad(o):
+        return math.hypot(o[0], o[1])
+    r = [rad(r32(orc, "radial_shell", x, 0.0, (1.0, 0.0, 1.0))) for x in (0.2, 0.8, 1.5)]
+    assert r[0] > r[1] > r[2]
+    assert rad(r32(orc, "radial_shell", 2.0, 0.0, (2.0, 0.0, 1.0))) < 0.02
+    assert rad(r32(orc, "radial_shell", 0.5, 0.0, (2.0, 0.0, 1.0))) > 0.05
+    for n, l in ((1, 0), (2, 1), (3, 0), (3, 2), (4, 3)):
+        for k in range(11):
+            assert rad(r32(orc, "radial_shell", 0.3 * k, 0.4 * k, (n, l, 1.0))) <= 1.5 + 1e-4
+    for x, y in ((0.5, 0.3), (-0.7, 0.4), (0.2, -0.9)):
+        o = r32(orc, "radial_shell", x, y, (3.0, 0.0, 1.0))
+        if rad(o) >= 1e-4:
+            assert abs(math.atan2(o[1], o[0]) - math.atan2(y, x)) < 5e-5
+        h = r32(orc, "hydrogen_orbital", x, y, (3.0, 2.0, 0.0, 1.0, 0.0))
+        s = r32(orc, "radial_shell", x, y, (3.0, 2.0, 1.0))
+        assert abs(h[0] - s[0]) < 5e-6 and abs(h[1] - s[1]) < 5e-6
+    assert rad(r32(orc, "hydrogen_orbital", 0.0, 0.5, (2.0, 1.0, 0.0, 1.0, 1.0))) < 0.02
+    assert rad(r32(orc, "hydrogen_orbital", 0.5, 0.0, (2.0, 1.0, 0.0, 1.0, 1.0))) > 0.02
+    for n, l, m in ((1, 0, 0), (2, 1, 0), (3, 2, 0), (4, 3, 0)):
+        for i in range(9):
+            for j in range(8):
+                rr, a = 0.2 + 0.6 * i, 0.78 * j
+                o = r32(orc, "hydrogen_orbital", rr * math.cos(a), rr * math.sin(a), (n, l, m, 1.0, 1.0))
+                assert np.isfinite(rad(o)) and rad(o) <= 1.5 + 1e-4
+
+
+def billiard_circle_ref(x, y, w, radius, step, angle):   # src/variations.ts:2475-2522
+    rc = max(radius, 1e-5)
+    px, py = x, y
+    r0 = math.hypot(px, py)
+    if r0 > rc:
+        px, py, r0 = px / r0 * rc, py / r0 * rc, rc
+    bx, by = (px / r0, py / r0) if r0 > 1e-6 else (1.0, 0.0)
+    vx = bx * math.cos(angle) - by * math.sin(angle)
+    vy = bx * math.sin(angle) + by * math.cos(angle)
+    dpv = px * vx + py * vy
+    t = -dpv + math.sqrt(max(dpv * dpv + rc * rc - (px * px + py * py), 0.0))
+    fx, fy = px + step * vx, py + step * vy
+    if 0.0 < t < step:
+        hx, hy = px + t * vx, py + t * vy
+        nx, ny = -hx / rc, -hy / rc
+        d = vx * nx + vy * ny
+        rx, ry = vx - 2 * d * nx, vy - 2 * d * ny
+        fx, fy = hx + (step - t) * rx, hy + (step - t) * ry
+    return w * fx, w * fy
+
+
+def billiard_polygon_ref(x, y, w, sides_f, radius, step, angle):   # src/variations.ts:2752-2840
+    sides = max(3, min(12, int(math.floor(sides_f + 0.5))))
+    R = max(radius, 1e-5)
+    inr = R * math.cos(PI / sides)
+    px, py = x, y
+    r0 = math.hypot(px, py)
+    if r0 > inr:
+        s = inr / max(r0, 1e-6)
+        px, py = px * s, py * s
+    vx, vy = math.cos(angle), math.sin(angle)
+    vs = [(R * math.cos(2 * PI * k / sides), R * math.sin(2 * PI * k / sides)) for k in range(sides)]
+    t_hit, seg = math.inf, -1
+    for k in range(sides):
+        (ax, ay), (bx, by) = vs[k], vs[(k + 1) % sides]
+        dx, dy = bx - ax, by - ay
+        det = -vx * dy + vy * dx
+        if abs(det) > 1e-6:
+            t = (-(ax - px) * dy + (ay - py) * dx) / det
+            u = (vx * (ay - py) - vy * (ax - px)) / det
+            if t >= 0 and 0 <= u <= 1 and t < t_hit:
+                t_hit, seg = t, k
+    fx, fy = px + step * vx, py + step * vy
+    if 0.0 < t_hit < step and seg != -1:
+        hx, hy = px + t_hit * vx, py + t_hit * vy
+        (ax, ay), (bx, by) = vs[seg], vs[(seg + 1) % sides]
+        ex, ey = bx - ax, by - ay
+        el = max(math.hypot(ex, ey), 1e-5)
+        nx, ny = -ey / el, ex / el
+        if nx * (ax + bx) / 2 + ny * (ay + by) / 2 > 0:
+            nx, ny = -nx, -ny
+        d = vx * nx + vy * ny
+        rx, ry = vx - 2 * d * nx, vy - 2 * d * ny
+        fx, fy = hx + (step - t_hit) * rx, hy + (step - t_hit) * ry
+    return w * fx, w * fy
+
+
+def _reflect(px, py, vx, vy, step, t, nx, ny):
+    """Flight of `step` from p along v, reflected off a wall met at 0 < t < step."""
+    if not 0.0 < t < step:
+        return px + step * vx, py + step * vy
+    hx, hy = px + t * vx, py + t * vy
+    d = vx * nx + vy * ny
+    return hx + (step - t) * (vx - 2 * d * nx), hy + (step - t) * (vy - 2 * d * ny)
+
+
+def billiard_stadium_ref(x, y, w, width, height, step, angle):   # src/variations.ts:2525-2636
+    R, w2 = max(height / 2, 1e-5), max(width / 2, 1e-5)
+    px, py = x, y
+    if abs(px) > w2:
+        cx = w2 if px > 0 else -w2
+        d = math.hypot(px - cx, py)
+        if d > R:
+            px, py = cx + (px - cx) / d * R, py / d * R
+    elif abs(py) > R:
+        py = R if py > 0 else -R
+    vx, vy = math.cos(angle), math.sin(angle)
+    best, wall = math.inf, 0
+    for k, (sgn, ok) in enumerate(((1, vy > 0), (-1, vy < 0)), start=1):   # top, bottom
+        if ok:
+            t = (sgn * R - py) / vy
+            if t >= 0 and -w2 <= px + t * vx <= w2 and t < best:
+                best, wall = t, k
+    for k, c in ((3, w2), (4, -w2)):   # right, left caps
+        dx = px - c
+        dpv = dx * vx + py * vy
+        disc = dpv * dpv + R * R - (dx * dx + py * py)
+        if disc >= 0:
+            t = -dpv + math.sqrt(disc)
+            xx = px + t * vx
+            if t >= 0 and (xx >= w2 - 1e-5 if k == 3 else xx <= -w2 + 1e-5) and t < best:
+                best, wall = t, k
+    nx = ny = 0.0
+    if 0.0 < best < step:
+        hx, hy = px + best * vx, py + best * vy
+        if wall in (1, 2):
+            ny = -1.0 if wall == 1 else 1.0
+        else:
+            c = w2 if wall == 3 else -w2
+            rl = max(math.hypot(hx - c, hy), 1e-5)
+            nx, ny = -(hx - c) / rl, -hy / rl
+    fx, fy = _reflect(px, py, vx, vy, step, best, nx, ny)
+    return w * fx, w * fy
+
+
+def billiard_sinai_ref(x, y, w, length, radius, step, angle):   # src/variations.ts:2639-2750
+    L2, R = max(length / 2, 1e-5), max(radius, 1e-5)
+    px, py = min(max(x, -L2), L2), min(max(y, -L2), L2)
+    r0 = math.hypot(px, py)
+    assert r0 > 1e-6   # the origin start takes a hashed direction: not restated here
+    if r0 < R:
+        px, py = px / r0 * R, py / r0 * R
+    vx, vy = math.cos(angle), math.sin(angle)
+    best, wall = math.inf, 0
+    if vx < 0:
+        t = (-L2 - px) / vx
+        if t >= 0 and -L2 <= py + t * vy <= L2 and t < best:
+            best, wall = t, 1
+    if vx > 0:
+        t = (L2 - px) / vx
+        if t >= 0 and -L2 <= py + t * vy <= L2 and t < best:
+            best, wall = t, 2
+    if vy < 0:
+        t = (-L2 - py) / vy
+        if t >= 0 and -L2 <= px + t * vx <= L2 and t < best:
+            best, wall = t, 3
+    if vy > 0:
+        t = (L2 - py) / vy
+        if t >= 0 and -L2 <= px + t * vx <= L2 and t < best:
+            best, wall = t, 4
+    dpv = px * vx + py * vy
+    disc = dpv * dpv - (px * px + py * py - R * R)
+    if disc >= 0:
+        t1 = -dpv - math.sqrt(disc)
+        if 0 <= t1 < best:
+            best, wall = t1, 5
+    nx, ny = {1: (1.0, 0.0), 2: (-1.0, 0.0), 3: (0.0, 1.0), 4: (0.0, -1.0)}.get(wall, (0.0, 0.0))
+    if wall == 5:
+        nx, ny = (px + best * vx) / R, (py + best * vy) / R
+    fx, fy = _reflect(px, py, vx, vy, step, best, nx, ny)
+    return w * fx, w * fy
+
+
+def test_billiards(orc):
+    # src/issue150-billiards.gpu.test.ts:128-206 (toBeCloseTo 4) against the
+    # reference's f64 oracles (src/variations.ts), at the test's points and
+    # parameters and at the shared points with other ones
+    check_pin(orc, "billiard_circle", billiard_circle_ref, 1.0, (1.0, 0.5, 0.0),
+              [(0.0, 0.0), (0.5, 0.0), (0.9, 0.1), (1.2, -0.5)], 4)
+    # extra points and parameters in f64 only: a start clamped onto a wall meets it
+    # at t ~ 0, where f32 rounding decides whether the flight reflects
+    ex = dict(f64_rel=1e-7, f32=False)
+    check_pin(orc, "billiard_circle", billiard_circle_ref, 1.0, (1.2, 0.7, 2.2), P8, 4, **ex)
+    check_pin(orc, "billiard_polygon", billiard_polygon_ref, 1.0, (5.0, 1.0, 0.5, 0.7),
+              [(0.0, 0.0), (0.5, -0.2), (-0.8, 0.8)], 4)
+    check_pin(orc, "billiard_polygon", billiard_polygon_ref, 1.0, (7.0, 1.3, 0.9, -1.1), P8, 4, **ex)
+    check_pin(orc, "billiard_stadium", billiard_stadium_ref, 1.0, (1.5, 1.0, 0.5, 0.5),
+              [(0.0, 0.0), (0.7, 0.4), (1.0, -0.2), (-1.5, 1.2)], 4)
+    check_pin(orc, "billiard_stadium", billiard_stadium_ref, 1.0, (1.5, 1.0, 0.5, 0.5), P8, 4, **ex)
+    check_pin(orc, "billiard_stadium", billiard_stadium_ref, 1.0, (2.2, 1.4, 0.9, -2.4), P8, 4, **ex)
+    check_pin(orc, "billiard_sinai", billiard_sinai_ref, 1.0, (2.0, 0.5, 0.5, 0.3),
+              [(0.7, 0.7), (-0.9, -0.9), (0.3, 0.4)], 4)
+    check_pin(orc, "billiard_sinai", billiard_sinai_ref, 1.0, (2.0, 0.5, 0.5, 0.3), P8, 4, **ex)
+    check_pin(orc, "billiard_sinai", billiard_sinai_ref, 1.0, (2.4, 0.3, 0.8, 2.0), P8, 4, **ex)
+    # the origin start (:170) is finite
+    assert np.isfinite(r32(orc, "billiard_sinai", 0.0, 0.0, (2.0, 0.5, 0.5, 0.3))[:2]).all()
+
+
+# ------------------------------------------- radial inverse CDFs (V292-V295) --
+# The quantile functions are the textbook ones (Weibull, logistic, Cauchy,
+# Pareto) applied to u = r^2 / (1 + r^2), clamped as the test's uMap does
+# (src/issue151-cdf.gpu.test.ts:81).
+def _umap(r2):
+    return min(max(r2 / (1.0 + r2), 1e-4), 1.0 - 1e-4)
+
+
+def _radial(x, y, w, rp):
+    r = math.hypot(x, y)
+    if r < 1e-9:
+        return 0.0, 0.0
+    return w * rp * x / r, w * rp * y / r
+
+
+def weibull_cdf_ref(x, y, w, lam, k):
+    u = _umap(x * x + y * y)
+    return _radial(x, y, w, lam * (-math.log1p(-u)) ** (1.0 / max(abs(k), 0.05)))
+
+
+def logistic_cdf_ref(x, y, w, mu, s):
+    u = _umap(x * x + y * y)
+    return _radial(x, y, w, max(mu + s * min(max(math.log(u / (1.0 - u)), -12.0), 12.0), 0.0))
+
+
+def cauchy_cdf_ref(x, y, w, gamma, tail):
+    u = _umap(x * x + y * y)
+    cap = max(tail, 0.01)
+    return _radial(x, y, w, min(max(gamma * math.tan(PI * (u - 0.5)), -cap), cap))
+
+
+def pareto_cdf_ref(x, y, w, xm, alpha, tail):
+    u = _umap(x * x + y * y)
+    return _radial(x, y, w, min(xm * (1.0 - u) ** (-1.0 / max(abs(alpha), 0.05)), max(tail, 0.01)))
+
+
+def test_radial_inverse_cdfs(orc):
+    # src/issue151-cdf.gpu.test.ts:84-150 (toBeCloseTo 4 / 3)
+    pts = [(0.0, 0.0), (0.5, 0.0), (0.0, 0.7), (0.3, 0.0), (0.9, 0.0)]
+    check_pin(orc, "weibull_cdf", weibull_cdf_ref, 1.0, (0.6, 1.5), pts + P8, 4, f64_rel=1e-7)
+    o = [r32(orc, "weibull_cdf", x, y, (0.6, 1.5)) for x, y in pts]
+    assert math.hypot(*o[0][:2]) < 1e-6 and abs(o[1][1]) < 1e-5 and abs(o[2][0]) < 1e-5
+    assert math.hypot(*o[4][:2]) > math.hypot(*o[3][:2])        # monotone in r
+    check_pin(orc, "logistic_cdf", logistic_cdf_ref, 1.0, (0.0, 0.35),
+              [(1.0, 0.0), (0.3, 0.0), (50.0, 0.0)] + P8, 3, f64_rel=1e-7)
+    assert math.hypot(*r32(orc, "logistic_cdf", 1.0, 0.0, (0.0, 0.35))[:2]) < 1e-4
+    assert math.hypot(*r32(orc, "logistic_cdf", 50.0, 0.0, (0.0, 0.35))[:2]) < \
+        0.35 * math.log(9999) + 1e-3                                   # the logit cap
+    check_pin(orc, "logistic_cdf", logistic_cdf_ref, 1.0, (0.4, -0.8), P8, 4, f64_rel=1e-7)
+    check_pin(orc, "cauchy_cdf", cauchy_cdf_ref, 1.0, (0.25, 4.0),
+              [(1.0, 0.0), (0.5, 0.0), (30.0, 0.0)] + P8, 4, f64_rel=1e-7)
+    assert math.hypot(*r32(orc, "cauchy_cdf", 1.0, 0.0, (0.25, 4.0))[:2]) < 1e-4   # median
+    assert abs(math.hypot(*r32(orc, "cauchy_cdf", 30.0, 0.0, (0.25, 4.0))[:2]) - 4.0) < 5e-5
+    check_pin(orc, "pareto_cdf", pareto_cdf_ref, 1.0, (0.3, 1.6, 4.0),
+              [(0.0, 0.0), (0.2, 0.0), (50.0, 0.0)] + P8, 4, f64_rel=1e-7)
+    assert math.hypot(*r32(orc, "pareto_cdf", 0.0, 0.0, (0.3, 1.6, 4.0))[:2]) == 0.0
+    assert math.hypot(*r32(orc, "pareto_cdf", 0.2, 0.0, (0.3, 1.6, 4.0))[:2]) >= 0.3 - 1e-4
+
+
+# ---------------------------------------------------- wavelets (V296-V298) --
+def morlet_ref(x, y, w, freq, sigma, amp):
+    r = math.hypot(x, y)
+    t = r / max(sigma, 0.05)
+    return w * (1.0 + amp * math.exp(-0.5 * min(t * t, 80.0)) * math.cos(freq * r)) * x, \
+        w * (1.0 + amp * math.exp(-0.5 * min(t * t, 80.0)) * math.cos(freq * r)) * y
+
+
+def mexican_hat_ref(x, y, w, sigma, amp):
+    t2 = min((x * x + y * y) / max(sigma, 0.05) ** 2, 80.0)
+    s = 1.0 + amp * (1.0 - t2) * math.exp(-0.5 * t2)
+    return w * s * x, w * s * y
+
+
+def chirp_ref(x, y, w, alpha, amp, decay):
+    r2 = x * x + y * y
+    s = 1.0 + amp * math.exp(-decay * min(r2, 160.0)) * math.sin(alpha * r2)
+    return w * s * x, w * s * y
+
+
+def test_wavelets(orc):
+    # src/issue152-wavelets.gpu.test.ts:81-140 (toBeCloseTo 4 / 5): the
+    # Morlet, Ricker and linear-chirp wavelets as radial gains 1 + amp * psi
+    pts = [(0.0, 0.0), (0.5, 0.3), (-0.7, 0.4), (3.0, 4.0)]
+    check_pin(orc, "morlet", morlet_ref, 1.0, (6.0, 0.5, 0.35), pts + P8, 4, f64_rel=1e-7)
+    for x, y in pts[1:3]:
+        o = r32(orc, "morlet", x, y, (6.0, 0.5, 0.35))
+        assert abs(o[0] * y - o[1] * x) < 1e-5                 # direction kept
+    o = r32(orc, "morlet", 1.0, 0.0, (2.0e6, 0.5, 0.35))      # past the trig cliff
+    assert np.isfinite(o[:2]).all() and abs(o[0]) <= 1.35 + 1e-3
+    sig = 0.45
+    hat_pts = [(0.0, 0.0), (sig, 0.0), (sig * math.sqrt(3), 0.0), (0.3, 0.5), (-0.3, -0.5)]
+    check_pin(orc, "mexican_hat", mexican_hat_ref, 1.0, (sig, 0.4), hat_pts + P8, 5, f64_rel=1e-7)
+    assert abs(r32(orc, "mexican_hat", sig, 0.0, (sig, 0.4))[0] - sig) < 5e-6   # zero crossing
+    assert abs(r32(orc, "mexican_hat", sig * math.sqrt(3), 0.0, (sig, 0.4))[0]) < sig * math.sqrt(3)
+    a = r32(orc, "mexican_hat", 0.3, 0.5, (sig, 0.4))
+    b = r32(orc, "mexican_hat", -0.3, -0.5, (sig, 0.4))
+    assert a[0] == -b[0] and a[1] == -b[1]                     # odd map, exactly
+    check_pin(orc, "chirp", chirp_ref, 1.0, (8.0, 0.3, 0.6), [(0.0, 0.0)] + P8, 4, f64_rel=1e-7)
+    check_pin(orc, "chirp", chirp_ref, 1.0, (8.0, 0.3, 4.0), [(2.0, 0.0)], 4)
+    o = r32(orc, "chirp", 0.0, 0.0, (8.0, 0.3, 0.6))
+    assert o[0] == 0.0 and o[1] == 0.0
+    o = r32(orc, "chirp", 1.0, 0.0, (2.0e7, 0.3, 0.0))        # alpha r^2 past the cliff
+    assert np.isfinite(o[:2]).all() and abs(o[0]) <= 1.3 + 1e-3
+
+
+# -------------------------------------------- celestial mechanics (V299-V301) --
+def kepler_ref(x, y, w, e_in, scale):
+    """Two Newton steps on Kepler's equation M = E - e sin E from E = M, the
+    polar angle as mean anomaly (src/issue153-celestial.gpu.test.ts:79-84)."""
+    e = min(max(e_in, 0.0), 0.95)
+    m = math.atan2(y, x)
+    a = scale * (math.hypot(x, y) + 1e-6)
+    ea = m
+    for _ in range(2):
+        ea -= (ea - e * math.sin(ea) - m) / max(1.0 - e * math.cos(ea), 0.05)
+    return w * a * (math.cos(ea) - e), w * a * math.sqrt(max(1.0 - e * e, 0.0)) * math.sin(ea)
+
+
+def three_body_ref(x, y, w, mu_in, step, coriolis):
+    """A step along the gradient of the rotating-frame effective potential of
+    the restricted three-body problem, softened by 1e-3, plus a Coriolis turn,
+    the displacement capped at 2 (src/issue153-celestial.gpu.test.ts:85-92)."""
+    mu = min(max(mu_in, 0.01), 0.5)
+    r1x, r2x = x + mu, x - (1.0 - mu)
+    d1 = (r1x * r1x + y * y + 1e-3) ** 1.5
+    d2 = (r2x * r2x + y * y + 1e-3) ** 1.5
+    gx = x - (1.0 - mu) * r1x / d1 - mu * r2x / d2
+    gy = y - (1.0 - mu) * y / d1 - mu * y / d2
+    dx, dy = step * (gx - coriolis * gy), step * (gy + coriolis * gx)
+    dl = math.hypot(dx, dy)
+    if dl > 2.0:
+        dx, dy = dx * 2.0 / dl, dy * 2.0 / dl
+    return w * (x + dx), w * (y + dy)
+
+
+def hill_ref(x, y, w, kappa, phase, shear):
+    """Hill's linearised epicycle propagator over phase*kappa plus the
+    guiding-centre shear (src/issue153-celestial.gpu.test.ts:93-96)."""
+    k = min(max(kappa, 0.2), 2.0)
+    c, s = math.cos(phase * k), math.sin(phase * k)
+    return w * (x * c - s / k * y), w * (2.0 * k * s * x + y * c + shear * x)
+
+
+def test_celestial(orc):
+    # src/issue153-celestial.gpu.test.ts:98-170
+    circ = [(0.5, 0.3), (-0.7, 0.4), (0.9, -0.5)]
+    for x, y in circ:                                            # e = 0: the circle
+        o = r32(orc, "kepler_orbit", x, y, (0.0, 1.0))
+        assert abs(o[0] - x) < 5e-5 and abs(o[1] - y) < 5e-5
+    check_pin(orc, "kepler_orbit", kepler_ref, 1.0, (0.5, 0.6),
+              [(0.5, 0.0), (0.4, 0.6), (-0.8, 0.2)] + P8, 3, f64_rel=1e-7)
+    o = r32(orc, "kepler_orbit", 0.5, 0.0, (0.5, 0.6))           # perihelion a(1-e)
+    assert abs(o[0] - 0.15) < 5e-4 and abs(o[1]) < 1e-3
+    check_pin(orc, "kepler_orbit", kepler_ref, 1.0, (0.9, 1.3), P8, 3, f64_rel=1e-7)
+    check_pin(orc, "restricted_3body", three_body_ref, 1.0, (0.2, 0.15, 0.0),
+              [(0.5, 0.3), (-0.4, 0.5), (0.2, -0.6)] + P8, 4, f64_rel=1e-7)
+    check_pin(orc, "restricted_3body", three_body_ref, 1.0, (0.2, 0.15, 0.4),
+              [(0.8, 0.0), (0.5, 0.3)] + P8, 4, f64_rel=1e-7)
+    o = r32(orc, "restricted_3body", 0.5, 0.3, (0.2, 0.15, 0.4))
+    assert math.hypot(o[0] - 0.5, o[1] - 0.3) <= 0.15 * 2.0 + 1e-4
+    for x, y in [(0.5, 0.3), (-0.7, 0.4)]:                       # phase 0: shear only
+        o = r32(orc, "hill_epicyclic", x, y, (1.0, 0.0, 0.5))
+        assert abs(o[0] - x) < 5e-6 and abs(o[1] - (y + 0.5 * x)) < 5e-6
+    check_pin(orc, "hill_epicyclic", hill_ref, 1.0, (1.0, 1.2, 0.5), [(0.5, 0.3), (1.0, 0.6)] + P8, 4)
+    a = r32(orc, "hill_epicyclic", 0.5, 0.3, (1.0, 1.2, 0.5))
+    b = r32(orc, "hill_epicyclic", 1.0, 0.6, (1.0, 1.2, 0.5))
+    assert abs(b[0] - 2 * a[0]) < 5e-5 and abs(b[1] - 2 * a[1]) < 5e-5   # linear
+    check_pin(orc, "hill_epicyclic", hill_ref, 1.0, (3.0, -2.2, -0.7), P8, 4)
+
+
+# -- end of pins --
