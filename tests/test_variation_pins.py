@@ -1758,4 +1758,477 @@ def test_celestial(orc):
     check_pin(orc, "hill_epicyclic", hill_ref, 1.0, (3.0, -2.2, -0.7), P8, 4)
 
 
+# ------------------------------------- conformal maps (V317 SC, V318 Doyle) --
+def sc_series_ref(x, y, w, sides):
+    """Ten terms of the Schwarz-Christoffel disk -> regular n-gon map
+    w(z) = int_0^z (1 - t^n)^(-2/n) dt as its binomial series
+    sum_k (2/n)_k / k! z^(nk+1) / (nk+1), |z| clamped to 0.999
+    (src/shaders/chaos.wgsl:7012-7020 describes the series)."""
+    n = max(round(sides), 3)
+    z = complex(x, y)
+    if abs(z) > 0.999:
+        z *= 0.999 / abs(z)
+    acc, c = 0j, 1.0
+    for k in range(10):
+        acc += c / (n * k + 1) * z ** (n * k + 1)
+        c *= (2.0 / n + k) / (k + 1)
+    return w * acc.real, w * acc.imag
+
+
+def _sc_integral(z, n):
+    """The defining integral along the segment 0 -> z (Gauss-Legendre)."""
+    t, wt = np.polynomial.legendre.leggauss(64)
+    s = 0.5 * (t + 1.0)
+    return complex(np.sum(0.5 * wt * z * (1.0 - (s * z) ** n) ** (-2.0 / n)))
+
+
+def doyle_ref(x, y, w, dp, dq):
+    pp = max(abs(dp), 1.0)
+    u, v = 0.5 * math.log(x * x + y * y + 1e-20), math.atan2(y, x)
+    pitch = dq / pp
+    us = u + pitch * v
+    rip = 0.12 * (math.sin(pp * v) + math.cos(pp * v - 2 * PI * us))
+    vs = v + pitch * rip
+    e = math.exp(min(max(us + rip, -20.0), 20.0))
+    return w * e * math.cos(vs), w * e * math.sin(vs)
+
+
+def test_conformal_maps(orc):
+    # src/issue154-sc-doyle.gpu.test.ts:87-150
+    grid = [(0.0, 0.0), (0.3, 0.4), (-0.6, 0.2), (0.5, -0.5), (0.99, 0.0), (1.5, 1.5),
+            (8.0, -8.0), (0.001, 0.001)]
+    for n in (5.0, 3.0, 8.0):
+        check_pin(orc, "schwarz_christoffel", sc_series_ref, 1.0, (n,), grid + P8, 5, f64_rel=1e-7)
+        # well inside the disk ten terms are the integral itself
+        for x, y in [(0.3, 0.4), (-0.6, 0.2), (0.5, -0.5), (0.1, 0.05)]:
+            g = run(orc, "schwarz_christoffel", x, y, 1.0, (n,))
+            want = _sc_integral(complex(x, y), int(n))
+            assert abs(complex(g[0], g[1]) - want) < 1e-6, (n, x, y)
+    for x, y in grid:
+        o = r32(orc, "schwarz_christoffel", x, y, (5.0,))
+        assert np.isfinite(o[:2]).all() and math.hypot(o[0], o[1]) < 4
+    o = r32(orc, "schwarz_christoffel", 0.01, 0.02, (5.0,))
+    assert abs(o[0] - 0.01) < 5e-5 and abs(o[1] - 0.02) < 5e-5           # w(z) ~ z
+    dgrid = [(0.5, 0.7), (-1.2, 0.4), (0.3, -2.1), (3.0, 3.0), (-3.0, -3.0), (8.0, -8.0),
+             (0.05, 0.05), (1.0, 0.0)]
+    check_pin(orc, "doyle", doyle_ref, 1.0, (2.0, 1.0), dgrid + P8, 3, f64_rel=1e-7)
+    check_pin(orc, "doyle", doyle_ref, 1.0, (3.7, -1.4), dgrid + P8, 3, f64_rel=1e-7)
+    assert abs(r32(orc, "doyle", 1.0, 0.0, (2.0, 0.0))[1]) < 5e-5        # pitch 0 on the axis
+    a = r32(orc, "doyle", 0.6, 0.8, (2.0, 1.0))
+    b = r32(orc, "doyle", 0.6, 0.8, (2.001, 1.0))
+    assert math.hypot(a[0] - b[0], a[1] - b[1]) < 0.05                   # continuous in p
+
+
+# ------------------------------------------------- knots (V302, V303) --
+def torus_knot_ref(x, y, w, pp, qq, radius, tube):
+    phi = math.atan2(y, x)
+    pf, qf = max(round(pp), 1.0), max(round(qq), 1.0)
+    rr = radius + tube * min(max(math.hypot(x, y), 0.0), 1.0) * math.cos(qf * phi)
+    return w * rr * math.cos(pf * phi), w * rr * math.sin(pf * phi)
+
+
+def braid_ref(x, y, w, strands, twist, crossings):
+    n, cr = max(round(strands), 2.0), max(round(crossings), 1.0)
+    r, th = math.hypot(x, y), math.atan2(y, x)
+    a = (th + PI) / (2 * PI)
+    lane = math.floor(a * n)
+    fr = a * n - lane
+    sgn = -1.0 if int(lane) & 1 else 1.0
+    nt = th + twist * sgn * math.sin(PI * fr) * math.sin(cr * PI * fr) * min(r, 1.0)
+    return w * r * math.cos(nt), w * r * math.sin(nt)
+
+
+def test_knots(orc):
+    # src/issue155-knots.gpu.test.ts: torus_knot bounded by radius + tube, on
+    # the radius circle at tube 0, p-fold symmetric; braid_warp keeps |p|
+    pts = [(x * 0.5, float(y)) for x in range(-4, 5) for y in range(-2, 3)]
+    check_pin(orc, "torus_knot", torus_knot_ref, 1.0, (3.0, 2.0, 0.6, 0.3), pts + P8, 4, f64_rel=1e-7)
+    assert abs(math.hypot(*r32(orc, "torus_knot", 0.0, 0.0, (3.0, 2.0, 0.6, 0.3))[:2]) - 0.6) < 5e-4
+    for x, y in pts:
+        assert math.hypot(*r32(orc, "torus_knot", x, y, (3.0, 2.0, 0.6, 0.3))[:2]) <= 0.95
+    for x, y in [(0.5, 0.3), (-0.7, 0.4), (0.9, -0.5)]:
+        assert abs(math.hypot(*r32(orc, "torus_knot", x, y, (3.0, 2.0, 0.6, 0.0))[:2]) - 0.6) < 5e-4
+    pa = (0.8 * math.cos(0.4), 0.8 * math.sin(0.4))
+    pb = (0.8 * math.cos(0.4 + 2 * PI / 3), 0.8 * math.sin(0.4 + 2 * PI / 3))
+    a, b = r32(orc, "torus_knot", *pa, (3.0, 2.0, 0.6, 0.0)), r32(orc, "torus_knot", *pb, (3.0, 2.0, 0.6, 0.0))
+    assert abs(a[0] - b[0]) < 5e-4 and abs(a[1] - b[1]) < 5e-4
+    ring = [(r * math.cos(0.4 * i), r * math.sin(0.4 * i)) for i in range(16) for r in (0.3, 0.8, 1.5)]
+    check_pin(orc, "braid_warp", braid_ref, 1.0, (3.0, 0.6, 2.0), ring + P8, 4, f64_rel=1e-7)
+    check_pin(orc, "braid_warp", braid_ref, 1.0, (2.0, 2.0, 8.0), ring + P8, 4, f64_rel=1e-7)
+    for x, y in ring:
+        o = r32(orc, "braid_warp", x, y, (3.0, 0.6, 2.0))
+        assert abs(math.hypot(o[0], o[1]) - math.hypot(x, y)) < 5e-5
+    th = 2 * PI / 3 - PI                                                 # a lane edge
+    o = r32(orc, "braid_warp", 0.7 * math.cos(th), 0.7 * math.sin(th), (3.0, 0.6, 2.0))
+    assert abs(o[0] - 0.7 * math.cos(th)) < 5e-4 and abs(o[1] - 0.7 * math.sin(th)) < 5e-4
+    o = r32(orc, "braid_warp", 0.0, 0.0, (3.0, 0.6, 2.0))
+    assert o[0] == 0.0 and o[1] == 0.0
+
+
+# -------------------------------------------------- airy_caustic (V304) --
+def airy_ai_ref(x):
+    """Ai(x): the Maclaurin series c1 f(x) - c2 g(x) with twelve terms on
+    [-5, 4] (A&S 10.4.2), the leading asymptotic terms outside (A&S 10.4.59
+    and 10.4.60); src/issue216-airy-caustic.gpu.test.ts:90-97."""
+    if x > 4.0:
+        xi = 2.0 / 3.0 * x ** 1.5
+        return math.exp(-xi) / (2.0 * math.sqrt(PI) * x ** 0.25)
+    if x < -5.0:
+        xi = 2.0 / 3.0 * (-x) ** 1.5
+        return math.sin(xi + PI / 4) / (math.sqrt(PI) * (-x) ** 0.25)
+    x3 = x ** 3
+    f = tf = 1.0
+    g = tg = x
+    for k in range(1, 12):
+        tf *= x3 / ((3 * k - 1) * (3 * k))
+        tg *= x3 / ((3 * k) * (3 * k + 1))
+        f += tf
+        g += tg
+    return 0.355028053887817 * f - 0.258819403792807 * g
+
+
+def airy_caustic_ref(x, y, w, scale, r0, amp):   # :98-103
+    env = min(max(1.0 + amp * airy_ai_ref(scale * (math.hypot(x, y) - r0)), 0.05), 3.0)
+    return w * env * x, w * env * y
+
+
+def test_airy_caustic(orc):
+    from scipy.special import airy
+    # the series and the asymptotic forms against Ai itself
+    # (twelve terms truncate to 1e-11 on [-3, 3], 3.4e-4 at x = -5)
+    for x in np.linspace(-3.0, 3.0, 25):
+        assert abs(airy_ai_ref(x) - airy(x)[0]) < 1e-10, x
+    for x in np.linspace(-5.0, 4.0, 37):
+        assert abs(airy_ai_ref(x) - airy(x)[0]) < 4e-4, x
+    for x in (4.5, 6.0, 9.0):
+        assert abs(airy_ai_ref(x) / airy(x)[0] - 1.0) < 0.02, x   # 5/(72 xi) relative
+    for x in (-5.5, -8.0, -12.0):
+        assert abs(airy_ai_ref(x) - airy(x)[0]) < 0.01, x
+    # src/issue216-airy-caustic.gpu.test.ts:106-143
+    pts = [(0.3, 0.0), (0.0, 0.7), (0.9, 0.4), (1.1, 0.0), (1.5, -0.6)]
+    check_pin(orc, "airy_caustic", airy_caustic_ref, 1.0, (4.5, 1.1, 2.2), pts + P8, 4, f64_rel=1e-7)
+    check_pin(orc, "airy_caustic", airy_caustic_ref, 1.0, (2.0, 0.4, -1.3), P8, 4, f64_rel=1e-7)
+    for x in (0.05, 0.5, 1.0, 2.0, 5.0):
+        m = math.hypot(*r32(orc, "airy_caustic", x, 0.0, (4.5, 1.1, 2.2))[:2])
+        assert 0.05 * x - 1e-4 <= m <= 3.0 * x + 1e-4
+    assert abs(r32(orc, "airy_caustic", 6.0, 0.0, (4.5, 1.1, 2.2))[0] - 6.0) < 5e-3
+
+
+# ---------------------------------------------- copulas (V315, V316) --
+def winitzki_erfinv(x):
+    """Winitzki's closed-form erfinv with a = 0.147, argument clamped to
+    +-0.999999 (src/issue218-inverse-cdf.gpu.test.ts:81-87)."""
+    xc = min(max(x, -0.999999), 0.999999)
+    a = 0.147
+    ln1 = math.log(1.0 - xc * xc)
+    t1 = 2.0 / (PI * a) + 0.5 * ln1
+    inner = math.sqrt(max(t1 * t1 - ln1 / a, 0.0)) - t1
+    return math.copysign(math.sqrt(max(inner, 0.0)), xc) if xc != 0 else 0.0
+
+
+def _sig_clamped(t):
+    return min(max(1.0 / (1.0 + math.exp(-t)), 1e-4), 1.0 - 1e-4)
+
+
+def copula_gaussian_ref(x, y, w, strength, rho_in):
+    s = max(strength, 1e-3)
+    zx = math.sqrt(2.0) * winitzki_erfinv(2.0 * _sig_clamped(s * x) - 1.0)
+    zy = math.sqrt(2.0) * winitzki_erfinv(2.0 * _sig_clamped(s * y) - 1.0)
+    rho = min(max(rho_in, -0.95), 0.95)
+    return w * x, w * (rho * zx + math.sqrt(max(1.0 - rho * rho, 0.0)) * zy) / s
+
+
+def _clayton_c(u, v, th):
+    return (u ** -th + v ** -th - 1.0) ** (-1.0 / th)
+
+
+def copula_clayton_ref(x, y, w, strength, theta_in):
+    s = max(strength, 1e-3)
+    u, v = _sig_clamped(s * x), _sig_clamped(s * y)
+    th = min(max(theta_in, 0.05), 8.0)
+    base = max(u ** -th + v ** -th - 1.0, 1e-6)
+    h = min(max(u ** (-th - 1.0) * base ** (-1.0 - 1.0 / th), 1e-4), 1.0 - 1e-4)
+    return w * x, w * min(max(math.log(h / (1.0 - h)), -12.0), 12.0) / s
+
+
+def test_copulas(orc):
+    from scipy.special import erfinv
+    # Winitzki's approximation stays within its published 2e-3 relative error
+    # off the clamp, and is odd
+    for x in np.linspace(-0.99, 0.99, 45):
+        if x != 0:
+            assert abs(winitzki_erfinv(x) / erfinv(x) - 1.0) < 2e-3, x
+    assert winitzki_erfinv(0.0) == 0.0
+    assert abs(winitzki_erfinv(0.8427008) - 1.0) < 5e-3
+    # the Clayton conditional h(v|u) is dC/du of the Clayton copula C(u, v)
+    for u, v, th in [(0.3, 0.6, 2.0), (0.8, 0.2, 0.7), (0.5, 0.5, 5.0)]:
+        d = 1e-6
+        dc = (_clayton_c(u + d, v, th) - _clayton_c(u - d, v, th)) / (2 * d)
+        h = u ** (-th - 1.0) * (u ** -th + v ** -th - 1.0) ** (-1.0 - 1.0 / th)
+        assert abs(dc - h) < 1e-6, (u, v, th)
+    # src/issue217-copula.gpu.test.ts:88-140
+    grid = [(0.0, 0.0), (0.5, 0.7), (-1.2, 0.4), (0.3, -2.1), (3.0, 3.0), (-3.0, -3.0),
+            (8.0, -8.0), (0.001, 0.001)]
+    for prm in ((1.0, 0.0), (1.0, 0.6), (2.3, -0.8)):
+        check_pin(orc, "copula_gaussian", copula_gaussian_ref, 1.0, prm, grid + P8, 3, f64_rel=1e-7)
+    for x, y in grid:
+        o = r32(orc, "copula_gaussian", x, y, (1.0, 0.6))
+        assert abs(o[0] - f32(x)) < 5e-6 and abs(o[1]) < 8
+    o = r32(orc, "copula_gaussian", 2.0, -1.0, (1.0, 0.0))
+    want = math.sqrt(2.0) * erfinv(2.0 * _sig_clamped(-1.0) - 1.0)
+    assert abs(o[1] - want) < 5e-3                       # decoupled at rho 0
+    for prm in ((1.0, 2.0), (1.0, 50.0), (0.6, 0.3)):
+        check_pin(orc, "copula_clayton", copula_clayton_ref, 1.0, prm, grid + P8, 3, f64_rel=1e-6)
+        for x, y in grid:
+            o = r32(orc, "copula_clayton", x, y, prm)
+            assert np.isfinite(o[:2]).all() and abs(o[1]) <= 12.001 / prm[0]
+
+
+# ------------------------------- gaussian_cdf, levy_cdf (V307, V308) --
+def gaussian_cdf_ref(x, y, w, mu, sigma):
+    u = _umap(x * x + y * y)
+    return _radial(x, y, w, max(mu + sigma * math.sqrt(2.0) * winitzki_erfinv(2.0 * u - 1.0), 0.0))
+
+
+def levy_cdf_ref(x, y, w, c, tail):
+    u = _umap(x * x + y * y)
+    e = winitzki_erfinv(1.0 - u)
+    return _radial(x, y, w, min(max(c, 1e-4) / max(2.0 * e * e, 1e-6), max(tail, 0.01)))
+
+
+def test_normal_and_levy_cdfs(orc):
+    # src/issue218-inverse-cdf.gpu.test.ts:88-118
+    check_pin(orc, "gaussian_cdf", gaussian_cdf_ref, 1.0, (0.9, 0.5),
+              [(1.0, 0.0), (0.0, 0.0), (0.5, 0.0)] + P8, 3, f64_rel=1e-6)
+    assert abs(math.hypot(*r32(orc, "gaussian_cdf", 1.0, 0.0, (0.9, 0.5))[:2]) - 0.9) < 5e-4
+    assert math.hypot(*r32(orc, "gaussian_cdf", 0.0, 0.0, (0.9, 0.5))[:2]) < 1e-6
+    check_pin(orc, "levy_cdf", levy_cdf_ref, 1.0, (0.35, 3.0), [(0.0, 0.0), (40.0, 0.0)] + P8, 4,
+              f64_rel=1e-6)
+    assert math.hypot(*r32(orc, "levy_cdf", 0.0, 0.0, (0.35, 3.0))[:2]) == 0.0
+    assert abs(math.hypot(*r32(orc, "levy_cdf", 40.0, 0.0, (0.35, 3.0))[:2]) - 3.0) < 5e-5
+    # the Levy quantile c / (2 erfinv(1 - u)^2) with the exact erfinv, where
+    # Winitzki's relative error 2e-3 bounds the gap
+    from scipy.special import erfinv
+    for x, y in P8[:5]:
+        u = _umap(x * x + y * y)
+        exact = 0.35 / (2.0 * erfinv(1.0 - u) ** 2)
+        if exact < 3.0:
+            g = math.hypot(*run(orc, "levy_cdf", x, y, 1.0, (0.35, 3.0))[:2])
+            assert abs(g / exact - 1.0) < 5e-3, (x, y)
+
+
+# ----------------------------------------------------- lichtenberg (V314) --
+def _lich_vnoise(px, py, period):
+    fx, fy = math.floor(px), math.floor(py)
+    ix, iy = int(fx), int(fy)
+    tx, ty = px - fx, py - fy
+    ux, uy = tx * tx * (3 - 2 * tx), ty * ty * (3 - 2 * ty)
+    ix0, ix1 = ix % period, (ix + 1) % period
+    c00, c10 = _corner(ix0, iy), _corner(ix1, iy)
+    c01, c11 = _corner(ix0, iy + 1), _corner(ix1, iy + 1)
+    bot, top = c00 + (c10 - c00) * ux, c01 + (c11 - c01) * ux
+    return bot + (top - bot) * uy
+
+
+def _lich_field(x, y, freq, branches, radial, octaves):
+    r = math.hypot(x, y)
+    up, vp = (math.atan2(y, x) / (2 * PI) + 0.5) * branches, r * freq
+    ui, vi = x * freq, y * freq
+    u, v = ui + (up - ui) * radial, vi + (vp - vi) * radial
+    period = max(round(branches), 1)
+    s = tot = 0.0
+    amp = 1.0
+    for _ in range(octaves):
+        s += amp * _lich_vnoise(u, v, period)
+        tot += amp
+        amp *= 0.5
+        u, v, period = 2 * u, 2 * v, 2 * period
+    return s / tot
+
+
+def lichtenberg_ref(x, y, w, freq_in, br_in, rad_in, det_in, str_in):
+    """One Newton step toward the zero set of an angularly periodic value-noise
+    field, central differences h = 1e-2, step capped at 2
+    (src/issue219-lichtenberg.gpu.test.ts:100-160)."""
+    freq, br = max(freq_in, 1e-3), min(max(round(br_in), 1), 16)
+    rad, octs, st = min(max(rad_in, 0.0), 1.0), min(max(round(det_in), 1), 4), max(str_in, 0.0)
+    if math.hypot(x, y) < 1e-4:
+        return w * x, w * y
+    h = 1e-2
+    f = lambda a, b: _lich_field(a, b, freq, br, rad, octs)
+    f0 = f(x, y)
+    gx, gy = (f(x + h, y) - f(x - h, y)) / (2 * h), (f(x, y + h) - f(x, y - h)) / (2 * h)
+    g2 = max(gx * gx + gy * gy, 1e-3)
+    sx, sy = -f0 / g2 * gx, -f0 / g2 * gy
+    sl = math.hypot(sx, sy)
+    if sl > 2.0:
+        sx, sy = sx * 2.0 / sl, sy * 2.0 / sl
+    return w * (x + st * sx), w * (y + st * sy)
+
+
+def test_lichtenberg(orc):
+    prm = (1.5, 5.0, 0.8, 3.0, 0.5)
+    pts = [(0.5, 0.3), (-0.7, 0.4), (-1.2, -0.9), (0.9, -0.5)]
+    # (the oracle carries the field's constants in f32, which moves the
+    # Newton step by a few 1e-7)
+    check_pin(orc, "lichtenberg", lichtenberg_ref, 1.0, prm, pts, 3, f64_rel=1e-5)
+    check_pin(orc, "lichtenberg", lichtenberg_ref, 1.0, prm, P8, 3, f64_rel=1e-5, f32=False)
+    check_pin(orc, "lichtenberg", lichtenberg_ref, 1.0, (0.7, 11.0, 0.2, 4.0, 1.3), P8, 3,
+              f64_rel=1e-5, f32=False)
+    for x, y in pts[:2]:
+        o = r32(orc, "lichtenberg", x, y, (1.5, 5.0, 0.8, 3.0, 0.0))
+        assert abs(o[0] - x) < 5e-6 and abs(o[1] - y) < 5e-6
+    for k in range(16):
+        x, y = 1.3 * math.cos(0.4 * k), 1.3 * math.sin(0.4 * k)
+        o = r32(orc, "lichtenberg", x, y, prm)
+        assert np.isfinite(o[:2]).all() and math.hypot(o[0] - x, o[1] - y) <= 0.5 * 2.0 + 1e-3
+    o = r32(orc, "lichtenberg", 0.0, 0.0, prm)
+    assert o[0] == 0.0 and o[1] == 0.0
+
+
+# --------------------------------------- elliptic_E, elliptic_K (V305, V306) --
+def ellip_e_ref(m):   # A&S 17.3.36 with m1 = 1 - m
+    m1 = min(max(1.0 - m, 0.0), 1.0)
+    lg = math.log(1.0 / max(m1, 1e-6))
+    return 1.0 + m1 * (0.4630151 + m1 * 0.1077812) + m1 * (0.2452727 + m1 * 0.0412496) * lg
+
+
+def ellip_k_ref(m):   # A&S 17.3.34, m1 floored at 1e-3
+    m1 = min(max(1.0 - m, 1e-3), 1.0)
+    lg = math.log(1.0 / m1)
+    return 1.3862944 + m1 * (0.1119723 + m1 * 0.0725296) + (0.5 + m1 * (0.1213478 + m1 * 0.0288729)) * lg
+
+
+def elliptic_e_var_ref(x, y, w, scale):
+    r2 = x * x + y * y
+    return _radial(x, y, w, scale * ellip_e_ref(r2 / (1.0 + r2)))
+
+
+def elliptic_k_var_ref(x, y, w, scale, tail):
+    r2 = x * x + y * y
+    return _radial(x, y, w, min(scale * ellip_k_ref(r2 / (1.0 + r2)), max(tail, 0.01)))
+
+
+def test_elliptic(orc):
+    from scipy.special import ellipe, ellipk
+    # the Hastings fits keep their published error bounds (4e-5 for E,
+    # 3e-5 for K) against the complete elliptic integrals of parameter m
+    for m in np.linspace(0.0, 0.999, 60):
+        assert abs(ellip_e_ref(m) - ellipe(m)) < 4e-5, m
+        assert abs(ellip_k_ref(m) - ellipk(m)) < 3e-5, m
+    # src/issue220-elliptic.gpu.test.ts:84-120
+    pts = [(0.0, 0.0), (0.5, 0.0), (0.0, 0.7), (2.0, 0.0)]
+    check_pin(orc, "elliptic_E", elliptic_e_var_ref, 1.0, (0.8,), pts + P8, 4, f64_rel=1e-7)
+    o = [r32(orc, "elliptic_E", x, y, (0.8,)) for x, y in pts]
+    assert math.hypot(*o[0][:2]) < 1e-6 and abs(o[1][1]) < 1e-5 and abs(o[2][0]) < 1e-5
+    assert math.hypot(*o[3][:2]) <= 0.8 * PI / 2 + 1e-3
+    check_pin(orc, "elliptic_K", elliptic_k_var_ref, 1.0, (0.5, 1.5), [(0.5, 0.0), (40.0, 0.0)] + P8,
+              4, f64_rel=1e-7)
+    assert abs(math.hypot(*r32(orc, "elliptic_K", 40.0, 0.0, (0.5, 1.5))[:2]) - 1.5) < 5e-5
+
+
+# ----------------------------------------------------------- peano (V309) --
+def peano_ref(x, y, w, extent, trits_f):
+    """Per-axis base-3 fixed-point code, Peano reflected-ternary digit flip,
+    decode to the cell centre (src/issue221-peano.gpu.test.ts:78-86)."""
+    t = int(min(max(trits_f, 2.0), 15.0))
+    cells, s = 3 ** t, max(extent, 1e-4)
+
+    def enc(c):
+        nrm = (c + s) / (2 * s)
+        return min(int((nrm - math.floor(nrm)) * cells), cells - 1)
+
+    def scr(i):
+        flip, out = False, 0
+        for k in range(t - 1, -1, -1):
+            d = (i // 3 ** k) % 3
+            e = 2 - d if flip else d
+            out += e * 3 ** k
+            flip ^= bool(e & 1)
+        return out
+
+    def dec(i):
+        return (i + 0.5) / cells * 2 * s - s
+    return w * dec(scr(enc(x))), w * dec(scr(enc(y)))
+
+
+def test_peano(orc):
+    # the digit flip permutes the cells (a bijection on 3^t indices) and, at
+    # two trits, is the serpentine order of the Peano curve's first level
+    prm = lambda ext, t: (ext, float(t))
+    for t in (2, 3, 4):
+        cells = 3 ** t
+        xs = [((k + 0.5) / cells) * 2 - 1 for k in range(cells)]
+        outs = sorted(round((peano_ref(x, 0.0, 1.0, 1.0, t)[0] + 1) / 2 * cells - 0.5) for x in xs)
+        assert outs == list(range(cells))
+    xs = [((k + 0.5) / 9) * 2 - 1 for k in range(9)]
+    order = [round((peano_ref(x, 0.0, 1.0, 1.0, 2)[0] + 1) / 2 * 9 - 0.5) for x in xs]
+    assert order == [0, 1, 2, 5, 4, 3, 6, 7, 8]
+    # src/issue221-peano.gpu.test.ts:88-125
+    pts = [(-0.8, 0.3), (0.0, 0.0), (0.55, -0.91), (0.99, 0.99), (-0.4, -0.6)]
+    check_pin(orc, "peano", peano_ref, 1.0, prm(1.0, 5), pts + P8, 5, f64_rel=1e-7)
+    check_pin(orc, "peano", peano_ref, 1.0, prm(1.0, 2), [(x, 0.0) for x in xs], 5)
+    ext = 1.35
+    for x, y in [(5.0, -7.0), (0.01, 0.01), (-3.3, 2.2), (100.0, 100.0)]:
+        o = r32(orc, "peano", x, y, prm(ext, 6))
+        assert abs(o[0]) <= ext and abs(o[1]) <= ext
+    check_pin(orc, "peano", peano_ref, 1.0, prm(ext, 6), P8, 5, f64_rel=1e-7)
+
+
+# ------------------------------------------ parallel weight invariant (V213) --
+def test_parallel_weight_invariant(orc):
+    # src/issue233-parallel-weight.gpu.test.ts:27-97: with zero widths both
+    # branches reduce to out_x = tilesize * x and out_y = height * y +- w * move,
+    # so out_x ignores w and out_y(2) - 2 out_y(1) = -height * y, for any draws
+    prm = (0.0, 0.5, 10.0, 1.0, 0.5, 3.0, 0.0, 0.5, 10.0, 1.0)
+    for s in range(8):
+        words = WORDS[s * 32:(s + 1) * 32]
+        for x, y in [(0.7, 0.4), (-0.6, 0.3), (0.9, -0.5), (1.2, 0.2)]:
+            a = run(orc, "parallel", f32(x), f32(y), 1.0, prm, np.float32, words)
+            b = run(orc, "parallel", f32(x), f32(y), 2.0, prm, np.float32, words)
+            assert abs(a[0] - 0.5 * x) < 5e-6 and abs(b[0] - 0.5 * x) < 5e-6
+            assert abs(b[1] - 2 * a[1] + 0.5 * y) < 5e-6
+            assert min(abs(a[1] - (0.5 * y + 3.0)), abs(a[1] - (0.5 * y - 1.0))) < 5e-6
+    assert run(orc, "parallel", 0.7, 0.4, 0.0, prm)[:2] == (0.0, 0.0)   # w = 0 is a no-op
+
+
+# --------------------------------------- bwraps, crackle smoke (V107, V108) --
+def test_bwraps_crackle_smoke(orc):
+    # src/issue243-bwraps-crackle.gpu.test.ts:53-110 at its seven points
+    pts = [(0.5, 0.3), (-0.5, 0.7), (1.2, -0.4), (-1.0, -1.0), (0.1, 0.05), (0.0, 2.0), (-2.5, 1.7)]
+    check_finite(orc, "bwraps", 1.0, (1.0, 0.0, 1.0, -1.04, 0.71), pts)
+    check_finite(orc, "crackle", 1.0, (1.0, 0.2, 0.0, 1.0), pts)
+    check_finite(orc, "crackle", 1.0, (1.0, 0.2, 1.0, 1.0), pts)
+    # bwraps with zero twists and zero space: every point is a bubble-wrap
+    # cell centre plus its offset scaled by gain inside the cell's radius;
+    # the identity holds outside the bubble
+    o = run(orc, "bwraps", -1.0, -1.0, 1.0, (1.0, 0.0, 1.0, 0.0, 0.0))
+    assert np.isfinite(o[:2]).all()
+
+
+# --------------------------------------------------- parameter domains --
+def test_parameter_domains(orc):
+    # src/issue246-param-domain.gpu.test.ts:77-115
+    pts = [(0.5, 0.3), (-0.6, 0.7), (1.1, -0.4), (0.2, 0.9)]
+    m2 = [run(orc, "atan", f32(x), f32(y), 1.0, (2.0, 1.0), np.float32) for x, y in pts]
+    m5 = [run(orc, "atan", f32(x), f32(y), 1.0, (5.0, 1.0), np.float32) for x, y in pts]
+    assert all(o[0] == 0.0 and o[1] == 0.0 for o in m5)          # out-of-range mode
+    assert any(abs(v) > 1e-6 for o in m2 for v in o[:2])
+    for x, y in pts:
+        w2 = run(orc, "shredrad", f32(x), f32(y), 1.0, (4.0, 2.0), np.float32)
+        w1 = run(orc, "shredrad", f32(x), f32(y), 1.0, (4.0, 1.0), np.float32)
+        assert abs(w2[0] - w1[0]) < 5e-6 and abs(w2[1] - w1[1]) < 5e-6   # width clamped to 1
+    assert any(abs(run(orc, "shredrad", f32(x), f32(y), 1.0, (4.0, 2.0), np.float32)[k] -
+                   run(orc, "shredrad", f32(x), f32(y), 1.0, (4.0, 0.5), np.float32)[k]) > 1e-4
+               for x, y in pts for k in range(2))
+    # Y_l^m uses |m| (the associated Legendre domain): m = -2 matches m = +2
+    # through hydrogen_orbital, the caller whose m runs over -l..l
+    for x, y in pts + P8:
+        a = run(orc, "hydrogen_orbital", x, y, 1.0, (3.0, 2.0, -2.0, 0.5, 1.0))
+        b = run(orc, "hydrogen_orbital", x, y, 1.0, (3.0, 2.0, 2.0, 0.5, 1.0))
+        assert a[:2] == b[:2]
+    assert any(abs(v) > 1e-6 for x, y in pts
+               for v in run(orc, "hydrogen_orbital", x, y, 1.0, (3.0, 2.0, 2.0, 0.5, 1.0))[:2])
+
+
 # -- end of pins --
