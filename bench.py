@@ -79,14 +79,16 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-WALKER_CANDIDATES = (65536, 131072, 196608, 262144, 393216, 524288)
+# half a chip's worth of chaos blocks up to four (one block per CU each)
+WALKER_CANDIDATES = tuple(int(f * pack.CHIP_WALKERS) for f in (0.5, 1, 1.5, 2, 3, 4))
 
 
 def schedule(samples: int, walkers: int):
     """Walkers x iters for `samples` post-fuse samples (walkers capped so that
-    each walker splats at least once)."""
-    walkers = max(256, min(walkers, samples))
-    walkers = (walkers + 255) // 256 * 256
+    each walker splats at least once, rounded up to whole chaos blocks)."""
+    blk = pack.CHAOS_BLOCK
+    walkers = max(blk, min(walkers, samples))
+    walkers = (walkers + blk - 1) // blk * blk
     iters = -(-samples // walkers)
     return walkers, iters
 
@@ -95,7 +97,8 @@ class Frame:
     """One frame of a genome at a quality: reset, chaos, DE, tonemap (plain
     path), plus the learned denoiser when one is given (learned path)."""
 
-    def __init__(self, r: Renderer, g, fuse: int, walkers: int = 196608, denoiser=None):
+    def __init__(self, r: Renderer, g, fuse: int, walkers: int = pack.CHIP_WALKERS,
+                 denoiser=None):
         self.r, self.g, self.fuse = r, g, fuse
         self.walkers = walkers
         self.packed = pack_genome(g)
@@ -305,7 +308,7 @@ def rooflines(kms, samples, bins, pixels, fuse, walkers, iters):
 
 
 def measure_config(name, g, W, H, os_, quality, steps, streams, fuse=pack.FUSE,
-                   walkers=131072):
+                   walkers=pack.CHIP_WALKERS):
     """One benchmark configuration on the GPU schedule: ms/frame and rooflines."""
     rs = [Renderer(W, H, os_) for _ in range(streams)]
     frames = [Frame(r, g, fuse, walkers=walkers) for r in rs]

@@ -248,11 +248,12 @@ class BatchRenderer:
 def gpu_dispatch(quality: float, width: int, height: int) -> tuple:
     """(walkers, iters) for `quality` samples per output pixel on one MI355X:
     about 768 post-fuse turns per walker (the 200-turn fuse stays a small
-    share), at least 64 blocks of 256 walkers when the frame has the samples,
-    at most 131,072 walkers (the autotuned optimum at 1080p)."""
+    share), whole chaos blocks, at most one block per CU (the autotuned
+    optimum at 1080p)."""
+    blk = pack.CHAOS_BLOCK
     samples = max(1, int(round(quality * width * height)))
-    walkers = min(131072, max(256, samples // 768))
-    walkers = max(256, walkers // 256 * 256)
+    walkers = min(pack.CHIP_WALKERS, max(blk, samples // 768))
+    walkers = max(blk, walkers // blk * blk)
     iters = -(-samples // walkers)
     return walkers, iters
 

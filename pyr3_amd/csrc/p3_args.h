@@ -9,7 +9,7 @@ constexpr int kMaxXforms = 128;    // regular xforms; the fallback row index
 constexpr int kXformFloats = 116;  // 464-byte xform record
 constexpr int kMaxVars = 8;        // variation slots per xform
 constexpr int kTraceFloats = 16;   // floats per trace entry
-constexpr int kChaosBlock = 256;   // walkers per chaos block
+constexpr int kChaosBlock = 576;   // walkers per chaos block (one block per CU)
 constexpr int kCatalogSize = 326;  // catalog indices 0..325 (97 = pre_blur)
 constexpr int kPickRows = 17;      // pick tables of up to 16 xforms (+ fallback row)
 constexpr int kPickCols = 16;      //   also travel as step boundaries, 16 per row
@@ -62,7 +62,8 @@ struct MultiArgs {
 // LDS image of a chaos block's walkers (structure of arrays, kB walkers):
 // ISAAC words [48][kB] (mem 0..15, two rsl rounds 16..47) and the
 // per-walker scalars: 244 bytes per walker; the pick boundaries (544 B);
-// the splat aggregation's 12 bytes per lane.
+// the splat aggregation's 12 bytes per lane. 256 bytes per walker in all:
+// 576 walkers fit one CU's 160 KiB.
 template <int kB>
 struct BlockLds {
   alignas(16) uint16_t pick_b[kPickRows * kPickCols];

@@ -489,7 +489,7 @@ P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L, uint32_t block)
       rng.m.stride = kB;
       rng.s.cnt = L.cnt[wl]; rng.s.a = L.ra[wl]; rng.s.b = L.rb[wl]; rng.s.c = L.rc[wl];
       rng.draws = L.draws[wl];
-      isaac_top_up(rng.s, rng.m);   // the wave's lanes generate rounds together here
+      isaac_top_up_wave(rng.s, rng.m);   // the wave's lanes generate rounds together here
       Walker wk{L.x[wl], L.y[wl], L.z[wl], L.prev[wl], L.consec[wl], L.iter[wl]};
       run_turn<Spec>(A, wk, rng, L.fn[wl], L.raw[wl], w0 + wl, sp);
       if (wk.iter < total) {

@@ -102,6 +102,18 @@ P3_INLINE void isaac_top_up(IsaacRegs& s, const IsaacLds& m) {
   }
 }
 
+// Wave-level top-up: the wave generates rounds only when one of its lanes
+// has fewer than kIsaacLow values left (none generated ahead), and then every
+// lane with a free half generates together. A lane refills about every four
+// turns, so topping up whenever any lane had a free half ran a round on most
+// turns with a quarter of the lanes; this way a round runs on fewer turns with
+// more lanes. A turn that draws more than is left still generates inline.
+constexpr uint32_t kIsaacLow = 8u;
+P3_INLINE void isaac_top_up_wave(IsaacRegs& s, const IsaacLds& m) {
+  const bool low = !(s.cnt & kIsaacReady) && (s.cnt & kIsaacUnread) < kIsaacLow;
+  if (__any(low)) isaac_top_up(s, m);
+}
+
 P3_INLINE void isaac_mix8(uint32_t v[8]) {
   uint32_t a = v[0], b = v[1], c = v[2], d = v[3], e = v[4], f = v[5], g = v[6], h = v[7];
   a ^= b << 11; d += a; b += c;

@@ -19,8 +19,9 @@ at the genome's quality (or `--quality`), present (DE + tonemap), then:
         [--walkers N] [--oversample N] [--no-de] [--transparent]
         [--format png8|png16|exr|exr-linear] [--no-auto-exposure]
 
-Walkers default to 131,072 (the reference's `--walkers` override; its own
-default of ~1,024 long walkers fills only a few CUs of this GPU).
+Walkers default to 147,456, one chaos block of 576 on each CU (the
+reference's `--walkers` override; its own default of ~1,024 long walkers
+fills only a few CUs of this GPU).
 """
 from __future__ import annotations
 
@@ -42,7 +43,7 @@ def main(argv=None) -> int:
     ap.add_argument("output", nargs="?")
     ap.add_argument("--quality", type=float, default=None, help="samples per output pixel")
     ap.add_argument("--seed", type=int, default=None)
-    ap.add_argument("--walkers", type=int, default=131072)
+    ap.add_argument("--walkers", type=int, default=pack.CHIP_WALKERS)
     ap.add_argument("--oversample", type=int, default=None)
     ap.add_argument("--no-de", action="store_true", help="skip density estimation")
     ap.add_argument("--transparent", action="store_true")

@@ -106,3 +106,18 @@ def test_manifest_keys_parse():
         for rec in xf:
             for s in range(int(min(rec[7], 8))):
                 assert int(rec[20 + 4 * s]) in SUPPORTED
+
+
+def test_chaos_block_matches_the_host_schedule():
+    """The host's walker schedules (whole chaos blocks, one block per CU) use
+    the kernel's block size, and that block's LDS image fits one CU."""
+    import re
+    from conftest import ROOT
+    src = (ROOT / "pyr3_amd" / "csrc" / "p3_args.h").read_text()
+    blk = int(re.search(r"kChaosBlock = (\d+);", src).group(1))
+    assert pack.CHAOS_BLOCK == blk and blk % 64 == 0
+    assert pack.CHIP_WALKERS == 256 * blk
+    # 192 B of ISAAC words, 52 B of walker scalars, 12 B of splat aggregation
+    # per walker, plus the pick boundaries and two bucket rows
+    lds = 256 * blk + 2 * 17 * 16 + 2 * 4 * (128 + 2)
+    assert lds <= 160 * 1024

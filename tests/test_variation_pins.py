@@ -2231,4 +2231,158 @@ def test_parameter_domains(orc):
                for v in run(orc, "hydrogen_orbital", x, y, 1.0, (3.0, 2.0, 2.0, 0.5, 1.0))[:2])
 
 
+# ------------------------------------------- direct colours (V99 - V102) --
+# Ken Perlin's permutation of 0..255 ("Improving Noise", SIGGRAPH 2002, and
+# his reference implementation), which src/noise-perlin-oracle.ts:12-29 uses
+PERLIN_P = [
+    151, 160, 137, 91, 90, 15, 131, 13, 201, 95, 96, 53, 194, 233, 7, 225, 140, 36, 103, 30, 69,
+    142, 8, 99, 37, 240, 21, 10, 23, 190, 6, 148, 247, 120, 234, 75, 0, 26, 197, 62, 94, 252, 219,
+    203, 117, 35, 11, 32, 57, 177, 33, 88, 237, 149, 56, 87, 174, 20, 125, 136, 171, 168, 68, 175,
+    74, 165, 71, 134, 139, 48, 27, 166, 77, 146, 158, 231, 83, 111, 229, 122, 60, 211, 133, 230,
+    220, 105, 92, 41, 55, 46, 245, 40, 244, 102, 143, 54, 65, 25, 63, 161, 1, 216, 80, 73, 209, 76,
+    132, 187, 208, 89, 18, 169, 200, 196, 135, 130, 116, 188, 159, 86, 164, 100, 109, 198, 173,
+    186, 3, 64, 52, 217, 226, 250, 124, 123, 5, 202, 38, 147, 118, 126, 255, 82, 85, 212, 207, 206,
+    59, 227, 47, 16, 58, 17, 182, 189, 28, 42, 223, 183, 170, 213, 119, 248, 152, 2, 44, 154, 163,
+    70, 221, 153, 101, 155, 167, 43, 172, 9, 129, 22, 39, 253, 19, 98, 108, 110, 79, 113, 224, 232,
+    178, 185, 112, 104, 218, 246, 97, 228, 251, 34, 242, 193, 238, 210, 144, 12, 191, 179, 162,
+    241, 81, 51, 145, 235, 249, 14, 239, 107, 49, 192, 214, 31, 181, 199, 106, 157, 184, 84, 204,
+    176, 115, 121, 50, 45, 127, 4, 150, 254, 138, 236, 205, 93, 222, 114, 67, 29, 24, 72, 243, 141,
+    128, 195, 78, 66, 215, 61, 156, 180]
+
+
+def perlin2d_ref(x, y):
+    """Improved noise in 2-D: quintic fade, gradients +-u +-2v chosen by the
+    hash's low three bits (src/noise-perlin-oracle.ts:43-70)."""
+    perm = PERLIN_P + PERLIN_P
+
+    def grad(h, u, v):
+        h &= 7
+        a, b = (u, v) if h < 4 else (v, u)
+        return (-a if h & 1 else a) + (-2 * b if h & 2 else 2 * b)
+
+    fx, fy = math.floor(x), math.floor(y)
+    X, Y = int(fx) & 255, int(fy) & 255
+    xf, yf = x - fx, y - fy
+    fade = lambda t: t * t * t * (t * (t * 6 - 15) + 10)
+    u, v = fade(xf), fade(yf)
+    A, B = (perm[X] + Y) & 255, (perm[X + 1] + Y) & 255
+    g00, g10 = grad(perm[A], xf, yf), grad(perm[B], xf - 1, yf)
+    g01, g11 = grad(perm[A + 1], xf, yf - 1), grad(perm[B + 1], xf - 1, yf - 1)
+    lo, hi = g00 + (g10 - g00) * u, g01 + (g11 - g01) * u
+    return lo + (hi - lo) * v
+
+
+def perlin_fbm_ref(x, y, octaves, scale):   # src/noise-perlin-oracle.ts:72-90
+    tot = mx = 0.0
+    amp, fr = 1.0, scale
+    for _ in range(max(1, min(8, int(octaves)))):
+        tot += perlin2d_ref(x * fr, y * fr) * amp
+        mx += amp
+        amp *= 0.5
+        fr *= 2
+    return tot / mx
+
+
+def _hue_of(rgb):
+    import colorsys
+    return colorsys.rgb_to_hls(*rgb)
+
+
+def test_direct_colours(orc):
+    import colorsys
+    assert sorted(PERLIN_P) == list(range(256))
+    # dc_linear (src/dc-variations.gpu.test.ts:36-41, toBeCloseTo 5)
+    for x, y in [(0.0, 0.0), (0.5, 0.5), (-0.5, -0.5), (1.0, 1.0), (-1.0, 1.0), (5.0, 5.0), (-5.0, -5.0)]:
+        want = [min(max(v, 0.0), 1.0) for v in (0.5 + 0.5 * x, 0.5 + 0.5 * y, 0.5 - 0.25 * (x + y))]
+        got = orc.dc_color(V["dc_linear"], x, y)
+        assert all(abs(g - w) < 5e-6 for g, w in zip(got, want)), (x, y)
+    # dc_perlin: HSL(fract(0.5 + 0.5 n + seed), 1, 0.55) of the fBm n, read
+    # back through the textbook HSL inverse (src/dc-variations.gpu.test.ts)
+    samples = [(0.0, 0.0, 1.0, 3, 0.0), (0.5, 0.5, 1.0, 3, 0.0), (1.5, 2.7, 1.0, 3, 0.25),
+               (-3.3, 4.8, 2.0, 4, 0.5), (10.0, 10.0, 0.5, 2, 0.0), (5.5, -3.3, 0.5, 4, 0.0),
+               (-7.7, 4.2, 3.0, 3, 0.1), (0.31, -0.72, 1.3, 8, 0.7)]
+    fps = set()
+    for x, y, sc, octv, seed in samples:
+        rgb = orc.dc_color(V["dc_perlin"], x, y, (sc, float(octv), seed))
+        n = perlin_fbm_ref(f32(x), f32(y), octv, f32(sc))
+        hue = (0.5 + 0.5 * n + seed) % 1.0
+        want = colorsys.hls_to_rgb(hue, 0.55, 1.0)
+        assert all(abs(g - w) < 2e-4 for g, w in zip(rgb, want)), (x, y, rgb, want)
+        assert all(0.0 <= v <= 1.0 for v in rgb)
+        fps.add(tuple(round(v * 10) for v in rgb))
+    assert len(fps) > 1
+    r0 = orc.dc_color(V["dc_perlin"], 0.0, 0.0, (1.0, 3.0, 0.0))
+    assert r0[0] < r0[1] and r0[0] < r0[2]                         # hue 0.5: cyan
+    assert perlin_fbm_ref(0.0, 0.0, 3, 1.0) == 0.0
+    # the noise vanishes on the lattice and stays within its bound
+    for x, y in [(3.0, -2.0), (17.0, 5.0), (-128.0, 255.0)]:
+        assert perlin2d_ref(x, y) == 0.0
+    g = np.random.default_rng(5).uniform(-50, 50, (400, 2))
+    assert max(abs(perlin2d_ref(a, b)) for a, b in g) <= 1.5
+    # dc_gridout: one colour per cell of the n x n lattice, from a hash of
+    # the cell (src/dc-variations.gpu.test.ts:203-262)
+    c = [orc.dc_color(V["dc_gridout"], x, y, (4.0,)) for x, y in
+         [(0.1, 0.1), (0.2, 0.2), (0.3, 0.1), (0.1, 0.3), (0.5, 0.5)]]
+    assert c[0] == c[1] and c[2] != c[0] and c[3] != c[0]
+    assert all(0.0 <= v <= 1.0 for rgb in c for v in rgb)
+    for x, y, n in [(0.1, 0.1, 4.0), (-0.7, 0.35, 4.0), (2.3, -1.9, 7.0), (0.0, 0.0, 1.0)]:
+        cx, cy = math.floor(f32(x) * f32(n)), math.floor(f32(y) * f32(n))
+        mixed = ((((cx * 73856093) ^ (cy * 19349663)) & 0xFFFFFFFF) + 0x9E3779B9) & 0xFFFFFFFF
+        want = (_hash01(mixed), _hash01(mixed ^ 0xDEADBEEF), _hash01(mixed ^ 0x13579BDF))
+        got = orc.dc_color(V["dc_gridout"], x, y, (n,))
+        assert all(abs(a - b) < 1e-7 for a, b in zip(got, want)), (x, y, got, want)
+    # dc_cylinder: HSL(fract(0.5 + 0.5 sin x), 0.9, clamp(0.5 + 0.25 tanh(y / 2), 0.2, 0.85))
+    for x, y in P8 + [(1e3, -1e3), (0.0, 40.0)]:
+        hue = (0.5 + 0.5 * math.sin(x)) % 1.0 if abs(x) <= 1e6 else None
+        lit = min(max(0.5 + 0.25 * math.tanh(y * 0.5), 0.2), 0.85)
+        want = colorsys.hls_to_rgb(hue, lit, 0.9)
+        got = orc.dc_color(V["dc_cylinder"], x, y)
+        assert all(abs(a - b) < 2e-4 for a, b in zip(got, want)), (x, y, got, want)
+
+
+# ------------------------------------- quadratic and hopping maps (V323 - V325) --
+def sprott_poly_ref(x, y, w, a1, a2, a3, a4, a5, b1, b2, b3, b4, b5):
+    """Sprott's general 2-D quadratic map without constant terms
+    (src/sprott-poly.cpu.test.ts:18-27)."""
+    return (w * (a1 * x + a2 * x * x + a3 * x * y + a4 * y + a5 * y * y),
+            w * (b1 * x + b2 * x * x + b3 * x * y + b4 * y + b5 * y * y))
+
+
+def hopalong_ref(x, y, w, a, b, c):
+    """Barry Martin's Hopalong map x' = y - sgn(x) sqrt|b x - c|, y' = a - x."""
+    s = (x > 0) - (x < 0)
+    return w * (y - s * math.sqrt(abs(b * x - c))), w * (a - x)
+
+
+def gumowski_mira_ref(x, y, w, a, b):
+    """The Gumowski-Mira map x' = b y + G(x), y' = -x + G(x'),
+    G(t) = a t + 2 (1 - a) t^2 / (1 + t^2)."""
+    g = lambda t: a * t + 2.0 * (1.0 - a) * t * t / (1.0 + t * t)
+    xp = b * y + g(x)
+    return w * xp, w * (-x + g(xp))
+
+
+def test_quadratic_and_hopping_maps(orc):
+    # src/sprott-poly.gpu.test.ts:60-90, src/hopalong.gpu.test.ts:55-70,
+    # src/gumowski-mira.gpu.test.ts:55-68 (toBeCloseTo 4)
+    for co, pts in [((0.5, -0.3, 0.2, 0.9, -0.4, 0.1, 0.6, -0.7, 0.3, 0.8),
+                     [(0.4, -0.2), (-0.7, 0.5), (0.1, 0.9)]),
+                    ((0.31, 0.88, -0.42, 0.17, -0.93, 0.55, -0.21, 0.64, 0.39, -0.76),
+                     [(0.3, 0.7), (-0.5, -0.1), (0.9, -0.6)])]:
+        check_pin(orc, "sprott_poly", sprott_poly_ref, 1.0, co, pts + P8, 4)
+        check_pin(orc, "sprott_poly", sprott_poly_ref, 2.5, co, pts, 4)    # weight outside
+    pts = [(0.3, -0.7), (-1.2, 0.4), (0.0, 0.9), (-0.05, -0.05)]
+    check_pin(orc, "hopalong", hopalong_ref, 1.0, (1.0, 2.0, 0.5), pts + P8, 4)
+    check_pin(orc, "hopalong", hopalong_ref, 0.7, (-0.4, 1.3, -2.0), pts + P8, 4)
+    o = r32(orc, "hopalong", 0.0, 0.9, (1.0, 2.0, 0.5))
+    assert o[0] == f32(0.9) and o[1] == 1.0                             # sgn(0) = 0
+    pts = [(0.3, -0.7), (-1.2, 0.4), (0.0, 0.9), (2.5, -1.1)]
+    check_pin(orc, "gumowski_mira", gumowski_mira_ref, 1.0, (0.31, 0.99), pts + P8, 4)
+    check_pin(orc, "gumowski_mira", gumowski_mira_ref, 1.0, (-0.8, 0.05), pts + P8, 4)
+    # G(0) = 0: the origin is a fixed point; a = 1 makes G the identity
+    assert run(orc, "gumowski_mira", 0.0, 0.0, 1.0, (0.31, 0.99))[:2] == (0.0, 0.0)
+    o = run(orc, "gumowski_mira", 0.3, -0.7, 1.0, (1.0, 0.5))
+    assert abs(o[0] - (0.5 * -0.7 + 0.3)) < 1e-12 and abs(o[1] - (-0.3 + o[0])) < 1e-12
+
+
 # -- end of pins --
