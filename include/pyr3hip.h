@@ -37,6 +37,7 @@ extern "C" {
 #define P3_DE_UNIFORM_WORDS 8
 #define P3_VIZ_UNIFORM_WORDS 24
 #define P3_TRACE_FLOATS 16
+#define P3_NOALLOC_SCRATCH_BYTES (96u * 1024u * 1024u)  /* p3_iterate_noalloc's device array */
 
 typedef struct p3_renderer p3_renderer;
 
@@ -204,8 +205,30 @@ int p3_denoiser_create(const float* params, size_t n, p3_denoiser** out);
 int p3_denoiser_destroy(p3_denoiser* d);
 int p3_denoise(p3_renderer* r, p3_denoiser* d);
 
-/* Message for the last failing call on this handle (or of p3_create when
- * r is null). Never null. */
+/* p3_iterate without any device allocation: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * inputs are those of p3_iterate plus the 256 x RGBA f32 palette (host
+ * memory); the output is hist, a caller-owned device buffer of
+ * n_hist = u[0] x u[1] x 4 u32 (the super-res grid) to which the dispatch's
+ * splats are added, saturating, on `stream` (a hipStream_t; null = the null
+ * stream) before the call returns. The packed accumulator, its carry counts
+ * and the uploaded blobs live in a static device array of
+ * P3_NOALLOC_SCRATCH_BYTES (96 MiB) that every GPU the library runs on holds
+ * until the process ends: 40 bytes per bin plus the blobs, so grids up to
+ * about 2.3 M bins (1920 x 1080 at oversample 1); a larger grid is P3_E_ARG.
+ * The kernel of the genome's shape must already be loaded on the device
+ * (one p3_iterate of that shape on it; loading code allocates), else
+ * P3_E_STATE. The colour-index capture (u[14] = 1) is P3_E_ARG. Calls share
+ * the device's array, which keeps the last call's contents (each call
+ * clears what it needs): calls that overlap on one GPU race. Errors are
+ * read with p3_last_error(NULL). */
+int p3_iterate_noalloc(const float* xforms, int n_slots, const uint32_t* distrib, int n_rows,
+                       const uint32_t* uniforms, const float* palette_rgba, uint32_t* hist,
+                       size_t n_hist, void* stream);
+
+/* Message for the last failing call on this handle (or of p3_create, and of
+ * p3_iterate_noalloc, when r is null). Never null. */
 const char* p3_last_error(p3_renderer* r);
 
 #ifdef __cplusplus

@@ -27,6 +27,7 @@ SIGNATURES = {
     "p3_iterate": (_I, [_P, _FP, _I, _UP, _I, _UP]),
     "p3_iterate_multi": (_I, [_P, _I, C.POINTER(_FP), _I, C.POINTER(_UP), _I, C.POINTER(_UP)]),
     "p3_iterate_states": (_I, [_P, _FP, _I, _UP, _I, _UP, _UP]),
+    "p3_iterate_noalloc": (_I, [_FP, _I, _UP, _I, _UP, _FP, _P, _SZ, _P]),
     "p3_present": (_I, [_P, _UP, _UP]),
     "p3_read_histogram": (_I, [_P, _UP, _SZ]),
     "p3_read_index": (_I, [_P, _UP, _SZ]),

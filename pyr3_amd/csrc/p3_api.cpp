@@ -14,6 +14,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -273,21 +274,33 @@ void release(p3_renderer* r) {
   r->linear.release(); r->curves.release();
 }
 
-int check_records(p3_renderer* r, const float* xforms, int n_slots) {
+// Empty when every record's variation count and active slots are valid,
+// else the message.
+std::string records_error(const float* xforms, int n_slots) {
+  char msg[128];
   for (int s = 0; s < n_slots; ++s) {
     const float* xf = xforms + (size_t)s * P3_XFORM_FLOATS;
     const float nvf = xf[7];
-    if (!(nvf >= 0.0f) || nvf > 1e9f)
-      return r->fail(P3_E_ARG, "xform %d: bad variation count", s);
+    if (!(nvf >= 0.0f) || nvf > 1e9f) {
+      std::snprintf(msg, sizeof msg, "xform %d: bad variation count", s);
+      return msg;
+    }
     const int nv = std::min((int)nvf, 8);
     for (int k = 0; k < nv; ++k) {
       const float idxf = xf[20 + 4 * k];
-      if (!(idxf >= 0.0f) || idxf > 1e9f || !supported((uint32_t)idxf))
-        return r->fail(P3_E_ARG, "xform %d slot %d: catalog entry %g is not implemented", s, k,
-                       (double)idxf);
+      if (!(idxf >= 0.0f) || idxf > 1e9f || !supported((uint32_t)idxf)) {
+        std::snprintf(msg, sizeof msg, "xform %d slot %d: catalog entry %g is not implemented",
+                      s, k, (double)idxf);
+        return msg;
+      }
     }
   }
-  return P3_OK;
+  return std::string();
+}
+
+int check_records(p3_renderer* r, const float* xforms, int n_slots) {
+  const std::string e = records_error(xforms, n_slots);
+  return e.empty() ? P3_OK : r->fail(P3_E_ARG, "%s", e.c_str());
 }
 
 // Checks one dispatch's blobs against the renderer (sizes, final slot, uniform
@@ -379,16 +392,18 @@ int usable(p3_renderer* r) {
                              "p3_resize again or destroy it");
 }
 
-// Kernel arguments of one dispatch whose records and table are at dxf / dds.
-p3::ChaosArgs dispatch_args(p3_renderer* r, const uint32_t* u, int n_regular, int32_t final_idx,
-                            const float* dxf, const uint32_t* dds) {
+// Kernel arguments of one dispatch whose records, table, palette and
+// histogram are at dxf / dds / pal / hist.
+p3::ChaosArgs dispatch_args_at(const uint32_t* u, int n_regular, int32_t final_idx,
+                               const float* dxf, const uint32_t* dds, const float* pal,
+                               uint32_t* hist) {
   p3::ChaosArgs a{};
   a.isaac_in = nullptr;
   a.trace = nullptr;
   a.xforms = dxf;
   a.distrib = dds;
-  a.palette = (const float4*)r->palette.p;
-  a.hist = r->hist.p;
+  a.palette = (const float4*)pal;
+  a.hist = hist;
   a.fallback_row = (uint32_t)n_regular;
   a.width = u[0]; a.height = u[1]; a.iters = u[2]; a.fuse = u[3];
   a.scale = bits_f(u[4]); a.cx = bits_f(u[5]); a.cy = bits_f(u[6]);
@@ -410,6 +425,12 @@ p3::ChaosArgs dispatch_args(p3_renderer* r, const uint32_t* u, int n_regular, in
   a.phase_strength = bits_f(u[28]); a.phase_freq = bits_f(u[29]);
   a.idx_sum = nullptr;
   return a;
+}
+
+// Kernel arguments of one dispatch on the renderer's palette and histogram.
+p3::ChaosArgs dispatch_args(p3_renderer* r, const uint32_t* u, int n_regular, int32_t final_idx,
+                            const float* dxf, const uint32_t* dds) {
+  return dispatch_args_at(u, n_regular, final_idx, dxf, dds, r->palette.p, r->hist.p);
 }
 
 // Capture buffer of the colour index (allocated and cleared on first use,
@@ -616,6 +637,54 @@ int read_dev(p3_renderer* r, void* dst, const void* src, size_t bytes, const cha
   int rc = r->hip(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, r->stream), what);
   if (rc) return rc;
   return r->hip(hipStreamSynchronize(r->stream), what);
+}
+
+// ------------------------------------------- allocation-free dispatch --
+// Device memory of p3_iterate_noalloc, one copy per GPU that loads the
+// library, held until the process ends. Laid out per call as 256-byte
+// aligned parts: packed accumulator (8 B per bin), carry counts (32 B per
+// bin), xform records, pick table, pick-table step boundaries, palette.
+// In host code `scratch` names a host-side stand-in, not the device array:
+// kernels, fills and copies take the address that hipGetSymbolAddress
+// returns for the current device (scratch_base), never the name.
+static __device__ __align__(256) unsigned char scratch[P3_NOALLOC_SCRATCH_BYTES];
+
+std::mutex g_scratch_mu;
+std::vector<unsigned char*> g_scratch_dev;                       // per device
+std::vector<std::vector<uint16_t>> g_scratch_bounds;             // per device: host staging
+
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+int noalloc_fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_create_error = buf;
+  return code;
+}
+
+// This device's scratch address, looked up once per device with the device
+// current; null on failure (message in g_create_error).
+unsigned char* scratch_base(int dev) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((size_t)dev >= g_scratch_dev.size()) {
+    g_scratch_dev.resize((size_t)dev + 1, nullptr);
+    g_scratch_bounds.resize((size_t)dev + 1);
+  }
+  if (!g_scratch_dev[(size_t)dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      noalloc_fail(P3_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                   hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_dev[(size_t)dev] = (unsigned char*)p;
+    g_scratch_bounds[(size_t)dev].assign((size_t)p3::kPickRows * p3::kPickCols, 0);
+  }
+  return g_scratch_dev[(size_t)dev];
 }
 
 }  // namespace
@@ -1117,6 +1186,100 @@ int p3_denoise(p3_renderer* r, p3_denoiser* d) {
   r->tic(T_DENOISE);
   if ((rc = r->hip(p3::launch_denoise(a, r->stream), "denoise"))) return rc;
   r->toc(T_DENOISE);
+  return P3_OK;
+}
+
+int p3_iterate_noalloc(const float* xforms, int n_slots, const uint32_t* distrib, int n_rows,
+                       const uint32_t* u, const float* palette_rgba, uint32_t* hist,
+                       size_t n_hist, void* stream_) {
+  // every check that needs no device comes first
+  if (!xforms || !distrib || !u || !palette_rgba || !hist)
+    return noalloc_fail(P3_E_ARG, "null argument");
+  if (n_rows < 2 || n_rows > P3_MAX_XFORMS + 1)
+    return noalloc_fail(P3_E_ARG, "n_rows must be 2..%d (got %d)", P3_MAX_XFORMS + 1, n_rows);
+  const int n_regular = n_rows - 1;
+  if (n_slots < n_regular || n_slots > n_regular + 1)
+    return noalloc_fail(P3_E_ARG, "n_slots %d inconsistent with %d regular xforms", n_slots,
+                        n_regular);
+  const int32_t final_idx = (int32_t)u[8];
+  if (final_idx != -1 && (final_idx < 0 || final_idx >= n_slots))
+    return noalloc_fail(P3_E_ARG, "final xform slot %d outside %d slots", final_idx, n_slots);
+  if (u[15] > 3u) return noalloc_fail(P3_E_ARG, "colour mode %u is not 0..3", u[15]);
+  if (u[14] != 0u)
+    return noalloc_fail(P3_E_ARG, "the colour-index capture needs a renderer (p3_iterate)");
+  const size_t bins = (size_t)u[0] * u[1];
+  if (bins == 0) return noalloc_fail(P3_E_ARG, "empty grid %ux%u", u[0], u[1]);
+  if (n_hist != bins * 4)
+    return noalloc_fail(P3_E_ARG, "histogram holds %zu u32, the %ux%u grid needs %zu", n_hist,
+                        u[0], u[1], bins * 4);
+  const std::string rec = records_error(xforms, n_slots);
+  if (!rec.empty()) return noalloc_fail(P3_E_ARG, "%s", rec.c_str());
+  const size_t n_xf = (size_t)n_slots * P3_XFORM_FLOATS;
+  const size_t table = (size_t)n_rows * P3_GRAIN;
+  for (size_t i = 0; i < table; ++i)
+    if (distrib[i] >= (uint32_t)n_regular)
+      return noalloc_fail(P3_E_ARG, "pick table entry %zu = %u outside %d xforms", i, distrib[i],
+                          n_regular);
+  const size_t off_carry = align256(bins * 8);
+  const size_t off_xf = off_carry + align256(bins * 32);
+  const size_t off_ds = off_xf + align256(n_xf * sizeof(float));
+  const size_t off_bd = off_ds + align256(table * sizeof(uint32_t));
+  const size_t off_pal = off_bd + align256((size_t)p3::kPickRows * p3::kPickCols * 2);
+  const size_t need = off_pal + 256 * 4 * sizeof(float);
+  if (need > (size_t)P3_NOALLOC_SCRATCH_BYTES)
+    return noalloc_fail(P3_E_ARG,
+                        "a %ux%u grid with %d pick rows needs %zu bytes of scratch, above the "
+                        "%zu-byte maximum (40 bytes per bin)",
+                        u[0], u[1], n_rows, need, (size_t)P3_NOALLOC_SCRATCH_BYTES);
+  if (u[12] == 0) return P3_OK;
+  // the device, its scratch, and the shape's module loaded earlier
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return noalloc_fail(P3_E_HIP, "hipGetDevice: %s", hipGetErrorString(e));
+  if (!p3::chaos_module_loaded(dev, xforms, n_regular, final_idx))
+    return noalloc_fail(P3_E_STATE,
+                        "the chaos kernel of this genome shape is not loaded on device %d: run "
+                        "one p3_iterate of the shape there first (loading code allocates)",
+                        dev);
+  unsigned char* base = scratch_base(dev);
+  if (!base) return P3_E_HIP;
+  hipStream_t stream = (hipStream_t)stream_;
+  uint64_t* acc = (uint64_t*)base;
+  uint64_t* carry = (uint64_t*)(base + off_carry);
+  float* dxf = (float*)(base + off_xf);
+  uint32_t* dds = (uint32_t*)(base + off_ds);
+  uint16_t* dbd = (uint16_t*)(base + off_bd);
+  float* dpal = (float*)(base + off_pal);
+  std::vector<uint16_t> bounds;
+  const bool have_bounds = pick_bounds(distrib, n_rows, bounds);
+  p3::ChaosArgs a = dispatch_args_at(u, n_regular, final_idx, dxf, dds, dpal, hist);
+  a.pick_bounds = have_bounds ? dbd : nullptr;
+  a.acc = acc;
+  a.carry = carry;
+  // uploads (the bounds through this device's host staging, which outlives
+  // the asynchronous copy), then the cleared accumulator
+  if ((e = hipMemcpyAsync(dxf, xforms, n_xf * sizeof(float), hipMemcpyHostToDevice, stream)) ||
+      (e = hipMemcpyAsync(dds, distrib, table * sizeof(uint32_t), hipMemcpyHostToDevice, stream)) ||
+      (e = hipMemcpyAsync(dpal, palette_rgba, 256 * 4 * sizeof(float), hipMemcpyHostToDevice,
+                          stream)))
+    return noalloc_fail(P3_E_HIP, "upload: %s", hipGetErrorString(e));
+  if (have_bounds) {
+    std::vector<uint16_t>* stage;
+    {
+      std::lock_guard<std::mutex> lock(g_scratch_mu);
+      stage = &g_scratch_bounds[(size_t)dev];
+    }
+    std::copy(bounds.begin(), bounds.end(), stage->begin());
+    if ((e = hipMemcpyAsync(dbd, stage->data(), bounds.size() * sizeof(uint16_t),
+                            hipMemcpyHostToDevice, stream)))
+      return noalloc_fail(P3_E_HIP, "upload pick bounds: %s", hipGetErrorString(e));
+  }
+  if ((e = p3::launch_fill(acc, off_xf, 0u, stream)))
+    return noalloc_fail(P3_E_HIP, "clear packed accumulator: %s", hipGetErrorString(e));
+  if ((e = p3::launch_chaos_rtc_loaded(a, xforms, n_regular, final_idx, stream)))
+    return noalloc_fail(P3_E_HIP, "chaos launch: %s", hipGetErrorString(e));
+  if ((e = p3::launch_hist_merge(hist, acc, carry, bins, stream)))
+    return noalloc_fail(P3_E_HIP, "packed merge: %s", hipGetErrorString(e));
   return P3_OK;
 }
 

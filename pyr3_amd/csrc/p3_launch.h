@@ -55,6 +55,12 @@ hipError_t launch_atomic_probe(uint64_t* cells_base, uint32_t cells, uint32_t bl
 // Chaos kernel specialised for the shape of `host_xforms` (p3_rtc.cpp).
 hipError_t launch_chaos_rtc(const ChaosArgs& a, const float* host_xforms, int n_regular,
                             int final_slot, hipStream_t stream, std::string& err);
+// True when the shape's module is loaded on device `dev`.
+bool chaos_module_loaded(int dev, const float* host_xforms, int n_regular, int final_slot);
+// As launch_chaos_rtc when the shape's module is already loaded on the
+// current device; hipErrorNotReady otherwise (never compiles or loads).
+hipError_t launch_chaos_rtc_loaded(const ChaosArgs& a, const float* host_xforms, int n_regular,
+                                   int final_slot, hipStream_t stream);
 // Batched launch of sub-dispatches sharing the shape of `host_xforms`.
 hipError_t launch_chaos_rtc_multi(const MultiArgs& margs, const float* host_xforms, int n_regular,
                                   int final_slot, hipStream_t stream, std::string& err);

@@ -348,6 +348,39 @@ hipError_t launch_chaos_rtc(const ChaosArgs& a, const float* host_xforms, int n_
                                0, stream, params, nullptr);
 }
 
+// True when this shape's module is loaded on device `dev`.
+bool chaos_module_loaded(int dev, const float* host_xforms, int n_regular, int final_slot) {
+  const std::string key =
+      std::to_string(dev) + "|" + chaos_shape_key(host_xforms, n_regular, final_slot);
+  std::lock_guard<std::mutex> lock(g_mu);
+  return g_mods.count(key) != 0;
+}
+
+// Launch of the kernel of this shape only when its module is already loaded
+// on the current device (neither compiles nor loads: loading a code object
+// allocates device memory); hipErrorNotReady when it is not.
+hipError_t launch_chaos_rtc_loaded(const ChaosArgs& a, const float* host_xforms, int n_regular,
+                                   int final_slot, hipStream_t stream) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  const std::string key =
+      std::to_string(dev) + "|" + chaos_shape_key(host_xforms, n_regular, final_slot);
+  Module m;
+  {
+    std::lock_guard<std::mutex> lock(g_mu);
+    auto it = g_mods.find(key);
+    if (it == g_mods.end()) return hipErrorNotReady;
+    m = it->second;
+  }
+  const unsigned blk = (unsigned)chaos_block();
+  const unsigned grid = (a.walkers + blk - 1) / blk;
+  if (grid == 0) return hipSuccess;
+  ChaosArgs args = a;
+  void* params[] = {&args};
+  return hipModuleLaunchKernel(m.single, grid, 1, 1, blk, 1, 1, 0, stream, params, nullptr);
+}
+
 // Batched launch of n sub-dispatches sharing one genome shape: `dev_args`
 // holds their ChaosArgs in device memory, start[] their first blocks.
 hipError_t launch_chaos_rtc_multi(const MultiArgs& margs, const float* host_xforms, int n_regular,
