@@ -9,7 +9,7 @@ constexpr int kMaxXforms = 128;    // regular xforms; the fallback row index
 constexpr int kXformFloats = 116;  // 464-byte xform record
 constexpr int kMaxVars = 8;        // variation slots per xform
 constexpr int kTraceFloats = 16;   // floats per trace entry
-constexpr int kChaosBlock = 576;   // walkers per chaos block (one block per CU)
+constexpr int kChaosBlock = 640;   // walkers per chaos block (one block per CU)
 constexpr int kCatalogSize = 326;  // catalog indices 0..325 (97 = pre_blur)
 constexpr int kPickRows = 17;      // pick tables of up to 16 xforms (+ fallback row)
 constexpr int kPickCols = 16;      //   also travel as step boundaries, 16 per row
@@ -60,23 +60,24 @@ struct MultiArgs {
 };
 
 // LDS image of a chaos block's walkers (structure of arrays, kB walkers):
-// ISAAC words [48][kB] (mem 0..15, two rsl rounds 16..47) and the
-// per-walker scalars: 244 bytes per walker; the pick boundaries (544 B);
-// the splat aggregation's 12 bytes per lane. 256 bytes per walker in all:
-// 576 walkers fit one CU's 160 KiB.
+// ISAAC words [48][kB] (mem 0..15, two rsl rounds 16..47), the ISAAC
+// registers a, b, c, the loop state word (ISAAC unread count and flags in
+// bits 0-6, previous xform + 1 in bits 8-15, retry count in bits 16-18), the
+// position and colour, the turn index, the pick and the xform order; the
+// splat aggregation's group sums (its hash table of lanes reuses `order`,
+// dead by then): 236 bytes per walker, plus the pick boundaries (544 B) and
+// two bucket rows. 640 walkers fit one CU's 160 KiB.
 template <int kB>
 struct BlockLds {
   alignas(16) uint16_t pick_b[kPickRows * kPickCols];
   uint32_t isaac[48 * kB];
-  uint32_t cnt[kB], ra[kB], rb[kB], rc[kB];
+  uint32_t ra[kB], rb[kB], rc[kB];
+  uint32_t state[kB];
   float x[kB], y[kB], z[kB];
-  int32_t prev[kB];
-  uint32_t consec[kB], iter[kB], draws[kB], raw[kB];
+  uint32_t iter[kB];
   uint16_t fn[kB], order[kB];
   uint32_t bucket[2][kMaxXforms + 2];   // per-xform counts, two turns alternating
-  // splat aggregation, 64 entries per wave: hash slot -> lane, and the
-  // packed sum of each group's splats (indexed by the group's leader lane)
-  uint32_t agg_lane[kB];
+  // packed sum of each splat group (indexed by the group's leader lane)
   alignas(8) uint64_t agg_sum[kB];
 };
 

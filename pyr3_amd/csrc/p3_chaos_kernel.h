@@ -188,15 +188,38 @@ struct Splat {
   uint64_t v;
 };
 
-// One turn of walker `wid` (global id) after its pick `fn` (from draw `raw`)
-// is known: chain, blend, bad check and retry window, final lens, jitter,
-// splat. Advances `wk.iter` unless the turn is retried. A splat whose words
-// are all at most kAggWord (every splat of a palette and opacity in [0, 1]:
-// the words are at most 255) is returned in `sp` for splat_wave; a wider one
-// is added here (packed below 2^16, else four saturating u32 adds).
+// Trace entry (walker wid, turn i), or null outside the traced window.
+P3_INLINE P3_GLOBAL float* trace_entry(const ChaosArgs& A, uint32_t wid, uint32_t i) {
+  if (A.trace == nullptr || wid >= A.trace_walkers || i >= A.trace_iters) return nullptr;
+  return (P3_GLOBAL float*)A.trace + ((size_t)wid * A.trace_iters + i) * kTraceFloats;
+}
+
+// Values walker wid has drawn since its ISAAC state was seeded or loaded
+// (for the trace only).
+P3_INLINE uint32_t drawn_since_start(const ChaosArgs& A, const Rng& rng, uint32_t wid) {
+  if (A.isaac_in) {   // the loaded state: its c and unread count
+    const P3_CONST uint32_t* st = (const P3_CONST uint32_t*)A.isaac_in + (size_t)wid * 36u;
+    return isaac_drawn(rng.s, st[3], st[0]);
+  }
+  return isaac_drawn(rng.s, 1u, 16u);   // device seeding ends after one round
+}
+
+// Loop state word of a walker: ISAAC unread count and flags (bits 0-6),
+// previous xform + 1 (bits 8-15), retry count (bits 16-18).
+P3_INLINE uint32_t pack_state(uint32_t cnt, int prev, uint32_t consec) {
+  return (cnt & 127u) | ((uint32_t)(prev + 1) << 8) | (consec << 16);
+}
+
+// One turn of walker `wid` (global id) after its pick `fn` is known: chain,
+// blend, bad check and retry window, final lens, jitter, splat. Advances
+// `wk.iter` unless the turn is retried. A splat whose words are all at most
+// kAggWord (every splat of a palette and opacity in [0, 1]: the words are at
+// most 255) is returned in `sp` for splat_wave; a wider one is added here
+// (packed below 2^16, else four saturating u32 adds). The trace entry's
+// pick draw (field 11) was written when the pick was drawn.
 template <class Spec>
-P3_INLINE void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32_t fn,
-                        uint32_t raw, uint32_t wid, Splat& sp) {
+P3_INLINE void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32_t fn, uint32_t wid,
+                        Splat& sp) {
   const uint32_t i = wk.iter;
   const float p0x = wk.x, p0y = wk.y;   // position before this turn's chain
   const xrec_t X = (xrec_t)A.xforms;
@@ -229,20 +252,18 @@ P3_INLINE void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32_t fn,
   } else {
     wk.consec = 0u;
   }
-  P3_GLOBAL float* tr = nullptr;
-  if (A.trace != nullptr && wid < A.trace_walkers && i < A.trace_iters) {
-    tr = (P3_GLOBAL float*)A.trace + ((size_t)wid * A.trace_iters + i) * kTraceFloats;
+  P3_GLOBAL float* tr = trace_entry(A, wid, i);
+  if (tr) {
     tr[0] = (float)i; tr[1] = (float)fn;
     tr[2] = ca.pa.x; tr[3] = ca.pa.y; tr[4] = ca.pv_pre.x; tr[5] = ca.pv_pre.y;
     tr[6] = pv.x; tr[7] = pv.y; tr[8] = bad ? 1.0f : 0.0f; tr[9] = nz;
-    tr[11] = __uint_as_float(raw);
     tr[12] = pv.x; tr[13] = pv.y; tr[14] = 0.0f; tr[15] = 0.0f;
   }
   const float ppx = pv.x, ppy = pv.y, ppz = nz;
   if (wk.consec > 0u && wk.consec < 5u) {
     // Retried turn: the loop index only advances when it is 0.
     wk.x = ppx; wk.y = ppy; wk.z = ppz;
-    if (tr) tr[10] = __uint_as_float(rng.draws);
+    if (tr) tr[10] = __uint_as_float(drawn_since_start(A, rng, wid));
     if (i == 0u) wk.iter = 1u;
     return;
   }
@@ -272,7 +293,7 @@ P3_INLINE void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32_t fn,
   wk.x = ppx + jx; wk.y = ppy + jy; wk.z = ppz;
   if (tr) {
     tr[12] = wk.x; tr[13] = wk.y; tr[14] = sx; tr[15] = sy;
-    tr[10] = __uint_as_float(rng.draws);
+    tr[10] = __uint_as_float(drawn_since_start(A, rng, wid));
   }
   wk.iter = i + 1u;
   if (i >= A.fuse) {
@@ -336,15 +357,18 @@ P3_INLINE void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32_t fn,
 // below 2^16, so add_packed stays exact. Hot bins (a compact attractor
 // concentrating its samples on a few bins) then cost one atomic per wave
 // instead of one per lane.
+// The wave's hash table is its 64 entries of `order`, which the wave's lanes
+// read at the start of the turn and the next write of which follows the
+// block's barrier.
 template <int kB>
 P3_INLINE void splat_wave(const ChaosArgs& A, BlockLds<kB>& L, const Splat& sp, int t) {
   const int lane = t & 63;
-  P3_LDS uint32_t* tab = (P3_LDS uint32_t*)L.agg_lane + (t & ~63);
+  P3_LDS uint16_t* tab = (P3_LDS uint16_t*)L.order + (t & ~63);
   P3_LDS uint64_t* sum = (P3_LDS uint64_t*)L.agg_sum + (t & ~63);
   const bool on = sp.bin != kNoSplat;
   sum[lane] = 0ull;
   const uint32_t slot = (sp.bin * 2654435761u) >> 26;
-  if (on) tab[slot] = (uint32_t)lane;
+  if (on) tab[slot] = (uint16_t)lane;
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
   const uint32_t lead = on ? tab[slot] : (uint32_t)lane;
@@ -424,7 +448,6 @@ P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L, uint32_t block)
     Rng rng;
     rng.m.base = (P3_LDS uint32_t*)(L.isaac + t);
     rng.m.stride = kB;
-    rng.draws = 0u;
     rng.s.cnt = rng.s.a = rng.s.b = rng.s.c = 0u;
     float x = 0.0f, y = 0.0f, z = 0.0f;
     uint32_t iter = total;
@@ -444,11 +467,12 @@ P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L, uint32_t block)
     if (iter < total) {
       uint32_t raw;
       f = draw_pick<Spec>(A, L, rng, -1, raw);
-      L.raw[t] = raw;
+      if (P3_GLOBAL float* tr = trace_entry(A, wid, 0u)) tr[11] = __uint_as_float(raw);
     }
-    L.cnt[t] = rng.s.cnt; L.ra[t] = rng.s.a; L.rb[t] = rng.s.b; L.rc[t] = rng.s.c;
+    L.ra[t] = rng.s.a; L.rb[t] = rng.s.b; L.rc[t] = rng.s.c;
+    L.state[t] = pack_state(rng.s.cnt, -1, 0u);
     L.x[t] = x; L.y[t] = y; L.z[t] = z;
-    L.prev[t] = -1; L.consec[t] = 0u; L.iter[t] = iter; L.draws[t] = rng.draws;
+    L.iter[t] = iter;
     L.fn[t] = (uint16_t)f;
     pos = atomicAdd(&L.bucket[0][f], 1u);
   }
@@ -487,20 +511,22 @@ P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L, uint32_t block)
       Rng rng;
       rng.m.base = (P3_LDS uint32_t*)(L.isaac + wl);
       rng.m.stride = kB;
-      rng.s.cnt = L.cnt[wl]; rng.s.a = L.ra[wl]; rng.s.b = L.rb[wl]; rng.s.c = L.rc[wl];
-      rng.draws = L.draws[wl];
+      const uint32_t st = L.state[wl];
+      rng.s.cnt = st & 127u; rng.s.a = L.ra[wl]; rng.s.b = L.rb[wl]; rng.s.c = L.rc[wl];
       isaac_top_up_wave(rng.s, rng.m);   // the wave's lanes generate rounds together here
-      Walker wk{L.x[wl], L.y[wl], L.z[wl], L.prev[wl], L.consec[wl], L.iter[wl]};
-      run_turn<Spec>(A, wk, rng, L.fn[wl], L.raw[wl], w0 + wl, sp);
+      Walker wk{L.x[wl], L.y[wl], L.z[wl], (int)((st >> 8) & 255u) - 1, (st >> 16) & 7u,
+                L.iter[wl]};
+      const uint32_t wid = w0 + wl;
+      run_turn<Spec>(A, wk, rng, L.fn[wl], wid, sp);
       if (wk.iter < total) {
         uint32_t raw;
         f = draw_pick<Spec>(A, L, rng, wk.prev, raw);
-        L.raw[wl] = raw;
+        if (P3_GLOBAL float* tr = trace_entry(A, wid, wk.iter)) tr[11] = __uint_as_float(raw);
       }
-      L.cnt[wl] = rng.s.cnt; L.ra[wl] = rng.s.a; L.rb[wl] = rng.s.b; L.rc[wl] = rng.s.c;
-      L.draws[wl] = rng.draws;
+      L.ra[wl] = rng.s.a; L.rb[wl] = rng.s.b; L.rc[wl] = rng.s.c;
+      L.state[wl] = pack_state(rng.s.cnt, wk.prev, wk.consec);
       L.x[wl] = wk.x; L.y[wl] = wk.y; L.z[wl] = wk.z;
-      L.prev[wl] = wk.prev; L.consec[wl] = wk.consec; L.iter[wl] = wk.iter;
+      L.iter[wl] = wk.iter;
       L.fn[wl] = (uint16_t)f;
     }
     splat_wave<kB>(A, L, sp, t);

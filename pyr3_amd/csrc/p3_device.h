@@ -169,12 +169,19 @@ P3_INLINE void isaac_seed(IsaacRegs& s, const IsaacLds& m, uint32_t seed,
   s.cnt = 16u;
 }
 
+// Values a walker has drawn since its state (c0, unread0): every round adds
+// 16 values and every draw takes one, so drawn = 16 (c - c0) - (unread -
+// unread0), unread counting the round generated ahead.
+P3_INLINE uint32_t isaac_drawn(const IsaacRegs& s, uint32_t c0, uint32_t unread0) {
+  const uint32_t unread = (s.cnt & kIsaacUnread) + ((s.cnt & kIsaacReady) ? 16u : 0u);
+  return 16u * (s.c - c0) - unread + unread0;
+}
+
 // A walker's RNG as the variations see it.
 struct Rng {
   IsaacRegs s;
   IsaacLds m;
-  uint32_t draws;
-  P3_INLINE uint32_t irand() { ++draws; return isaac_irand(s, m); }
+  P3_INLINE uint32_t irand() { return isaac_irand(s, m); }
   // chaos.wgsl:319-331: low 28 bits scaled by 1/(2^28-1) and 1/(2^27-1).
   P3_INLINE float rand01() {
     return (float)(irand() & 0x0fffffffu) * (1.0f / 268435455.0f);

@@ -234,14 +234,14 @@ bool compile(const std::string& src, std::vector<char>& code, std::string& err) 
 
 namespace p3 {
 
-// Walkers per chaos block: 576, or P3_CHAOS_BLOCK (a multiple of 64 up to
-// 576), read once per process.
+// Walkers per chaos block: 640, or P3_CHAOS_BLOCK (a multiple of 64 up to
+// 640), read once per process.
 int chaos_block() {
   static const int b = [] {
     const char* e = std::getenv("P3_CHAOS_BLOCK");
     const int v = e ? std::atoi(e) : kChaosBlock;
-    // whole waves; 576 walkers x 256 B of block state fit one CU's 160 KiB LDS
-    return (v >= 64 && v <= 576 && v % 64 == 0) ? v : kChaosBlock;
+    // whole waves; 640 walkers x 236 B of block state fit one CU's 160 KiB LDS
+    return (v >= 64 && v <= 640 && v % 64 == 0) ? v : kChaosBlock;
   }();
   return b;
 }

@@ -45,7 +45,7 @@ DE_UNIFORM_WORDS = 8
 VIZ_UNIFORM_WORDS = 24
 DEFAULT_WALKER_JITTER = 1e-7        # src/chaos.ts:96
 FUSE = 200                          # src/renderer.ts:40
-CHAOS_BLOCK = 576                   # walkers per chaos block (kChaosBlock in p3_args.h)
+CHAOS_BLOCK = 640                   # walkers per chaos block (kChaosBlock in p3_args.h)
 CHIP_WALKERS = 256 * CHAOS_BLOCK    # one chaos block on each of the 256 CUs
 TARGET_WALKERS = 1024               # src/renderer.ts:26
 MIN_ITERS_PER_WALKER = 4096

@@ -19,7 +19,7 @@ at the genome's quality (or `--quality`), present (DE + tonemap), then:
         [--walkers N] [--oversample N] [--no-de] [--transparent]
         [--format png8|png16|exr|exr-linear] [--no-auto-exposure]
 
-Walkers default to 147,456, one chaos block of 576 on each CU (the
+Walkers default to 163,840, one chaos block of 640 on each CU (the
 reference's `--walkers` override; its own default of ~1,024 long walkers
 fills only a few CUs of this GPU).
 """
