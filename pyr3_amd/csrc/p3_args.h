@@ -65,11 +65,14 @@ struct MultiArgs {
 // bits 0-6, previous xform + 1 in bits 8-15, retry count in bits 16-18), the
 // position and colour, the turn index, the pick and the xform order; the
 // splat aggregation's group sums (its hash table of lanes reuses `order`,
-// dead by then): 236 bytes per walker, plus the pick boundaries (544 B) and
-// two bucket rows. 640 walkers fit one CU's 160 KiB.
+// dead by then): 236 bytes per walker, plus the palette (4 KiB), the pick
+// boundaries (544 B), two bucket rows and two live counts. 640 walkers fit
+// one CU's 160 KiB.
 template <int kB>
 struct BlockLds {
+  float4 palette[256];
   alignas(16) uint16_t pick_b[kPickRows * kPickCols];
+  uint32_t live[2];                     // walkers still turning, two turns alternating
   uint32_t isaac[48 * kB];
   uint32_t ra[kB], rb[kB], rc[kB];
   uint32_t state[kB];

@@ -73,8 +73,7 @@ P3_INLINE void pin_max(P3_GLOBAL uint32_t* p) {
 // 64-bit wrap) in the bin's carry[0..3] makes the field sums exact:
 // S_k = field_k + 2^16 carry[k] - carry[k-1]. hist_merge (p3_image.hip) adds
 // them to the u32 histogram, saturating.
-P3_INLINE void add_packed(P3_GLOBAL uint64_t* acc, P3_GLOBAL uint64_t* carry, uint64_t v) {
-  const uint64_t o = __hip_atomic_fetch_add(acc, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+P3_INLINE void count_carries(P3_GLOBAL uint64_t* carry, uint64_t o, uint64_t v) {
   const uint64_t n = o + v;
   const uint64_t c = o ^ v ^ n;
   const bool c0 = (c >> 16) & 1u, c1 = (c >> 32) & 1u, c2 = (c >> 48) & 1u, c3 = n < o;
@@ -84,6 +83,12 @@ P3_INLINE void add_packed(P3_GLOBAL uint64_t* acc, P3_GLOBAL uint64_t* carry, ui
     if (c2) (void)__hip_atomic_fetch_add(carry + 2, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (c3) (void)__hip_atomic_fetch_add(carry + 3, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
+}
+P3_INLINE uint64_t add_packed_issue(P3_GLOBAL uint64_t* acc, uint64_t v) {
+  return __hip_atomic_fetch_add(acc, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+P3_INLINE void add_packed(P3_GLOBAL uint64_t* acc, P3_GLOBAL uint64_t* carry, uint64_t v) {
+  count_carries(carry, add_packed_issue(acc, v), v);
 }
 P3_INLINE uint64_t pack_words(uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3) {
   return (uint64_t)d0 | ((uint64_t)d1 << 16) | ((uint64_t)d2 << 32) | ((uint64_t)d3 << 48);
@@ -103,11 +108,9 @@ P3_INLINE void hist_add4_sat(P3_GLOBAL uint32_t* b, uint32_t d0, uint32_t d1, ui
   if (d3 && o3 > 0xffffffffu - d3) pin_max(b + 3);
 }
 
-// Palette entry i (four floats read through the constant address space).
-P3_INLINE float4 palette_at(const ChaosArgs& A, uint32_t i) {
-  const P3_CONST float* p = (const P3_CONST float*)A.palette + 4u * i;
-  return make_float4(p[0], p[1], p[2], p[3]);
-}
+// The block's LDS copy of the 256-entry palette.
+typedef const P3_LDS float4* pal_t;
+P3_INLINE float4 palette_at(pal_t pal, uint32_t i) { return pal[i]; }
 
 // chaos.wgsl:1662-1666: k = (1, 2/3, 1/3), p = |fract(h + k) * 6 - 3|,
 // rgb = v * mix(1, clamp(p - 1, 0, 1), s); fract(x) = x - floor(x).
@@ -128,8 +131,8 @@ P3_INLINE float3 hsv_to_rgb(float h, float s, float v) {
 // magnitude -> value), trap distance (palette at a glow / ring coordinate of
 // the distance to a point, circle or line) and phase (arg z -> hue,
 // log-modulus rings -> value), blended over the palette colour by strength.
-P3_INLINE void colour_mode(const ChaosArgs& A, float sx, float sy, float p0x, float p0y,
-                           float4& pal) {
+P3_INLINE void colour_mode(const ChaosArgs& A, pal_t palette, float sx, float sy, float p0x,
+                           float p0y, float4& pal) {
   float3 rgb;
   float t;
   if (A.color_mode == 1u) {
@@ -154,7 +157,7 @@ P3_INLINE void colour_mode(const ChaosArgs& A, float sx, float sy, float p0x, fl
       tt = f - floorf(f);
     }
     const uint32_t idx = min(f2u_sat(tt * 255.0f), 255u);
-    const float4 c = palette_at(A, idx);
+    const float4 c = palette_at(palette, idx);
     rgb = make_float3(c.x, c.y, c.z);
     t = fminf(fmaxf(A.trap_strength, 0.0f), 1.0f);
   } else if (A.color_mode == 3u) {
@@ -218,8 +221,8 @@ P3_INLINE uint32_t pack_state(uint32_t cnt, int prev, uint32_t consec) {
 // (packed below 2^16, else four saturating u32 adds). The trace entry's
 // pick draw (field 11) was written when the pick was drawn.
 template <class Spec>
-P3_INLINE void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32_t fn, uint32_t wid,
-                        Splat& sp) {
+P3_INLINE void run_turn(const ChaosArgs& A, pal_t palette, Walker& wk, Rng& rng, uint32_t fn,
+                        uint32_t wid, Splat& sp) {
   const uint32_t i = wk.iter;
   const float p0x = wk.x, p0y = wk.y;   // position before this turn's chain
   const xrec_t X = (xrec_t)A.xforms;
@@ -312,15 +315,15 @@ P3_INLINE void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32_t fn, u
         uint32_t i0 = f2u_sat(cxf);
         float frac = cxf - (float)i0;
         if (i0 >= 255u) { i0 = 254u; frac = 1.0f; }
-        const float4 p0 = palette_at(A, i0), p1 = palette_at(A, i0 + 1u);
+        const float4 p0 = palette_at(palette, i0), p1 = palette_at(palette, i0 + 1u);
         pal = make_float4(mixf(p0.x, p1.x, frac), mixf(p0.y, p1.y, frac),
                           mixf(p0.z, p1.z, frac), mixf(p0.w, p1.w, frac));
       } else {
         const uint32_t pi = min(f2u_sat(cxf), 255u);
-        pal = palette_at(A, pi);
+        pal = palette_at(palette, pi);
       }
       if (dc_on) { pal.x = dc_rgb.x; pal.y = dc_rgb.y; pal.z = dc_rgb.z; }
-      if (A.color_mode != 0u) colour_mode(A, sx, sy, p0x, p0y, pal);
+      if (A.color_mode != 0u) colour_mode(A, palette, sx, sy, p0x, p0y, pal);
       const float wgt = opacity * 255.0f;
       const size_t bin = ((size_t)yi * A.width + xi) * 4u;
       P3_GLOBAL uint32_t* b = (P3_GLOBAL uint32_t*)A.hist + bin;
@@ -360,8 +363,23 @@ P3_INLINE void run_turn(const ChaosArgs& A, Walker& wk, Rng& rng, uint32_t fn, u
 // The wave's hash table is its 64 entries of `order`, which the wave's lanes
 // read at the start of the turn and the next write of which follows the
 // block's barrier.
+//
+// A lane's packed add is issued here and its carries are counted at the
+// lane's next call (or by settle_add after the block's last turn), so the
+// wave does not wait for the atomic's returned value within the turn.
+struct PendingAdd {
+  uint64_t o, v;   // the add's returned old value and its increment
+  uint32_t bin;    // kNoSplat: none pending
+};
+P3_INLINE void settle_add(const ChaosArgs& A, PendingAdd& q) {
+  if (q.bin != kNoSplat) count_carries((P3_GLOBAL uint64_t*)A.carry + (size_t)4 * q.bin, q.o, q.v);
+  q.bin = kNoSplat;
+}
+
 template <int kB>
-P3_INLINE void splat_wave(const ChaosArgs& A, BlockLds<kB>& L, const Splat& sp, int t) {
+P3_INLINE void splat_wave(const ChaosArgs& A, BlockLds<kB>& L, const Splat& sp, int t,
+                          PendingAdd& pend) {
+  settle_add(A, pend);
   const int lane = t & 63;
   P3_LDS uint16_t* tab = (P3_LDS uint16_t*)L.order + (t & ~63);
   P3_LDS uint64_t* sum = (P3_LDS uint64_t*)L.agg_sum + (t & ~63);
@@ -379,8 +397,9 @@ P3_INLINE void splat_wave(const ChaosArgs& A, BlockLds<kB>& L, const Splat& sp, 
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
   if (on && (!grouped || lead == (uint32_t)lane)) {
-    const uint64_t v = grouped ? sum[lane] : sp.v;
-    add_packed((P3_GLOBAL uint64_t*)A.acc + sp.bin, (P3_GLOBAL uint64_t*)A.carry + (size_t)4 * sp.bin, v);
+    pend.v = grouped ? sum[lane] : sp.v;
+    pend.o = add_packed_issue((P3_GLOBAL uint64_t*)A.acc + sp.bin, pend.v);
+    pend.bin = sp.bin;
   }
 }
 
@@ -436,10 +455,13 @@ P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L, uint32_t block)
   const uint32_t idle = A.fallback_row + 1u;   // bucket of finished walkers
 
   for (int k = t; k < 2 * (kMaxXforms + 2); k += kB) (&L.bucket[0][0])[k] = 0u;
+  if (t < 2) L.live[t] = 0u;
+  for (int k = t; k < 256; k += kB) L.palette[k] = A.palette[k];
   if (Spec::kRegular < kPickRows && A.pick_bounds != nullptr)
     for (uint32_t k = t; k < (A.fallback_row + 1u) * kPickCols; k += kB)
       L.pick_b[k] = ((const P3_CONST uint16_t*)A.pick_bounds)[k];
   __syncthreads();
+  PendingAdd pend{0ull, 0ull, kNoSplat};
   int wl = t;          // walker this lane holds
   uint32_t f = idle;   // its pick
   uint32_t pos;        // its rank among the walkers with that pick
@@ -478,7 +500,14 @@ P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L, uint32_t block)
   }
 
   for (uint32_t p = 0;; p ^= 1u) {
-    if (__syncthreads_count(f != idle) == 0) break;
+    {  // walkers still turning: one LDS add per wave, one barrier; count p ^ 1
+       // was last read before the previous turn's second barrier
+      const uint64_t busy = __ballot(f != idle);
+      if ((t & 63) == 0 && busy) atomicAdd(&L.live[p], (uint32_t)__builtin_popcountll(busy));
+      __syncthreads();
+      if (L.live[p] == 0u) break;
+      if (t == 0) L.live[p ^ 1u] = 0u;
+    }
     // every wave scans the bucket counts itself (64 buckets per step,
     // carried across steps), takes its walkers' bucket offsets from the lane
     // holding each bucket, and lists its walkers in xform order; the other
@@ -517,7 +546,7 @@ P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L, uint32_t block)
       Walker wk{L.x[wl], L.y[wl], L.z[wl], (int)((st >> 8) & 255u) - 1, (st >> 16) & 7u,
                 L.iter[wl]};
       const uint32_t wid = w0 + wl;
-      run_turn<Spec>(A, wk, rng, L.fn[wl], wid, sp);
+      run_turn<Spec>(A, (pal_t)L.palette, wk, rng, L.fn[wl], wid, sp);
       if (wk.iter < total) {
         uint32_t raw;
         f = draw_pick<Spec>(A, L, rng, wk.prev, raw);
@@ -529,9 +558,10 @@ P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L, uint32_t block)
       L.iter[wl] = wk.iter;
       L.fn[wl] = (uint16_t)f;
     }
-    splat_wave<kB>(A, L, sp, t);
+    splat_wave<kB>(A, L, sp, t, pend);
     pos = atomicAdd(&L.bucket[p ^ 1u][f], 1u);
   }
+  settle_add(A, pend);
 }
 
 }  // namespace p3
