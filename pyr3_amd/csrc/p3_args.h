@@ -73,6 +73,7 @@ struct BlockLds {
   float4 palette[256];
   alignas(16) uint16_t pick_b[kPickRows * kPickCols];
   uint32_t live[2];                     // walkers still turning, two turns alternating
+  uint32_t probe[2];                    // values drawn and walkers at the probe turn
   uint32_t isaac[48 * kB];
   uint32_t ra[kB], rb[kB], rc[kB];
   uint32_t state[kB];

@@ -455,7 +455,7 @@ P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L, uint32_t block)
   const uint32_t idle = A.fallback_row + 1u;   // bucket of finished walkers
 
   for (int k = t; k < 2 * (kMaxXforms + 2); k += kB) (&L.bucket[0][0])[k] = 0u;
-  if (t < 2) L.live[t] = 0u;
+  if (t < 2) L.live[t] = L.probe[t] = 0u;
   for (int k = t; k < 256; k += kB) L.palette[k] = A.palette[k];
   if (Spec::kRegular < kPickRows && A.pick_bounds != nullptr)
     for (uint32_t k = t; k < (A.fallback_row + 1u) * kPickCols; k += kB)
@@ -499,7 +499,14 @@ P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L, uint32_t block)
     pos = atomicAdd(&L.bucket[0][f], 1u);
   }
 
-  for (uint32_t p = 0;; p ^= 1u) {
+  // The ISAAC top-up rhythm (isaac_top_up_turn): every walker's draws over
+  // the first kProbe turns give the block's mean draws per turn, and the
+  // rhythm is the number of such turns that fit in 14 of the 16 values a
+  // top-up leaves ahead, at most 3 (1 until then, and for walkers with fewer
+  // turns).
+  constexpr uint32_t kProbe = 12u;
+  uint32_t every = 1u;
+  for (uint32_t p = 0, turn = 0;; p ^= 1u, ++turn) {
     {  // walkers still turning: one LDS add per wave, one barrier; count p ^ 1
        // was last read before the previous turn's second barrier
       const uint64_t busy = __ballot(f != idle);
@@ -507,6 +514,11 @@ P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L, uint32_t block)
       __syncthreads();
       if (L.live[p] == 0u) break;
       if (t == 0) L.live[p ^ 1u] = 0u;
+      if (turn == kProbe + 1u) {   // the probe sums were added before the barrier
+        const uint32_t n = L.probe[1], d = L.probe[0];
+        const uint32_t fit = d ? (uint32_t)((14ull * n * kProbe) / d) : 3u;
+        every = fit >= 3u ? 3u : (fit >= 2u ? 2u : 1u);
+      }
     }
     // every wave scans the bucket counts itself (64 buckets per step,
     // carried across steps), takes its walkers' bucket offsets from the lane
@@ -542,7 +554,7 @@ P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L, uint32_t block)
       rng.m.stride = kB;
       const uint32_t st = L.state[wl];
       rng.s.cnt = st & 127u; rng.s.a = L.ra[wl]; rng.s.b = L.rb[wl]; rng.s.c = L.rc[wl];
-      isaac_top_up_wave(rng.s, rng.m);   // the wave's lanes generate rounds together here
+      isaac_top_up_turn(rng.s, rng.m, turn, every);   // rounds generated together
       Walker wk{L.x[wl], L.y[wl], L.z[wl], (int)((st >> 8) & 255u) - 1, (st >> 16) & 7u,
                 L.iter[wl]};
       const uint32_t wid = w0 + wl;
@@ -557,6 +569,10 @@ P3_INLINE void chaos_binned(const ChaosArgs& A, BlockLds<kB>& L, uint32_t block)
       L.x[wl] = wk.x; L.y[wl] = wk.y; L.z[wl] = wk.z;
       L.iter[wl] = wk.iter;
       L.fn[wl] = (uint16_t)f;
+      if (turn == kProbe) {
+        atomicAdd(&L.probe[0], drawn_since_start(A, rng, wid));
+        atomicAdd(&L.probe[1], 1u);
+      }
     }
     splat_wave<kB>(A, L, sp, t, pend);
     pos = atomicAdd(&L.bucket[p ^ 1u][f], 1u);

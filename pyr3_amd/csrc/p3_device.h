@@ -102,14 +102,20 @@ P3_INLINE void isaac_top_up(IsaacRegs& s, const IsaacLds& m) {
   }
 }
 
-// Wave-level top-up: the wave generates rounds only when one of its lanes
-// has fewer than kIsaacLow values left (none generated ahead), and then every
-// lane with a free half generates together. A lane refills about every four
-// turns, so topping up whenever any lane had a free half ran a round on most
-// turns with a quarter of the lanes; this way a round runs on fewer turns with
-// more lanes. A turn that draws more than is left still generates inline.
+// Top-up policy of a chaos block. With every = N > 1 (a genome that draws
+// few values per turn, measured by the block) the whole block generates on
+// every N-th turn, each lane with a free half, so rounds run on one turn in N
+// and the other turns' barriers wait for no round; with N = 1 a wave
+// generates when one of its lanes has fewer than kIsaacLow values left (none
+// generated ahead), again every lane with a free half. Either way a turn that
+// draws more than is left still generates inline, so the policy changes only
+// when rounds run.
 constexpr uint32_t kIsaacLow = 8u;
-P3_INLINE void isaac_top_up_wave(IsaacRegs& s, const IsaacLds& m) {
+P3_INLINE void isaac_top_up_turn(IsaacRegs& s, const IsaacLds& m, uint32_t turn, uint32_t every) {
+  if (every > 1u) {
+    if (turn % every == 0u) isaac_top_up(s, m);
+    return;
+  }
   const bool low = !(s.cnt & kIsaacReady) && (s.cnt & kIsaacUnread) < kIsaacLow;
   if (__any(low)) isaac_top_up(s, m);
 }
