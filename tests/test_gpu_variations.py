@@ -164,3 +164,40 @@ def test_direct_colour_histogram_bit_exact(orc, R):
         assert xf[:, 11].max() == 1.0  # dc flag packed
         ho, _ = orc.chaos(xf, ds, u, pal, trace_shape=(1, 1))
         assert np.array_equal(hg, ho), final is not None
+
+
+def test_hash_entries_on_well_conditioned_inputs(orc, R):
+    """Values of the five entries held to draw counts above, where their
+    arithmetic is well conditioned: circular, circular2 and waves4 with
+    inputs scaled to ~1e-6, where the sine inside their hash is its
+    argument in f32 on both sides (circular2's seeds make its hash argument
+    tiny too); hypertile2 and phoenix_julia at ordinary inputs, where only
+    a 1-ulp difference of the large rotation angle's sum separates the two."""
+    names = ("circular", "circular2", "waves4", "hypertile2", "phoenix_julia")
+    tiny = {"circular", "circular2", "waves4"}
+    xs = []
+    for n in names:
+        i = cat.VARIATION_INDEX[n]
+        s = 1e-6 if n in tiny else 2.0
+        xs.append(xform(1.0, (s, 0.0, 0.0, 0.0, s, 0.0), vars=((i, 1.0),), params=[list(_params(i))]))
+    g = genome(xs, scale=20.0)
+    W = H = 32
+    walkers, seed = 8192, 4321
+    r = R(W, H, 1)
+    r.reset(g)
+    r.set_trace(walkers, 1)
+    r.iterate(g, seed, walkers, 1, fuse=0)
+    tg = r.read_trace()[:, 0]
+    r.destroy()
+    xf, ds, u, pal = blobs(g, W, H, iters=1, fuse=0, seed=seed, walkers=walkers)
+    _, to = orc.chaos(xf, ds, u, pal, trace_shape=(walkers, 1))
+    to = to[:, 0]
+    assert np.array_equal(tg[:, 1], to[:, 1])
+    rates = {}
+    for j, n in enumerate(names):
+        m = to[:, 1] == j
+        assert m.sum() >= 500
+        assert np.array_equal(tg[m, 10].view(np.uint32), to[m, 10].view(np.uint32)), n
+        rates[n] = float(_rel_ok(tg[m, 4:6], to[m, 4:6], 1e-4).mean())
+    print("share within 1e-4:", rates)
+    assert all(v >= 0.99 for v in rates.values()), rates

@@ -2385,4 +2385,439 @@ def test_quadratic_and_hopping_maps(orc):
     assert abs(o[0] - (0.5 * -0.7 + 0.3)) < 1e-12 and abs(o[1] - (-0.3 + o[0])) < 1e-12
 
 
+# ------------------------------- JWildfire ports, V152 - V219 (WGSL restated) --
+# The reference's only tests of these entries are finite-output smokes at
+# seven points (src/issue121-Lall.gpu.test.ts:50-98, 197-240 and
+# src/issue121-Lall-tail.gpu.test.ts:50-72, 139-160); their definitions are
+# the WGSL bodies in src/shaders/chaos.wgsl, restated here independently of
+# the oracle's C++ and compared at those points and more.
+LALL_PTS = [(0.5, 0.3), (-0.5, 0.7), (1.2, -0.4), (-1.0, -1.0), (0.1, 0.05), (0.0, 2.0),
+            (-2.5, 1.7)]
+
+
+def _sel(f, t, c):   # WGSL select(f, t, cond)
+    return t if c else f
+
+
+def ennepers_ref(x, y, w):
+    return w * (x - x ** 3 / 3.0) + x * y * y, w * (y - y ** 3 / 3.0) + y * x * x
+
+
+def circus_ref(x, y, w, scale):
+    s1 = 1e30 if scale == 0 else 1.0 / scale
+    r, a = math.hypot(x, y), math.atan2(y, x)
+    rs = r * scale if r <= 1.0 else r * s1
+    return w * rs * math.cos(a), w * rs * math.sin(a)
+
+
+def clifford_js_ref(x, y, w, a, b, c, d):
+    return (w * (math.sin(a * y) + c * math.cos(a * x)),
+            w * (math.sin(b * x) + d * math.cos(b * y)))
+
+
+def devil_warp_ref(x, y, w, a, b, effect, warp, rmin, rmax):
+    r2 = 1.0 / max(x * x + y * y, 1e-30)
+    ba, bb = x * x + r2 * b * y * y, y * y + r2 * a * x * x
+    pa = ba ** warp if ba > 0 else 0.0
+    pb = bb ** warp if bb > 0 else 0.0
+    r = effect * min(max(pa - pb, rmin), rmax)
+    return x * (1 + r), y * (1 + r)
+
+
+def cardioid_ref(x, y, w, a):
+    th = math.atan2(y, x)
+    r = w * math.sqrt(max(x * x + y * y + math.sin(a * th) + 1.0, 0.0))
+    return r * math.cos(th), r * math.sin(th)
+
+
+def bcollide_ref(x, y, w, num_p, a):
+    num = max(1.0, num_p)
+    bcn_pi, pi_bcn, bca_bcn = num / PI, PI / num, PI * a / num
+    xp1, xm1, y2 = x + 1, x - 1, y * y
+    tau = 0.5 * (math.log(max(xp1 * xp1 + y2, 1e-30)) - math.log(max(xm1 * xm1 + y2, 1e-30)))
+    sr = PI - math.atan2(y, xp1) - math.atan2(y, 1 - x)
+    alt = int(sr * bcn_pi)
+    off = bca_bcn if alt & 1 == 0 else -bca_bcn
+    fold = sr + off - math.floor((sr + off) / pi_bcn) * pi_bcn
+    sig = alt * pi_bcn + fold
+    t = math.cosh(tau) - math.cos(sig)
+    t = 1e-30 if abs(t) < 1e-30 else t
+    return w * math.sinh(tau) / t, w * math.sin(sig) / t
+
+
+def bsplit_ref(x, y, w, sx, sy):
+    ax = x + sx
+    s = math.sin(ax)
+    if abs(s) < 1e-6:
+        return 0.0, 0.0
+    c = math.cos(ax)
+    tn = s / c
+    tn = math.copysign(1e6, tn) if abs(c) < 1e-6 else tn
+    return w / tn * math.cos(y + sy), w / s * (-y + sy)
+
+
+def bulge_ref(x, y, w, n):
+    r = max(math.hypot(x, y), 1e-30)
+    s = r ** n / r
+    return w * x * s, w * y * s
+
+
+def corners_ref(x, y, w, cx, cy, mx, my, xp, yp, xya, log_mode, log_base):
+    xs, ys = x * x, y * y
+    if log_mode == 0:
+        ex, ey = xs ** (xp + xya) * mx, ys ** (yp + xya) * my
+    else:
+        lb = math.log(max(abs(log_base), 1.000001))
+        ex = (math.log(xs * mx + 3.0) / lb) ** (xp + 2.25 + xya) - 1.33
+        ey = (math.log(ys * my + 3.0) / lb) ** (yp + 2.25 + xya) - 1.33
+    return (w * ex + cx if x > 0 else -w * ex - cx), (w * ey + cy if y > 0 else -w * ey - cy)
+
+
+def fibonacci2_ref(x, y, w, sc, sc2):
+    ff, fl = 0.447213595, 0.481211825
+    a, b = y * fl, -(x * PI + y * fl)
+    e1, e2 = sc * math.exp(sc2 * (x * fl)), sc * math.exp(sc2 * -(x * fl - y * PI))
+    return (w * (e1 * math.cos(a) - e2 * math.cos(b)) * ff,
+            w * (e1 * math.sin(a) - e2 * math.sin(b)) * ff)
+
+
+def hypertile_ref(x, y, w, p_p, q, n):
+    pa, qa = 2 * PI / max(3.0, p_p), 2 * PI / max(3.0, q)
+    den = math.cos(pa) + math.cos(qa)
+    r2 = (1 - math.cos(pa)) / den + 1 if abs(den) > 1e-30 else 1.0
+    r = 1 / math.sqrt(max(r2, 1e-30)) if r2 > 0 else 1.0
+    re, im = r * math.cos(n * pa), r * math.sin(n * pa)
+    a, b = x + re, y - im
+    c, d = re * x - im * y + 1, re * y + im * x
+    vr = w / max(c * c + d * d, 1e-30)
+    return vr * (a * c + b * d), vr * (b * c - a * d)
+
+
+def idisc_ref(x, y, w):
+    a = PI / (math.hypot(x, y) + 1)
+    v = math.atan2(y, x) * w / PI
+    return v * math.cos(a), v * math.sin(a)
+
+
+def hole_ref(x, y, w, a, inside_p):
+    al = math.atan2(y, x)
+    de = max(al / PI + 1, 1e-30) ** a
+    s = x * x + y * y
+    r = w * de / max(s + de, 1e-30) if int(inside_p) != 0 else w * math.sqrt(max(s + de, 0.0))
+    return r * math.cos(al), r * math.sin(al)
+
+
+def kaleidoscope_ref(x, y, w, pull, rotate, line_up, off_x, off_y):
+    c45, s45 = math.cos(45.0), math.sin(45.0)   # 45 radians, as the reference has it
+    ox = w * ((rotate * x * c45 - y * s45 + line_up) + off_x)
+    oy = (w * ((rotate * y * c45 + x * s45 + pull + line_up) + off_y) if y > 0
+          else w * (rotate * y * c45 + x * s45 - pull - line_up))
+    return ox, oy
+
+
+def layered_spiral_ref(x, y, w, radius):
+    t = x * x + y * y + 1e-30
+    return w * x * radius * math.cos(t), w * x * radius * math.sin(t)
+
+
+def linear_t_ref(x, y, w, px, py):
+    return (w * (1 if x >= 0 else -1) * max(abs(x), 1e-30) ** px,
+            w * (1 if y >= 0 else -1) * max(abs(y), 1e-30) ** py)
+
+
+def ovoid_ref(x, y, w, px, py):
+    r = w / (x * x + y * y + 1e-6)
+    return x * r * px, y * r * py
+
+
+def unpolar_ref(x, y, w):
+    v = w / PI * 0.5
+    return v * math.exp(y) * math.sin(x), v * math.exp(y) * math.cos(x)
+
+
+def target_ref(x, y, w, even, odd, size):
+    a, r = math.atan2(y, x), math.hypot(x, y)
+    t = math.log(max(r, 1e-30))
+    if t < 0:
+        t -= 0.5 * size
+    ss = max(abs(size), 1e-30)
+    t = abs(t) - math.floor(abs(t) / ss) * ss
+    a += even if t < 0.5 * size else odd
+    return r * math.cos(a) * w, r * math.sin(a) * w
+
+
+def funnel_ref(x, y, w, effect):
+    cx, cy = math.cos(x), math.cos(y)
+    sx = 1 / (1e-30 if abs(cx) < 1e-30 else cx)
+    sy = 1 / (1e-30 if abs(cy) < 1e-30 else cy)
+    off = effect * PI
+    return (w * math.tanh(min(max(x, -20.0), 20.0)) * (sx + off),
+            w * math.tanh(min(max(y, -20.0), 20.0)) * (sy + off))
+
+
+def holesq_ref(x, y, w):
+    x, y = w * x, w * y
+    fx, fy = abs(x), abs(y)
+    if fx + fy > 1:
+        return x, y
+    if fx > fy:
+        return ((x - fy + 1) * 0.5 if x >= 0 else (x + fy - 1) * 0.5), y
+    return x, ((y - fx + 1) * 0.5 if y >= 0 else (y + fx - 1) * 0.5)
+
+
+def fourth_ref(x, y, w, spin, space, twist, off_x, off_y):
+    if x > 0 and y > 0:
+        th, r = math.atan2(y, x), 1 / max(math.hypot(x, y), 1e-30)
+        return w * r * math.cos(th), w * r * math.sin(th)
+    if x > 0 and y < 0:
+        r2 = x * x + y * y
+        if r2 < w * w:
+            r = w * math.sqrt(max(w * w / max(r2, 1e-30) - 1, 0.0))
+            return r * x, r * y
+        return w * x, w * y
+    if x < 0 and y > 0:
+        xx, yy = x - off_x, y + off_y
+        r0 = math.hypot(xx, yy)
+        if r0 < w:
+            th = math.atan2(yy, xx) + spin + twist * (w - r0)
+            return w * r0 * math.cos(th) + off_x, w * r0 * math.sin(th) - off_y
+        r = w * (1 + space / max(r0, 1e-30))
+        return r * xx + off_x, r * yy - off_y
+    return w * x, w * y
+
+
+def pulse_ref(x, y, w, fx, fy, sx, sy):
+    return w * (x + sx * math.sin(x * fx)), w * (y + sy * math.sin(y * fy))
+
+
+def rays1_ref(x, y, w):
+    t = x * x + y * y
+    tv = math.tan(math.sqrt(max(t, 1e-30)))
+    u = 1 / (1e-30 if abs(tv) < 1e-30 else tv) + w * (2 / PI) ** 2
+    return w * u * t / (x if x != 0 else 1e-30), w * u * t / (y if y != 0 else 1e-30)
+
+
+def rays3_ref(x, y, w):
+    t = x * x + y * y
+    ts = max(t, 1e-30)
+    inner = math.sin(t * t + 1e-6) * math.sin(1 / (ts * ts) + 1e-6)
+    u = 1 / max(math.sqrt(max(math.cos(inner), 1e-30)), 1e-30)
+    c = w / 10
+    return (c * u * math.cos(t) * t / (x if x != 0 else 1e-30),
+            c * u * math.tan(t) * t / (y if y != 0 else 1e-30))
+
+
+def tancos_ref(x, y, w):
+    d1 = 1e-6 + x * x + y * y
+    d2 = w / d1
+    return d2 * math.tanh(min(d1, 20.0)) * 2 * x, d2 * math.cos(d1) * 2 * y
+
+
+def twoface_ref(x, y, w):
+    v = w / max(x * x + y * y, 1e-30) if x > 0 else w
+    return v * x, v * y
+
+
+def _elliptic_xmax(x, y):
+    tmp, tmp2 = y * y + x * x + 1, 2 * x
+    xmax = (math.sqrt(max(tmp + tmp2, 0.0)) + math.sqrt(max(tmp - tmp2, 0.0))) * 0.5
+    return max(xmax, 1.0)
+
+
+def e_collide_ref(x, y, w, num_p, a):
+    num = max(1.0, num_p)
+    ecn_pi, pi_ecn, eca_ecn = num / PI, PI / num, PI * a / num
+    xmax = _elliptic_xmax(x, y)
+    nu = math.acos(min(max(x / xmax, -1.0), 1.0))
+    alt = int(nu * ecn_pi)
+    arg = nu + (eca_ecn if alt & 1 == 0 else -eca_ecn)
+    nu = alt * pi_ecn + (arg - math.floor(arg / pi_ecn) * pi_ecn)
+    if y <= 0:
+        nu = -nu
+    return (w * xmax * math.cos(nu),
+            w * math.sqrt(max(xmax - 1, 0.0)) * math.sqrt(xmax + 1) * math.sin(nu))
+
+
+def e_mod_ref(x, y, w, radius, distance):
+    xmax = _elliptic_xmax(x, y)
+    mu = math.acosh(xmax)
+    nu = math.acos(min(max(x / xmax, -1.0), 1.0))
+    if y < 0:
+        nu = -nu
+    if mu < radius and -mu < radius:
+        r = max(radius, 1e-30)
+        if nu > 0:
+            mu = math.fmod(mu + r + distance * r, 2 * r) - r
+        else:
+            mu = math.fmod(mu - r - distance * r, 2 * r) + r
+    return w * math.cosh(mu) * math.cos(nu), w * math.sinh(mu) * math.sin(nu)
+
+
+def inv_squircular_ref(x, y, w):
+    if w == 0:
+        return 0.0, 0.0
+    ri = x * x + y * y
+    r2 = math.sqrt(max(ri * (w * w * ri - 4 * x * x * y * y) / w, 0.0))
+    r = math.sqrt(max(ri - r2, 0.0)) / 1.41421356237
+    return r / (x if x != 0 else 1e-30), r / (y if y != 0 else 1e-30)
+
+
+def hypershift_ref(x, y, w, shift, stretch):
+    rad1 = 1 / max(x * x + y * y, 1e-30)
+    xx, yy = rad1 * x + shift, rad1 * y
+    rad = w * (1 - shift * shift) / max(xx * xx + yy * yy, 1e-30)
+    return rad * xx + shift, rad * yy * stretch
+
+
+def hex_modulus_ref(x, y, w, size):
+    s32, s3 = 0.8660254037844386, 1.7320508075688772
+    hs = s32 / max(size, 1e-30)
+    X, Y = x * hs, y * hs
+    cx = 0.5773502691896258 * X - Y / 3
+    cz = 2 * Y / 3
+    cy = -cx - cz
+    rx, ry, rz = round(cx), round(cy), round(cz)
+    dx, dy, dz = abs(rx - cx), abs(ry - cy), abs(rz - cz)
+    if dx > dy and dx > dz:
+        rx = -ry - rz
+    elif dy > dz:
+        ry = -rx - rz
+    else:
+        rz = -rx - ry
+    fx, fy = s3 * rx + s32 * rz, 1.5 * rz
+    return (X - fx) * w / s32, (Y - fy) * w / s32
+
+
+def b_mod_ref(x, y, w, radius, distance):
+    xp1, xm1, y2 = x + 1, x - 1, y * y
+    tau = 0.5 * (math.log(max(xp1 * xp1 + y2, 1e-30)) - math.log(max(xm1 * xm1 + y2, 1e-30)))
+    sig = PI - math.atan2(y, xp1) - math.atan2(y, 1 - x)
+    if tau < radius and -tau < radius:
+        r = max(radius, 1e-30)
+        arg = tau + r + distance * r
+        tau = arg - math.floor(arg / (2 * r)) * 2 * r - r
+    t = math.cosh(tau) - math.cos(sig)
+    t = 1e-30 if abs(t) < 1e-30 else t
+    return w * math.sinh(tau) / t, w * math.sin(sig) / t
+
+
+def waves3_ref(x, y, w, sx, sy, fx, fy, sxf, syf):
+    ssx = 0.5 * sx * (1 + math.sin(y * sxf))
+    ssy = 0.5 * sy * (1 + math.sin(x * syf))
+    return w * (x + math.sin(y * fx) * ssx), w * (y + math.sin(x * fy) * ssy)
+
+
+def scry2_ref(x, y, w, sides_f, star, circle):
+    sides = min(max(int(sides_f), 1), 50)
+    a = 2 * PI / sides
+    sina, cosa = math.sin(a), math.cos(a)
+    sins, coss = math.sin(-star * PI * 0.5), math.cos(star * PI * 0.5)
+    sinc, cosc = math.sin(circle * PI * 0.5), math.cos(circle * PI * 0.5)
+    xr, yr = x, y
+    r2 = xr * coss + abs(yr) * sins
+    cr = math.hypot(xr, yr)
+    iters = 0
+    for i in range(sides - 1):
+        xr, yr = xr * cosa - yr * sina, xr * sina + yr * cosa
+        r2 = max(r2, xr * coss + abs(yr) * sins)
+        iters = i + 1
+    r2 = r2 * cosc + cr * sinc
+    r1 = r2
+    r2 = r2 * r2 if iters > 1 else abs(r2) * r2
+    if w == 0:
+        return 0.0, 0.0
+    d = r1 * (r2 + 1 / w)
+    return (0.0, 0.0) if d == 0 else (x / d, y / d)
+
+
+def ennepers2_ref(x, y, w, ap, bp, cp):
+    r2 = 1 / max(x * x + y * y, 1e-30)
+    dxy = (ap * x) ** 2 - (bp * y) ** 2
+    return (w * x * (ap * ap - dxy * r2 - cp * math.sqrt(abs(x))),
+            w * y * (bp * bp - dxy * r2 - cp * math.sqrt(abs(y))))
+
+
+def hole2_ref(x, y, w, a, b, c, d, inside_p, shape_p):
+    rho = x * x + y * y
+    th = math.atan2(y, x) * d
+    de = max(th / PI + 1, 1e-30) ** a * c
+    s = int(shape_p)
+    f = {0: lambda: math.sqrt(max(rho, 0.0)) + de,
+         1: lambda: math.sqrt(max(rho + de, 0.0)),
+         2: lambda: math.sqrt(max(rho + math.sin(b * th) + de, 0.0)),
+         3: lambda: math.sqrt(max(rho + math.sin(th) + de, 0.0)),
+         4: lambda: math.sqrt(max(rho + th * th - de + 1, 0.0)),
+         5: lambda: math.sqrt(max(rho + abs(math.tan(th)) + de, 0.0)),
+         6: lambda: math.sqrt(max(rho * (1 + math.sin(b * th)) + de, 0.0)),
+         7: lambda: math.sqrt(max(rho + abs(math.sin(0.5 * b * th)) + de, 0.0)),
+         8: lambda: math.sqrt(max(rho + math.sin(PI * math.sin(b * th)) + de, 0.0)),
+         9: lambda: math.sqrt(max(rho + (math.sin(b * th) + math.sin(2 * b * th + PI * 0.5)) * 0.5
+                                   + de, 0.0))}
+    r1 = f.get(s, lambda: 1.0)()
+    rf = w / max(r1, 1e-30) if int(inside_p) != 0 else w * r1
+    return rf * math.cos(th), rf * math.sin(th)
+
+
+JWF = [   # (entry, oracle, the Lall row's parameters, more parameters)
+    ("ennepers", ennepers_ref, (), None),
+    ("circus", circus_ref, (0.92,), (1.7,)),
+    ("clifford_js", clifford_js_ref, (-1.4, 1.6, 1.0, 0.7), (2.1, -0.3, -0.6, 1.2)),
+    ("devil_warp", devil_warp_ref, (1.0, 1.0, 1.5, 0.07, 0.005, 0.005), (0.7, 1.3, 0.9, 0.5, -0.4, 0.6)),
+    ("cardioid", cardioid_ref, (1.0,), (2.7,)),
+    ("bcollide", bcollide_ref, (1.0, 1.0), (3.0, 0.35)),
+    ("bsplit", bsplit_ref, (0.0, 0.0), (0.3, -0.2)),
+    ("bulge", bulge_ref, (1.0,), (1.7,)),
+    ("corners", corners_ref, (0.0, 0.0, 1.0, 1.0, 1.0, 1.0, 0.0, 0.0, 2.71828),
+     (0.1, -0.2, 0.8, 1.1, 0.75, 1.25, 0.1, 1.0, 2.71828)),
+    ("fibonacci2", fibonacci2_ref, (1.0, 1.0), (0.6, 0.4)),
+    ("hypertile", hypertile_ref, (3.0, 7.0, 0.0), (4.0, 6.0, 2.0)),
+    ("idisc", idisc_ref, (), None),
+    ("hole", hole_ref, (1.0, 0.0), (0.6, 1.0)),
+    ("kaleidoscope", kaleidoscope_ref, (1.0, 1.0, 1.0, 0.0, 0.0), (0.3, 0.8, 0.2, 0.1, -0.2)),
+    ("layered_spiral", layered_spiral_ref, (1.0,), (2.5,)),
+    ("linear_t", linear_t_ref, (1.0, 1.0), (1.4, 0.6)),
+    ("ovoid", ovoid_ref, (1.0, 1.0), (0.7, 1.3)),
+    ("unpolar", unpolar_ref, (), None),
+    ("target", target_ref, (1.0, 2.0, 0.0), (0.3, -0.5, 0.8)),
+    ("funnel", funnel_ref, (4.0,), (0.6,)),
+    ("holesq", holesq_ref, (), None),
+    ("fourth", fourth_ref, (1.0, 1.0, 1.0, 1.0, 1.0), (0.4, 0.3, 0.7, 0.2, -0.1)),
+    ("pulse", pulse_ref, (1.0, 1.0, 0.1, 0.1), (2.0, 3.0, 0.3, -0.2)),
+    ("rays1", rays1_ref, (), None),
+    ("rays3", rays3_ref, (), None),
+    ("tancos", tancos_ref, (), None),
+    ("twoface", twoface_ref, (), None),
+    ("e_collide", e_collide_ref, (1.0, 0.0), (3.0, 0.4)),
+    ("e_mod", e_mod_ref, (1.0, 0.0), (0.8, 0.3)),
+    ("inv_squircular", inv_squircular_ref, (), None),
+    ("hypershift", hypershift_ref, (0.5, 1.0), (-0.3, 1.4)),
+    ("hex_modulus", hex_modulus_ref, (0.5,), (0.8,)),
+    ("b_mod", b_mod_ref, (1.0, 0.0), (0.6, 0.4)),
+    ("waves3", waves3_ref, (0.05, 0.05, 7.0, 13.0, 0.0, 2.0), (0.2, 0.1, 3.0, 5.0, 1.0, 0.5)),
+    ("scry2", scry2_ref, (4.0, 0.15, 0.25), (6.0, 0.4, 0.6)),
+    ("ennepers2", ennepers2_ref, (1.0, 0.3333, 0.075), (0.8, 0.5, 0.2)),
+    ("hole2", hole2_ref, (1.0, 0.0, 0.0, 0.0, 0.5, 0.5), None),
+]
+
+
+@pytest.mark.parametrize("case", JWF, ids=lambda c: c[0])
+def test_jwildfire_ports(orc, case):
+    name, ref, p0, p1 = case
+    # the reference's smoke row: finite at its seven points (any draws)
+    check_finite(orc, name, 1.0, p0, LALL_PTS)
+    pts = LALL_PTS + P8 + [(0.3, -0.8), (-0.2, -0.45), (1.7, 0.9)]
+    for prm in [p0] + ([p1] if p1 is not None else []):
+        for w in (1.0, 0.7):
+            check_pin(orc, name, ref, w, prm, pts, 3, f64_rel=1e-6, f32=False)
+
+
+def test_hole2_shapes(orc):
+    # every shape of hole2 (src/shaders/chaos.wgsl, var_hole2's switch)
+    pts = LALL_PTS + P8
+    for shape in range(11):
+        for inside in (0.0, 1.0):
+            check_pin(orc, "hole2", hole2_ref, 1.0, (0.8, 1.3, 0.6, 1.2, inside, float(shape)),
+                      pts, 3, f64_rel=1e-6, f32=False)
+
+
 # -- end of pins --
