@@ -106,6 +106,35 @@ def test_draw_contract_xform_blend(orc):
     assert set(np.unique(d)) == {4, 6}
 
 
+def test_pre_blur_brute_force(orc):
+    """pre_blur (chaos.wgsl:8164-8181, flam3's pre_blur): five draws per turn
+    after the pick, r0..r3 giving the radius weight * (r0 + r1 + r2 + r3 - 2)
+    and r4 the angle 2 pi r4, added to the affine image before the variation
+    sum; with linear at weight 0.8 beside it the sum is 0.8 * (pa + offset).
+    Replayed from each walker's own ISAAC words, the affine image from the
+    previous turn's position."""
+    aff = (0.6, -0.2, 0.1, 0.3, 0.7, -0.2)
+    g = genome([xform(1.0, aff, vars=((97, 0.3), (0, 0.8)))], scale=8.0)
+    walkers, iters = 6, 20
+    xf, ds, u, pal = blobs(g, 16, 16, iters=iters, fuse=0, seed=5, walkers=walkers)
+    _, tr = orc.chaos(xf, ds, u, pal, trace_shape=(walkers, iters))
+    st = orc.seed_walkers(5, walkers)
+    for w in range(walkers):
+        assert np.all(_draws_per_turn(tr[w]) == 8)   # pick + 5 pre_blur + 2 jitter
+        words, _ = orc.isaac_draw(st[w], 3 + 8 * iters)
+        r01 = orc.rand_transforms(words)[0].astype(np.float64)
+        x, y = (float(v) for v in orc.rand_transforms(words[:2])[1])
+        for i in range(iters):
+            pa = (aff[0] * x + aff[1] * y + aff[2], aff[3] * x + aff[4] * y + aff[5])
+            assert np.allclose(tr[w, i, 2:4], pa, rtol=1e-5, atol=1e-6), (w, i)
+            r = r01[3 + 8 * i + 1:3 + 8 * i + 6]
+            rad = 0.3 * (r[0] + r[1] + r[2] + r[3] - 2.0)
+            ang = r[4] * 2 * np.pi
+            want = (0.8 * (pa[0] + np.cos(ang) * rad), 0.8 * (pa[1] + np.sin(ang) * rad))
+            assert np.allclose(tr[w, i, 4:6], want, rtol=1e-5, atol=1e-6), (w, i)
+            x, y = float(tr[w, i, 12]), float(tr[w, i, 13])
+
+
 def test_retry_window_all_bad(orc):
     """Every turn is bad: turns 1..4 retry, the 5th is accepted."""
     g = genome([xform(1.0, (1, 0, 0, 0, 1, 0), vars=((0, 1e12),))])

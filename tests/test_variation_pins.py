@@ -2820,4 +2820,759 @@ def test_hole2_shapes(orc):
                       pts, 3, f64_rel=1e-6, f32=False)
 
 
+# ------------------------- JWildfire ports with draws, V153 - V219 (WGSL restated) --
+class Draws:
+    """The draws a variation takes, in order: rand01 of each word as the
+    kernels form it (low 28 bits times 1/(2^28 - 1), in f32)."""
+
+    def __init__(self, words):
+        self.words, self.k = list(words), 0
+
+    def r01(self):
+        w = self.words[self.k] if self.k < len(self.words) else 0
+        self.k += 1
+        return float(np.float32(np.float32(w & 0x0FFFFFFF) * np.float32(1.0 / 268435455.0)))
+
+
+def erf_ref(x, y, w, D):   # A&S 7.1.26, as src/shaders/chaos.wgsl's erf_approx has it
+    def e(v):
+        t = 1 / (1 + 0.3275911 * abs(v))
+        poly = ((((1.061405429 * t - 1.453152027) * t + 1.421413741) * t - 0.284496736) * t
+                + 0.254829592) * t
+        return math.copysign(1.0, v) * (1 - poly * math.exp(-v * v)) if v != 0 else 1 - poly
+    return w * e(x), w * e(y)
+
+
+def asteria_ref(x, y, w, alpha, D):
+    sa, ca = math.sin(PI * alpha), math.cos(PI * alpha)
+    x0, y0 = w * x, w * y
+    r = x0 * x0 + y0 * y0
+    r2 = math.sqrt((abs(y0) - 1) ** 2 + (abs(x0) - 1) ** 2)
+    inside = (D.r01() > 0.35) if (r < 1 and r2 < 1) else not (r < 1)
+    if inside:
+        return x0, y0
+    rx, ry = x0 * ca - y0 * sa, x0 * sa + y0 * ca
+    nx = rx / math.sqrt(max(1 - ry * ry, 1e-30)) * (1 - math.sqrt(max(1 - (1 - abs(ry)) ** 2, 0.0)))
+    return nx * ca + ry * sa, -nx * sa + ry * ca
+
+
+def chrysanthemum_ref(x, y, w, D):
+    u = 21 * PI * D.r01()
+    p4, p8 = math.sin(17 * u / 3), math.sin(2 * math.cos(3 * u) - 28 * u)
+    r = w * 0.1 * (5 * (1 + math.sin(11 * u / 5)) - 4 * p4 ** 4 * p8 ** 8)
+    return r * math.cos(u), r * math.sin(u)
+
+
+def checks_ref(x, y, w, cx, cy, size, rnd, D):
+    cs = 1 / (size + 1e-6)
+    is_xy = int(round(x * cs)) + int(round(y * cs))
+    rnx, rny = rnd * D.r01(), rnd * D.r01()
+    dx, dy = (-cx + rnx, -cy) if is_xy & 1 == 0 else (cx, cy + rny)
+    return w * (x + dx), w * (y + dy)
+
+
+def circleblur_ref(x, y, w, D):
+    rad = math.sqrt(D.r01())
+    a = D.r01() * 2 * PI
+    return w * math.cos(a) * rad, w * math.sin(a) * rad
+
+
+def hypertile1_ref(x, y, w, p_p, q, D):
+    pa = 2 * PI / max(3.0, p_p)
+    den = math.cos(pa) + math.cos(2 * PI / max(3.0, q))
+    r2 = 1 - (math.cos(pa) - 1) / den if abs(den) > 1e-30 else 1.0
+    r = 1 / math.sqrt(max(r2, 1e-30)) if r2 > 0 else 1.0
+    rpa = math.floor(D.r01() * 10) * pa
+    re, im = r * math.cos(rpa), r * math.sin(rpa)
+    a, b = x + re, y - im
+    c, d = re * x - im * y + 1, re * y + im * x
+    vr = w / max(c * c + d * d, 1e-30)
+    return vr * (a * c + b * d), vr * (b * c - a * d)
+
+
+def line_ref(x, y, w, delta, phi, D):
+    ux, uy = math.cos(delta * PI) * math.cos(phi * PI), math.sin(delta * PI) * math.cos(phi * PI)
+    r = math.sqrt(max(ux * ux + uy * uy + math.sin(phi * PI) ** 2, 1e-30))
+    rnd = D.r01() * w
+    return ux / r * rnd, uy / r * rnd
+
+
+def vogel_ref(x, y, w, n, scale, D):
+    phi = 1.61803398874989
+    i = math.floor(D.r01() * max(1.0, n)) + 1
+    a = i * 2 * PI / (phi * phi)
+    r = w * (math.hypot(x, y) + math.sqrt(i))
+    return r * (math.cos(a) + scale * x), r * (math.sin(a) + scale * y)
+
+
+def yin_yang_ref(x, y, w, radius, ang1, ang2, dual_t, outside, D):
+    sa, ca = math.sin(PI * ang1), math.cos(PI * ang1)
+    sb, cb = math.sin(PI * ang2), math.cos(PI * ang2)
+    inv, RR = 1.0, radius
+    if x * x + y * y < 1:
+        nx, ny = x * ca - y * sa, x * sa + y * ca
+        if int(dual_t) == 1 and D.r01() > 0.5:
+            inv, RR = -1.0, 1 - radius
+            nx, ny = x * cb - y * sb, x * sb + y * cb
+        if ny > 0:
+            t = math.sqrt(max(1 - ny * ny, 0.0))
+            k = nx / max(abs(t), 1e-30) * (1.0 if t > 0 else 0.0)
+            t1 = (t - 0.5) * 2
+            alfa = (1 - k) * 0.5
+            k1 = alfa * RR + (1 - alfa)
+            return w * (t1 * k1 + alfa * (RR - 1)) * inv, w * math.sqrt(max(1 - t1 * t1, 0.0)) * k1 * inv
+        return w * (nx * (1 - RR) + RR) * inv, w * ny * (1 - RR) * inv
+    return (w * x, w * y) if int(outside) == 1 else (0.0, 0.0)
+
+
+def squish_ref(x, y, w, power_p, D):
+    pw = max(2.0, power_p)
+    ax, ay = abs(x), abs(y)
+    if ax > ay:
+        s, pp = ax, (y if x > 0 else 4 * ax - y)
+    else:
+        s, pp = ay, (2 * ay - x if y > 0 else 6 * ay + x)
+    pp = (pp + 8 * s * math.floor(pw * D.r01())) / pw
+    if pp <= s:
+        return w * s, w * pp
+    if pp <= 3 * s:
+        return w * (2 * s - pp), w * s
+    if pp <= 5 * s:
+        return -w * s, w * (4 * s - pp)
+    if pp <= 7 * s:
+        return -w * (6 * s - pp), -w * s
+    return w * s, -w * (8 * s - pp)
+
+
+def lace_js_ref(x, y, w, D):
+    r0, wt, s32 = math.hypot(x, y), D.r01(), 0.8660254037844386
+    if wt > 0.75:
+        th = math.atan2(y, x - 1)
+        return w * (-r0 * math.sin(th) / 2), w * (-r0 * math.cos(th) / 2 + 1)
+    if wt > 0.5:
+        th = math.atan2(y - s32, x + 0.5)
+        return w * (-r0 * math.sin(th) / 2 + s32), w * (-r0 * math.cos(th) / 2 - 0.5)
+    if wt > 0.25:
+        th = math.atan2(y + s32, x + 0.5)
+        return w * (-r0 * math.sin(th) / 2 - s32), w * (-r0 * math.cos(th) / 2 - 0.5)
+    th = math.atan2(y, x)
+    return w * (-r0 * math.sin(th) / 2), w * (-r0 * math.cos(th) / 2)
+
+
+def _csqrt(z):   # WGSL complex_sqrt: the sign of the imaginary part from z.y < 0
+    rad = abs(z)
+    im = math.sqrt(max(0.5 * (rad - z.real), 0.0))
+    return complex(math.sqrt(max(0.5 * (rad + z.real), 0.0)), -im if z.imag < 0 else im)
+
+
+def julia_outside_ref(x, y, w, re_div, im_div, mode_p, D):
+    m = int(mode_p)
+    z, z2 = complex(x, y), complex(x, y)
+    if m in (0, 2):
+        z = _csqrt(z)
+    z += 1
+    if m in (0, 2):
+        z, z2 = z * z, _csqrt(z2)
+    z2 -= 1
+    if m in (0, 2):
+        z2 = z2 * z2
+    z = z / z2
+    if m in (0, 1):
+        z = _csqrt(z)
+    z = z / complex(re_div, im_div)
+    if m in (0, 1):
+        z = z * (1 if D.r01() < 0.5 else -1)
+    return w * z.real, w * z.imag
+
+
+def rays2_ref(x, y, w, D):
+    t = x * x + y * y
+    ts = max(t, 1e-30)
+    ci = math.cos((ts + 1e-6) * math.tan(1 / ts + 1e-6))
+    u = 1 / (1e-30 if abs(ci) < 1e-30 else ci)
+    c = w / 10
+    return c * u * t / (x if x != 0 else 1e-30), c * u * t / (y if y != 0 else 1e-30)
+
+
+def e_julia_ref(x, y, w, power_p, D):
+    ps = 1.0 if power_p == 0 else power_p
+    r2 = x * x + y * y
+    xx = x if ps > 0 else x / max(r2, 1e-30)
+    if ps <= 0:
+        r2 = 1 / max(r2, 1e-30)
+    tmp, tmp2 = r2 + 1, 2 * xx
+    xmax = max((math.sqrt(max(tmp + tmp2, 0.0)) + math.sqrt(max(tmp - tmp2, 0.0))) * 0.5, 1.0)
+    mu = math.acosh(xmax)
+    nu = math.acos(min(max(xx / xmax, -1.0), 1.0))
+    if y < 0:
+        nu = -nu
+    nu = nu / ps + 2 * PI / ps * math.floor(D.r01() * abs(ps))
+    mu = mu / ps
+    return w * math.cosh(mu) * math.cos(nu), w * math.sinh(mu) * math.sin(nu)
+
+
+def cannabis_ref(x, y, w, filled, D):
+    a = math.atan2(y, x)
+    r = ((1 + 0.9 * math.cos(8 * a)) * (1 + 0.1 * math.cos(24 * a)) * (0.9 + 0.1 * math.cos(200 * a))
+         * (1 + math.sin(a)))
+    a += PI * 0.5
+    if filled > 0 and filled > D.r01():
+        r *= D.r01()
+    return w * math.sin(a) * r, w * math.cos(a) * r
+
+
+def intersection_ref(x, y, w, xw, xt, xm1, xm2, xh, yh, yt, ym1, ym2, yw, D):
+    if w == 0:
+        return 0.0, 0.0
+    if D.r01() < 0.5:
+        s = xw if D.r01() < 0.5 else -xw
+        ox = xt * (x + round(s * math.log(max(D.r01(), 1e-30))))
+        r1 = max(xm2 * xm1, 1e-30)
+        if y > xm1:
+            oy = xh * (-xm1 + math.fmod(y + xm1, r1))
+        elif y < -xm1:
+            oy = xh * (xm1 - math.fmod(xm1 - y, r1))
+        else:
+            oy = xh * y
+        return ox, oy
+    s = yh if D.r01() < 0.5 else -yh
+    oy = yt * (y + round(s * math.log(max(D.r01(), 1e-30))))
+    r1 = max(ym2 * ym1, 1e-30)
+    if x > ym1:
+        ox = yw * (-ym1 + math.fmod(x + ym1, r1))
+    elif x < -ym1:
+        ox = yw * (ym1 - math.fmod(ym1 - x, r1))
+    else:
+        ox = yw * x
+    return ox, oy
+
+
+def boarders2_ref(x, y, w, c, left, right, D):
+    ac, al, ar = max(abs(c), 1e-6), max(abs(left), 1e-6), max(abs(right), 1e-6)
+    cl, cr = ac * al, ac + ac * ar
+    rx, ry = round(x), round(y)
+    ox, oy = x - rx, y - ry
+    if D.r01() >= cr:
+        return w * (ox * ac + rx), w * (oy * ac + ry)
+    nz = lambda v: 1e-30 if v == 0 else v
+    if abs(ox) >= abs(oy):
+        sg = 1 if ox >= 0 else -1
+        return w * (ox * ac + rx + sg * cl), w * (oy * ac + ry + sg * cl * oy / nz(ox))
+    sg = 1 if oy >= 0 else -1
+    return w * (ox * ac + rx + sg * ox / nz(oy) * cl), w * (oy * ac + ry + sg * cl)
+
+
+def b_transform_ref(x, y, w, rotate, power_p, move_v, split, D):
+    pw = max(1.0, power_p)
+    xp1, xm1, y2 = x + 1, x - 1, y * y
+    tau = 0.5 * (math.log(max(xp1 * xp1 + y2, 1e-30)) - math.log(max(xm1 * xm1 + y2, 1e-30))) / pw + move_v
+    sig = PI - math.atan2(y, xp1) - math.atan2(y, 1 - x) + rotate
+    sig = sig / pw + 2 * PI / pw * math.floor(D.r01() * pw)
+    tau += split if x >= 0 else -split
+    t = math.cosh(tau) - math.cos(sig)
+    t = 1e-30 if abs(t) < 1e-30 else t
+    return w * math.sinh(tau) / t, w * math.sin(sig) / t
+
+
+def apollony_ref(x, y, w, D):
+    s3 = 1.7320508075688772
+    dx = 1 + s3 - x
+    den = max(dx * dx + y * y, 1e-30)
+    a0, b0 = 3 * dx / den - (1 + s3) / (2 + s3), 3 * y / den
+    ab = max(a0 * a0 + b0 * b0, 1e-30)
+    f1x, f1y = a0 / ab, -b0 / ab
+    br = int(math.floor(4 * D.r01())) % 3
+    if br == 0:
+        return w * a0, w * b0
+    if br == 1:
+        return w * (-f1x * 0.5 - f1y * s3 * 0.5), w * (f1x * s3 * 0.5 - f1y * 0.5)
+    return w * (-f1x * 0.5 + f1y * s3 * 0.5), w * (-f1x * s3 * 0.5 - f1y * 0.5)
+
+
+def circlecrop_ref(x, y, w, radius, cx, cy, scatter, zero_p, D):
+    ca = min(max(scatter, -1.0), 1.0)
+    dx, dy = x - cx, y - cy
+    rad, ang = math.hypot(dx, dy), math.atan2(dy, dx)
+    rdc = radius + D.r01() * 0.5 * ca
+    esc = rad > radius
+    if zero_p > 0.5 and esc:
+        return 0.0, 0.0
+    if esc:
+        return w * rdc * math.cos(ang) + cx, w * rdc * math.sin(ang) + cy
+    return w * dx + cx, w * dy + cy
+
+
+def henon_ref(x, y, w, a, b, c, D):
+    return w * (c - a * x * x + y), w * b * x
+
+
+def lozi_ref(x, y, w, a, b, c, D):
+    return w * (c - a * abs(x) + y), w * b * x
+
+
+def _i32(v):
+    v &= 0xFFFFFFFF
+    return v - (1 << 32) if v >= 1 << 31 else v
+
+
+def _voron_noise(x):   # JWildfire's DiscretNoise in wrapping 32-bit integers
+    s = _i32((x << 13) ^ x)
+    r = _i32(s * _i32(_i32(s * s) * 15731 + 789221) + 1376312589) & 0x7FFFFFFF
+    return r / 2147483647.0
+
+
+def voron_ref(x, y, w, k, step, num_f, xseed_f, yseed_f, D):
+    xs, ys = int(xseed_f), int(yseed_f)
+    st = 1e-30 if step == 0 else step
+    M, N = math.floor(x / st), math.floor(y / st)
+    best = (20.0, 0.0, 0.0)
+    for i in (-1, 0, 1):
+        for j in (-1, 0, 1):
+            m1, n1 = M + i, N + j
+            cnt = min(max(1 + math.floor(num_f * _voron_noise(19 * m1 + 257 * n1 + xs)), 1), 26)
+            for l in range(cnt):
+                cx = (_voron_noise(l + 64 * m1 + 15 * n1 + xs) + m1) * step
+                cy = (_voron_noise(l + 21 * m1 + 33 * n1 + ys) + n1) * step
+                r = math.hypot(x - cx, y - cy)
+                if r < best[0]:
+                    best = (r, cx, cy)
+    _, x0, y0 = best
+    return w * (k * (x - x0) + x0), w * (k * (y - y0) + y0)
+
+
+JWF_DRAWS = [   # (entry, oracle, the reference's smoke-row parameters, more parameters)
+    ("voron", voron_ref, (0.99, 0.25, 1.0, 3.0, 7.0), (0.5, 0.4, 3.0, -11.0, 5.0)),
+    ("erf", erf_ref, (), None),
+    ("asteria", asteria_ref, (0.5,), (0.2,)),
+    ("henon", henon_ref, (1.4, 0.3, 0.6), (1.1, -0.4, 1.0)),
+    ("chrysanthemum", chrysanthemum_ref, (), None),
+    ("checks", checks_ref, (0.3, 0.3, 0.0, 0.0), (0.2, -0.1, 0.5, 0.7)),
+    ("circleblur", circleblur_ref, (), None),
+    ("hypertile1", hypertile1_ref, (3.0, 7.0), (5.0, 4.0)),
+    ("line", line_ref, (0.5, 0.5), (0.2, -0.3)),
+    ("vogel", vogel_ref, (8.0, 0.3), (13.0, -0.2)),
+    ("yin_yang", yin_yang_ref, (1.0, 1.0, 0.5, 0.5, 0.5), (0.4, 0.3, -0.6, 1.0, 1.0)),
+    ("squish", squish_ref, (2.0,), (5.0,)),
+    ("lace_js", lace_js_ref, (), None),
+    ("julia_outside", julia_outside_ref, (2.0, 0.0, 0.0), (1.5, 0.5, 1.0)),
+    ("rays2", rays2_ref, (), None),
+    ("e_julia", e_julia_ref, (2.0,), (-3.0,)),
+    ("cannabis_curve_wf", cannabis_ref, (1.0,), (0.4,)),
+    ("intersection", intersection_ref, (1.0, 0.0, 0.5, 1.0, 0.5, 1.0, 0.5, 0.5, 0.0, 0.5),
+     (0.7, 0.4, 0.6, 2.0, 0.9, 0.3, 0.8, 0.4, 1.5, 0.6)),
+    ("lozi", lozi_ref, (1.7, 0.5, 0.5), (1.2, -0.3, 1.0)),
+    ("boarders2", boarders2_ref, (0.4, 0.65, 0.35), (0.6, 0.3, 0.5)),
+    ("b_transform", b_transform_ref, (1.0, 0.0, 0.0, 0.0), (0.3, 3.0, 0.2, 0.1)),
+    ("apollony", apollony_ref, (), None),
+    ("circlecrop", circlecrop_ref, (1.0, 0.0, 0.0, 0.0, 1.0), (0.8, 0.1, -0.2, 0.6, 0.0)),
+]
+
+
+# flam3's draw-taking variations (flam3 variations.c, as src/shaders/chaos.wgsl:
+# 1036-1170 and 1540-1620 state them). The reference's fixture rows for these
+# only check that the output stays finite, since flam3's own RNG stream is not
+# reproduced; with the draws scripted, the values are fixed.
+def noise_ref(x, y, w, D):
+    a = D.r01() * 2 * PI
+    r = w * D.r01()
+    return x * r * math.cos(a), y * r * math.sin(a)
+
+
+def blur_ref(x, y, w, D):
+    a = D.r01() * 2 * PI
+    r = w * D.r01()
+    return r * math.cos(a), r * math.sin(a)
+
+
+def gaussian_blur_ref(x, y, w, D):
+    a = D.r01() * 2 * PI
+    r = w * (sum(D.r01() for _ in range(4)) - 2)
+    return r * math.cos(a), r * math.sin(a)
+
+
+def arch_ref(x, y, w, D):
+    a = D.r01() * w * PI
+    return w * math.sin(a), w * math.sin(a) ** 2 / math.cos(a)
+
+
+def radial_blur_ref(x, y, w, angle, D):
+    spin, zoom = math.sin(angle * PI / 2), math.cos(angle * PI / 2)
+    g = w * (sum(D.r01() for _ in range(4)) - 2)
+    ra, a = math.hypot(x, y), math.atan2(y, x) + spin * g
+    rz = zoom * g - 1
+    return ra * math.cos(a) + rz * x, ra * math.sin(a) + rz * y
+
+
+def square_ref(x, y, w, D):
+    r0 = D.r01()
+    return w * (r0 - 0.5), w * (D.r01() - 0.5)
+
+
+def rays_ref(x, y, w, D):
+    a = w * D.r01() * PI
+    t = w * math.tan(a) * w / (x * x + y * y + 1e-10)
+    return t * math.cos(x), t * math.sin(y)
+
+
+def blade_ref(x, y, w, D):
+    r = D.r01() * w * math.hypot(x, y)
+    return w * x * (math.cos(r) + math.sin(r)), w * x * (math.cos(r) - math.sin(r))
+
+
+def twintrian_ref(x, y, w, D):
+    r = D.r01() * w * math.hypot(x, y)
+    s = math.sin(r)
+    diff = math.log10(s * s) + math.cos(r) if s != 0 else -30.0
+    if not -1e10 <= diff <= 1e10:
+        diff = -30.0
+    return w * x * diff, w * x * (diff - s * PI)
+
+
+def super_shape_ref(x, y, w, rnd, m, n1, n2, n3, holes, D):
+    r0 = math.hypot(x, y)
+    th = m / 4 * math.atan2(y, x) + PI / 4
+    t = abs(math.cos(th)) ** n2 + abs(math.sin(th)) ** n3
+    r = w * ((rnd * D.r01() + (1 - rnd) * r0) - holes) * t ** (-1 / n1) / r0
+    return r * x, r * y
+
+
+def flower_ref(x, y, w, petals, holes, D):
+    r0 = math.hypot(x, y)
+    r = w * (D.r01() - holes) * math.cos(petals * math.atan2(y, x)) / r0
+    return r * x, r * y
+
+
+def conic_ref(x, y, w, ecc, holes, D):
+    r0 = math.hypot(x, y)
+    r = w * (D.r01() - holes) * ecc / (1 + ecc * x / r0) / r0
+    return r * x, r * y
+
+
+def parabola_ref(x, y, w, height, width, D):
+    r = math.hypot(x, y)
+    r0 = D.r01()
+    return height * w * math.sin(r) ** 2 * r0, width * w * math.cos(r) * D.r01()
+
+
+def pie_ref(x, y, w, slices, rotation, thickness, D):
+    sl = math.trunc(D.r01() * slices + 0.5)
+    a = rotation + 2 * PI * (sl + D.r01() * thickness) / slices
+    r = w * D.r01()
+    return r * math.cos(a), r * math.sin(a)
+
+
+def boarders_ref(x, y, w, D):
+    r0 = D.r01()
+    rx, ry = math.floor(x + 0.5), math.floor(y + 0.5)
+    ox, oy = x - rx, y - ry
+    if r0 >= 0.75:
+        return w * (ox * 0.5 + rx), w * (oy * 0.5 + ry)
+    if abs(ox) >= abs(oy):
+        s = 0.25 if ox >= 0 else -0.25
+        return w * (ox * 0.5 + rx + s), w * (oy * 0.5 + ry + s * oy / ox)
+    s = 0.25 if oy >= 0 else -0.25
+    return w * (ox * 0.5 + rx + s * ox / oy), w * (oy * 0.5 + ry + s)
+
+
+# The spatial-hash entries (src/shaders/chaos.wgsl:4046-4077, 4200-4222,
+# 4331-4352, 5219-5233): the hash multiplies a sine by 43758.5453, so an f32
+# result moves with the last bit of its input; the f64 instantiation, pinned
+# here, does not.
+def _frac_trunc(v):
+    return v - math.trunc(v)
+
+
+def circular_ref(x, y, w, angle_deg, seed, D, kx=12.9898, ky=78.233):
+    ca = angle_deg * PI / 180
+    aux = _frac_trunc(math.sin(x * kx + y * ky + seed) * 43758.5453)
+    rnd = (2 * (D.r01() + aux) - 2) * ca
+    rad, ang = math.hypot(x, y), math.atan2(y, x)
+    return w * math.cos(ang + rnd) * rad, w * math.sin(ang + rnd) * rad
+
+
+def circular2_ref(x, y, w, angle_deg, seed, xx, yy, D):
+    return circular_ref(x, y, w, angle_deg, seed, D, kx=xx, ky=yy)
+
+
+def hypertile2_ref(x, y, w, p_p, q, D):
+    pa = 2 * PI / max(3.0, p_p)
+    den = math.cos(pa) + math.cos(2 * PI / max(3.0, q))
+    r2 = 1 - (math.cos(pa) - 1) / den if abs(den) > 1e-30 else 1.0
+    r = 1 / math.sqrt(max(r2, 1e-30)) if r2 > 0 else 1.0
+    a, b, c, d = x + r, y, r * x + 1, r * y
+    xx, yy = a * c + b * d, b * c - a * d
+    vr = w / max(c * c + d * d, 1e-30)
+    rpa = math.floor(D.r01() * 32767) * pa
+    return vr * (xx * math.cos(rpa) + yy * math.sin(rpa)), vr * (yy * math.cos(rpa) - xx * math.sin(rpa))
+
+
+def phoenix_julia_ref(x, y, w, power, dist, xd, yd, D):
+    pw = 1.0 if power == 0 else power
+    k = math.floor(D.r01() * 32768)
+    a = math.atan2(y * (yd + 1), x * (xd + 1)) * dist / pw + k * 2 * PI / pw
+    r = w * max(x * x + y * y, 1e-30) ** (dist / (2 * pw))
+    return r * math.cos(a), r * math.sin(a)
+
+
+def waves4_ref(x, y, w, sx, sy, fx, fy, cont, yfact, D):
+    cell = math.floor(y * fx / (2 * PI))
+    ax = _frac_trunc(math.sin(cell * 12.9898 + cell * 78.233 + 1 + y * 0.001 * yfact) * 43758.5453)
+    if cont > 0.5:
+        ax = 1.0 if ax > 0.5 else 0.0
+    return w * (x + math.sin(y * fx) * ax * ax * sx), w * (y + math.sin(x * fy) * sy)
+
+
+# Entries the reference's own tests check only for finite output
+# (src/post-flam3-batch2b-a.gpu.test.ts:186-317, issue243-bwraps-crackle.gpu.
+# test.ts:53-110, issue120-b5-glynnsim.gpu.test.ts:122-144), restated from
+# src/shaders/chaos.wgsl:1988-2140 (bwraps, crackle), 2149-2344 (juliaq,
+# glynnia, falloff, falloff2, falloff3) and 3453-3545 (glynnSim1-3); and the
+# position part of the direct-colour entries (:7849-7856).
+def bwraps_ref(x, y, w, cellsize, space, gain, inner, outer, D):
+    if abs(cellsize) < 1e-30:
+        return w * x, w * y
+    radius = 0.5 * cellsize / (1 + space * space)
+    g2 = gain * gain / cellsize + 1e-6
+    mb = g2 * radius
+    mb = 1.0 if mb > 2 else mb / (mb * mb / 4 + 1)
+    r2, rfac = radius * radius, radius / max(mb, 1e-30)
+    cx, cy = (math.floor(x / cellsize) + 0.5) * cellsize, (math.floor(y / cellsize) + 0.5) * cellsize
+    lx, ly = x - cx, y - cy
+    if lx * lx + ly * ly > r2:
+        return w * x, w * y
+    lx, ly = lx * g2, ly * g2
+    rd = rfac / ((lx * lx + ly * ly) / 4 + 1)
+    lx, ly = lx * rd, ly * rd
+    f = (lx * lx + ly * ly) / max(r2, 1e-30)
+    th = inner * (1 - f) + outer * f
+    return w * (cx + math.cos(th) * lx + math.sin(th) * ly), w * (cy - math.sin(th) * lx + math.cos(th) * ly)
+
+
+def crackle_ref(x, y, w, cellsize, power, distort, scale, D):
+    if abs(cellsize) < 1e-30:
+        return w * x, w * y
+    s = cellsize / 2
+    r = [D.r01() for _ in range(4)]
+    blurr = (r[0] + r[1]) / 2 + (r[2] - 0.5) / 4
+    ux, uy = blurr * math.sin(2 * PI * r[3]), blurr * math.cos(2 * PI * r[3])
+
+    def centre(cx, cy):   # a cell's centre moved by two noise samples
+        return ((cx + distort * perlin2d_ref(cx * 2.5, cy * 2.5)) * s,
+                (cy + distort * perlin2d_ref(cy * 2.5 + 30.2, cx * 2.5 - 12.1)) * s)
+
+    def grid(gx, gy):
+        return [centre(gx + i // 3 - 1, gy + i % 3 - 1) for i in range(9)]
+
+    gx, gy = math.floor(ux / s), math.floor(uy / s)
+    P = grid(gx, gy)
+    d2 = [(px - ux) ** 2 + (py - uy) ** 2 for px, py in P]
+    q = d2.index(min(d2))
+    P = grid(gx + q // 3 - 1, gy + q % 3 - 1)
+    (qx, qy), L = P[4], -1e30
+    for i in range(9):
+        if i != 4:
+            ex, ey = P[i][0] - qx, P[i][1] - qy
+            den = ex * ex + ey * ey
+            L = max(L, 1.0 if den < 1e-30 else 2 * ((ux - qx) * ex + (uy - qy) * ey) / den)
+    ls = L + 1e-30
+    R = abs(ls) ** power * scale / ls
+    return w * (qx + (ux - qx) * R), w * (qy + (uy - qy) * R)
+
+
+def juliaq_ref(x, y, w, power, divisor, D):
+    pw = 1.0 if abs(power) < 1e-30 else power
+    n = math.floor(D.r01() * max(1.0, abs(pw)))
+    a = math.atan2(y, x) * divisor / pw + n * 2 * PI / pw
+    r = w * (x * x + y * y) ** (0.5 * divisor / pw)
+    return r * math.cos(a), r * math.sin(a)
+
+
+def glynnia_ref(x, y, w, D):
+    v2 = w * math.sqrt(0.5)
+    r, coin = math.hypot(x, y), D.r01()
+    sg = 1.0 if r >= 1 else -1.0
+    if coin > 0.5:
+        if r + x <= 0:
+            return 0.0, 0.0
+        d = math.sqrt(r + x)
+        return sg * v2 * d, -(v2 / d) * y
+    d = r + x
+    rad = r * (y * y + d * d)
+    if rad <= 0:
+        return 0.0, 0.0
+    rr = w / math.sqrt(rad)
+    return sg * rr * d, rr * y
+
+
+def _falloff_dist(x, y, scatter, mindist, x0, y0):
+    return max((max(math.hypot(x - x0, y - y0), 0.0) - mindist) * 0.04 * scatter, 0.0)
+
+
+def falloff_ref(x, y, w, scatter, mindist, mx, my, x0, y0, D):
+    d = _falloff_dist(x, y, scatter, mindist, x0, y0)
+    r0 = D.r01()
+    return w * (x + mx * r0 * d), w * (y + my * D.r01() * d)
+
+
+def falloff2_ref(x, y, w, scatter, typ, mx, my, x0, y0, mindist, D):
+    d = _falloff_dist(x, y, scatter, mindist, x0, y0)
+    r0, r1, r2 = D.r01(), D.r01(), D.r01()
+    t = int(min(max(typ, 0.0), 2.0))
+    if t == 1:
+        phi = math.atan2(y, x) + my * d * r1
+        rr = math.hypot(x, y) + mx * r0 * d
+        return w * rr * math.cos(phi), w * rr * math.sin(phi)
+    if t == 2:
+        sc, phi, rad = math.cos(d * r1 * 2 * PI), d * r2 * PI, d * r0
+        return w * (x + mx * rad * sc * math.cos(phi)), w * (y + my * rad * sc * math.sin(phi))
+    return w * (x + mx * r0 * d), w * (y + my * r1 * d)
+
+
+def falloff3_ref(x, y, w, scatter, mx, my, x0, y0, mindist, inv, D):
+    rad0 = math.hypot(x - x0, y - y0)
+    base = max(1 - rad0, 0.0) if inv > 0.5 else max(rad0, 0.0)
+    d = max((base - mindist) * 0.04 * scatter, 0.0)
+    r0, r1, r2 = D.r01() - 0.5, D.r01() - 0.5, D.r01() - 0.5
+    sc, phi, rad = math.cos(d * r1 * 2 * PI), d * r2 * PI, d * r0
+    return w * (x + mx * rad * sc * math.cos(phi)), w * (y + my * rad * sc * math.sin(phi))
+
+
+def _glynn_inside(w, radius1, thickness, x1, y1, D):
+    ri = radius1 * (thickness + (1 - thickness) * D.r01())
+    phi = 2 * PI * D.r01()
+    return w * (ri * math.cos(phi) + x1), w * (ri * math.sin(phi) + y1)
+
+
+def glynnsim1_ref(x, y, w, radius, radius1, phi1, thickness, pw, contrast, D):
+    a = phi1 * PI / 180
+    x1, y1 = radius * math.cos(a), radius * math.sin(a)
+    r = math.hypot(x, y)
+    if r < radius:
+        return _glynn_inside(w, radius1, thickness, x1, y1, D)
+    alpha = radius / max(r, 1e-30)
+    if D.r01() > contrast * abs(alpha) ** abs(pw):
+        px, py = x, y
+    else:
+        px, py = alpha * alpha * x, alpha * alpha * y
+    if (px - x1) ** 2 + (py - y1) ** 2 < radius1 * radius1:
+        return _glynn_inside(w, radius1, thickness, x1, y1, D)
+    return w * px, w * py
+
+
+def glynnsim2_ref(x, y, w, radius, thickness, contrast, pw, phi1, phi2, D):
+    p10, p20 = phi1 * PI / 180, phi2 * PI / 180
+    rpt = radius + thickness
+    gamma = thickness * (2 * radius + thickness) / max(rpt, 1e-30)
+    r = math.hypot(x, y)
+    if r < radius:
+        ri = rpt - gamma * D.r01()
+        ph = p10 + (p20 - p10) * D.r01()
+        return w * ri * math.cos(ph), w * ri * math.sin(ph)
+    alpha = radius / max(r, 1e-30)
+    if D.r01() > contrast * abs(alpha) ** abs(pw):
+        return w * x, w * y
+    return w * alpha * alpha * x, w * alpha * alpha * y
+
+
+def glynnsim3_ref(x, y, w, radius, thickness, contrast, pw, D):
+    r1 = radius + thickness
+    r2 = radius * radius / max(r1, 1e-30)
+    gamma = r1 / max(r1 + r2, 1e-30)
+    r = math.hypot(x, y)
+    if r < r1:
+        phi = 2 * PI * D.r01()
+        ri = r1 if D.r01() < gamma else r2
+        return w * ri * math.cos(phi), w * ri * math.sin(phi)
+    alpha = radius / max(r, 1e-30)
+    if D.r01() > contrast * abs(alpha) ** abs(pw):
+        return w * x, w * y
+    return w * alpha * alpha * x, w * alpha * alpha * y
+
+
+def dc_zero_ref(x, y, w, *rest):   # colour-only entries add nothing to the position
+    return 0.0, 0.0
+
+
+def dc_cylinder_ref(x, y, w, D):
+    return w * math.sin(x), w * y
+
+
+SMOKE_DRAWS = [
+    ("bwraps", bwraps_ref, (1.0, 0.0, 1.0, -1.04, 0.71), (0.7, 0.3, 1.6, 0.5, -0.9)),
+    ("crackle", crackle_ref, (1.0, 0.2, 0.0, 1.0), (1.0, 0.2, 1.0, 1.0)),
+    ("juliaq", juliaq_ref, (3.0, 2.0), (-4.0, 1.5)),
+    ("glynnia", glynnia_ref, (), None),
+    ("falloff", falloff_ref, (1.0, 0.5, 1.0, 1.0, 0.0, 0.0), (3.0, 0.2, 0.7, -1.3, 0.4, -0.2)),
+    ("falloff2", falloff2_ref, (1.0, 0.0, 1.0, 1.0, 0.0, 0.0, 0.5), (2.0, 1.0, 0.8, 1.2, 0.3, 0.1, 0.2)),
+    ("falloff2", falloff2_ref, (1.0, 2.0, 1.0, 1.0, 0.0, 0.0, 0.5), (2.5, 2.7, 0.6, -0.9, -0.3, 0.2, 0.1)),
+    ("falloff3", falloff3_ref, (1.0, 1.0, 1.0, 0.0, 0.0, 0.5, 0.0), (2.0, 0.8, 1.1, 0.2, -0.1, 0.1, 1.0)),
+    ("glynnSim1", glynnsim1_ref, (1.0, 0.1, 110.0, 0.1, 1.5, 0.5), (0.45, 0.43, 110.0, 0.25, 1.5, 0.5)),
+    ("glynnSim2", glynnsim2_ref, (1.0, 0.1, 0.5, 1.5, 110.0, 150.0), (0.25, 0.1, 0.8, 1.5, 30.0, 200.0)),
+    ("glynnSim3", glynnsim3_ref, (1.0, 0.1, 0.5, 1.5), (0.5, 0.2, 0.9, 1.3)),
+    ("dc_linear", dc_zero_ref, (), None),
+    ("dc_perlin", dc_zero_ref, (1.0, 3.0, 0.0), None),
+    ("dc_gridout", dc_zero_ref, (4.0,), None),
+    ("dc_cylinder", dc_cylinder_ref, (), None),
+]
+
+
+HASH_DRAWS = [
+    ("circular", circular_ref, (90.0, 0.0), (35.0, 2.5)),
+    ("circular2", circular2_ref, (90.0, 0.0, 12.9898, 78.233), (40.0, 1.0, 7.3, -3.1)),
+    ("hypertile2", hypertile2_ref, (3.0, 7.0), (5.0, 4.0)),
+    ("phoenix_julia", phoenix_julia_ref, (2.0, 1.0, 0.0, 0.0), (-3.0, 0.7, 0.4, -0.2)),
+    ("waves4", waves4_ref, (0.05, 0.05, 7.0, 13.0, 0.0, 1.0), (0.3, 0.2, 3.0, 5.0, 1.0, 2.0)),
+]
+
+
+FLAM3_DRAWS = [   # (entry, oracle, the fixture row's parameters, more parameters)
+    ("noise", noise_ref, (), None),
+    ("blur", blur_ref, (), None),
+    ("gaussian_blur", gaussian_blur_ref, (), None),
+    ("arch", arch_ref, (), None),
+    ("radial_blur", radial_blur_ref, (0.5,), (-1.3,)),
+    ("square", square_ref, (), None),
+    ("rays", rays_ref, (), None),
+    ("blade", blade_ref, (), None),
+    ("twintrian", twintrian_ref, (), None),
+    ("super_shape", super_shape_ref, (0.5, 6.0, 2.0, 2.0, 2.0, 0.0), (0.2, 5.0, 3.0, 1.5, 0.7, 0.3)),
+    ("flower", flower_ref, (6.0, 0.3), (3.0, -0.6)),
+    ("conic", conic_ref, (0.7, 0.2), (0.4, -0.1)),
+    ("parabola", parabola_ref, (0.5, 0.5), (0.55, -0.8)),
+    ("pie", pie_ref, (6.0, 0.5, 0.5), (5.0, -0.3, 0.9)),
+    ("boarders", boarders_ref, (), None),
+]
+
+
+@pytest.mark.parametrize("case", JWF_DRAWS + SMOKE_DRAWS + HASH_DRAWS + FLAM3_DRAWS, ids=lambda c: c[0])
+def test_values_with_scripted_draws(orc, case):
+    """Entries restated with scripted RNG words (those that draw nothing take
+    none): the value for every point, weight, parameter set and six word
+    sequences, f64 within 1e-6 relative, draws consumed in the order the
+    restatement takes them. Where the restatement divides by zero (a lattice
+    point of boarders, the origin), the oracle must give a non-finite value
+    there."""
+    name, ref, p0, p1 = case
+    if case in JWF_DRAWS:   # the reference's smoke rows for these ports
+        check_finite(orc, name, 1.0, p0, LALL_PTS)
+    pts = LALL_PTS + P8 + [(0.3, -0.8), (-0.2, -0.45), (1.7, 0.9)]
+    n = 0
+    for prm in [p0] + ([p1] if p1 is not None else []):
+        for s in range(6):
+            words = WORDS[s * 16:(s + 1) * 16]
+            for w in (1.0, 0.7):
+                for x, y in pts:
+                    got = run(orc, name, x, y, w, prm, words=words)
+                    try:
+                        want = ref(x, y, w, *prm, Draws(words))
+                    except ZeroDivisionError:
+                        assert not all(math.isfinite(v) for v in got), (name, prm, x, y, got)
+                        continue
+                    for k in range(2):
+                        assert abs(got[k] - want[k]) <= 1e-6 * max(1.0, abs(want[k])), \
+                            (name, prm, s, w, x, y, k, got[k], want[k])
+                    n += 1
+    assert n > 0
+
+
 # -- end of pins --
