@@ -1,8 +1,9 @@
 """The Node-API binding (pyr3_amd/napi) a TypeScript host would load.
 
 CPU: the addon loads under the installed Node and turns C-ABI errors into JS
-exceptions. GPU: one frame rendered from packed blobs through Node matches
-the same blobs rendered through the Python facade bit for bit.
+exceptions. GPU: one frame rendered from packed blobs through Node (its
+readbacks awaited as promises) matches the same blobs rendered through the
+Python facade bit for bit.
 """
 import json
 import shutil
@@ -41,7 +42,7 @@ def test_addon_loads_and_exports_stage_interface():
               str(NAPI / "pyr3hip.node"))
     assert r.returncode == 0, r.stderr
     assert r.stdout.strip().split(",") == ["create", "reset", "iterate", "present",
-                                           "readHistogram", "readImage", "sync"]
+                                           "readHistogram", "readImage", "readIndexMap", "sync"]
 
 
 def test_addon_raises_c_abi_errors():
@@ -57,6 +58,8 @@ def test_node_render_matches_python_render(tmp_path):
     g = genome(sierpinski(), scale=20.0, cx=0.5, cy=0.5)
     W, H, os_ = 40, 30, 1
     xf, ds, u, pal = blobs(g, W, H, iters=200, fuse=10, seed=77, walkers=300)
+    u = np.array(u, dtype=np.uint32)
+    u[14] = 1   # colour-index capture, read back through readIndexMap
     taps = pack.spatial_kernel(0.5, os_)
     k1, k2 = pack.calibration(g.scale, 300 * 200, 1.0, os_)
     de = pack.de_uniforms(W, H, 9, 0, 0.4, k1, k2, os_)
@@ -64,14 +67,20 @@ def test_node_render_matches_python_render(tmp_path):
     for name, arr in (("taps.f32", taps), ("palette.f32", pal), ("xforms.f32", xf),
                       ("distrib.u32", ds), ("uniforms.u32", u), ("de.u32", de), ("viz.u32", viz)):
         np.ascontiguousarray(arr).tofile(tmp_path / name)
-    (tmp_path / "meta.json").write_text(json.dumps({"width": W, "height": H, "oversample": os_}))
+    (tmp_path / "meta.json").write_text(json.dumps({"width": W, "height": H, "oversample": os_,
+                                                     "capture": True}))
     r = _node(str(NAPI / "render_blobs.js"), str(tmp_path))
     assert r.returncode == 0, r.stderr
     hn = np.fromfile(tmp_path / "hist_node.u32", dtype=np.uint32).reshape(H, W, 4)
     imn = np.fromfile(tmp_path / "image_node.u8", dtype=np.uint8).reshape(H, W, 4)
     rp = Renderer(W, H, os_)
     rp.reset(g)
-    rp.iterate(g, 77, 300, 200, fuse=10)
+    rp.iterate(g, 77, 300, 200, fuse=10, color=pack.ColorOpts(capture_index=True))
     rp.present(g, 300 * 200)
-    assert np.array_equal(hn, rp.read_histogram())
+    hp = rp.read_histogram()
+    assert np.array_equal(hn, hp)
     assert np.array_equal(imn, rp.read_image())
+    idx = np.fromfile(tmp_path / "idx_node.u32", dtype=np.uint32).reshape(H, W)
+    cnt = np.fromfile(tmp_path / "cnt_node.u32", dtype=np.uint32).reshape(H, W)
+    assert np.array_equal(idx, rp.read_index()) and idx.sum() > 0
+    assert np.array_equal(cnt, hp[..., 3])
