@@ -12,8 +12,8 @@ learned path (low quality + BF16-MFMA denoiser) beside the plain one.
 
 The other benchmark configurations are timed in the same run and reported
 under "configs" (C0 256x256 q50, C1 1024x768 os2 q200, C2 3840x2160 os2
-q200 from the whole catalog), each with its own chaos / DE / tonemap
-rooflines; the CPU baseline is the oracle on this host's cores for C0 in
+q200 from the whole catalog, each with its own chaos / DE / tonemap
+rooflines; C3 the 512-genome batch on a captured HIP graph, genomes/s); the CPU baseline is the oracle on this host's cores for C0 in
 full and for a bounded sample of the headline frame.
 
 Prints ONE JSON line. --gpus above 1 exits: this build targets one GPU.
@@ -107,11 +107,14 @@ class Frame:
     def render(self, quality: float, seed: int, walkers: int = None):
         samples_target = int(round(quality * self.r.width * self.r.height))
         walkers, iters = schedule(samples_target, walkers or self.walkers)
-        self.r.set_image_f32(self.denoiser is not None)
+        learned_frame = self.denoiser is not None
+        self.r.set_image_f32(learned_frame)
         self.r.reset(self.g)
         s = self.r.iterate(self.g, seed, walkers, iters, fuse=self.fuse, packed=self.packed)
-        self.r.present(self.g, s)
-        if self.denoiser is not None:
+        # the learned path's input: tonemap without DE into the f32 image,
+        # then the denoiser (pyr3_amd/learned.py)
+        self.r.present(self.g, s, force_de_off=learned_frame)
+        if learned_frame:
             self.denoiser.apply(self.r)
         return walkers, iters, s
 
@@ -348,34 +351,65 @@ def run_configs(steps, streams):
     return out
 
 
-def batch_main(a):
-    """--workload batch: the 512-genome batch (long edge 128..1024, 2..16
-    xforms, q 20..200, half motion-blurred with 8 sub-frames) through the
-    batch driver (per-shape kernels compiled in worker processes, each job
-    rendered as soon as its kernel is ready); one JSON line with genomes/s of
-    the second of two runs (the first compiles the shapes and allocates
-    buffers; the second finds them in the code-object cache)."""
+def measure_batch(n_jobs: int = 512, steps: int = 5, autotune: bool = True):
+    """C3: the n-genome batch (long edge 128..1024, 2..16 xforms, q 20..200,
+    half of them motion-blurred with 8 sub-frames) on one captured HIP graph
+    (batch.GraphBatch: one union-kernel chaos launch per launch group of 16
+    genome shapes, merges, DE and tonemap per frame). The plan (post-fuse
+    turns per walker, graph streams) is autotuned on the batch's first 64
+    genomes; the timed value is the device time of whole-batch graph
+    replays (HIP events around each replay), every input resident in HBM."""
     from pyr3_amd import batch
-    jobs = batch.batch_workload(n=a.batch_size)
-    samples = sum(j.quality * j.width * j.height for j in jobs)
-    out = {}
-    for run in ("first", "timed"):
-        br = batch.BatchRenderer(n_streams=a.streams)
+    t0 = time.perf_counter()
+    jobs = batch.batch_workload(n=n_jobs)
+    packed = [[batch.pack_for(g) for g in (j.subframes or [j.genome])] for j in jobs]
+    t_pack = time.perf_counter() - t0
+    if autotune:
+        plan, table = batch.autotune(jobs, packed=packed, log=log)
+    else:
+        plan, table = batch.BatchPlan(), {}
+    t0 = time.perf_counter()
+    gb = batch.GraphBatch(jobs, plan, packed=packed)
+    t_commit = time.perf_counter() - t0
+    try:
+        gb.launch()
+        gb.sync()
+        times = []
+        for _ in range(steps):
+            gb.launch()
+            times.append(gb.elapsed_ms())
         t0 = time.perf_counter()
-        br.render(jobs, progress=lambda k: log(f"{run}: {k}/{len(jobs)} genomes"))
-        wall = time.perf_counter() - t0
-        br.close()
-        out[run] = {"s": round(wall, 3), "genomes_per_s": round(len(jobs) / wall, 2),
-                    "shapes": br.timings["shapes"],
-                    "gsamples_per_s": round(samples / wall / 1e9, 3)}
-        log(f"{run}: {wall:.2f}s for {len(jobs)} genomes")
+        for _ in range(steps):
+            gb.launch()
+        gb.sync()
+        wall = (time.perf_counter() - t0) * 1e3 / steps
+        st = gb.stats()
+        samples = int(sum(gb.samples))
+        turns = int(sum(w * (i + pack.FUSE) for w, i in zip(gb.walkers, gb.iters)))
+    finally:
+        gb.close()
+    ms = float(np.median(times))
+    log(f"batch of {n_jobs}: {ms:.1f} ms per graph replay ({n_jobs / ms * 1e3:.1f} genomes/s), "
+        f"plan {plan}, {st}")
+    return {"workload": f"batch-{n_jobs}", "genomes": n_jobs, "motion_blur_subframes": 8,
+            "genomes_per_s": round(n_jobs / ms * 1e3, 2), "ms_per_batch": round(ms, 2),
+            "ms_per_batch_wall": round(wall, 2), "steps": steps,
+            "plan": {"turns_per_walker": plan.turns, "max_shapes": plan.max_shapes,
+                     "lanes": plan.lanes},
+            "autotune_ms_per_64": table, **st,
+            "gsamples_per_s": round(samples / (ms * 1e-3) / 1e9, 3),
+            "giters_per_s": round(turns / (ms * 1e-3) / 1e9, 3),
+            "host_pack_s": round(t_pack, 2), "commit_s": round(t_commit, 2)}
+
+
+def batch_main(a):
+    """--workload batch: the batch measurement alone, as one JSON line."""
+    out = measure_batch(a.batch_size, a.steps)
     line = {
-        "metric": "genomes/s, 512-genome batch (plain path)", "value": out["timed"]["genomes_per_s"],
-        "unit": "genomes/s", "n_gpus": 1, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32+u32", "data": "synthetic",
-        "config": {"workload": f"batch-{len(jobs)}", "motion_blur_subframes": 8,
-                   "streams": a.streams},
-        "runs": out, "total_gsamples": round(samples / 1e9, 3),
+        "metric": "genomes/s, 512-genome batch (plain path)", "value": out["genomes_per_s"],
+        "unit": "genomes/s", "n_gpus": 1, "steps": a.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+u32", "data": "synthetic",
+        "config": {"workload": out["workload"]}, "batch": out,
     }
     print(json.dumps(line), flush=True)
 
@@ -387,9 +421,10 @@ LEARNED_LADDER = (2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64)
 def learned_path(r: Renderer, g, a, ref_img, walkers: int):
     """Low-quality render + the BF16-MFMA denoiser: smallest q reaching the
     target PSNR, then ms/frame of render + denoise at that q."""
+    from pyr3_amd import learned
     from pyr3_amd.renderer import Denoiser
     den = Denoiser.from_npz(WEIGHTS)
-    fl = Frame(r, g, a.fuse, walkers=walkers, denoiser=den)
+    fl = Frame(r, g, learned.FUSE, walkers=walkers, denoiser=den)
     ql, scores = quality_to_target(fl, ref_img, a.target_db, ladder=LEARNED_LADDER)
     if ql is None:
         log(f"learned path: no q up to {max(scores)} reached {a.target_db} dB")
@@ -416,7 +451,7 @@ def learned_path(r: Renderer, g, a, ref_img, walkers: int):
     # synchronize of every stream after the last frame
     extra = [Renderer(r.width, r.height, r.oversample) for _ in range(a.streams - 1)]
     dens = [Denoiser.from_npz(WEIGHTS) for _ in extra]
-    frames = [fl] + [Frame(rx, g, a.fuse, walkers=walkers, denoiser=dx)
+    frames = [fl] + [Frame(rx, g, learned.FUSE, walkers=walkers, denoiser=dx)
                      for rx, dx in zip(extra, dens)]
     for rx in extra:
         rx.set_image_f32(True)
@@ -451,10 +486,11 @@ def learned_path(r: Renderer, g, a, ref_img, walkers: int):
     den.destroy()
     log(f"learned path: q={ql} {scores[ql]:.2f} dB, {ms:.3f} ms/frame ({serial:.3f} with a "
         f"sync per frame; chaos {chaos / a.steps:.3f} ms, denoise {dn / a.steps:.3f} ms)")
-    from pyr3_amd.denoise.model import macs_per_pixel
+    from pyr3_amd.denoise.net import macs_per_pixel
     flops = 2.0 * macs_per_pixel() * r.width * r.height
     dms = dn / a.steps
     return {"reached": True, "ms_per_frame": round(ms, 3), "quality": ql, "walkers": walkers,
+            "fuse": learned.FUSE,
             "walker_autotune_ms": {str(k): v for k, v in tune_ms.items()},
             "ms_per_frame_serial": round(serial, 3), "chaos_ms": round(chaos / a.steps, 4),
             "psnr_db": round(scores[ql], 2), "denoise_ms": round(dms, 4),
@@ -544,6 +580,8 @@ def main():
         learned = learned_path(r, g, a, ref_img, frame.walkers)
     r.destroy()
     configs = None if a.no_configs else run_configs(max(2, a.steps // 2), a.streams)
+    if configs is not None:
+        configs["C3"] = measure_batch(512, 5)
     traffic = profiled_traffic(CHAOS_KERNEL, f"plain-{W}x{H}-os1", q, walkers)
     chaos_ms = kms["chaos"]
     out = {

@@ -192,13 +192,16 @@ int p3_chaos_ready(const float* xforms, int n_slots, int n_rows, int final_idx);
 int p3_chaos_source(const float* xforms, int n_slots, int n_rows, int final_idx, char* out,
                     size_t cap, size_t* len);
 
-/* Learned denoiser (no counterpart in the reference): a small U-Net run in
- * BF16 with MFMA kernels on the handle's stream. Parameters are the float32
- * weights in the order of pyr3_amd/denoise/model.py (each 3x3 layer's weight
- * [c_out][c_in][3][3] then bias, then the 1x1 head's weight [3][16] and
- * bias); p3_denoiser_param_count() gives their number. p3_denoise reads the
- * renderer's last present (oversample 1, f32 image enabled) and overwrites
- * its RGBA8 image with the denoised one. */
+/* Learned denoiser (no counterpart in the reference): a 4-level U-Net
+ * (pyr3_amd/denoise/net.py, ~0.49 M parameters) run in BF16 with MFMA
+ * kernels on the handle's stream. Parameters are float32, per 3x3 layer
+ * (e0 e1 d1 e2 d2 e3 d3 b0 u3 c3 u2 c2 u1 c1) its weight [c_out][c_in][3][3]
+ * then its bias, then the 1x1 head's weight [3][16] and bias;
+ * p3_denoiser_param_count() gives their number. p3_denoise reads the
+ * renderer's histogram (oversample 1) and its last present, which must have
+ * run without DE and with the f32 image enabled (the base image the network
+ * corrects; the present's k2 scales the density input), and overwrites the
+ * RGBA8 image with the denoised one. */
 typedef struct p3_denoiser p3_denoiser;
 size_t p3_denoiser_param_count(void);
 int p3_denoiser_create(const float* params, size_t n, p3_denoiser** out);
@@ -226,6 +229,57 @@ int p3_denoise(p3_renderer* r, p3_denoiser* d);
 int p3_iterate_noalloc(const float* xforms, int n_slots, const uint32_t* distrib, int n_rows,
                        const uint32_t* uniforms, const float* palette_rgba, uint32_t* hist,
                        size_t n_hist, void* stream);
+
+/* Batched rendering (no counterpart in the reference, whose batch loop
+ * renders one genome or sub-frame per dispatch, src/animate-render.ts:144-160):
+ * many output frames of any genome shapes and sizes on one captured HIP
+ * graph with a few persistent chaos launches.
+ *
+ * p3_batch_add: one output frame, with the arguments of p3_create (size,
+ * oversample, taps), p3_reset (palette), p3_iterate_multi (n_sub
+ * sub-dispatches of one genome shape, e.g. the sub-frames of a motion-blurred
+ * frame, each with its own records, pick table and uniform block: seed,
+ * walkers, iterations, camera) and p3_present (DE uniforms or null, tonemap
+ * uniforms). Returns the frame's index (>= 0) or a P3_E* code. Colour-index
+ * capture and channel curves are refused (P3_E_ARG).
+ *
+ * p3_batch_commit(b, max_shapes, lanes): allocates every frame's buffers,
+ * uploads the blobs, compiles one union chaos kernel per launch group (the
+ * frames in order, at most max_shapes distinct genome shapes per group;
+ * frames with more than 16 xforms group apart) and captures the batch in a
+ * HIP graph: per group a histogram fill, one chaos launch over all of its
+ * sub-dispatches, the packed merge, then DE and tonemap per frame; groups
+ * alternate over `lanes` (1..4) streams. Frames cannot be added afterwards.
+ *
+ * p3_batch_launch replays the graph on the batch's stream (returns at once);
+ * every replay renders the same frames, bit-identically, and each frame's
+ * histogram equals that of p3_reset + p3_iterate_multi of its sub-dispatches
+ * on a renderer. p3_batch_elapsed gives the last launch's device time (ms);
+ * p3_batch_stats {groups, shapes, sub-dispatches, chaos blocks}; the reads
+ * wait for the launch and copy one frame's RGBA8 image (w x h x 4 bytes) or
+ * histogram (W_s x H_s x 4 u32). Errors: p3_batch_last_error(b). */
+typedef struct p3_batch p3_batch;
+int p3_batch_create(p3_batch** out);
+void p3_batch_destroy(p3_batch* b);
+int p3_batch_add(p3_batch* b, int width, int height, int oversample, const float* taps,
+                 int n_taps, const float* palette_rgba, int n_sub, const float* const* xforms,
+                 int n_slots, const uint32_t* const* distribs, int n_rows,
+                 const uint32_t* const* uniforms, const uint32_t* de_uniforms,
+                 const uint32_t* viz_uniforms);
+int p3_batch_commit(p3_batch* b, int max_shapes, int lanes);
+int p3_batch_launch(p3_batch* b);
+int p3_batch_sync(p3_batch* b);
+int p3_batch_elapsed(p3_batch* b, float* ms);
+int p3_batch_frames(p3_batch* b, int* out);
+int p3_batch_stats(p3_batch* b, uint32_t* out4);
+int p3_batch_read_image(p3_batch* b, int frame, uint8_t* out, size_t n);
+int p3_batch_read_histogram(p3_batch* b, int frame, uint32_t* out, size_t n);
+const char* p3_batch_last_error(p3_batch* b);
+/* Compiles the union chaos kernel of n shapes ahead of time (no device
+ * needed), as p3_batch_commit would for a group of those shapes in that
+ * order; each shape is given as for p3_chaos_prepare. */
+int p3_union_prepare(int n, const float* const* xforms, const int* n_slots, const int* n_rows,
+                     const int* final_idx);
 
 /* Message for the last failing call on this handle (or of p3_create, and of
  * p3_iterate_noalloc, when r is null). Never null. */

@@ -55,6 +55,31 @@ SIGNATURES = {
     "p3_chaos_source": (_I, [_FP, _I, _I, _I, C.c_char_p, _SZ, C.POINTER(_SZ)]),
     "p3_variation_supported": (_I, [_I]),
     "p3_last_error": (C.c_char_p, [_P]),
+    "p3_batch_create": (_I, [C.POINTER(_P)]),
+    "p3_batch_destroy": (None, [_P]),
+    "p3_batch_add": (_I, [_P, _I, _I, _I, _FP, _I, _FP, _I, C.POINTER(_FP), _I, C.POINTER(_UP),
+                          _I, C.POINTER(_UP), _UP, _UP]),
+    "p3_batch_commit": (_I, [_P, _I, _I]),
+    "p3_batch_launch": (_I, [_P]),
+    "p3_batch_sync": (_I, [_P]),
+    "p3_batch_elapsed": (_I, [_P, _FP]),
+    "p3_batch_frames": (_I, [_P, C.POINTER(_I)]),
+    "p3_batch_stats": (_I, [_P, _UP]),
+    "p3_batch_read_image": (_I, [_P, _I, _BP, _SZ]),
+    "p3_batch_read_histogram": (_I, [_P, _I, _UP, _SZ]),
+    "p3_batch_last_error": (C.c_char_p, [_P]),
+    "p3_union_prepare": (_I, [_I, C.POINTER(_FP), C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
+    # training kernels of the learned denoiser (include/pyr3nn.h)
+    "p3nn_gemm": (_I, [_P, _I, _I, _I, _I, _P]),
+    "p3nn_im2col": (_I, [_P, _P, _P, _P, _I, _P]),
+    "p3nn_col2im": (_I, [_P, _P, _I, _P, _P, _P]),
+    "p3nn_relu_bwd": (_I, [_P, _P, _P, _SZ, _P]),
+    "p3nn_bias_grad": (_I, [_P, _SZ, _I, _P, _P]),
+    "p3nn_features": (_I, [_P, _P, C.c_float, _SZ, _P, _P]),
+    "p3nn_head": (_I, [_P, _P, _P, _P, _SZ, _P, _P, _P, _P, _P]),
+    "p3nn_adam": (_I, [_P, _P, _P, _P, _SZ, C.c_float, C.c_float, C.c_float, C.c_float, _I, _P,
+                       _P]),
+    "p3nn_cast_bf16": (_I, [_P, _P, _SZ, _P]),
 }
 
 _lib = None

@@ -17,7 +17,8 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libpyr3hip.so"
-SOURCES = ["p3_api.cpp", "p3_rtc.cpp", "p3_image.hip", "p3_denoise.hip"]
+SOURCES = ["p3_api.cpp", "p3_batch.cpp", "p3_rtc.cpp", "p3_image.hip", "p3_denoise.hip",
+           "p3_nn.hip"]
 # Device headers embedded as text for the run-time chaos specialisation, in
 # dependency order (their #include lines are dropped).
 RTC_HEADERS = ["p3_args.h", "p3_device.h", "p3_variations.h", "p3_variations_ext.h",

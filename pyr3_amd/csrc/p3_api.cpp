@@ -21,6 +21,7 @@
 #include "../../include/pyr3hip.h"
 #include "p3_denoise.h"
 #include "p3_launch.h"
+#include "p3_host.h"
 
 
 namespace {
@@ -49,8 +50,6 @@ struct DevBuf {
   }
 };
 
-float bits_f(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
-
 // Catalog entries the chaos kernel implements: all of 0..325 (index 97 is
 // pre_blur, applied by the chain itself).
 bool supported(uint32_t idx) {
@@ -60,9 +59,10 @@ bool supported(uint32_t idx) {
 // Every active slot of every record must name an implemented entry.
 int check_records(p3_renderer* r, const float* xforms, int n_slots);
 
-// Step boundaries of a pick table (n_rows x 16 entries, see ChaosArgs), or
-// false when the table has more than 16 xforms or a row whose entries
-// decrease somewhere (then the kernel reads the table itself).
+}  // namespace
+
+namespace p3h {
+
 bool pick_bounds(const uint32_t* table, int n_rows, std::vector<uint16_t>& out) {
   if (n_rows > p3::kPickRows) return false;
   out.assign((size_t)n_rows * p3::kPickCols, (uint16_t)P3_GRAIN);
@@ -79,7 +79,9 @@ bool pick_bounds(const uint32_t* table, int n_rows, std::vector<uint16_t>& out) 
   return true;
 }
 
-}  // namespace
+}  // namespace p3h
+
+using namespace p3h;
 
 struct p3_renderer {
   int width = 0, height = 0, os = 1, sw = 0, sh = 0;
@@ -125,6 +127,8 @@ struct p3_renderer {
   bool timing = false;
   bool want_image_f32 = false;
   bool have_image_f32 = false;
+  float last_k2 = 0.0f;        // calibration k2 of the last present (denoiser input)
+  bool last_de = false;        // the last present ran DE
   std::vector<float> last_xforms;       // host copies of the last uploads
   std::vector<uint32_t> last_distrib;
   hipEvent_t ev[kTimers][2] = {};
@@ -156,7 +160,7 @@ struct p3_denoiser {
   DevBuf<uint16_t> w;
   DevBuf<float> b, head;
   size_t w_off[p3::kDenoiseLayers] = {}, b_off[p3::kDenoiseLayers] = {};
-  DevBuf<uint16_t> s0a, s0, h1a, s1, h1b, q0, q1;  // activations, grown on demand
+  DevBuf<uint16_t> act[p3::kDenoiseLayers - 1];  // activations, grown on demand
 };
 
 namespace {
@@ -169,6 +173,10 @@ uint16_t bf16_bits(float f) {
   return (uint16_t)(u >> 16);
 }
 
+
+}  // namespace
+
+namespace p3h {
 
 // Per-radius kernel sums over the integer disc, sigma = r / 3, in f64
 // (src/density.ts:78-99), stored as f32: ~22 M exp() for radii up to 256,
@@ -191,6 +199,11 @@ const float* kernel_norm_lut() {
   }();
   return lut.data();
 }
+
+}  // namespace p3h
+
+namespace {
+
 int build(p3_renderer* r, int width, int height, int os, const float* taps, int n_taps) {
   if (width <= 0 || height <= 0 || os < 1 || os > 16)
     return r->fail(P3_E_ARG, "invalid size %dx%d oversample %d", width, height, os);
@@ -274,8 +287,10 @@ void release(p3_renderer* r) {
   r->linear.release(); r->curves.release();
 }
 
-// Empty when every record's variation count and active slots are valid,
-// else the message.
+}  // namespace
+
+namespace p3h {
+
 std::string records_error(const float* xforms, int n_slots) {
   char msg[128];
   for (int s = 0; s < n_slots; ++s) {
@@ -297,6 +312,10 @@ std::string records_error(const float* xforms, int n_slots) {
   }
   return std::string();
 }
+
+}  // namespace p3h
+
+namespace {
 
 int check_records(p3_renderer* r, const float* xforms, int n_slots) {
   const std::string e = records_error(xforms, n_slots);
@@ -392,8 +411,10 @@ int usable(p3_renderer* r) {
                              "p3_resize again or destroy it");
 }
 
-// Kernel arguments of one dispatch whose records, table, palette and
-// histogram are at dxf / dds / pal / hist.
+}  // namespace
+
+namespace p3h {
+
 p3::ChaosArgs dispatch_args_at(const uint32_t* u, int n_regular, int32_t final_idx,
                                const float* dxf, const uint32_t* dds, const float* pal,
                                uint32_t* hist) {
@@ -426,6 +447,32 @@ p3::ChaosArgs dispatch_args_at(const uint32_t* u, int n_regular, int32_t final_i
   a.idx_sum = nullptr;
   return a;
 }
+
+std::string de_params(const uint32_t* de, p3::DeArgs& a) {
+  a.max_rad = bits_f(de[2]); a.min_rad = bits_f(de[3]); a.curve = bits_f(de[4]);
+  a.k1 = bits_f(de[5]); a.k2 = bits_f(de[6]);
+  if (!(a.max_rad >= 0.0f) || std::isinf(a.max_rad))
+    return "DE max radius must be finite and >= 0";
+  a.R = (int)std::ceil(std::min(std::max(a.max_rad, 0.0f), 256.0f));
+  if (a.R > 254)
+    return "DE search radius " + std::to_string(a.R) + " above the 254 limit";
+  return std::string();
+}
+
+void tm_params(const uint32_t* viz, p3::TmArgs& t) {
+  t.fwidth = viz[9];
+  t.k1 = bits_f(viz[2]); t.k2 = bits_f(viz[3]); t.gamma = bits_f(viz[4]);
+  t.vibrancy = bits_f(viz[5]); t.highpow = bits_f(viz[6]); t.linrange = bits_f(viz[7]);
+  t.bg0 = bits_f(viz[12]); t.bg1 = bits_f(viz[13]); t.bg2 = bits_f(viz[14]);
+  t.transparent = viz[21];
+  t.curves_mask = viz[16];
+  t.hsl_on = viz[17];
+  t.hsl_hue = bits_f(viz[18]); t.hsl_sat = bits_f(viz[19]); t.hsl_light = bits_f(viz[20]);
+}
+
+}  // namespace p3h
+
+namespace {
 
 // Kernel arguments of one dispatch on the renderer's palette and histogram.
 p3::ChaosArgs dispatch_args(p3_renderer* r, const uint32_t* u, int n_regular, int32_t final_idx,
@@ -797,12 +844,8 @@ int p3_present(p3_renderer* r, const uint32_t* de, const uint32_t* viz) {
       return r->fail(P3_E_ARG, "DE uniforms %ux%u do not match super-res grid %dx%d", de[0], de[1],
                      r->sw, r->sh);
     p3::DeArgs a{};
-    a.max_rad = bits_f(de[2]); a.min_rad = bits_f(de[3]); a.curve = bits_f(de[4]);
-    a.k1 = bits_f(de[5]); a.k2 = bits_f(de[6]);
-    if (!(a.max_rad >= 0.0f) || std::isinf(a.max_rad))
-      return r->fail(P3_E_ARG, "DE max radius must be finite and >= 0");
-    a.R = (int)std::ceil(std::min(std::max(a.max_rad, 0.0f), 256.0f));
-    if (a.R > 254) return r->fail(P3_E_ARG, "DE search radius %d above the 254 limit", a.R);
+    const std::string bad = de_params(de, a);
+    if (!bad.empty()) return r->fail(P3_E_ARG, "%s", bad.c_str());
     a.hist = (const uint4*)r->hist.p;
     a.pre = r->pre.p;
     a.rad = r->rad.p;
@@ -824,19 +867,15 @@ int p3_present(p3_renderer* r, const uint32_t* de, const uint32_t* viz) {
   t.out8 = r->image8.p;
   t.outf = r->want_image_f32 ? r->imagef.p : nullptr;
   t.width = (uint32_t)r->width; t.height = (uint32_t)r->height;
-  t.os = (uint32_t)r->os; t.fwidth = viz[9];
-  t.k1 = bits_f(viz[2]); t.k2 = bits_f(viz[3]); t.gamma = bits_f(viz[4]);
-  t.vibrancy = bits_f(viz[5]); t.highpow = bits_f(viz[6]); t.linrange = bits_f(viz[7]);
-  t.bg0 = bits_f(viz[12]); t.bg1 = bits_f(viz[13]); t.bg2 = bits_f(viz[14]);
-  t.transparent = viz[21];
+  t.os = (uint32_t)r->os;
+  tm_params(viz, t);
+  r->last_k2 = t.k2;
+  r->last_de = de != nullptr;
   if (viz[16] > 31u) return r->fail(P3_E_ARG, "curves mask %u has bits above 4", viz[16]);
   if (viz[16] != 0u && !r->curves_set)
     return r->fail(P3_E_STATE, "curves mask %u set but no curves LUT uploaded (p3_set_curves)",
                    viz[16]);
   t.curves = r->curves.p;
-  t.curves_mask = viz[16];
-  t.hsl_on = viz[17];
-  t.hsl_hue = bits_f(viz[18]); t.hsl_sat = bits_f(viz[19]); t.hsl_light = bits_f(viz[20]);
   r->tic(T_TM);
   if ((rc = r->hip(p3::launch_tonemap(t, de != nullptr, r->stream), "tonemap launch"))) return rc;
   r->toc(T_TM);
@@ -1103,26 +1142,22 @@ int p3_denoiser_create(const float* params, size_t n, p3_denoiser** out) {
     return P3_E_ARG;
   }
   p3_denoiser* d = new p3_denoiser();
-  // Pack each 3x3 layer as bf16 [tap][k-chunk][out][16 k] (input channels
-  // zero-padded to a multiple of 16), biases as f32.
+  // Each 3x3 layer's weight [c_out][c_in][3][3] packed as bf16 [c_out][kp]
+  // with k = tap * c_in + c (zero past 9 c_in); biases f32.
   std::vector<uint16_t> w;
   std::vector<float> b;
   const float* q = params;
   for (int l = 0; l < p3::kDenoiseLayers; ++l) {
     const int ci = p3::kDenoiseShape[l][0], co = p3::kDenoiseShape[l][1];
-    const int chunks = (ci + 15) / 16;
+    const int kp = p3::denoise_kp(l);
     d->w_off[l] = w.size();
     d->b_off[l] = b.size();
-    w.resize(w.size() + (size_t)9 * chunks * co * 16, 0);
+    w.resize(w.size() + (size_t)co * kp, 0);
     uint16_t* dst = w.data() + d->w_off[l];
-    for (int t = 0; t < 9; ++t)
-      for (int kc = 0; kc < chunks; ++kc)
-        for (int o = 0; o < co; ++o)
-          for (int k = 0; k < 16; ++k) {
-            const int c = kc * 16 + k;
-            const float v = c < ci ? q[((size_t)o * ci + c) * 9 + t] : 0.0f;
-            dst[(((size_t)t * chunks + kc) * co + o) * 16 + k] = bf16_bits(v);
-          }
+    for (int o = 0; o < co; ++o)
+      for (int c = 0; c < ci; ++c)
+        for (int t = 0; t < 9; ++t)
+          dst[(size_t)o * kp + t * ci + c] = bf16_bits(q[((size_t)o * ci + c) * 9 + t]);
     q += (size_t)co * ci * 9;
     b.insert(b.end(), q, q + co);
     q += co;
@@ -1146,7 +1181,7 @@ int p3_denoiser_create(const float* params, size_t n, p3_denoiser** out) {
 int p3_denoiser_destroy(p3_denoiser* d) {
   if (!d) return P3_OK;
   d->w.release(); d->b.release(); d->head.release();
-  for (auto* a : {&d->s0a, &d->s0, &d->h1a, &d->s1, &d->h1b, &d->q0, &d->q1}) a->release();
+  for (auto& a : d->act) a.release();
   delete d;
   return P3_OK;
 }
@@ -1156,21 +1191,21 @@ int p3_denoise(p3_renderer* r, p3_denoiser* d) {
   if (usable(r)) return P3_E_STATE;
   if (!d) return r->fail(P3_E_ARG, "null denoiser");
   if (r->os != 1) return r->fail(P3_E_STATE, "the denoiser runs at oversample 1");
-  if (!r->have_image || !r->have_image_f32)
-    return r->fail(P3_E_STATE, "denoise needs a present with the f32 image enabled");
+  if (!r->have_image || !r->have_image_f32 || r->last_de)
+    return r->fail(P3_E_STATE,
+                   "denoise needs a present without DE and with the f32 image enabled");
   const int W = r->width, H = r->height;
-  const size_t W2 = (W + 1) / 2, H2 = (H + 1) / 2, W4 = (W2 + 1) / 2, H4 = (H2 + 1) / 2;
-  const size_t full = (size_t)W * H, half = W2 * H2, qtr = W4 * H4;
+  size_t lw[4] = {(size_t)W, 0, 0, 0}, lh[4] = {(size_t)H, 0, 0, 0};
+  for (int i = 1; i < 4; ++i) { lw[i] = (lw[i - 1] + 1) / 2; lh[i] = (lh[i - 1] + 1) / 2; }
   int rc;
-  if ((rc = r->hip(d->s0a.ensure(full * 16), "hipMalloc(denoise)")) ||
-      (rc = r->hip(d->s0.ensure(full * 16), "hipMalloc(denoise)")) ||
-      (rc = r->hip(d->h1a.ensure(half * 32), "hipMalloc(denoise)")) ||
-      (rc = r->hip(d->s1.ensure(half * 32), "hipMalloc(denoise)")) ||
-      (rc = r->hip(d->h1b.ensure(half * 32), "hipMalloc(denoise)")) ||
-      (rc = r->hip(d->q0.ensure(qtr * 64), "hipMalloc(denoise)")) ||
-      (rc = r->hip(d->q1.ensure(qtr * 64), "hipMalloc(denoise)")))
-    return rc;
   p3::DenoiseArgs a{};
+  for (int l = 0; l < p3::kDenoiseLayers - 1; ++l) {
+    const int lev = p3::kDenoiseShape[l][2];
+    if ((rc = r->hip(d->act[l].ensure(lw[lev] * lh[lev] * p3::kDenoiseShape[l][1]),
+                     "hipMalloc(denoise activations)")))
+      return rc;
+    a.act[l] = d->act[l].p;
+  }
   for (int l = 0; l < p3::kDenoiseLayers; ++l) {
     a.w[l] = d->w.p + d->w_off[l];
     a.b[l] = d->b.p + d->b_off[l];
@@ -1179,8 +1214,7 @@ int p3_denoise(p3_renderer* r, p3_denoiser* d) {
   a.img = r->imagef.p;
   a.hist = r->hist.p;
   a.out8 = r->image8.p;
-  a.s0a = d->s0a.p; a.s0 = d->s0.p; a.h1a = d->h1a.p; a.s1 = d->s1.p;
-  a.h1b = d->h1b.p; a.q0 = d->q0.p; a.q1 = d->q1.p;
+  a.k2 = r->last_k2;
   a.width = W;
   a.height = H;
   r->tic(T_DENOISE);

@@ -48,6 +48,10 @@ struct ChaosArgs {
   // (the last: out of bit 63)
   uint64_t* acc;
   uint64_t* carry;
+  // Kernels built for several genome shapes at once (a batch's union
+  // kernel): this dispatch's shape among them and the case number of its
+  // xform 0 in the union's xform switch (0 and 0 in a one-shape kernel).
+  uint32_t shape, case_base;
 };
 
 // Batched launch: up to kMaxSub sub-dispatches of one genome shape in one
@@ -57,6 +61,16 @@ struct MultiArgs {
   const ChaosArgs* args;  // n entries in device memory
   uint32_t n;
   uint32_t start[kMaxSub + 1];
+};
+
+// Batch launch of a union kernel: any number of sub-dispatches (of any of
+// the union's shapes, each with its own histogram); sub-dispatch k owns
+// blocks [start[k], start[k+1]) and start[] (n + 1 entries) is in device
+// memory.
+struct BatchArgs {
+  const ChaosArgs* args;  // n entries in device memory
+  const uint32_t* start;  // n + 1 entries in device memory
+  uint32_t n;
 };
 
 // LDS image of a chaos block's walkers (structure of arrays, kB walkers):

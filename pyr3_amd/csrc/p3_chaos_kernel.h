@@ -230,21 +230,38 @@ P3_INLINE void run_turn(const ChaosArgs& A, pal_t palette, Walker& wk, Rng& rng,
   const xrec_t xf = X + (size_t)fn * kXformFloats;
   bool dc_on = false;
   float3 dc_rgb = make_float3(0.0f, 0.0f, 0.0f);
-  const Chain ca = Spec::chain(fn, X, wk.x, wk.y, wk.z, rng, dc_on, dc_rgb);
+  // One call site of the chain, so its straight-line xform cases are
+  // inlined once: pass 0 runs the picked xform; with xform blend, a second
+  // pass runs the blended-in xform bi (whose direct colour is dropped).
+  // Draw order as chaos.wgsl:8372-8380: chain(fn), blend test, bi's draw,
+  // chain(bi), then the blend factor.
+  Chain ca{}, cb{};
+  uint32_t cur = fn;
+  bool blended = false;
+#pragma clang loop unroll(disable)
+  for (int pass = 0; pass < 2; ++pass) {
+    bool dcp = false;
+    float3 rgbp = make_float3(0.0f, 0.0f, 0.0f);
+    const Chain c = Spec::chain(A, cur, X, wk.x, wk.y, wk.z, rng, dcp, rgbp);
+    if (pass != 0) {
+      cb = c;
+      break;
+    }
+    ca = c;
+    dc_on = dcp;
+    dc_rgb = rgbp;
+    if (!(A.xform_blend > 0.0f) || !(rng.rand01() < A.xform_blend)) break;
+    const uint32_t rb = rng.irand();
+    cur = D[(size_t)A.fallback_row * kGrain + (rb & (kGrain - 1))];
+    blended = true;
+  }
   float2 pv = ca.pv;
   float nz = ca.z;
-  if (A.xform_blend > 0.0f) {
-    if (rng.rand01() < A.xform_blend) {
-      const uint32_t rb = rng.irand();
-      const uint32_t bi = D[(size_t)A.fallback_row * kGrain + (rb & (kGrain - 1))];
-      bool dc_b = false;
-      float3 rgb_b;
-      const Chain cb = Spec::chain(bi, X, wk.x, wk.y, wk.z, rng, dc_b, rgb_b);
-      const float t = rng.rand01();
-      pv.x = mixf(pv.x, cb.pv.x, t);
-      pv.y = mixf(pv.y, cb.pv.y, t);
-      nz = mixf(nz, cb.z, t);
-    }
+  if (blended) {
+    const float t = rng.rand01();
+    pv.x = mixf(pv.x, cb.pv.x, t);
+    pv.y = mixf(pv.y, cb.pv.y, t);
+    nz = mixf(nz, cb.z, t);
   }
   const bool bad = is_bad(pv);
   if (bad) {
@@ -284,7 +301,7 @@ P3_INLINE void run_turn(const ChaosArgs& A, pal_t palette, Walker& wk, Rng& rng,
     if (apply && needed) {
       const float2 fpa = make_float2(fx[0] * pv.x + fx[1] * pv.y + fx[2],
                                      fx[4] * pv.x + fx[5] * pv.y + fx[6]);
-      const float2 fpv = Spec::final_sum(X, fslot, fpa, rng, dc_on, dc_rgb);
+      const float2 fpv = Spec::final_sum(A, X, fslot, fpa, rng, dc_on, dc_rgb);
       const float fz = mixf(nz, fx[8], fx[9]);
       if (!is_bad(fpv)) { sx = fpv.x; sy = fpv.y; sz = fz; }
     }
@@ -408,6 +425,19 @@ P3_INLINE uint32_t multi_find(const MultiArgs& M, uint32_t b) {
   uint32_t k = 0;
   while (k + 1 < M.n && b >= M.start[k + 1]) ++k;
   return k;
+}
+
+// Sub-dispatch owning block b of a batch launch: the last k with
+// start[k] <= b (binary search over the device array; start[0] = 0 and b is
+// below start[n]).
+P3_INLINE uint32_t batch_find(const BatchArgs& M, uint32_t b) {
+  const P3_CONST uint32_t* s = (const P3_CONST uint32_t*)M.start;
+  uint32_t lo = 0, hi = M.n;   // s[lo] <= b < s[hi]
+  while (hi - lo > 1u) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (s[mid] <= b) lo = mid; else hi = mid;
+  }
+  return lo;
 }
 
 // Pick of the walker's next turn: row = its previous accepted xform (or the

@@ -52,6 +52,24 @@ hipError_t launch_hist_merge(uint32_t* hist, uint64_t* acc, uint64_t* carry, siz
 hipError_t launch_atomic_probe(uint64_t* cells_base, uint32_t cells, uint32_t blocks,
                                uint32_t per_thread, uint32_t seed, hipStream_t stream);
 
+// One genome shape of a union kernel: a record blob of that shape (host
+// memory), its regular xform count and final slot (-1: none).
+struct ShapeRef {
+  const float* xforms;
+  int n_regular;
+  int final_slot;
+};
+// Union kernel of `shapes` (each shape once, in case order): key, source,
+// compile (cached like the one-shape kernels) and the batch launch of
+// `n_blocks` blocks over the sub-dispatches of `bargs`.
+std::string union_key(const std::vector<ShapeRef>& shapes);
+std::string union_source(const std::vector<ShapeRef>& shapes);
+bool union_code(const std::vector<ShapeRef>& shapes, const std::vector<char>** out,
+                std::string& err);
+hipError_t launch_union_batch(const BatchArgs& bargs, uint32_t n_blocks,
+                              const std::vector<ShapeRef>& shapes, hipStream_t stream,
+                              std::string& err);
+
 // Chaos kernel specialised for the shape of `host_xforms` (p3_rtc.cpp).
 hipError_t launch_chaos_rtc(const ChaosArgs& a, const float* host_xforms, int n_regular,
                             int final_slot, hipStream_t stream, std::string& err);

@@ -16,6 +16,7 @@
 #include <sys/stat.h>
 #include <unistd.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -34,7 +35,8 @@ namespace {
 
 struct Module {
   hipModule_t mod = nullptr;
-  hipFunction_t single = nullptr, multi = nullptr;
+  hipFunction_t single = nullptr, multi = nullptr;   // one-shape kernels
+  hipFunction_t batch = nullptr;                     // union kernels
 };
 
 std::mutex g_mu;
@@ -97,6 +99,25 @@ void emit_body(std::ostringstream& s, const float* xf, int slot, bool full) {
   }
 }
 
+// The xform cases of one shape, numbered from `base`.
+void emit_cases(std::ostringstream& s, const float* xforms, int n_regular, int base) {
+  for (int j = 0; j < n_regular; ++j) {
+    s << "    case " << base + j << ": {\n";
+    emit_body(s, xforms + (size_t)j * p3::kXformFloats, j, true);
+    s << "    } break;\n";
+  }
+}
+
+const char* const kChainHead =
+    "  static P3_INLINE Chain chain(const ChaosArgs& A, uint32_t fn, xrec_t X, float x,\n"
+    "      float y, float z, Rng& rng, bool& dc_on, float3& dc_rgb) {\n"
+    "    (void)A; (void)dc_on; (void)dc_rgb;\n"
+    "    Chain c;\n";
+const char* const kFinalHead =
+    "  static P3_INLINE float2 final_sum(const ChaosArgs& A, xrec_t X, int, float2 pa,\n"
+    "      Rng& rng, bool& dc_on, float3& dc_rgb) {\n"
+    "    (void)A; (void)dc_on; (void)dc_rgb;\n";
+
 std::string generate(const float* xforms, int n_regular, int final_slot) {
   std::ostringstream s;
   s << "namespace p3 {\nstruct Spec {\n"
@@ -107,23 +128,14 @@ std::string generate(const float* xforms, int n_regular, int final_slot) {
     << (final_slot >= 0 ? final_slot : 0) << "; }\n"
     << "  static P3_INLINE bool final_draw_free(const ChaosArgs&) { return "
     << (final_draw_free(xforms, final_slot) ? "true" : "false") << "; }\n"
-    << "  static P3_INLINE Chain chain(uint32_t fn, xrec_t X, float x,\n"
-    << "      float y, float z, Rng& rng, bool& dc_on, float3& dc_rgb) {\n"
-    << "    (void)dc_on; (void)dc_rgb;\n"
-    << "    Chain c;\n"
+    << kChainHead
     << "    switch (fn) {\n";
-  for (int j = 0; j < n_regular; ++j) {
-    s << "    case " << j << ": {\n";
-    emit_body(s, xforms + (size_t)j * p3::kXformFloats, j, true);
-    s << "    } break;\n";
-  }
+  emit_cases(s, xforms, n_regular, 0);
   s << "    default: __builtin_unreachable();\n"
     << "    }\n"
     << "    return c;\n"
     << "  }\n"
-    << "  static P3_INLINE float2 final_sum(xrec_t X, int, float2 pa,\n"
-    << "      Rng& rng, bool& dc_on, float3& dc_rgb) {\n"
-    << "    (void)dc_on; (void)dc_rgb;\n";
+    << kFinalHead;
   if (final_slot >= 0) {
     s << "    {\n";
     emit_body(s, xforms + (size_t)final_slot * p3::kXformFloats, final_slot, false);
@@ -147,6 +159,68 @@ std::string generate(const float* xforms, int n_regular, int final_slot) {
     << "  const uint32_t k = p3::multi_find(M, blockIdx.x);\n"
     << "  const p3::ChaosArgs A = ((const P3_CONST p3::ChaosArgs*)M.args)[k];\n"
     << "  p3::chaos_binned<kBlk, p3::Spec>(A, L, blockIdx.x - M.start[k]);\n"
+    << "}\n";
+  return s.str();
+}
+
+// Union kernel of several shapes: shape u's xform j is case
+// case_base(u) + j of one switch (the dispatch's ChaosArgs carry its shape
+// index and case base), its final xform case u of a second switch. A block
+// belongs to one sub-dispatch, so both switches are block-uniform. The pick
+// path is the step-boundary one when every shape has at most 16 xforms
+// (kRegular = the largest count: a table of fewer xforms has its unused
+// boundaries at 16384, which no column reaches).
+std::string generate_union(const std::vector<p3::ShapeRef>& shapes) {
+  std::ostringstream s;
+  int n_max = 1;
+  for (const auto& sh : shapes) n_max = std::max(n_max, sh.n_regular);
+  s << "namespace p3 {\nstruct Spec {\n"
+    << "  static constexpr int kRegular = " << n_max << ";\n"
+    << "  static P3_INLINE bool has_final(const ChaosArgs& A) { return A.final_idx >= 0; }\n"
+    << "  static P3_INLINE int final_slot(const ChaosArgs& A) { return A.final_idx; }\n"
+    << "  static P3_INLINE bool final_draw_free(const ChaosArgs& A) {\n"
+    << "    switch (A.shape) {\n";
+  for (size_t u = 0; u < shapes.size(); ++u)
+    if (final_draw_free(shapes[u].xforms, shapes[u].final_slot))
+      s << "    case " << u << ":\n";
+  s << "      return true;\n"
+    << "    default: return false;\n"
+    << "    }\n"
+    << "  }\n"
+    << kChainHead
+    << "    switch (A.case_base + fn) {\n";
+  int base = 0;
+  for (const auto& sh : shapes) {
+    emit_cases(s, sh.xforms, sh.n_regular, base);
+    base += sh.n_regular;
+  }
+  s << "    default: __builtin_unreachable();\n"
+    << "    }\n"
+    << "    return c;\n"
+    << "  }\n"
+    << kFinalHead
+    << "    switch (A.shape) {\n";
+  for (size_t u = 0; u < shapes.size(); ++u) {
+    if (shapes[u].final_slot < 0) continue;
+    s << "    case " << u << ": {\n";
+    emit_body(s, shapes[u].xforms + (size_t)shapes[u].final_slot * p3::kXformFloats,
+              shapes[u].final_slot, false);
+    s << "    }\n";
+  }
+  s << "    default: (void)X; (void)rng; return pa;\n"
+    << "    }\n"
+    << "  }\n};\n}  // namespace p3\n\n"
+    << "constexpr int kBlk = " << p3::chaos_block() << ";\n"
+    // Sub-dispatch k (any shape of the union, its own records, pick table,
+    // seed, camera, walker count and histogram) owns blocks
+    // [start[k], start[k+1]) of one grid.
+    << "extern \"C\" __global__ void __launch_bounds__(kBlk)\n"
+    << "p3_chaos_batch(p3::BatchArgs M) {\n"
+    << "  __shared__ p3::BlockLds<kBlk> L;\n"
+    << "  const uint32_t k = p3::batch_find(M, blockIdx.x);\n"
+    << "  const p3::ChaosArgs A = ((const P3_CONST p3::ChaosArgs*)M.args)[k];\n"
+    << "  p3::chaos_binned<kBlk, p3::Spec>(A, L,\n"
+    << "      blockIdx.x - ((const P3_CONST uint32_t*)M.start)[k]);\n"
     << "}\n";
   return s.str();
 }
@@ -398,6 +472,66 @@ hipError_t launch_chaos_rtc_multi(const MultiArgs& margs, const float* host_xfor
   void* params[] = {&args};
   return hipModuleLaunchKernel(m.multi, grid, 1, 1, chaos_block(), 1, 1, 0,
                                stream, params, nullptr);
+}
+
+std::string union_key(const std::vector<ShapeRef>& shapes) {
+  std::string key = "U";
+  for (const auto& sh : shapes)
+    key += "|" + chaos_shape_key(sh.xforms, sh.n_regular, sh.final_slot);
+  return key;
+}
+
+std::string union_source(const std::vector<ShapeRef>& shapes) {
+  return std::string(kRtcSource) + generate_union(shapes);
+}
+
+// Code object of the union kernel, compiled on first use (no device
+// needed) and cached like the one-shape kernels.
+bool union_code(const std::vector<ShapeRef>& shapes, const std::vector<char>** out,
+                std::string& err) {
+  const std::string key = union_key(shapes);
+  {
+    std::lock_guard<std::mutex> lock(g_mu);
+    auto it = g_code.find(key);
+    if (it != g_code.end()) {
+      *out = &it->second;
+      return true;
+    }
+  }
+  std::vector<char> code;
+  if (!compile(union_source(shapes), code, err)) return false;
+  std::lock_guard<std::mutex> lock(g_mu);
+  log_shape(key);
+  *out = &g_code.emplace(key, std::move(code)).first->second;
+  return true;
+}
+
+hipError_t launch_union_batch(const BatchArgs& bargs, uint32_t n_blocks,
+                              const std::vector<ShapeRef>& shapes, hipStream_t stream,
+                              std::string& err) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  const std::vector<char>* code = nullptr;
+  if (!union_code(shapes, &code, err)) return hipErrorInvalidImage;
+  const std::string key = std::to_string(dev) + "|" + union_key(shapes);
+  Module m;
+  {
+    std::lock_guard<std::mutex> lock(g_mu);
+    auto it = g_mods.find(key);
+    if (it != g_mods.end()) {
+      m = it->second;
+    } else {
+      if ((e = hipModuleLoadData(&m.mod, code->data())) != hipSuccess) return e;
+      if ((e = hipModuleGetFunction(&m.batch, m.mod, "p3_chaos_batch")) != hipSuccess) return e;
+      g_mods.emplace(key, m);
+    }
+  }
+  if (n_blocks == 0) return hipSuccess;
+  BatchArgs args = bargs;
+  void* params[] = {&args};
+  return hipModuleLaunchKernel(m.batch, n_blocks, 1, 1, chaos_block(), 1, 1, 0, stream, params,
+                               nullptr);
 }
 
 size_t chaos_compiled_shapes() {

@@ -1,173 +1,214 @@
-"""Train the denoiser on rendered pairs (runs on one GPU).
+"""Train the learned denoiser with the hand-written BF16-MFMA training
+kernels (include/pyr3nn.h, driven by net.HipNet) on one GPU.
 
-Pairs: seeded synthetic flames of the benchmark family (12 xforms + final,
-flam3 core set; genome seeds disjoint from the ones the benchmark uses),
-rendered by this package's renderer at `--width x --height`: a quality-4000
-target and inputs at a few low qualities with independent seeds. Loss: MSE
-on RGB (the benchmark scores PSNR). Writes `--out` (.npz: one float32 array
-per weight, plus `meta` JSON) and prints a progress line per phase.
+Pairs are generated by this package's plain path from synthetic genomes of
+the 4k workload's distribution (12 xforms + final from the whole catalog,
+log-normal(0.7) affine scales), framed at 1920x1080 oversample 1:
+  target  the plain render at quality `--ref-quality` (fuse 200, DE, tonemap)
+  inputs  learned-path renders at the qualities `--qualities` (fuse
+          learned.FUSE, tonemap without DE, independent seeds), as the
+          network sees them: p3nn_features of the histogram + the base image
+Genomes whose attractor covers less than 5% of the frame are skipped. Each
+step trains on `--batch` random 256x256 crops (input, base, target) with
+MSE on RGB and Adam (warm-up, then cosine decay). The kernel shapes of the
+genomes compile in worker processes while the GPU renders the ready ones.
 
-    python -m pyr3_amd.denoise.train --out denoiser.npz
+Writes `--out` (.npz: the parameters in the W[o][kp] layout of net.py and a
+`meta` JSON string) and prints a progress line per phase and every 100 steps.
+
+    python -m pyr3_amd.denoise.train --out weights.npz
 """
 from __future__ import annotations
 
 import argparse
 import json
 import math
+import os
 import sys
 import time
+from concurrent.futures import ProcessPoolExecutor, as_completed
+from multiprocessing import get_context
 
 import numpy as np
-import torch
 
-from .. import pack, synth
-from ..renderer import Renderer, pack_genome
-from .model import Denoiser, features
+from .. import learned, pack, synth
+from . import net
 
-GENOME_BASE = 1000   # bench_genome(k) seeds the benchmark searches start at k = 0
-
-
-def render(r: Renderer, g, quality: float, seed: int, walkers: int = 131072):
-    samples = int(round(quality * r.width * r.height))
-    walkers = max(256, min(walkers, samples) // 256 * 256)
-    iters = -(-samples // walkers)
-    r.reset(g)
-    s = r.iterate(g, seed, walkers, iters, packed=pack_genome(g))
-    r.present(g, s)
-    return r.read_image_f32(), r.read_histogram()[..., 3], r.read_image()
+TRAIN_BASE = 20000    # genome seeds of the training set (the held-out set starts at 40000)
+W, H = 1920, 1080
 
 
 def log(msg):
     print(f"[train] {msg}", flush=True)
 
 
-def make_pairs(n: int, width: int, height: int, qualities, ref_q: float, dev):
-    from ..batch import precompile
-    n_genomes = -(-n // len(qualities))
-    cands = [synth.bench_genome(k, width, height)
-             for k in range(GENOME_BASE, GENOME_BASE + n_genomes * 3 // 2 + 8)]
-    t0 = time.time()
-    log(f"compiling {len(cands)} genome shapes ...")
-    precompile(cands)
-    log(f"compiled in {time.time() - t0:.0f}s")
-    r = Renderer(width, height, 1)
-    r.set_image_f32(True)
-    X, Y = [], []
-    k = GENOME_BASE
-    while len(Y) < n and k - GENOME_BASE < len(cands):
-        g = cands[k - GENOME_BASE]
-        k += 1
-        ref, cnt, _ = render(r, g, 20, 7 + k)
-        if np.mean(cnt > 0) < 0.05:
+def shape_key(g) -> str:
+    from ..kcache import key_of
+    from ..renderer import pack_genome
+    pg = pack_genome(g)
+    return key_of(pg.xforms, pg.distrib.shape[0], pack.final_xform_slot(pg.expanded))
+
+
+def compile_ahead(genomes, workers: int):
+    """Process pool compiling the genomes' kernel shapes into the code-object
+    cache; returns (pool, {future: index})."""
+    from ..kcache import CACHE, _compile_key
+    cache = os.environ.get("P3_KERNEL_CACHE") or str(CACHE)
+    ex = ProcessPoolExecutor(max_workers=workers, mp_context=get_context("spawn"))
+    futs = {}
+    seen = {}
+    for i, g in enumerate(genomes):
+        k = shape_key(g)
+        if k in seen:
             continue
-        ref, _, _ = render(r, g, ref_q, 0x9E3779B9 + k)
-        ys = torch.from_numpy(np.ascontiguousarray(ref[..., :3].transpose(2, 0, 1))).to(dev)
-        for q in qualities:
-            img, cnt, _ = render(r, g, q, 1000 * k + int(q))
-            rgb = torch.from_numpy(np.ascontiguousarray(img[..., :3].transpose(2, 0, 1))).to(dev)
-            c = torch.from_numpy(cnt.astype(np.float32)[None]).to(dev)
-            X.append(features(rgb[None], c[None])[0].to(torch.bfloat16))
-            Y.append(ys.to(torch.bfloat16))
-        if len(Y) % (10 * len(qualities)) == 0:
-            log(f"{len(Y)} pairs from {k - GENOME_BASE} genomes ({time.time() - t0:.0f}s)")
-    r.destroy()
-    return torch.stack(X), torch.stack(Y)
+        seen[k] = i
+        futs[ex.submit(_compile_key, k, cache)] = k
+    return ex, futs, seen
 
 
-def train(X, Y, steps: int, crop: int, batch: int, lr: float, dev, seed: int = 0):
-    torch.manual_seed(seed)
-    net = Denoiser().to(dev)
-    opt = torch.optim.AdamW(net.parameters(), lr=lr, weight_decay=1e-4)
-    sched = torch.optim.lr_scheduler.OneCycleLR(opt, max_lr=lr, total_steps=steps, pct_start=0.1)
-    N, _, H, W = X.shape
-    g = torch.Generator(device="cpu").manual_seed(seed)
-    t0 = time.time()
-    for step in range(steps):
-        idx = torch.randint(0, N, (batch,), generator=g)
-        ys = torch.randint(0, H - crop + 1, (batch,), generator=g)
-        xs = torch.randint(0, W - crop + 1, (batch,), generator=g)
-        xb = torch.stack([X[i, :, a:a + crop, b:b + crop] for i, a, b in zip(idx, ys, xs)])
-        yb = torch.stack([Y[i, :, a:a + crop, b:b + crop] for i, a, b in zip(idx, ys, xs)])
-        with torch.autocast("cuda", dtype=torch.bfloat16):
-            out = net(xb)
-        loss = torch.mean((out.float() - yb.float()) ** 2)
-        opt.zero_grad(set_to_none=True)
-        loss.backward()
-        opt.step()
-        sched.step()
-        if step % 100 == 0 or step == steps - 1:
-            log(f"step {step} loss {loss.item():.3e} psnr~{-10 * math.log10(max(loss.item(), 1e-12)):.2f} "
-                f"({time.time() - t0:.0f}s)")
-    return net
-
-
-def psnr8(a: np.ndarray, b: np.ndarray) -> float:
-    d = a[..., :3].astype(np.float64) - b[..., :3].astype(np.float64)
-    mse = float(np.mean(d * d))
-    return float("inf") if mse == 0 else 10 * math.log10(255.0 ** 2 / mse)
-
-
-def to_rgba8(rgb: torch.Tensor) -> np.ndarray:
-    """(3, H, W) float -> (H, W, 4) u8 like the tonemap store (round to nearest)."""
-    v = (rgb.float().clamp(0, 1) * 255.0 + 0.5).floor().to(torch.uint8)
-    a = torch.full_like(v[:1], 255)
-    return torch.cat([v, a], 0).permute(1, 2, 0).cpu().numpy()
-
-
-def evaluate(net, dev, width: int, height: int, ks, qualities, ref_q: float):
-    r = Renderer(width, height, 1)
+def learned_input(r, hn, g, quality: float, seed: int):
+    """(features (1, H, W, 8) bf16, base (1, H, W, 4) f32, coverage) of one
+    learned-path input render."""
+    import torch
+    w, it = learned.dispatch(quality, r.width, r.height)
     r.set_image_f32(True)
-    rows = []
-    for k in ks:
-        g = synth.bench_genome(k, width, height)
-        _, _, ref8 = render(r, g, ref_q, 0x9E3779B9 + k)
-        for q in qualities:
-            img, cnt, img8 = render(r, g, q, 500 + int(q))
-            rgb = torch.from_numpy(np.ascontiguousarray(img[..., :3].transpose(2, 0, 1))).to(dev)
-            c = torch.from_numpy(cnt.astype(np.float32)[None]).to(dev)
-            with torch.no_grad(), torch.autocast("cuda", dtype=torch.bfloat16):
-                out = net(features(rgb[None], c[None]))[0]
-            rows.append({"k": k, "q": q, "plain_db": round(psnr8(img8, ref8), 2),
-                         "learned_db": round(psnr8(to_rgba8(out), ref8), 2)})
-            log(str(rows[-1]))
-    r.destroy()
-    return rows
+    r.reset(g)
+    s = r.iterate(g, seed, w, it, fuse=learned.FUSE)
+    r.present(g, s, force_de_off=True)
+    hist = r.read_histogram()
+    img = r.read_image_f32()
+    _, k2 = pack.calibration(g.scale, s, g.tonemap.brightness, 1)
+    feat = hn.features(torch.from_numpy(hist.view(np.int32)).cuda(), torch.from_numpy(img).cuda(),
+                       k2)
+    cover = float(np.mean(hist[..., 3] > 0))
+    return feat, torch.from_numpy(img).cuda()[None], cover
 
 
-def main(argv=None):
+def plain_target(r, g, quality: float, seed: int):
+    import torch
+    w, it = learned.dispatch(quality, r.width, r.height)
+    r.set_image_f32(False)
+    r.reset(g)
+    s = r.iterate(g, seed, w, it, fuse=pack.FUSE)
+    r.present(g, s)
+    img = r.read_image()
+    return torch.from_numpy(img[..., :3].astype(np.float32) / 255.0).cuda()[None]
+
+
+def make_data(genomes, hn, qualities, ref_quality, workers):
+    """Renders inputs and targets of every usable genome (kernels compiled
+    in worker processes as the GPU goes)."""
+    from ..renderer import Renderer
+    ex, futs, first = compile_ahead(genomes, workers)
+    ready = set()
+    r = Renderer(W, H, 1)
+    inputs, targets = [], []   # inputs: (feat, base, target index)
+    t0 = time.time()
+    todo = list(range(len(genomes)))
+    try:
+        for f in as_completed(futs):
+            key, rc, _ = f.result()
+            if rc != 0:
+                raise RuntimeError(f"kernel compile failed for {key}")
+            ready.add(key)
+            for i in [i for i in todo if shape_key(genomes[i]) in ready]:
+                todo.remove(i)
+                g = genomes[i]
+                feat, base, cover = learned_input(r, hn, g, qualities[0], 11 + i)
+                if cover < 0.05:
+                    continue
+                targets.append(plain_target(r, g, ref_quality, 0x9E3779B9 + i))
+                ti = len(targets) - 1
+                inputs.append((feat, base, ti))
+                for j, q in enumerate(qualities[1:]):
+                    f2, b2, _ = learned_input(r, hn, g, q, 1000 * (j + 2) + i)
+                    inputs.append((f2, b2, ti))
+            if len(targets) and len(targets) % 8 == 0:
+                log(f"data: {len(targets)} genomes rendered, {len(todo)} waiting "
+                    f"({time.time() - t0:.0f}s)")
+    finally:
+        ex.shutdown()
+        r.destroy()
+    log(f"data: {len(targets)} usable genomes of {len(genomes)}, {len(inputs)} inputs "
+        f"({time.time() - t0:.0f}s)")
+    return inputs, targets
+
+
+def batch_of(inputs, targets, rng, n: int, crop: int):
+    import torch
+    fs, bs, ts = [], [], []
+    for _ in range(n):
+        feat, base, ti = inputs[int(rng.integers(len(inputs)))]
+        y = int(rng.integers(0, H - crop + 1))
+        x = int(rng.integers(0, W - crop + 1))
+        fs.append(feat[:, y:y + crop, x:x + crop])
+        bs.append(base[:, y:y + crop, x:x + crop])
+        ts.append(targets[ti][:, y:y + crop, x:x + crop])
+    return (torch.cat(fs).contiguous(), torch.cat(bs).contiguous(), torch.cat(ts).contiguous())
+
+
+def lr_at(step: int, steps: int, peak: float, warm: int = 100) -> float:
+    if step < warm:
+        return peak * (step + 1) / warm
+    t = (step - warm) / max(1, steps - warm)
+    return 1e-5 + (peak - 1e-5) * 0.5 * (1.0 + math.cos(math.pi * t))
+
+
+def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--out", default="denoiser.npz")
-    ap.add_argument("--pairs-genomes", type=int, default=120)
-    ap.add_argument("--width", type=int, default=960)
-    ap.add_argument("--height", type=int, default=540)
-    ap.add_argument("--qualities", default="3,6,12")
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--genomes", type=int, default=96)
+    ap.add_argument("--qualities", default="4,2,3,6")
     ap.add_argument("--ref-quality", type=float, default=4000)
     ap.add_argument("--steps", type=int, default=3000)
-    ap.add_argument("--crop", type=int, default=128)
-    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--crop", type=int, default=256)
     ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--eval-k", default="0")
-    ap.add_argument("--eval-q", default="3,5,8,10,15")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--workers", type=int, default=16)
+    ap.add_argument("--init", default=None, help="start from these weights (.npz)")
     a = ap.parse_args(argv)
-    dev = torch.device("cuda")
-    qs = [float(v) for v in a.qualities.split(",")]
+    import torch
+    t_start = time.time()
+    qualities = [float(q) for q in a.qualities.split(",")]
+    genomes = [synth.config2_genome(k, W, H) for k in range(TRAIN_BASE, TRAIN_BASE + a.genomes)]
+    params = net.init_params(a.seed)
+    if a.init:
+        from .weights import load
+        params = load(a.init)
+    hn = net.HipNet(params)
+    inputs, targets = make_data(genomes, hn, qualities, a.ref_quality, a.workers)
+    rng = np.random.default_rng(a.seed)
     t0 = time.time()
-    X, Y = make_pairs(a.pairs_genomes * len(qs), a.width, a.height, qs, a.ref_quality, dev)
-    log(f"data: {tuple(X.shape)} in {time.time() - t0:.0f}s")
-    net = train(X, Y, a.steps, a.crop, a.batch, a.lr, dev)
-    del X, Y
-    rows = evaluate(net, dev, 1920, 1080, [int(v) for v in a.eval_k.split(",")],
-                    [float(v) for v in a.eval_q.split(",")], a.ref_quality)
-    arrays = {f"{n}.{p}": t.detach().float().cpu().numpy() for n, m in net.convs.items()
-              for p, t in (("weight", m.weight), ("bias", m.bias))}
-    arrays["out.weight"] = net.out.weight.detach().float().cpu().numpy()
-    arrays["out.bias"] = net.out.bias.detach().float().cpu().numpy()
-    meta = {"qualities": qs, "ref_quality": a.ref_quality, "steps": a.steps,
-            "pairs_genomes": a.pairs_genomes, "size": [a.width, a.height],
-            "genome_base": GENOME_BASE, "eval": rows}
-    np.savez(a.out, meta=np.array(json.dumps(meta)), **arrays)
-    log(f"wrote {a.out} ({time.time() - t0:.0f}s total)")
+    acc, n_acc = 0.0, 0
+    history = []
+    for step in range(a.steps):
+        feat, base, target = batch_of(inputs, targets, rng, a.batch, a.crop)
+        hn.forward(feat, base)
+        loss = hn.loss_backward(target)
+        hn.adam_step(lr_at(step, a.steps, a.lr))
+        if step % 10 == 0:
+            acc += loss.item()
+            n_acc += 1
+        if (step + 1) % 100 == 0:
+            mse = acc / max(1, n_acc)
+            history.append((step + 1, mse))
+            log(f"step {step + 1}/{a.steps}: mse {mse:.3e} ({10 * math.log10(1 / mse):.2f} dB "
+                f"on crops), {(time.time() - t0) / (step + 1) * 1e3:.1f} ms/step")
+            acc, n_acc = 0.0, 0
+    torch.cuda.synchronize()
+    out = hn.params_numpy()
+    meta = {"genomes": len(targets), "inputs": len(inputs), "qualities": qualities,
+            "ref_quality": a.ref_quality, "fuse": learned.FUSE, "steps": a.steps,
+            "batch": a.batch, "crop": a.crop, "lr": a.lr, "size": [W, H],
+            "distribution": "4k workload genomes (synth.config2_genome) framed at 1920x1080",
+            "genome_seeds": [TRAIN_BASE, TRAIN_BASE + a.genomes],
+            "train_s": round(time.time() - t0, 1), "total_s": round(time.time() - t_start, 1),
+            "loss_history": history, "kernels": "pyr3nn (hand-written BF16 MFMA)"}
+    np.savez(a.out, meta=np.array(json.dumps(meta)), **out)
+    log(f"wrote {a.out} ({net.param_count()} parameters) in {time.time() - t_start:.0f}s")
+    return 0
 
 
 if __name__ == "__main__":
-    main(sys.argv[1:])
+    sys.exit(main())

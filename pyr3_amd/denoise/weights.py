@@ -1,46 +1,34 @@
-"""Flat float32 parameter vector of the denoiser in the C ABI's order."""
+"""Denoiser parameters: the .npz the trainer writes and the flat float32
+vector of the C ABI (p3_denoiser_create)."""
 from __future__ import annotations
 
 from pathlib import Path
+from typing import Dict
 
 import numpy as np
 
-from .model import LAYERS
+from .net import HEAD, LAYERS, to_torch_weight
 
 DEFAULT = Path(__file__).resolve().parent / "weights.npz"
 
 
-def flat_params(arrays) -> np.ndarray:
-    """arrays: mapping name -> array (an .npz from train.py, or a state dict
-    converted with .numpy()). Order: each 3x3 layer's weight then bias, then
-    the head's weight and bias."""
+def load(path=DEFAULT) -> Dict[str, np.ndarray]:
+    """name -> float32 array (W[o][kp] per layer as `<name>.w`, `<name>.b`,
+    `head`), read without unpickling."""
+    with np.load(path, allow_pickle=False) as z:
+        return {k: np.asarray(z[k], np.float32) for k in z.files if k != "meta"}
+
+
+def flat_params(params: Dict[str, np.ndarray]) -> np.ndarray:
+    """The C ABI's order: per 3x3 layer its weight [c_out][c_in][3][3] then
+    its bias, then the head's weight [3][16] and bias."""
     parts = []
-    for n, ci, co, _ in LAYERS:
-        w = np.asarray(arrays[f"{n}.weight"], dtype=np.float32)
-        assert w.shape == (co, ci, 3, 3), (n, w.shape)
-        parts += [w.reshape(-1), np.asarray(arrays[f"{n}.bias"], dtype=np.float32).reshape(-1)]
-    parts += [np.asarray(arrays["out.weight"], dtype=np.float32).reshape(-1),
-              np.asarray(arrays["out.bias"], dtype=np.float32).reshape(-1)]
+    for L in LAYERS:
+        w = np.asarray(params[L.name + ".w"], np.float32)
+        assert w.shape == (L.cout, L.kp), (L.name, w.shape)
+        parts += [to_torch_weight(w, L).numpy().reshape(-1),
+                  np.asarray(params[L.name + ".b"], np.float32).reshape(-1)]
+    head = np.asarray(params["head"], np.float32).reshape(-1)
+    assert head.size == HEAD
+    parts.append(head)
     return np.concatenate(parts)
-
-
-def from_module(net) -> np.ndarray:
-    arrays = {f"{n}.{p}": getattr(m, p).detach().float().cpu().numpy()
-              for n, m in net.convs.items() for p in ("weight", "bias")}
-    arrays["out.weight"] = net.out.weight.detach().float().cpu().numpy()
-    arrays["out.bias"] = net.out.bias.detach().float().cpu().numpy()
-    return flat_params(arrays)
-
-
-def load_module(cls, path=DEFAULT):
-    """An eval-mode network of class `cls` holding the weights of `path`."""
-    import torch
-    arrays = np.load(path, allow_pickle=False)
-    net = cls()
-    with torch.no_grad():
-        for n, m in net.convs.items():
-            m.weight.copy_(torch.from_numpy(np.asarray(arrays[f"{n}.weight"], np.float32)))
-            m.bias.copy_(torch.from_numpy(np.asarray(arrays[f"{n}.bias"], np.float32)))
-        net.out.weight.copy_(torch.from_numpy(np.asarray(arrays["out.weight"], np.float32)))
-        net.out.bias.copy_(torch.from_numpy(np.asarray(arrays["out.bias"], np.float32)))
-    return net.eval()

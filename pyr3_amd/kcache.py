@@ -19,7 +19,9 @@ those counts, flags and indices (zero weights and parameters) compiles to the
 same code object as any genome of that shape. A process started with
 ``P3_SHAPE_LOG=<file>`` appends the key of each shape it compiles or loads,
 which is how ``kernel_shapes.txt`` (the shapes of the smoke test, the GPU
-tests and the benchmark) is collected.
+tests and the benchmark) is collected. A batch's union kernel (several
+shapes in one kernel, p3_batch) is keyed ``U|<shape>|<shape>|...`` with its
+shapes in case order.
 
 ``python -m pyr3_amd.kcache [manifest ...]`` compiles the manifest (default
 ``kernel_shapes.txt``) into the cache.
@@ -104,13 +106,23 @@ def read_manifest(paths: Iterable[Path]) -> List[str]:
 
 
 def _compile_key(key: str, cache: str) -> Tuple[str, int, float]:
-    """Worker process: compile one shape into the cache."""
+    """Worker process: compile one shape (or one union kernel, key
+    ``U|<shape>|<shape>...``) into the cache."""
     os.environ["P3_KERNEL_CACHE"] = cache
     os.environ.pop("P3_SHAPE_LOG", None)
     from . import _lib
     lib = _lib.load()
-    xf, n_rows, final_slot = blobs_from_key(key)
     t0 = time.perf_counter()
+    if key.startswith("U|"):
+        parts = [blobs_from_key(k) for k in key.split("|")[1:]]
+        n = len(parts)
+        xs = [np.ascontiguousarray(p[0]) for p in parts]
+        arr = (C.POINTER(C.c_float) * n)(*[x.ctypes.data_as(C.POINTER(C.c_float)) for x in xs])
+        rc = lib.p3_union_prepare(n, arr, (C.c_int * n)(*[x.shape[0] for x in xs]),
+                                  (C.c_int * n)(*[p[1] for p in parts]),
+                                  (C.c_int * n)(*[p[2] for p in parts]))
+        return key, rc, time.perf_counter() - t0
+    xf, n_rows, final_slot = blobs_from_key(key)
     rc = lib.p3_chaos_prepare(xf.ctypes.data_as(C.POINTER(C.c_float)), xf.shape[0], n_rows,
                               final_slot)
     return key, rc, time.perf_counter() - t0

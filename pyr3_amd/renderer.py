@@ -284,7 +284,9 @@ class Renderer:
 
 
 class Denoiser:
-    """The learned denoiser on the GPU (BF16 MFMA kernels)."""
+    """The learned denoiser on the GPU (BF16 MFMA kernels): corrects a
+    renderer's low-quality frame presented without DE and with the f32 image
+    enabled."""
 
     def __init__(self, params: np.ndarray):
         self._lib = load()
@@ -297,12 +299,13 @@ class Denoiser:
         return int(load().p3_denoiser_param_count())
 
     @classmethod
-    def from_npz(cls, path) -> "Denoiser":
-        from .denoise.weights import flat_params
-        return cls(flat_params(np.load(path, allow_pickle=False)))
+    def from_npz(cls, path=None) -> "Denoiser":
+        from .denoise import weights
+        return cls(weights.flat_params(weights.load(path or weights.DEFAULT)))
 
     def apply(self, r: Renderer) -> None:
-        """Denoise the renderer's last present (needs set_image_f32(True))."""
+        """Denoise the renderer's last present (DE off, f32 image enabled)
+        into its RGBA8 image."""
         check(self._lib.p3_denoise(r._h, self._h), r._h)
 
     def destroy(self) -> None:

@@ -128,3 +128,37 @@ def config2_genome(k: int = 0, width: int = 3840, height: int = 2160) -> Genome:
     return make_genome(rng, 12, implemented_pool(), counts, width, height, 200, 2,
                        post_fraction=0.5, final=True, lognormal_sigma=0.7,
                        name=f"4k-{k}")
+
+
+def redraw(g: Genome, seed: int, lognormal_sigma: float = 0.7) -> Genome:
+    """A genome of the same kernel shape as `g` (the same xforms, variation
+    indices per slot, post-affine presence and final xform) with its other
+    parameters drawn anew as make_genome draws them: affine coefficients
+    (regular xforms' linear parts scaled by a log-normal factor, as the 4k
+    workload's), xform weights and colours, variation weights and parameters
+    (entries past the flam3 core set keep their defaults), the palette."""
+    import copy
+    rng = np.random.default_rng(seed)
+    h = copy.deepcopy(g)
+    xs = list(h.xforms) + ([h.finalxform] if h.finalxform is not None else [])
+    for i, x in enumerate(xs):
+        final = h.finalxform is not None and i == len(xs) - 1
+        x.affine = _affine(rng)
+        if not final:
+            x.weight = float(rng.uniform(0.2, 1.0))
+            if lognormal_sigma > 0:
+                sc = float(rng.lognormal(0.0, lognormal_sigma))
+                x.affine.a *= sc; x.affine.b *= sc; x.affine.d *= sc; x.affine.e *= sc
+        if x.post is not None:
+            x.post = _affine(rng)
+        x.color = float(rng.random())
+        ws = rng.dirichlet(np.ones(len(x.variations))) if not final else [1.0] * len(x.variations)
+        for v, w in zip(x.variations, ws):
+            v.weight = float(w)
+            if v.index <= 98:
+                for j in range(len(v.params)):
+                    if v.params[j] != 0.0:
+                        v.params[j] = float(rng.uniform(-2.0, 2.0))
+    h.palette = random_palette(rng)
+    h.name = f"{g.name}-redraw{seed}"
+    return h

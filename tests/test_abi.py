@@ -6,8 +6,8 @@ from conftest import ROOT
 
 
 def _declared():
-    text = (ROOT / "include" / "pyr3hip.h").read_text()
-    return sorted(set(re.findall(r"\b(p3_[a-z0-9_]+)\s*\(", text)))
+    text = "".join((ROOT / "include" / h).read_text() for h in ("pyr3hip.h", "pyr3nn.h"))
+    return sorted(set(re.findall(r"\b(p3(?:nn)?_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_header_declares_the_stage_interface():

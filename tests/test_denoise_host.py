@@ -1,57 +1,66 @@
-"""Denoiser host side (CPU): parameter layout shared by the network
-definition, the weight flattener and the C ABI."""
+"""Denoiser host side (CPU): the network definition, the parameter layout
+shared by the trainer's .npz, the C ABI's flat vector and the PyTorch
+restatement."""
 import numpy as np
+import pytest
 import torch
 
-from pyr3_amd.denoise.model import LAYERS, Denoiser, macs_per_pixel
-from pyr3_amd.denoise.weights import flat_params, from_module
+from pyr3_amd.denoise import net
+from pyr3_amd.denoise.weights import DEFAULT, flat_params, load
 
 
 def test_param_count_matches_c_abi():
     from pyr3_amd import _lib
-    net = Denoiser()
-    n = sum(p.numel() for p in net.parameters())
-    assert _lib.load().p3_denoiser_param_count() == n == from_module(net).size
+    p = net.init_params(0)
+    assert _lib.load().p3_denoiser_param_count() == net.param_count() == flat_params(p).size
+    assert 480_000 < net.param_count() < 500_000     # ~0.5 M parameters
 
 
 def test_flat_order_is_layer_weight_then_bias():
-    torch.manual_seed(0)
-    net = Denoiser()
-    flat = from_module(net)
+    p = net.init_params(3)
+    flat = flat_params(p)
     off = 0
-    for name, ci, co, _ in LAYERS:
-        w = net.convs[name].weight.detach().numpy().reshape(-1)
-        assert np.array_equal(flat[off:off + w.size], w)
-        off += w.size
-        b = net.convs[name].bias.detach().numpy()
-        assert np.array_equal(flat[off:off + co], b)
-        off += co
-    assert off + 48 + 3 == flat.size
+    for L in net.LAYERS:
+        w = net.to_torch_weight(p[L.name + ".w"], L)
+        assert w.shape == (L.cout, L.cin, 3, 3)
+        # tap t = 3 (dy + 1) + (dx + 1) of input channel c is column t * cin + c
+        assert w[1, L.cin - 1, 2, 0] == p[L.name + ".w"][1, 6 * L.cin + L.cin - 1]
+        assert np.array_equal(flat[off:off + w.numel()], w.numpy().reshape(-1))
+        off += w.numel()
+        assert np.array_equal(flat[off:off + L.cout], p[L.name + ".b"])
+        off += L.cout
+    assert off + net.HEAD == flat.size
 
 
-def test_network_shapes_and_cost():
-    net = Denoiser()
-    for h, w in ((67, 93), (64, 64), (1, 3)):
-        assert net(torch.zeros(1, 4, h, w)).shape == (1, 3, h, w)
-    assert 25_000 < macs_per_pixel() < 32_000
+def test_network_shapes_levels_and_cost():
+    p = net.init_params(1)
+    for h, w in ((67, 93), (64, 64), (9, 3)):
+        out = net.torch_forward(p, torch.zeros(1, net.FEAT_CH, h, w), torch.zeros(1, 3, h, w))
+        assert out.shape == (1, 3, h, w)
+    assert net.level_sizes(1080, 1920) == [(1080, 1920), (540, 960), (270, 480), (135, 240)]
+    assert net.level_sizes(67, 93)[3] == (9, 12)
+    assert 40_000 < net.macs_per_pixel() < 42_000
 
 
-def test_zero_head_is_identity():
-    net = Denoiser()    # the head starts at zero: output = input RGB
-    x = torch.rand(1, 4, 20, 30)
-    assert torch.equal(net(x), x[:, :3])
+def test_zero_head_is_the_base_image():
+    p = net.init_params(2)      # the head starts at zero: output = base RGB
+    base = torch.rand(1, 3, 20, 30)
+    assert torch.equal(net.torch_forward(p, torch.rand(1, net.FEAT_CH, 20, 30), base), base)
+
+
+def test_padding_columns_are_zero():
+    p = net.init_params(4)
+    for L in net.LAYERS:
+        assert not p[L.name + ".w"][:, 9 * L.cin:].any()
 
 
 def test_shipped_weights_load():
-    from pathlib import Path
+    if not DEFAULT.exists():
+        pytest.skip("no trained weights in the tree")
     from pyr3_amd import _lib
-    path = Path(__file__).resolve().parents[1] / "pyr3_amd" / "denoise" / "weights.npz"
-    z = np.load(path, allow_pickle=False)
-    flat = flat_params(z)
+    p = load()
+    flat = flat_params(p)
     assert flat.size == _lib.load().p3_denoiser_param_count()
     assert np.isfinite(flat).all() and np.abs(flat).max() < 100
-    net = Denoiser()
-    net.load_state_dict({**{f"convs.{n}.{p}": torch.from_numpy(z[f"{n}.{p}"])
-                            for n, _, _, _ in LAYERS for p in ("weight", "bias")},
-                         "out.weight": torch.from_numpy(z["out.weight"]),
-                         "out.bias": torch.from_numpy(z["out.bias"])})
+    for L in net.LAYERS:
+        assert not p[L.name + ".w"][:, 9 * L.cin:].any()

@@ -190,31 +190,36 @@ def test_config2_low_opacity_takes_the_wide_de_branch(orc, R):
 
 
 def test_config3_batch_sample_matches_single_renders(R):
+    """Eight jobs of the 512-genome batch (four motion-blurred with 8
+    sub-frames) as one union-kernel launch on a captured graph, at the
+    batch's default plan: every frame equals a renderer's render of it."""
     from pyr3_amd import batch
     jobs = batch.batch_workload(512)
     sample = [jobs[i] for i in SAMPLE_JOBS]
     assert sum(j.subframes is not None for j in sample) == 4
-    br = batch.BatchRenderer(n_streams=2)
-    got = br.render(sample, read_images=True)
-    br.close()
-    for j, res in zip(sample, got):
-        r = R(j.width, j.height, 1)
-        walkers, iters = batch.gpu_dispatch(j.quality, j.width, j.height)
-        r.reset(j.genome)
-        if j.subframes:
-            k = len(j.subframes)
-            counts = batch.apportion(walkers, [1.0] * k)
-            s = 0
-            for i in range(k):
-                if counts[i]:
-                    s += r.iterate(j.subframes[i], (j.seed + i * batch.SUBFRAME_SEED_STEP)
-                                   & 0xFFFFFFFF, counts[i], iters)
-        else:
-            s = r.iterate(j.genome, j.seed, walkers, iters)
-        r.present(j.genome, s)
-        assert res.samples == s
-        assert np.array_equal(res.image, r.read_image()), j.name
-        r.destroy()
+    gb = batch.GraphBatch(sample, batch.BatchPlan())
+    try:
+        gb.launch()
+        gb.sync()
+        for i, j in enumerate(sample):
+            r = R(j.width, j.height, 1)
+            r.reset(j.genome)
+            walkers, iters = gb.walkers[i], gb.iters[i]
+            if j.subframes:
+                k = len(j.subframes)
+                counts = batch.apportion(walkers, [1.0] * k)
+                s = r.iterate_multi(j.subframes,
+                                    [(j.seed + q * batch.SUBFRAME_SEED_STEP) & 0xFFFFFFFF
+                                     for q in range(k)], counts, iters)
+            else:
+                s = r.iterate(j.genome, j.seed, walkers, iters)
+            r.present(j.genome, s)
+            assert gb.samples[i] == s
+            assert np.array_equal(gb.read_histogram(i), r.read_histogram()), j.name
+            assert np.array_equal(gb.read_image(i), r.read_image()), j.name
+            r.destroy()
+    finally:
+        gb.close()
 
 
 # jobs of the 512-genome batch in this test: four motion-blurred (8
@@ -223,32 +228,30 @@ SAMPLE_JOBS = (0, 1, 2, 3, 256, 257, 258, 259)
 
 
 def test_config4_learned_frame_1080p():
-    import torch
-    from pyr3_amd.denoise.model import Denoiser as Net, features
-    from pyr3_amd.denoise.weights import load_module
+    """The 1080p learned frame (q4 input, short fuse, DE off, the shipped
+    weights) against the network in PyTorch FP32 on the same features."""
+    from pyr3_amd import learned
+    from pyr3_amd.denoise.weights import flat_params, load
     from pyr3_amd.renderer import Denoiser, Renderer
+    from test_gpu_denoise import reference_output
+    if not WEIGHTS.exists():
+        pytest.skip("no trained weights in the tree")
     W, H = 1920, 1080
     g = synth.bench_genome(0, W, H)
-    walkers = 131072
-    iters = -(-10 * W * H // walkers)
     r = Renderer(W, H, 1)
     r.set_image_f32(True)
+    walkers, iters = learned.dispatch(4, W, H)
     r.reset(g)
-    s = r.iterate(g, 0x4C, walkers, iters, fuse=pack.FUSE)
-    r.present(g, s)
-    img = r.read_image_f32()
-    cnt = r.read_histogram()[..., 3].astype(np.float32)
-    net = load_module(Net)
-    d = Denoiser.from_npz(WEIGHTS)
+    s = r.iterate(g, 0x4C, walkers, iters, fuse=learned.FUSE)
+    r.present(g, s, force_de_off=True)
+    p = load()
+    torch.set_num_threads(THREADS)
+    ref, _ = reference_output(p, r, g, s)
+    d = Denoiser(flat_params(p))
     d.apply(r)
     got = r.read_image()
     d.destroy()
     r.destroy()
-    torch.set_num_threads(THREADS)
-    x = features(torch.from_numpy(img[..., :3].transpose(2, 0, 1).copy())[None],
-                 torch.from_numpy(cnt)[None, None])
-    with torch.no_grad():
-        ref = net(x)[0].clamp(0, 1).numpy().transpose(1, 2, 0)
     want = np.floor(ref * 255.0 + 0.5).astype(np.int32)
     diff = np.abs(got[..., :3].astype(np.int32) - want)
     assert diff.max() <= 3 and diff.mean() < 0.35, (diff.max(), diff.mean())

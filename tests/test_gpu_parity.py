@@ -384,33 +384,3 @@ def test_batched_iterate_rejects_mixed_shapes(R):
     r.reset(a)
     with pytest.raises((P3Error, ValueError)):
         r.iterate_multi([a, b], [1, 2], [64, 64], 10)
-
-
-def test_batch_renderer_matches_single_renders(R):
-    """Jobs of mixed sizes and shapes through the batch driver (process-pool
-    compiles, two alternating renderers, motion-blur sub-frames in one
-    batched launch) give the same images as one renderer per job."""
-    from pyr3_amd import batch
-    jobs = batch.batch_workload(n=6, seed=77, motion_fraction=0.34, subframes=4)
-    for j in jobs:
-        j.quality = 4.0
-    br = batch.BatchRenderer(n_streams=2, compile_workers=4)
-    res = br.render(jobs, read_images=True)
-    br.close()
-    for j, out in zip(jobs, res):
-        r = R(j.width, j.height, j.oversample)
-        walkers, iters = batch.gpu_dispatch(j.quality, j.width, j.height)
-        r.reset(j.genome)
-        if j.subframes:
-            counts = batch.apportion(walkers, [1.0] * len(j.subframes))
-            s = 0
-            for i, g in enumerate(j.subframes):
-                if counts[i]:
-                    s += r.iterate(g, (j.seed + i * batch.SUBFRAME_SEED_STEP) & 0xFFFFFFFF,
-                                   counts[i], iters)
-        else:
-            s = r.iterate(j.genome, j.seed, walkers, iters)
-        r.present(j.genome, s)
-        assert s == out.samples
-        assert np.array_equal(r.read_image(), out.image), j.name
-        r.destroy()

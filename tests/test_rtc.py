@@ -96,16 +96,20 @@ def test_shape_key_rebuilds_the_same_kernel(k):
 
 def test_manifest_keys_parse():
     """Every line of the shipped manifest names a valid shape of implemented
-    entries."""
+    entries, or a union kernel (``U|shape|shape...``) of such shapes."""
     from pyr3_amd import kcache
     keys = kcache.read_manifest([kcache.MANIFEST])
     assert len(keys) >= 10
-    for key in keys:
-        xf, n_rows, fs = kcache.blobs_from_key(key)
-        assert kcache.key_of(xf, n_rows, fs) == key
-        for rec in xf:
-            for s in range(int(min(rec[7], 8))):
-                assert int(rec[20 + 4 * s]) in SUPPORTED
+    assert any(k.startswith("U|") for k in keys)
+    for line in keys:
+        parts = line.split("|")[1:] if line.startswith("U|") else [line]
+        assert parts
+        for key in parts:
+            xf, n_rows, fs = kcache.blobs_from_key(key)
+            assert kcache.key_of(xf, n_rows, fs) == key
+            for rec in xf:
+                for s in range(int(min(rec[7], 8))):
+                    assert int(rec[20 + 4 * s]) in SUPPORTED
 
 
 def test_chaos_block_matches_the_host_schedule():
