@@ -631,6 +631,9 @@ def main():
                              "unit": "GB/s", "frac": roof["tonemap"]["frac"]},
     }
     if learned is not None:
+        if learned.get("reached"):
+            # the north-star ratio: learned frame time / plain frame time
+            learned["vs_plain"] = round(learned["ms_per_frame"] / ms, 4)
         out["learned"] = learned
     if configs is not None:
         out["configs"] = configs
