@@ -27,9 +27,6 @@
 namespace p3 {
 
 constexpr int kTile = 16;
-constexpr int kLdsRadius = 16;                        // staged-halo limit
-constexpr int kLdsSpan = kTile + 2 * kLdsRadius;      // 48
-constexpr int kKwStride = kLdsRadius * kLdsRadius + 1;  // d^2 = 0 .. 256 per radius
 constexpr uint8_t kEmpty = 255;
 
 __global__ void __launch_bounds__(256) de_prep(DeArgs A) {
@@ -81,10 +78,26 @@ __device__ inline float de_kw(int d2, int r) {
   return expf(-(float)d2 / (2.0f * sigma * sigma));
 }
 
+// Per 16 x 16 output tile with search radius Rt <= kR: the (16 + 2 Rt)^2
+// neighbourhood (pre-scaled RGBA, radius) is staged in LDS, and every staged
+// row gets the list of its occupied columns (in column order) and their
+// prefix counts, so an output visits only the occupied bins of its
+// (2 Rt + 1)^2 box, row by row in column order: the same bins in the same
+// order as a scan of the whole box that skips empty bins, so the sums are
+// bit-identical, while sparse regions (large radii, few occupied bins) cost
+// a fraction of the box. The Gaussian weights exp(-d^2 / 2 sigma^2) depend
+// only on (d^2, r): they are tabulated per block for r <= Rt with the same
+// expression. Tiles with Rt > kR read HBM directly.
+template <int kR>
 __global__ void __launch_bounds__(256) de_gather(DeArgs A) {
-  __shared__ float4 s_pre[kLdsSpan * kLdsSpan];
-  __shared__ uint8_t s_rad[kLdsSpan * kLdsSpan];
-  __shared__ float s_kw[(kLdsRadius + 1) * kKwStride];  // de_kw(d^2, r) for d^2 <= r^2
+  constexpr int kSpan = kTile + 2 * kR;
+  constexpr int kKw = kR * kR + 1;                 // d^2 = 0 .. kR^2 per radius
+  static_assert(kSpan < 64, "one wave lists a staged row and its total");
+  __shared__ float4 s_pre[kSpan * kSpan];
+  __shared__ uint8_t s_rad[kSpan * kSpan];
+  __shared__ uint8_t s_col[kSpan * kSpan];          // occupied columns, per row
+  __shared__ uint16_t s_cnt[kSpan * (kSpan + 1)];   // occupied columns left of c, per row
+  __shared__ float s_kw[(kR + 1) * kKw];
   __shared__ int s_rt;
   const int tid = threadIdx.y * kTile + threadIdx.x;
   const int x0 = blockIdx.x * kTile, y0 = blockIdx.y * kTile;
@@ -95,7 +108,7 @@ __global__ void __launch_bounds__(256) de_gather(DeArgs A) {
   const int W = (int)A.width, H = (int)A.height;
   float4 sum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 
-  if (Rt <= kLdsRadius) {
+  if (Rt <= kR) {
     const int span = kTile + 2 * Rt;
     const int kws = Rt * Rt + 1;
     for (int k = tid; k < (Rt + 1) * kws; k += 256) {
@@ -110,20 +123,34 @@ __global__ void __launch_bounds__(256) de_gather(DeArgs A) {
       if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
         const size_t g = (size_t)gy * W + gx;
         r8 = A.rad[g];
-        p = A.pre[g];
+        if (r8 != kEmpty) p = A.pre[g];
       }
       s_rad[k] = r8;
       s_pre[k] = p;
     }
     __syncthreads();
+    {  // one wave per staged row: ballot of the occupied columns
+      const int lane = tid & 63, wave = tid >> 6;
+      for (int ly = wave; ly < span; ly += 4) {
+        const bool occ = lane < span && s_rad[ly * span + lane] != kEmpty;
+        const uint64_t m = __ballot(occ);
+        const int before = __builtin_popcountll(m & ((1ull << lane) - 1ull));
+        if (occ) s_col[ly * span + before] = (uint8_t)lane;
+        if (lane <= span) s_cnt[ly * (span + 1) + lane] = (uint16_t)before;
+      }
+    }
+    __syncthreads();
     if (xi < W && yi < H) {
       const int cx = threadIdx.x + Rt, cy = threadIdx.y + Rt;
       for (int dy = -Rt; dy <= Rt; ++dy) {
-        const int rowk = (cy + dy) * span + cx;
-        for (int dx = -Rt; dx <= Rt; ++dx) {
-          const int k = rowk + dx;
+        const int ly = cy + dy;
+        const uint16_t* cnt = s_cnt + ly * (span + 1);
+        const int j1 = cnt[cx + Rt + 1];
+        for (int j = cnt[cx - Rt]; j < j1; ++j) {
+          const int lx = s_col[ly * span + j];
+          const int k = ly * span + lx;
           const int r = s_rad[k];
-          if (r == kEmpty) continue;
+          const int dx = lx - cx;
           const int d2 = dx * dx + dy * dy;
           if (d2 > r * r) continue;
           // r <= R (rint of a radius clamped to max_rad <= R) and every bin of the
@@ -161,11 +188,17 @@ __global__ void __launch_bounds__(256) de_gather(DeArgs A) {
   if (xi < W && yi < H) A.out[(size_t)yi * W + xi] = sum;
 }
 
+// The gather's LDS radius follows the search radius R: the smallest staged
+// halo that holds every tile (9: oversample 1 at the default 9-pixel radius,
+// 6 blocks per CU; 18: oversample 2, 2 blocks per CU); tiles whose radius
+// exceeds 18 read HBM.
 hipError_t launch_density(const DeArgs& a, hipStream_t stream) {
   const dim3 block(kTile, kTile);
   const dim3 grid(a.tiles_x, a.tiles_y);
   de_prep<<<grid, block, 0, stream>>>(a);
-  de_gather<<<grid, block, 0, stream>>>(a);
+  if (a.R <= 9) de_gather<9><<<grid, block, 0, stream>>>(a);
+  else if (a.R <= 16) de_gather<16><<<grid, block, 0, stream>>>(a);
+  else de_gather<18><<<grid, block, 0, stream>>>(a);
   return hipGetLastError();
 }
 

@@ -230,6 +230,7 @@ SAMPLE_JOBS = (0, 1, 2, 3, 256, 257, 258, 259)
 def test_config4_learned_frame_1080p():
     """The 1080p learned frame (q4 input, short fuse, DE off, the shipped
     weights) against the network in PyTorch FP32 on the same features."""
+    import torch
     from pyr3_amd import learned
     from pyr3_amd.denoise.weights import flat_params, load
     from pyr3_amd.renderer import Denoiser, Renderer
