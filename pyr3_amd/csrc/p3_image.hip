@@ -79,29 +79,35 @@ __device__ inline float de_kw(int d2, int r) {
 }
 
 // Per 16 x 16 output tile with search radius Rt <= kR: the (16 + 2 Rt)^2
-// neighbourhood (pre-scaled RGBA, radius) is staged in LDS, and every staged
-// row gets the list of its occupied columns (in column order) and their
-// prefix counts, so an output visits only the occupied bins of its
-// (2 Rt + 1)^2 box, row by row in column order: the same bins in the same
-// order as a scan of the whole box that skips empty bins, so the sums are
-// bit-identical, while sparse regions (large radii, few occupied bins) cost
-// a fraction of the box. The Gaussian weights exp(-d^2 / 2 sigma^2) depend
-// only on (d^2, r): they are tabulated per block for r <= Rt with the same
-// expression. Tiles with Rt > kR read HBM directly.
-template <int kR>
+// neighbourhood (pre-scaled RGBA, radius) is staged in LDS, and each output
+// sums kw * pre over the occupied bins of its (2 Rt + 1)^2 box in dy-then-dx
+// order (the reference's order, empty bins skipped). With kLists, a tile
+// whose staged neighbourhood is mostly empty (under a quarter occupied: the
+// large radii of sparse regions) walks, per row, a list of its occupied
+// columns (one wave ballots each staged row into the list and its prefix
+// counts); other tiles scan the box. Both walks visit the same bins in the same order, so
+// the sums are bit-identical. The Gaussian weights exp(-d^2 / 2 sigma^2)
+// depend only on (d^2, r): they are tabulated per block for r <= Rt with the
+// same expression. Tiles with Rt > kR read HBM directly.
+template <int kR, bool kLists>
 __global__ void __launch_bounds__(256) de_gather(DeArgs A) {
   constexpr int kSpan = kTile + 2 * kR;
   constexpr int kKw = kR * kR + 1;                 // d^2 = 0 .. kR^2 per radius
+  constexpr int kL = kLists ? kSpan : 1;           // list arrays (none without lists)
   static_assert(kSpan < 64, "one wave lists a staged row and its total");
   __shared__ float4 s_pre[kSpan * kSpan];
   __shared__ uint8_t s_rad[kSpan * kSpan];
-  __shared__ uint8_t s_col[kSpan * kSpan];          // occupied columns, per row
-  __shared__ uint16_t s_cnt[kSpan * (kSpan + 1)];   // occupied columns left of c, per row
+  __shared__ uint8_t s_col[kL * kL];                // occupied columns, per row
+  __shared__ uint16_t s_cnt[kL * (kL + 1)];         // occupied columns left of c, per row
   __shared__ float s_kw[(kR + 1) * kKw];
   __shared__ int s_rt;
+  __shared__ uint32_t s_occ;                        // occupied bins staged
   const int tid = threadIdx.y * kTile + threadIdx.x;
   const int x0 = blockIdx.x * kTile, y0 = blockIdx.y * kTile;
-  if (tid == 0) s_rt = tile_bound(A, blockIdx.x, blockIdx.y);
+  if (tid == 0) {
+    s_rt = tile_bound(A, blockIdx.x, blockIdx.y);
+    s_occ = 0u;
+  }
   __syncthreads();
   const int Rt = s_rt;
   const int xi = x0 + threadIdx.x, yi = y0 + threadIdx.y;
@@ -115,6 +121,7 @@ __global__ void __launch_bounds__(256) de_gather(DeArgs A) {
       const int r = k / kws, d2 = k - r * kws;
       s_kw[k] = d2 <= r * r ? de_kw(d2, r) : 0.0f;
     }
+    uint32_t occupied = 0u;   // counted only where the lists are built
     for (int k = tid; k < span * span; k += 256) {
       const int ly = k / span, lx = k - ly * span;
       const int gx = x0 - Rt + lx, gy = y0 - Rt + ly;
@@ -125,11 +132,14 @@ __global__ void __launch_bounds__(256) de_gather(DeArgs A) {
         r8 = A.rad[g];
         if (r8 != kEmpty) p = A.pre[g];
       }
+      if (kLists) occupied += r8 != kEmpty ? 1u : 0u;
       s_rad[k] = r8;
       s_pre[k] = p;
     }
+    if (kLists && occupied) atomicAdd(&s_occ, occupied);
     __syncthreads();
-    {  // one wave per staged row: ballot of the occupied columns
+    const bool sparse = kLists && 4u * s_occ < (uint32_t)(span * span);   // block-uniform
+    if (sparse) {  // one wave per staged row: ballot of the occupied columns
       const int lane = tid & 63, wave = tid >> 6;
       for (int ly = wave; ly < span; ly += 4) {
         const bool occ = lane < span && s_rad[ly * span + lane] != kEmpty;
@@ -138,9 +148,27 @@ __global__ void __launch_bounds__(256) de_gather(DeArgs A) {
         if (occ) s_col[ly * span + before] = (uint8_t)lane;
         if (lane <= span) s_cnt[ly * (span + 1) + lane] = (uint16_t)before;
       }
+      __syncthreads();
     }
-    __syncthreads();
-    if (xi < W && yi < H) {
+    if (xi < W && yi < H && !sparse) {
+      const int cx = threadIdx.x + Rt, cy = threadIdx.y + Rt;
+      for (int dy = -Rt; dy <= Rt; ++dy) {
+        const int rowk = (cy + dy) * span + cx;
+        for (int dx = -Rt; dx <= Rt; ++dx) {
+          const int k = rowk + dx;
+          const int r = s_rad[k];
+          if (r == kEmpty) continue;
+          const int d2 = dx * dx + dy * dy;
+          if (d2 > r * r) continue;
+          const float kw = r <= Rt ? s_kw[r * kws + d2] : de_kw(d2, r);
+          const float4 p = s_pre[k];
+          sum.x = sum.x + p.x * kw;
+          sum.y = sum.y + p.y * kw;
+          sum.z = sum.z + p.z * kw;
+          sum.w = sum.w + p.w * kw;
+        }
+      }
+    } else if (kLists && xi < W && yi < H) {
       const int cx = threadIdx.x + Rt, cy = threadIdx.y + Rt;
       for (int dy = -Rt; dy <= Rt; ++dy) {
         const int ly = cy + dy;
@@ -188,17 +216,20 @@ __global__ void __launch_bounds__(256) de_gather(DeArgs A) {
   if (xi < W && yi < H) A.out[(size_t)yi * W + xi] = sum;
 }
 
-// The gather's LDS radius follows the search radius R: the smallest staged
-// halo that holds every tile (9: oversample 1 at the default 9-pixel radius,
-// 6 blocks per CU; 18: oversample 2, 2 blocks per CU); tiles whose radius
-// exceeds 18 read HBM.
+// The gather's LDS halo follows the search radius R (9 os + 1 super-pixels
+// for a 9-pixel DE radius): 10 without the lists (oversample 1: 26 KB of
+// LDS, 6 blocks per CU) or 16 with them (2 blocks per CU; tiles with
+// Rt > 16 read HBM). Measured on the 1080p q30 frame (R = 10): 177 us at a
+// 10 halo against 422 at 16 (204 us at 10 with the lists: their LDS costs a
+// block per CU and the frame's tiles are mostly dense); on the 1024x768 os2
+// q200 frame (R = 19): 2.21 ms at 16 with the lists against 3.00 without
+// them, while a 19 halo (1 block per CU) took 4.34 ms.
 hipError_t launch_density(const DeArgs& a, hipStream_t stream) {
   const dim3 block(kTile, kTile);
   const dim3 grid(a.tiles_x, a.tiles_y);
   de_prep<<<grid, block, 0, stream>>>(a);
-  if (a.R <= 9) de_gather<9><<<grid, block, 0, stream>>>(a);
-  else if (a.R <= 16) de_gather<16><<<grid, block, 0, stream>>>(a);
-  else de_gather<18><<<grid, block, 0, stream>>>(a);
+  if (a.R <= 10) de_gather<10, false><<<grid, block, 0, stream>>>(a);
+  else de_gather<16, true><<<grid, block, 0, stream>>>(a);
   return hipGetLastError();
 }
 
