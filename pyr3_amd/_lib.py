@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libpyr3hip.so"
@@ -83,6 +84,12 @@ SIGNATURES = {
 }
 
 _lib = None
+# Whether torch's modules were loaded when the library was. torch ships its
+# own HIP and HSA runtimes (torch/lib); libpyr3hip.so binds to whichever
+# libamdhip64.so.7 the process already holds. Loaded after torch, the process
+# has one runtime; loaded first, it takes /opt/rocm's, and torch's own copy,
+# loaded later, cannot open the GPU ("No HIP GPUs are available").
+loaded_after_torch = False
 
 
 def load() -> C.CDLL:
@@ -97,6 +104,8 @@ def load() -> C.CDLL:
     # Compiled per-shape chaos kernels persist in the package's kcache/ unless
     # P3_KERNEL_CACHE names another directory ("" turns the cache off).
     os.environ.setdefault("P3_KERNEL_CACHE", str(LIB_PATH.parent / "kcache"))
+    global loaded_after_torch
+    loaded_after_torch = "torch" in sys.modules
     lib = C.CDLL(str(LIB_PATH))
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

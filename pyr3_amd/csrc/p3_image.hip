@@ -21,6 +21,8 @@
 // fwidth^2 super-pixels (clamp to edge), flam3 alpha/vibrancy/highlight
 // composite, background blend, optional channel curves and HSV adjust
 // (:240-266), RGBA8 store with round-to-nearest.
+#include <type_traits>
+
 #include "p3_device.h"
 #include "p3_launch.h"
 
@@ -78,36 +80,96 @@ __device__ inline float de_kw(int d2, int r) {
   return expf(-(float)d2 / (2.0f * sigma * sigma));
 }
 
+// Offset of radius r's weights in the per-block table: radius j holds
+// j^2 + 1 weights (d^2 = 0 .. j^2), so radii 0 .. r-1 take
+// sum (j^2 + 1) = (r - 1) r (2r - 1) / 6 + r.
+__host__ __device__ constexpr int kw_offset(int r) { return (r - 1) * r * (2 * r - 1) / 6 + r; }
+
+// A staged bin as one LDS word. Halos up to 10: 16 bits, its weights'
+// offset in the table (kw_offset(r), low 9 bits) and its reach r^2 + 1
+// above them (0: empty bin, which every test rejects). Larger halos: one
+// byte, its radius (255: empty bin; those halos are walked only through
+// masks, which leave empty bins out), the offset read from a table of the
+// Rt + 1 offsets. A visit at squared distance d2 takes the bin if
+// reaches(word, d2) and reads weight [offset + d2].
+template <int kR> struct DePack {
+  static constexpr bool kReach = kR <= 10;
+  using T = typename std::conditional<kReach, uint16_t, uint8_t>::type;
+  static constexpr int kShift = 9;
+  static constexpr uint32_t kOff = (1u << kShift) - 1u;
+  static constexpr uint32_t kEmptyWord = kReach ? 0u : 255u;
+  static_assert(!kReach || (kw_offset(kR) <= (int)kOff && kR * kR + 1 < 128), "16-bit word");
+  static_assert(kR < 255, "radius byte");
+  __device__ static uint32_t encode(int r) {
+    return kReach ? ((uint32_t)kw_offset(r) | ((uint32_t)(r * r + 1) << kShift)) : (uint32_t)r;
+  }
+  __device__ static bool reaches(uint32_t w, uint32_t d2) {
+    return kReach ? d2 < (w >> kShift) : d2 <= w * w;
+  }
+  __device__ static uint32_t offset(uint32_t w, const uint16_t* off) {
+    return kReach ? (w & kOff) : (uint32_t)off[w];
+  }
+  // radius >= t and the bin is occupied
+  __device__ static bool at_least(uint32_t w, int t) {
+    return kReach ? (w >> kShift) > (uint32_t)(t * t) : (w != 255u && w >= (uint32_t)t);
+  }
+};
+
+// One row of an output's box through its mask: bit j of m is staged
+// column (window start + j), dx = j + dx0; the bins are visited in
+// increasing x.
+template <class Pk, class M>
+__device__ __forceinline__ void de_walk(M m, const typename Pk::T* pk, const float4* pre,
+                                        const float* kw, const uint16_t* off, int dx0, int dy2,
+                                        float4& sum) {
+  while (m) {
+    const int j = sizeof(M) == 8 ? (int)__builtin_ctzll((uint64_t)m) : (int)__builtin_ctz((uint32_t)m);
+    m &= m - (M)1;
+    const uint32_t w = pk[j];
+    const int dx = j + dx0;
+    const uint32_t d2 = (uint32_t)(dx * dx + dy2);
+    if (!Pk::reaches(w, d2)) continue;
+    const float k = kw[Pk::offset(w, off) + d2];
+    const float4 p = pre[j];
+    sum.x = sum.x + p.x * k;
+    sum.y = sum.y + p.y * k;
+    sum.z = sum.z + p.z * k;
+    sum.w = sum.w + p.w * k;
+  }
+}
+
 // Per 16 x 16 output tile with search radius Rt <= kR: the (16 + 2 Rt)^2
-// neighbourhood (pre-scaled RGBA, radius) is staged in LDS, and each output
-// sums kw * pre over the occupied bins of its (2 Rt + 1)^2 box in dy-then-dx
-// order (the reference's order, empty bins skipped). With kLists, a tile
-// whose staged neighbourhood is mostly empty (under a quarter occupied: the
-// large radii of sparse regions) walks, per row, a list of its occupied
-// columns (one wave ballots each staged row into the list and its prefix
-// counts); other tiles scan the box. Both walks visit the same bins in the same order, so
-// the sums are bit-identical. The Gaussian weights exp(-d^2 / 2 sigma^2)
-// depend only on (d^2, r): they are tabulated per block for r <= Rt with the
-// same expression. Tiles with Rt > kR read HBM directly.
-template <int kR, bool kLists>
+// neighbourhood (pre-scaled RGBA, packed radius) is staged in LDS, and each
+// output sums kw * pre over the bins of its (2 Rt + 1)^2 box that reach it
+// (d^2 <= r^2), in dy-then-dx order (the reference's order; bins that do
+// not reach it add nothing). A bin reaches an output dy rows away only if
+// its radius is at least |dy|, so with kMask one wave ballots every staged
+// row into 64-bit masks of the columns whose radius is at least t, for
+// t = 0 .. Rt, and row dy of an output's box visits the set bits of mask
+// |dy| inside its 2 Rt + 1 columns in increasing x; without kMask every
+// column of the box is tested. Both visit the reaching bins in the same
+// order, so the sums are bit-identical. The Gaussian weights
+// exp(-d^2 / 2 sigma^2) depend only on (d^2, r): they are tabulated per
+// block for r <= Rt (packed, kw_offset) with the same expression. A staged
+// bin's radius is at most Rt (it is at most max_rad < R, and its tile is
+// one of those bounding Rt); the staging clamps it there all the same, so
+// no visit reads past the table. Tiles with Rt > kR read HBM directly.
+template <int kR, bool kMask>
 __global__ void __launch_bounds__(256) de_gather(DeArgs A) {
+  using Pk = DePack<kR>;
   constexpr int kSpan = kTile + 2 * kR;
-  constexpr int kKw = kR * kR + 1;                 // d^2 = 0 .. kR^2 per radius
-  constexpr int kL = kLists ? kSpan : 1;           // list arrays (none without lists)
-  static_assert(kSpan < 64, "one wave lists a staged row and its total");
+  constexpr int kM = kMask ? kSpan * (kR + 1) : 1;   // masks (none without kMask)
+  static_assert(kSpan <= 64, "one wave ballots a staged row into a 64-bit mask");
+  static_assert(kMask || Pk::kReach, "a box scan needs words whose test rejects empty bins");
   __shared__ float4 s_pre[kSpan * kSpan];
-  __shared__ uint8_t s_rad[kSpan * kSpan];
-  __shared__ uint8_t s_col[kL * kL];                // occupied columns, per row
-  __shared__ uint16_t s_cnt[kL * (kL + 1)];         // occupied columns left of c, per row
-  __shared__ float s_kw[(kR + 1) * kKw];
+  __shared__ typename Pk::T s_pk[kSpan * kSpan];
+  __shared__ uint64_t s_mask[kM];                  // [row][t]: columns with radius >= t
+  __shared__ float s_kw[kw_offset(kR + 1)];
+  __shared__ uint16_t s_off[Pk::kReach ? 1 : kR + 1];   // kw_offset(r), r <= Rt
   __shared__ int s_rt;
-  __shared__ uint32_t s_occ;                        // occupied bins staged
   const int tid = threadIdx.y * kTile + threadIdx.x;
   const int x0 = blockIdx.x * kTile, y0 = blockIdx.y * kTile;
-  if (tid == 0) {
-    s_rt = tile_bound(A, blockIdx.x, blockIdx.y);
-    s_occ = 0u;
-  }
+  if (tid == 0) s_rt = tile_bound(A, blockIdx.x, blockIdx.y);
   __syncthreads();
   const int Rt = s_rt;
   const int xi = x0 + threadIdx.x, yi = y0 + threadIdx.y;
@@ -116,79 +178,67 @@ __global__ void __launch_bounds__(256) de_gather(DeArgs A) {
 
   if (Rt <= kR) {
     const int span = kTile + 2 * Rt;
-    const int kws = Rt * Rt + 1;
-    for (int k = tid; k < (Rt + 1) * kws; k += 256) {
-      const int r = k / kws, d2 = k - r * kws;
-      s_kw[k] = d2 <= r * r ? de_kw(d2, r) : 0.0f;
+    for (int k = tid; k < kw_offset(Rt + 1); k += 256) {
+      int r = 0;
+      while (kw_offset(r + 1) <= k) ++r;
+      s_kw[k] = de_kw(k - kw_offset(r), r);
     }
-    uint32_t occupied = 0u;   // counted only where the lists are built
+    if (!Pk::kReach && tid <= Rt) s_off[tid] = (uint16_t)kw_offset(tid);
     for (int k = tid; k < span * span; k += 256) {
       const int ly = k / span, lx = k - ly * span;
       const int gx = x0 - Rt + lx, gy = y0 - Rt + ly;
-      uint8_t r8 = kEmpty;
+      uint32_t pk = Pk::kEmptyWord;
       float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
         const size_t g = (size_t)gy * W + gx;
-        r8 = A.rad[g];
-        if (r8 != kEmpty) p = A.pre[g];
+        const int r8 = A.rad[g];
+        if (r8 != kEmpty) {
+          p = A.pre[g];
+          pk = Pk::encode(min(r8, Rt));
+        }
       }
-      if (kLists) occupied += r8 != kEmpty ? 1u : 0u;
-      s_rad[k] = r8;
+      s_pk[k] = (typename Pk::T)pk;
       s_pre[k] = p;
     }
-    if (kLists && occupied) atomicAdd(&s_occ, occupied);
     __syncthreads();
-    const bool sparse = kLists && 4u * s_occ < (uint32_t)(span * span);   // block-uniform
-    if (sparse) {  // one wave per staged row: ballot of the occupied columns
+    if (kMask) {   // one wave per staged row
       const int lane = tid & 63, wave = tid >> 6;
       for (int ly = wave; ly < span; ly += 4) {
-        const bool occ = lane < span && s_rad[ly * span + lane] != kEmpty;
-        const uint64_t m = __ballot(occ);
-        const int before = __builtin_popcountll(m & ((1ull << lane) - 1ull));
-        if (occ) s_col[ly * span + before] = (uint8_t)lane;
-        if (lane <= span) s_cnt[ly * (span + 1) + lane] = (uint16_t)before;
+        const uint32_t w = lane < span ? (uint32_t)s_pk[ly * span + lane] : Pk::kEmptyWord;
+        for (int t = 0; t <= Rt; ++t) {
+          const uint64_t m = __ballot(Pk::at_least(w, t));
+          if (lane == 0) s_mask[ly * (kR + 1) + t] = m;
+        }
       }
       __syncthreads();
     }
-    if (xi < W && yi < H && !sparse) {
+    if (xi < W && yi < H) {
       const int cx = threadIdx.x + Rt, cy = threadIdx.y + Rt;
+      const int nb = 2 * Rt + 1;   // window: staged columns cx - Rt .. cx + Rt
       for (int dy = -Rt; dy <= Rt; ++dy) {
-        const int rowk = (cy + dy) * span + cx;
-        for (int dx = -Rt; dx <= Rt; ++dx) {
-          const int k = rowk + dx;
-          const int r = s_rad[k];
-          if (r == kEmpty) continue;
-          const int d2 = dx * dx + dy * dy;
-          if (d2 > r * r) continue;
-          const float kw = r <= Rt ? s_kw[r * kws + d2] : de_kw(d2, r);
-          const float4 p = s_pre[k];
-          sum.x = sum.x + p.x * kw;
-          sum.y = sum.y + p.y * kw;
-          sum.z = sum.z + p.z * kw;
-          sum.w = sum.w + p.w * kw;
-        }
-      }
-    } else if (kLists && xi < W && yi < H) {
-      const int cx = threadIdx.x + Rt, cy = threadIdx.y + Rt;
-      for (int dy = -Rt; dy <= Rt; ++dy) {
-        const int ly = cy + dy;
-        const uint16_t* cnt = s_cnt + ly * (span + 1);
-        const int j1 = cnt[cx + Rt + 1];
-        for (int j = cnt[cx - Rt]; j < j1; ++j) {
-          const int lx = s_col[ly * span + j];
-          const int k = ly * span + lx;
-          const int r = s_rad[k];
-          const int dx = lx - cx;
-          const int d2 = dx * dx + dy * dy;
-          if (d2 > r * r) continue;
-          // r <= R (rint of a radius clamped to max_rad <= R) and every bin of the
-          // box lies in a tile that bounds Rt, so r <= Rt: the table holds it
-          const float kw = r <= Rt ? s_kw[r * kws + d2] : de_kw(d2, r);
-          const float4 p = s_pre[k];
-          sum.x = sum.x + p.x * kw;
-          sum.y = sum.y + p.y * kw;
-          sum.z = sum.z + p.z * kw;
-          sum.w = sum.w + p.w * kw;
+        const int rowk = (cy + dy) * span;
+        if (kMask) {
+          const uint64_t m = s_mask[(cy + dy) * (kR + 1) + (dy < 0 ? -dy : dy)] >> threadIdx.x;
+          const int w0 = rowk + (int)threadIdx.x;
+          if (nb <= 32)   // block-uniform
+            de_walk<Pk, uint32_t>((uint32_t)m & (0xffffffffu >> (32 - nb)), s_pk + w0, s_pre + w0,
+                                  s_kw, s_off, -Rt, dy * dy, sum);
+          else
+            de_walk<Pk, uint64_t>(m & ((1ull << nb) - 1ull), s_pk + w0, s_pre + w0, s_kw, s_off,
+                                  -Rt, dy * dy, sum);
+        } else {
+          for (int dx = -Rt; dx <= Rt; ++dx) {
+            const int k = rowk + cx + dx;
+            const uint32_t pk = s_pk[k];
+            const uint32_t d2 = (uint32_t)(dx * dx + dy * dy);
+            if (!Pk::reaches(pk, d2)) continue;
+            const float kw = s_kw[Pk::offset(pk, s_off) + d2];
+            const float4 p = s_pre[k];
+            sum.x = sum.x + p.x * kw;
+            sum.y = sum.y + p.y * kw;
+            sum.z = sum.z + p.z * kw;
+            sum.w = sum.w + p.w * kw;
+          }
         }
       }
     }
@@ -217,13 +267,14 @@ __global__ void __launch_bounds__(256) de_gather(DeArgs A) {
 }
 
 // The gather's LDS halo follows the search radius R (9 os + 1 super-pixels
-// for a 9-pixel DE radius): 10 without the lists (oversample 1: 26 KB of
-// LDS, 6 blocks per CU) or 16 with them (2 blocks per CU; tiles with
-// Rt > 16 read HBM). Measured on the 1080p q30 frame (R = 10): 177 us at a
-// 10 halo against 422 at 16 (204 us at 10 with the lists: their LDS costs a
-// block per CU and the frame's tiles are mostly dense); on the 1024x768 os2
-// q200 frame (R = 19): 2.21 ms at 16 with the lists against 3.00 without
-// them, while a 19 halo (1 block per CU) took 4.34 ms.
+// for a 9-pixel DE radius). At R <= 10 (oversample 1) a 10 halo and the box
+// scan (24.9 KB of LDS, 6 blocks per CU); the mask walk, which visits 27 %
+// fewer bins per wave there, costs more per visit and in building its
+// masks than it saves (1080p q30: 148 us box against 196 us masks, q2:
+// 161 against 181, with de_prep). Larger R (oversample 2): a 16 halo and
+// the mask walk (51.8 KB, 3 blocks per CU), whose radius-thresholded rows
+// skip the bins that cannot reach across |dy| rows (tiles with Rt > 16 read
+// HBM).
 hipError_t launch_density(const DeArgs& a, hipStream_t stream) {
   const dim3 block(kTile, kTile);
   const dim3 grid(a.tiles_x, a.tiles_y);
@@ -340,39 +391,10 @@ __device__ inline uint32_t to_u8(float v) {
   return (uint32_t)(fminf(fmaxf(v, 0.0f), 1.0f) * 255.0f + 0.5f);
 }
 
-template <bool kDe>
-__global__ void __launch_bounds__(256) tonemap_kernel(TmArgs A) {
-  const uint32_t xi = blockIdx.x * 16 + threadIdx.x;
-  const uint32_t yi = blockIdx.y * 16 + threadIdx.y;
-  if (xi >= A.width || yi >= A.height) return;
-  const uint32_t N = A.os, fw = A.fwidth, half = fw / 2u;
-  const int SW = (int)(A.width * N), SH = (int)(A.height * N);
-  const uint32_t cx = xi * N + N / 2u, cy = yi * N + N / 2u;
-  float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, c3 = 0.0f;
-  for (uint32_t dy = 0; dy < fw; ++dy) {
-    const float ky = A.taps[dy];
-    const int sy = min(max((int)cy + (int)dy - (int)half, 0), SH - 1);
-    for (uint32_t dx = 0; dx < fw; ++dx) {
-      const float kx = A.taps[dx];
-      const int sx = min(max((int)cx + (int)dx - (int)half, 0), SW - 1);
-      const size_t sb = (size_t)sy * SW + sx;
-      const float w = kx * ky;
-      if (kDe) {
-        const float4 f = A.filtered[sb];
-        c0 = c0 + f.x * w; c1 = c1 + f.y * w; c2 = c2 + f.z * w; c3 = c3 + f.w * w;
-      } else {
-        const uint4 h = A.hist[sb];
-        const float cnt = (float)h.w;
-        if (cnt > 0.0f) {
-          const float ls = (A.k1 * logf(1.0f + cnt * A.k2)) / cnt;
-          c0 = c0 + ((float)h.x * ls) * w;
-          c1 = c1 + ((float)h.y * ls) * w;
-          c2 = c2 + ((float)h.z * ls) * w;
-          c3 = c3 + (cnt * ls) * w;
-        }
-      }
-    }
-  }
+// visualize_f32.wgsl's per-pixel composite of the filtered sums (c0..c3)
+// into output pixel (xi, yi): RGBA8, and RGBA f32 when A.outf is set.
+__device__ inline void tm_store(const TmArgs& A, uint32_t xi, uint32_t yi, float c0, float c1,
+                                float c2, float c3) {
   float4 o;
   if (c3 <= 0.0f) {
     o = A.transparent ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(A.bg0, A.bg1, A.bg2, 1.0f);
@@ -401,6 +423,83 @@ __global__ void __launch_bounds__(256) tonemap_kernel(TmArgs A) {
   const size_t oi = (size_t)yi * A.width + xi;
   A.out8[oi] = to_u8(o.x) | (to_u8(o.y) << 8) | (to_u8(o.z) << 16) | (to_u8(o.w) << 24);
   if (A.outf) A.outf[oi] = o;
+}
+
+// Log-density scaling of one bin without DE (visualize_u32.wgsl:175-184):
+// the bin's RGBA counts times k1 log(1 + cnt k2) / cnt, zero for an empty bin.
+__device__ inline float4 log_scaled(const TmArgs& A, uint4 h) {
+  const float cnt = (float)h.w;
+  if (!(cnt > 0.0f)) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const float ls = (A.k1 * logf(1.0f + cnt * A.k2)) / cnt;
+  return make_float4((float)h.x * ls, (float)h.y * ls, (float)h.z * ls, cnt * ls);
+}
+
+// Spatial filter (fw x fw taps centred on the output pixel's os x os
+// super-pixels, edges clamped) and composite, one thread per output pixel,
+// bins read from HBM per tap. Used when the filter's tile does not fit the
+// LDS budget of tonemap_tiled.
+template <bool kDe>
+__global__ void __launch_bounds__(256) tonemap_kernel(TmArgs A) {
+  const uint32_t xi = blockIdx.x * 16 + threadIdx.x;
+  const uint32_t yi = blockIdx.y * 16 + threadIdx.y;
+  if (xi >= A.width || yi >= A.height) return;
+  const uint32_t N = A.os, fw = A.fwidth, half = fw / 2u;
+  const int SW = (int)(A.width * N), SH = (int)(A.height * N);
+  const uint32_t cx = xi * N + N / 2u, cy = yi * N + N / 2u;
+  float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, c3 = 0.0f;
+  for (uint32_t dy = 0; dy < fw; ++dy) {
+    const float ky = A.taps[dy];
+    const int sy = min(max((int)cy + (int)dy - (int)half, 0), SH - 1);
+    for (uint32_t dx = 0; dx < fw; ++dx) {
+      const float kx = A.taps[dx];
+      const int sx = min(max((int)cx + (int)dx - (int)half, 0), SW - 1);
+      const size_t sb = (size_t)sy * SW + sx;
+      const float w = kx * ky;
+      const float4 f = kDe ? A.filtered[sb] : log_scaled(A, A.hist[sb]);
+      c0 = c0 + f.x * w; c1 = c1 + f.y * w; c2 = c2 + f.z * w; c3 = c3 + f.w * w;
+    }
+  }
+  tm_store(A, xi, yi, c0, c1, c2, c3);
+}
+
+// The same filter and composite over a 32 x 8 tile of output pixels whose
+// super-pixels (plus the filter's halo, edges clamped) are staged in LDS
+// once: each bin is read from HBM once per tile (and, without DE, log-scaled
+// once instead of once per tap), then every tap reads LDS. The per-pixel
+// sums take the same terms in the same order as tonemap_kernel (a bin that
+// is empty adds +0), so both kernels give identical bits.
+constexpr uint32_t kTmTileX = 32, kTmTileY = 8;
+template <bool kDe>
+__global__ void __launch_bounds__(256) tonemap_tiled(TmArgs A, uint32_t span_x, uint32_t span_y) {
+  extern __shared__ float4 tm_tile[];
+  const uint32_t N = A.os, fw = A.fwidth, half = fw / 2u;
+  const int SW = (int)(A.width * N), SH = (int)(A.height * N);
+  const int x0 = (int)(blockIdx.x * kTmTileX * N + N / 2u) - (int)half;
+  const int y0 = (int)(blockIdx.y * kTmTileY * N + N / 2u) - (int)half;
+  const uint32_t t = threadIdx.x;
+  const uint32_t cells = span_x * span_y;
+  for (uint32_t k = t; k < cells; k += 256u) {
+    const uint32_t ty = k / span_x, tx = k - ty * span_x;
+    const int sy = min(max(y0 + (int)ty, 0), SH - 1);
+    const int sx = min(max(x0 + (int)tx, 0), SW - 1);
+    const size_t sb = (size_t)sy * SW + sx;
+    tm_tile[k] = kDe ? A.filtered[sb] : log_scaled(A, A.hist[sb]);
+  }
+  __syncthreads();
+  const uint32_t lx = t % kTmTileX, ly = t / kTmTileX;
+  const uint32_t xi = blockIdx.x * kTmTileX + lx, yi = blockIdx.y * kTmTileY + ly;
+  if (xi >= A.width || yi >= A.height) return;
+  float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, c3 = 0.0f;
+  for (uint32_t dy = 0; dy < fw; ++dy) {
+    const float ky = A.taps[dy];
+    const float4* row = tm_tile + (ly * N + dy) * span_x + lx * N;
+    for (uint32_t dx = 0; dx < fw; ++dx) {
+      const float w = A.taps[dx] * ky;
+      const float4 f = row[dx];
+      c0 = c0 + f.x * w; c1 = c1 + f.y * w; c2 = c2 + f.z * w; c3 = c3 + f.w * w;
+    }
+  }
+  tm_store(A, xi, yi, c0, c1, c2, c3);
 }
 
 // ------------------------------------------------------- linear export --
@@ -504,7 +603,22 @@ hipError_t launch_atomic_probe(uint64_t* grid, uint32_t cells, uint32_t blocks,
   return hipGetLastError();
 }
 
+// LDS budget of one tonemap_tiled block: 48 KiB holds the 32 x 8 tile up to
+// os 2 with 16 taps (66 x 34 cells would be 35.9 KiB) and os 1 with up to
+// 20 taps; wider filters take the per-tap HBM kernel.
+constexpr size_t kTmTileBytes = 48 * 1024;
+
 hipError_t launch_tonemap(const TmArgs& a, bool de, hipStream_t stream) {
+  if (a.width == 0 || a.height == 0) return hipSuccess;
+  const uint32_t span_x = (kTmTileX - 1) * a.os + a.fwidth;
+  const uint32_t span_y = (kTmTileY - 1) * a.os + a.fwidth;
+  const size_t bytes = (size_t)span_x * span_y * sizeof(float4);
+  if (bytes <= kTmTileBytes) {
+    const dim3 grid((a.width + kTmTileX - 1) / kTmTileX, (a.height + kTmTileY - 1) / kTmTileY);
+    if (de) tonemap_tiled<true><<<grid, 256, bytes, stream>>>(a, span_x, span_y);
+    else tonemap_tiled<false><<<grid, 256, bytes, stream>>>(a, span_x, span_y);
+    return hipGetLastError();
+  }
   const dim3 block(16, 16);
   const dim3 grid((a.width + 15) / 16, (a.height + 15) / 16);
   if (de) tonemap_kernel<true><<<grid, block, 0, stream>>>(a);

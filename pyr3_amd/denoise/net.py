@@ -166,9 +166,14 @@ class HipNet:
     tensors (torch is the allocator and the stream, not the arithmetic)."""
 
     def __init__(self, params: Dict[str, np.ndarray], device: str = "cuda"):
+        from .. import _lib
+        if _lib._lib is not None and not _lib.loaded_after_torch:
+            raise RuntimeError(
+                "pyr3_amd: libpyr3hip.so was loaded before torch, so torch's own HIP runtime "
+                "cannot open the GPU in this process; import torch before the first use of "
+                "pyr3_amd's renderer or kernels")
         import torch
-        from .._lib import load
-        self.lib = load()
+        self.lib = _lib.load()
         self.torch = torch
         self.dev = device
         self.p = {k: torch.from_numpy(v.copy()).to(device) for k, v in params.items()}

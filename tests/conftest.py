@@ -11,6 +11,15 @@ for p in (ROOT, ROOT / "oracle"):
 
 GOLDEN = ROOT / "tests" / "golden"
 
+# torch ships its own HIP runtime, and libpyr3hip.so binds to the one the
+# process holds when it loads: importing torch first keeps one runtime in
+# the test process whichever tests run (the torch-reference tests need
+# torch's device).
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 # Specialised chaos kernels compiled by one test process are reused by the
 # next (and by a host of the same arch) from this directory.
 os.environ.setdefault("P3_KERNEL_CACHE", str(ROOT / "pyr3_amd" / "kcache"))
