@@ -30,6 +30,8 @@
 
 #include <cstdint>
 
+#include <type_traits>
+
 #include "p3_denoise.h"
 #include "p3_nn_common.h"
 
@@ -116,40 +118,56 @@ __global__ void __launch_bounds__(256) conv3x3_mfma(ConvArgs A) {
 #pragma unroll
     for (int g = 0; g < kGW; ++g) acc[j][g] = v4f32{0.0f, 0.0f, 0.0f, 0.0f};
   constexpr int kUnroll = kChunks * kNtW <= 18 ? kChunks : 4;
+  // The K loop. An upsampling layer reads output-resolution tap (uy, ux) at
+  // staged low-resolution pixel (uy >> 1, ux >> 1); with an even tile origin
+  // that is row ((ty + dy) >> 1) + 1, column ((px + dx) >> 1) + 1 of the
+  // tile. Taps outside the output read zeros from the staged border, except
+  // the row or column just past an odd output size, which halves onto the
+  // input's last row or column: blocks on such a border test every tap
+  // (kChecked).
+  auto k_loop = [&](auto checked) {
+    constexpr bool kChecked = decltype(checked)::value;
 #pragma unroll kUnroll
-  for (int q = 0; q < kChunks; ++q) {
-    const int k = q * 32 + kg * 8;               // this lane's first flat k
-    const int tap = k / kCin < 9 ? k / kCin : 8;  // padding k: any tap (B is zero there)
-    const int ch = k / kCin < 9 ? k % kCin : 0;
-    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-    v8bf16 b[kNtW];
-#pragma unroll
-    for (int j = 0; j < kNtW; ++j)
-      b[j] = __builtin_bit_cast(
-          v8bf16, *(const uint4*)(A.w + (size_t)((nt0 + j) * 16 + m) * kKp + k));
-#pragma unroll
-    for (int g = 0; g < kGW; ++g) {
-      const int gg = wi + kWPN * g;
-      const int ty = gg / kGX, px = (gg % kGX) * 16 + m;
-      int r, c;
-      bool ok = true;
-      if (kUp) {  // output-resolution tap, cropped to the output, then halved
-        const int uy = y0 + ty + dy, ux = x0 + px + dx;
-        ok = uy >= 0 && uy < A.out_h && ux >= 0 && ux < A.out_w;
-        r = (uy >> 1) - ((y0 >> 1) - 1);
-        c = (ux >> 1) - ((x0 >> 1) - 1);
-      } else {
-        r = ty * kStride + dy + 1;
-        c = px * kStride + dx + 1;
-      }
-      uint4 av = make_uint4(0u, 0u, 0u, 0u);
-      if (ok) av = *(const uint4*)&lds[(r * T::kSX + c) * T::kPS + ch];
-      const v8bf16 a = __builtin_bit_cast(v8bf16, av);
+    for (int q = 0; q < kChunks; ++q) {
+      const int k = q * 32 + kg * 8;               // this lane's first flat k
+      const int tap = k / kCin < 9 ? k / kCin : 8;  // padding k: any tap (B is zero there)
+      const int ch = k / kCin < 9 ? k % kCin : 0;
+      const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+      v8bf16 b[kNtW];
 #pragma unroll
       for (int j = 0; j < kNtW; ++j)
-        acc[j][g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[j][g], 0, 0, 0);
+        b[j] = __builtin_bit_cast(
+            v8bf16, *(const uint4*)(A.w + (size_t)((nt0 + j) * 16 + m) * kKp + k));
+#pragma unroll
+      for (int g = 0; g < kGW; ++g) {
+        const int gg = wi + kWPN * g;
+        const int ty = gg / kGX, px = (gg % kGX) * 16 + m;
+        int r, c;
+        bool ok = true;
+        if (kUp) {
+          r = ((ty + dy) >> 1) + 1;
+          c = ((px + dx) >> 1) + 1;
+          if (kChecked) {
+            const int uy = y0 + ty + dy, ux = x0 + px + dx;
+            ok = uy < A.out_h && ux < A.out_w;
+          }
+        } else {
+          r = ty * kStride + dy + 1;
+          c = px * kStride + dx + 1;
+        }
+        uint4 av = make_uint4(0u, 0u, 0u, 0u);
+        if (ok) av = *(const uint4*)&lds[(r * T::kSX + c) * T::kPS + ch];
+        const v8bf16 a = __builtin_bit_cast(v8bf16, av);
+#pragma unroll
+        for (int j = 0; j < kNtW; ++j)
+          acc[j][g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[j][g], 0, 0, 0);
+      }
     }
-  }
+  };
+  const bool border = kUp && (((A.out_h & 1) && y0 + kTY >= A.out_h) ||
+                              ((A.out_w & 1) && x0 + kTX >= A.out_w));   // block-uniform
+  if (border) k_loop(std::true_type{});
+  else k_loop(std::false_type{});
   __syncthreads();  // the input tile is dead; its LDS now holds the output tile
 #pragma unroll
   for (int j = 0; j < kNtW; ++j) {

@@ -286,14 +286,22 @@ hipError_t launch_density(const DeArgs& a, hipStream_t stream) {
 
 // ------------------------------------------------------------- tonemap --
 
+// pow as WGSL defines it, exp2(y log2 x), on the hardware's log2 and exp2
+// (v_log_f32, v_exp_f32): the tonemap's powers all have x >= 0 and y > 0
+// (or x > 0 and y = 0), where this is exact in form and within ~3e-7
+// relative of a correctly rounded pow.
+__device__ inline float pow_wgsl(float x, float y) {
+  return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+}
+
 __device__ inline float calc_alpha(float d, float g, float lin) {
   if (d <= 0.0f) return 0.0f;
   if (d < lin) {
-    const float fv = powf(lin, g);
+    const float fv = pow_wgsl(lin, g);
     const float frac = d / lin;
-    return (1.0f - frac) * d * (fv / lin) + frac * powf(d, g);
+    return (1.0f - frac) * d * (fv / lin) + frac * pow_wgsl(d, g);
   }
-  return powf(d, g);
+  return pow_wgsl(d, g);
 }
 
 __device__ inline float3 rgb2hsv(float3 c) {
@@ -334,7 +342,7 @@ __device__ inline float3 calc_newrgb(float3 c, float ls, float highpow) {
   if (sc.z > maxa) { maxa = sc.z; maxc = cw.z; }
   if (maxa > 255.0f && highpow >= 0.0f) {
     const float newls = 255.0f / maxc;
-    const float lsratio = powf(newls / ls, highpow);
+    const float lsratio = pow_wgsl(newls / ls, highpow);
     float3 n = make_float3(newls * cw.x / 255.0f, newls * cw.y / 255.0f, newls * cw.z / 255.0f);
     float3 hsv = rgb2hsv(n);
     hsv.y *= lsratio;
@@ -411,7 +419,7 @@ __device__ inline void tm_store(const TmArgs& A, uint32_t xi, uint32_t yi, float
     float res[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      const float perch = (1.0f - A.vibrancy) * 256.0f * powf(fmaxf(cc[i] / 255.0f, 0.0f), g);
+      const float perch = (1.0f - A.vibrancy) * 256.0f * pow_wgsl(fmaxf(cc[i] / 255.0f, 0.0f), g);
       const float bgt = A.transparent ? 0.0f : (1.0f - alpha) * 256.0f * bg[i];
       const float composed = nn[i] + perch + bgt;
       res[i] = fminf(fmaxf(composed / 256.0f, 0.0f), 1.0f);
