@@ -8,8 +8,9 @@ palette), so the set needs few kernel compiles. For each genome: the plain
 render at `--ref-quality` is the judge, the learned frame (learned-path
 input at each of `--qualities`, then the denoiser through the C ABI) is
 scored by PSNR over RGB, next to the plain path at the same quality for
-comparison. Genomes whose attractor covers less than 5% of the frame are
-skipped (as in training). Writes a JSON report (per quality: mean, median,
+comparison; with several `--fuses` the learned input is rendered (same
+seed) at each fuse. Genomes whose attractor covers less than 5% of the
+frame are skipped (as in training). Writes a JSON report (per quality: mean, median,
 10th percentile PSNR and the share of genomes at or above the 38 dB target)
 and prints a progress line every 25 genomes.
 
@@ -30,6 +31,12 @@ from .train import H, W, compile_ahead, log, shape_key
 HELDOUT_BASE = 40000
 
 
+def _lkey(q: float, fuse: int, fuses) -> str:
+    """Row key of a learned score: the quality, with the fuse when several
+    fuses are scored."""
+    return str(q) if len(fuses) == 1 else f"{q}/f{fuse}"
+
+
 def heldout_genomes(n_shapes: int, per_shape: int):
     out = []
     for k in range(HELDOUT_BASE, HELDOUT_BASE + n_shapes):
@@ -46,6 +53,8 @@ def main(argv=None) -> int:
     ap.add_argument("--shapes", type=int, default=125)
     ap.add_argument("--per-shape", type=int, default=8)
     ap.add_argument("--qualities", default="3,4,6")
+    ap.add_argument("--fuses", default=str(learned.FUSE),
+                    help="input fuses of the learned path to score (comma list)")
     ap.add_argument("--ref-quality", type=float, default=4000)
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--max-genomes", type=int, default=0)
@@ -55,6 +64,7 @@ def main(argv=None) -> int:
     from ..renderer import Denoiser, Renderer
     t0 = time.time()
     qs = [float(q) for q in a.qualities.split(",")]
+    fuses = [int(f) for f in a.fuses.split(",")]
     genomes = heldout_genomes(a.shapes, a.per_shape)
     if a.max_genomes:
         genomes = genomes[:a.max_genomes]
@@ -88,12 +98,13 @@ def main(argv=None) -> int:
         row = {"genome": g.name, "learned": {}, "plain": {}}
         for q in qs:
             w, it = learned.dispatch(q, W, H)
-            r.set_image_f32(True)
-            r.reset(g)
-            s = r.iterate(g, 101 + i, w, it, fuse=learned.FUSE)
-            r.present(g, s, force_de_off=True)
-            den.apply(r)
-            row["learned"][str(q)] = round(learned.psnr(r.read_image(), ref), 3)
+            for fz in fuses:
+                r.set_image_f32(True)
+                r.reset(g)
+                s = r.iterate(g, 101 + i, w, it, fuse=fz)
+                r.present(g, s, force_de_off=True)
+                den.apply(r)
+                row["learned"][_lkey(q, fz, fuses)] = round(learned.psnr(r.read_image(), ref), 3)
             r.set_image_f32(False)
             r.reset(g)
             s = r.iterate(g, 101 + i, w, it, fuse=pack.FUSE)
@@ -102,21 +113,22 @@ def main(argv=None) -> int:
         rows.append(row)
         if len(rows) % 25 == 0:
             log(f"eval: {len(rows)} genomes scored ({time.time() - t1:.0f}s); learned "
-                + ", ".join(f"q{q:g} mean {np.mean([x['learned'][str(q)] for x in rows]):.2f} dB"
-                            for q in qs))
+                + ", ".join(f"{k} mean {np.mean([x['learned'][k] for x in rows]):.2f} dB"
+                            for k in rows[0]["learned"]))
     den.destroy()
     r.destroy()
     summary = {}
-    for path in ("learned", "plain"):
-        for q in qs:
-            v = np.array([x[path][str(q)] for x in rows])
-            v = v[np.isfinite(v)]
-            summary[f"{path}_q{q:g}"] = {
-                "mean_db": round(float(v.mean()), 3), "median_db": round(float(np.median(v)), 3),
-                "p10_db": round(float(np.percentile(v, 10)), 3),
-                "share_at_target": round(float(np.mean(v >= learned.TARGET_DB)), 4)}
+    keys = [("learned", _lkey(q, fz, fuses), f"learned_q{q:g}" + (f"_fuse{fz}" if len(fuses) > 1 else ""))
+            for q in qs for fz in fuses] + [("plain", str(q), f"plain_q{q:g}") for q in qs]
+    for path, key, name in keys:
+        v = np.array([x[path][key] for x in rows])
+        v = v[np.isfinite(v)]
+        summary[name] = {
+            "mean_db": round(float(v.mean()), 3), "median_db": round(float(np.median(v)), 3),
+            "p10_db": round(float(np.percentile(v, 10)), 3),
+            "share_at_target": round(float(np.mean(v >= learned.TARGET_DB)), 4)}
     rep = {"genomes_scored": len(rows), "genomes_drawn": len(genomes), "shapes": a.shapes,
-           "per_shape": a.per_shape, "ref_quality": a.ref_quality, "fuse": learned.FUSE,
+           "per_shape": a.per_shape, "ref_quality": a.ref_quality, "fuses": fuses,
            "size": [W, H], "target_db": learned.TARGET_DB, "summary": summary,
            "seconds": round(time.time() - t0, 1), "rows": rows}
     with open(a.report, "w") as f:

@@ -2,11 +2,12 @@
 denoiser, gated per genome by the plain path as judge.
 
 A learned frame is one chaos dispatch at a low quality with a short fuse
-(`FUSE` turns instead of the plain path's 200: the network is trained on
-inputs rendered this way, and the fuse's sequential turns would otherwise
-dominate a frame of ~60 post-fuse turns per walker), the tonemap without DE
-into the f32 image, then the denoiser, which reads the histogram and that
-image and writes the RGBA8 frame (pyr3_amd/denoise/net.py).
+(`FUSE` turns instead of the plain path's 200: the fuse's sequential turns
+would otherwise dominate a frame of ~25 post-fuse turns per walker; the
+network was trained on 16-turn inputs and scores the same on 8- and 4-turn
+ones, measured on held-out genomes), the tonemap without DE into the f32
+image, then the denoiser, which reads the histogram and that image and
+writes the RGBA8 frame (pyr3_amd/denoise/net.py).
 
 Fallback: the plain high-quality render stays the judge. `LearnedRenderer`
 scores a genome's learned frame against its plain render at the reference
@@ -23,7 +24,7 @@ import numpy as np
 from . import pack
 from .genome import Genome
 
-FUSE = 16             # fuse of the learned path's input renders
+FUSE = 8              # fuse of the learned path's input renders
 QUALITY = 4           # default input quality (samples per pixel)
 TARGET_DB = 38.0
 REF_QUALITY = 4000
