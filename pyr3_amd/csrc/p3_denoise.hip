@@ -248,24 +248,27 @@ hipError_t launch_denoise(const DenoiseArgs& d, hipStream_t s) {
   uint16_t* const* t = d.act;
   // layer order and tensors as kDenoiseShape (p3_denoise.h); tile shapes
   // (pixels x rows per block) measured per layer at 1920x1080
-  // (16 x 16 x 32 MFMA, random data): e.g. c3 47.7 us at 64x2 against 66.4
-  // at 64x1, c2 48.8 at 64x4 against 61.5 at 64x2, b0 26.8 at 64x2 against
-  // 36.3 at 64x1, c1 53.0 at 32x8 against 59.0 at 64x4; the upsampling
-  // layers re-measured with their compile-time tap indexing: u2 32.3 at 32x8
-  // against 50.1 at 32x4, u1 32.7 at 32x16 against 41.0 at 64x4, u3 26.3 at
-  // 32x4 (35.96 at 64x4)
+  // (16 x 16 x 32 MFMA, random data, each layer alone): narrow tall tiles
+  // for the low-resolution layers (d3 14.4 us at 16x8 against 26.1 at 32x1,
+  // b0 17.2 at 16x8 against 26.3 at 64x2, c3 37.3 at 32x4 against 49.8 at
+  // 64x2, c2 45.6 at 16x16 against 51.1 at 64x4, e3 19.4 at 16x8 against
+  // 22.6 at 64x2, d1 24.7 at 32x8 against 26.3 at 64x2), wide ones at full
+  // resolution (e1 34.3 at 64x8, e2 23.5 at 64x4, c1 53.0 at 32x8 against
+  // 59.0 at 64x4); the upsampling layers with their compile-time tap
+  // indexing: u2 32.3 at 32x8 against 50.1 at 32x4, u1 32.7 at 32x16
+  // against 41.0 at 64x4, u3 26.3 at 32x4 (36.0 at 64x4)
   if ((e = run<8, 0, 16, 1, false, true, 64, 8>(conv(nullptr, nullptr, 0, t[0], W0, H0, W0, H0), s))) return e;
   if ((e = run<16, 0, 16, 1, false, false, 64, 8>(conv(t[0], nullptr, 1, t[1], W0, H0, W0, H0), s))) return e;
-  if ((e = run<16, 0, 32, 2, false, false, 64, 2>(conv(t[1], nullptr, 2, t[2], W0, H0, W1, H1), s))) return e;
+  if ((e = run<16, 0, 32, 2, false, false, 32, 8>(conv(t[1], nullptr, 2, t[2], W0, H0, W1, H1), s))) return e;
   if ((e = run<32, 0, 32, 1, false, false, 64, 4>(conv(t[2], nullptr, 3, t[3], W1, H1, W1, H1), s))) return e;
   if ((e = run<32, 0, 64, 2, false, false, 64, 1>(conv(t[3], nullptr, 4, t[4], W1, H1, W2, H2), s))) return e;
-  if ((e = run<64, 0, 64, 1, false, false, 64, 2>(conv(t[4], nullptr, 5, t[5], W2, H2, W2, H2), s))) return e;
-  if ((e = run<64, 0, 128, 2, false, false, 32, 1>(conv(t[5], nullptr, 6, t[6], W2, H2, W3, H3), s))) return e;
-  if ((e = run<128, 0, 128, 1, false, false, 64, 2>(conv(t[6], nullptr, 7, t[7], W3, H3, W3, H3), s))) return e;
+  if ((e = run<64, 0, 64, 1, false, false, 16, 8>(conv(t[4], nullptr, 5, t[5], W2, H2, W2, H2), s))) return e;
+  if ((e = run<64, 0, 128, 2, false, false, 16, 8>(conv(t[5], nullptr, 6, t[6], W2, H2, W3, H3), s))) return e;
+  if ((e = run<128, 0, 128, 1, false, false, 16, 8>(conv(t[6], nullptr, 7, t[7], W3, H3, W3, H3), s))) return e;
   if ((e = run<128, 0, 64, 1, true, false, 32, 4>(conv(t[7], nullptr, 8, t[8], W3, H3, W2, H2), s))) return e;
-  if ((e = run<64, 64, 64, 1, false, false, 64, 2>(conv(t[8], t[5], 9, t[9], W2, H2, W2, H2), s))) return e;
+  if ((e = run<64, 64, 64, 1, false, false, 32, 4>(conv(t[8], t[5], 9, t[9], W2, H2, W2, H2), s))) return e;
   if ((e = run<64, 0, 32, 1, true, false, 32, 8>(conv(t[9], nullptr, 10, t[10], W2, H2, W1, H1), s))) return e;
-  if ((e = run<32, 32, 32, 1, false, false, 64, 4>(conv(t[10], t[3], 11, t[11], W1, H1, W1, H1), s))) return e;
+  if ((e = run<32, 32, 32, 1, false, false, 16, 16>(conv(t[10], t[3], 11, t[11], W1, H1, W1, H1), s))) return e;
   if ((e = run<32, 0, 16, 1, true, false, 32, 16>(conv(t[11], nullptr, 12, t[12], W1, H1, W0, H0), s))) return e;
   return run<16, 16, 16, 1, false, false, 32, 8, true>(conv(t[12], t[1], 13, nullptr, W0, H0, W0, H0), s);
 }
