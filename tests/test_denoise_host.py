@@ -64,3 +64,14 @@ def test_shipped_weights_load():
     assert np.isfinite(flat).all() and np.abs(flat).max() < 100
     for L in net.LAYERS:
         assert not p[L.name + ".w"][:, 9 * L.cin:].any()
+
+
+def test_hipnet_refuses_a_library_loaded_before_torch(monkeypatch):
+    """torch's own HIP runtime cannot open the GPU once the native library has
+    bound /opt/rocm's: HipNet says so instead of failing inside torch."""
+    from pyr3_amd import _lib
+    from pyr3_amd.denoise import net
+    monkeypatch.setattr(_lib, "_lib", object())
+    monkeypatch.setattr(_lib, "loaded_after_torch", False)
+    with pytest.raises(RuntimeError, match="loaded before torch"):
+        net.HipNet(net.init_params(0))
