@@ -632,7 +632,7 @@ def main():
     }
     if learned is not None:
         if learned.get("reached"):
-            # the north-star ratio: learned frame time / plain frame time
+            # learned frame time / plain frame time at the same PSNR target
             learned["vs_plain"] = round(learned["ms_per_frame"] / ms, 4)
         out["learned"] = learned
     if configs is not None:
